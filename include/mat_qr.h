@@ -1,0 +1,185 @@
+/*
+ * mat_qr.h -- C ABI of the MI355X (gfx950) Householder QR library.
+ *
+ * The reference (Mat-8.3.1) is a header-only C++ library with no FFI layer:
+ * its "operator API" is the template call QR(A, Q, R, QR_Householder)
+ * (QR/QR.h:7-46) on column-major MAT<T> (Base/__Matrix.h:2-174).  This header
+ * is the boundary that call is routed through: plain pointers, 64-bit sizes,
+ * explicit leading dimensions, no C++ or HIP types.  include/mat/Mat.h keeps
+ * the reference's MAT<T> / QR() signatures on top of it.
+ *
+ * Conventions
+ *  - Column-major storage, element (r, c) at A[r + c * lda], lda >= max(1, m).
+ *  - Every function returns an int status: 0 = success; -i = the i-th
+ *    argument is invalid (LAPACK info style, counting from 1); positive codes
+ *    are runtime errors (MQR_ERR_*), described by mqr_strerror().  No C++
+ *    exception crosses this boundary.
+ *  - Functions taking a handle run on the handle's device and HIP stream and
+ *    are stream-ordered: they return once the work is queued unless the
+ *    description says otherwise.  Device pointers must be device memory of the
+ *    handle's device.  Distinct handles may be used from distinct threads.
+ *  - Reflector conventions follow the reference panel
+ *    (QR/__Householder_Blocked.h:62-84): R_jj = -sign(a_jj) * ||a_j:m,j||
+ *    with sign(0) = +, v_j = 1 implicit, tau_j = (|a_jj| + ||.||) / ||.||;
+ *    a column whose norm is 0 gets tau = 0; a column with nothing below the
+ *    diagonal is still reflected (tau = 2).
+ *  - Compact-WY T blocks (forward, columnwise) use the LAPACK dgeqrt layout:
+ *    for the block of nb columns starting at column j, T(0:b, j:j+b) is the
+ *    b x b upper-triangular factor with I - V T V^T = H_j ... H_{j+b-1}.
+ *    The reference keeps the same factors at T_all + j * BS with leading
+ *    dimension b (QR/__Householder_Blocked.h:236).
+ */
+#ifndef MAT_QR_H
+#define MAT_QR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define MQR_SUCCESS 0
+#define MQR_ERR_NO_DEVICE 1001     /* no HIP device / device index out of range */
+#define MQR_ERR_HIP 1002           /* a HIP runtime call failed */
+#define MQR_ERR_ALLOC 1003         /* device allocation failed */
+#define MQR_ERR_NOT_CONVERGED 1004 /* iterative refinement did not converge */
+#define MQR_ERR_UNSUPPORTED 1005   /* valid request this build does not support */
+#define MQR_ERR_NOT_INITIALIZED 1006
+#define MQR_ERR_INTERNAL 1007
+
+typedef struct mqr_handle_st *mqr_handle_t;
+
+typedef enum { MQR_F64 = 0, MQR_F32 = 1 } mqr_dtype_t;
+
+/* Arithmetic of the trailing updates in the mixed-precision least squares.
+ * Panels, T factors and refinement residuals always stay FP64. */
+typedef enum {
+    MQR_PREC_FP64 = 0,   /* plain FP64 MFMA */
+    MQR_PREC_FP32 = 1,   /* FP32 MFMA, FP32 accumulation */
+    MQR_PREC_BF16X3 = 2, /* split-BF16 (hi*hi + hi*lo + lo*hi) MFMA, FP32 accumulation */
+    MQR_PREC_BF16 = 3    /* single BF16 MFMA product, FP32 accumulation */
+} mqr_precision_t;
+
+/* ---- library / handle ---------------------------------------------------- */
+/* No reference counterpart (the reference has no runtime state). */
+const char *mqr_strerror(int status);
+int mqr_get_device_count(int *count);
+int mqr_create(mqr_handle_t *handle, int device);
+int mqr_destroy(mqr_handle_t handle);
+/* stream: a hipStream_t, or NULL for the handle's own stream */
+int mqr_set_stream(mqr_handle_t handle, void *stream);
+int mqr_get_stream(mqr_handle_t handle, void **stream);
+/* Block size nb of the blocked factorisation (the reference's __BlockSize_QR,
+ * Mat.h:79-81,105-107); 16 <= nb <= 64, multiple of 16. */
+int mqr_set_block_size(mqr_handle_t handle, int nb);
+int mqr_get_block_size(mqr_handle_t handle, int *nb);
+int mqr_synchronize(mqr_handle_t handle);
+/* Device memory helpers so that callers need no HIP headers. */
+int mqr_malloc(mqr_handle_t handle, void **ptr, size_t bytes);
+int mqr_free(mqr_handle_t handle, void *ptr);
+int mqr_memcpy_htod(mqr_handle_t handle, void *dst, const void *src, size_t bytes);
+int mqr_memcpy_dtoh(mqr_handle_t handle, void *dst, const void *src, size_t bytes);
+
+/* ---- kernel timing (no reference counterpart; the reference has no profiler) */
+typedef struct {
+    char name[24];    /* kernel kind: panel, vta_gemm, larft, w2, update_gemm, ... */
+    int64_t launches; /* launches timed */
+    double total_ms;  /* sum of HIP-event durations */
+    double flops;     /* algorithmic flops of those launches */
+    double bytes;     /* algorithmic HBM bytes of those launches */
+} mqr_kernel_stat_t;
+
+/* Turn per-launch HIP event timing on or off for calls on this handle
+ * (clears what was recorded). */
+int mqr_profile_enable(mqr_handle_t handle, int on);
+/* Synchronise, then return one entry per kernel kind timed since the last
+ * read or enable (at most max_stats are written; *count gets the number of
+ * kinds), and clear the records. */
+int mqr_profile_read(mqr_handle_t handle, mqr_kernel_stat_t *stats, int max_stats, int *count);
+
+/* ---- FP64 GEMM tile (device pointers) ----------------------------------- */
+/* C = alpha * op(A) * op(B) + beta * C with op(X) = X ('N') or X^T ('T');
+ * op(A) is m x k, op(B) k x n.  The device counterpart of the packed
+ * MR x NR micro-kernel and its drivers behind operator*
+ * (Mul/__micro_kernel_packed_double.h:301-585, Mul/Mul.h:123-235). */
+int mqr_dgemm(mqr_handle_t handle, char transa, char transb, int64_t m, int64_t n, int64_t k,
+              double alpha, const double *A, int64_t lda, const double *B, int64_t ldb,
+              double beta, double *C, int64_t ldc);
+
+/* ---- FP64 blocked Householder QR (device pointers) ------------------------ */
+/* Factor the m x n matrix A in place: R on and above the diagonal, the
+ * reflectors below it, tau[min(m,n)], and the T blocks in T (ldt >= nb, ldt x
+ * min(m,n)).  Replaces __QR_HouseHolder_Blocked(R, buf, m, n)
+ * (QR/__Householder_Blocked.h:214-245) with its panel (:62-84), T build
+ * (:86-103) and trailing update (:143-212). */
+int mqr_dgeqrf(mqr_handle_t handle, int64_t m, int64_t n, double *A, int64_t lda,
+               double *tau, double *T, int64_t ldt);
+
+/* Q(:, 0:nq) = H_0 ... H_{k-1} applied to the first nq columns of I_m
+ * (nq = k: thin Q, nq = m: full Q), from a factorisation by mqr_dgeqrf.
+ * Replaces __QR_HouseHolder_Blocked_CalculateQ (QR/__Householder_Blocked.h:247-333);
+ * the known-zero part of Q is skipped. */
+int mqr_dorgqr(mqr_handle_t handle, int64_t m, int64_t nq, int64_t k, const double *VR,
+               int64_t ldvr, const double *T, int64_t ldt, double *Q, int64_t ldq);
+
+/* C <- Q^T C (trans = 'T') or Q C (trans = 'N') for the m x nrhs matrix C,
+ * with Q held as k reflectors by mqr_dgeqrf.  The block-reflector pattern of
+ * QR/__Householder_Blocked.h:143-212 with C as the right-hand side. */
+int mqr_dormqr(mqr_handle_t handle, char trans, int64_t m, int64_t nrhs, int64_t k,
+               const double *VR, int64_t ldvr, const double *T, int64_t ldt, double *C,
+               int64_t ldc);
+
+/* Solve R X = B (trans = 'N') or R^T X = B (trans = 'T') in place for the
+ * upper triangle of the n x n matrix R (the diagonal must be nonzero).
+ * Pattern of the blocked back substitution LU/__Solve_direct.h:39-65. */
+int mqr_dtrsm_upper(mqr_handle_t handle, char trans, int64_t n, int64_t nrhs, const double *R,
+                    int64_t ldr, double *B, int64_t ldb);
+
+/* Plain FP64 least squares min ||A X - B|| for m >= n: A is overwritten by
+ * its factorisation, B (m x nrhs) by Q^T B, whose first n rows hold X.
+ * No reference counterpart (new on top of the QR path). */
+int mqr_dgels(mqr_handle_t handle, int64_t m, int64_t n, int64_t nrhs, double *A, int64_t lda,
+              double *B, int64_t ldb);
+
+/* ---- mixed precision least squares with iterative refinement ------------ */
+typedef struct {
+    int max_iter;    /* refinement steps allowed (default 30) */
+    double tol;      /* stop when the backward-error estimate <= tol (default 1e-15) */
+    int fallback;    /* 1: on stagnation refactor in plain FP64 and solve (default 1) */
+} mqr_ir_opts_t;
+
+typedef struct {
+    int iterations;       /* refinement steps taken */
+    int converged;        /* 1 when tol was met */
+    int fell_back;        /* 1 when the FP64 fallback ran */
+    double backward_error;/* ||A^T r|| / (||A||_F (||A||_F ||x|| + ||b||)) at exit */
+    double residual_norm; /* ||b - A x|| at exit */
+} mqr_ir_info_t;
+
+/* x = argmin ||A x - b|| for one right-hand side, m >= n.  A (m x n, FP64,
+ * device) and b (m, device) are not modified; x (n, device) is written.
+ * The factorisation runs its trailing updates in `prec`; refinement works on
+ * the augmented system [I A; A^T 0][r; x] = [b; 0] with FP64 residuals.
+ * Blocks until the solve is done (the stopping test is read on the host).
+ * No reference counterpart. */
+int mqr_gels_mixed(mqr_handle_t handle, mqr_precision_t prec, int64_t m, int64_t n,
+                   const double *A, int64_t lda, const double *b, double *x,
+                   const mqr_ir_opts_t *opts, mqr_ir_info_t *info);
+
+/* ---- host-memory convenience (used by include/mat/Mat.h) ------------------ */
+/* QR(A, Q, R, QR_Householder) semantics (QR/QR.h:7-24 with
+ * QR/__Householder_Blocked.h:335-353): A m x n (lda = m) is read, Q m x m and
+ * R m x n (R zero below the diagonal) are written; all host memory.  Blocks. */
+int mqr_qr_householder_host(mqr_handle_t handle, int64_t m, int64_t n, const double *A,
+                            double *Q, double *R);
+/* Host least squares: A m x n (lda = m), b (m), x (n) host memory; prec as
+ * above (MQR_PREC_FP64 = plain solve, no refinement).  Blocks. */
+int mqr_gels_host(mqr_handle_t handle, mqr_precision_t prec, int64_t m, int64_t n,
+                  const double *A, const double *b, double *x, mqr_ir_info_t *info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAT_QR_H */

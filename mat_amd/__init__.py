@@ -1,0 +1,328 @@
+"""Python mirror of the MI355X Householder-QR library's C ABI (include/mat_qr.h).
+
+The product is ``mat_amd/lib/libmatqr.so`` (HIP, gfx950).  This module binds
+its ``extern "C"`` entry points with ctypes and adds thin helpers for
+column-major device buffers held in torch tensors (torch supplies device
+memory and streams only).  There is no CPU fallback: every call goes to the
+HIP library, and loading fails loudly when the library is missing.
+
+Column-major convention: a logical m x n matrix is a torch tensor of shape
+(m, n) with strides (1, ld), as returned by :func:`colmajor`; element (r, c)
+sits at ``data[r + c * ld]`` like ``MAT<T>`` (Base/__Matrix.h:141-146).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmatqr.so")
+
+# QR_type of the reference (Mat.h:43-47); only QR_Householder is on the GPU path.
+QR_Householder, QR_Givens, QR_GramSchmidt = 0, 1, 2
+
+# status codes (include/mat_qr.h)
+MQR_SUCCESS = 0
+MQR_ERR_NO_DEVICE = 1001
+MQR_ERR_HIP = 1002
+MQR_ERR_ALLOC = 1003
+MQR_ERR_NOT_CONVERGED = 1004
+MQR_ERR_UNSUPPORTED = 1005
+MQR_ERR_NOT_INITIALIZED = 1006
+MQR_ERR_INTERNAL = 1007
+
+MQR_F64, MQR_F32 = 0, 1
+PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2, "bf16": 3}
+
+# Every exported symbol of include/mat_qr.h with its ctypes signature.
+_i64, _i32, _vp, _dbl = C.c_int64, C.c_int, C.c_void_p, C.c_double
+SIGNATURES = {
+    "mqr_strerror": (C.c_char_p, [_i32]),
+    "mqr_get_device_count": (_i32, [_vp]),
+    "mqr_create": (_i32, [_vp, _i32]),
+    "mqr_destroy": (_i32, [_vp]),
+    "mqr_set_stream": (_i32, [_vp, _vp]),
+    "mqr_get_stream": (_i32, [_vp, _vp]),
+    "mqr_set_block_size": (_i32, [_vp, _i32]),
+    "mqr_get_block_size": (_i32, [_vp, _vp]),
+    "mqr_synchronize": (_i32, [_vp]),
+    "mqr_malloc": (_i32, [_vp, _vp, C.c_size_t]),
+    "mqr_free": (_i32, [_vp, _vp]),
+    "mqr_memcpy_htod": (_i32, [_vp, _vp, _vp, C.c_size_t]),
+    "mqr_memcpy_dtoh": (_i32, [_vp, _vp, _vp, C.c_size_t]),
+    "mqr_profile_enable": (_i32, [_vp, _i32]),
+    "mqr_profile_read": (_i32, [_vp, _vp, _i32, _vp]),
+    "mqr_dgemm": (_i32, [_vp, C.c_char, C.c_char, _i64, _i64, _i64, _dbl, _vp, _i64, _vp, _i64,
+                         _dbl, _vp, _i64]),
+    "mqr_dgeqrf": (_i32, [_vp, _i64, _i64, _vp, _i64, _vp, _vp, _i64]),
+    "mqr_dorgqr": (_i32, [_vp, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64]),
+    "mqr_dormqr": (_i32, [_vp, C.c_char, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64]),
+    "mqr_dtrsm_upper": (_i32, [_vp, C.c_char, _i64, _i64, _vp, _i64, _vp, _i64]),
+    "mqr_dgels": (_i32, [_vp, _i64, _i64, _i64, _vp, _i64, _vp, _i64]),
+    "mqr_gels_mixed": (_i32, [_vp, _i32, _i64, _i64, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "mqr_qr_householder_host": (_i32, [_vp, _i64, _i64, _vp, _vp, _vp]),
+    "mqr_gels_host": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp]),
+}
+
+
+class IROpts(C.Structure):
+    _fields_ = [("max_iter", C.c_int), ("tol", C.c_double), ("fallback", C.c_int)]
+
+
+class IRInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("fell_back", C.c_int),
+                ("backward_error", C.c_double), ("residual_norm", C.c_double)]
+
+
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 24), ("launches", C.c_int64), ("total_ms", C.c_double),
+                ("flops", C.c_double), ("bytes", C.c_double)]
+
+
+def profile_enable(h: "Handle", on: bool = True) -> None:
+    check(lib().mqr_profile_enable(h.h, 1 if on else 0), "mqr_profile_enable")
+
+
+def profile_read(h: "Handle") -> dict:
+    """{kind: {launches, total_ms, flops, bytes}} since the last read/enable."""
+    arr = (KernelStat * 16)()
+    cnt = C.c_int(0)
+    check(lib().mqr_profile_read(h.h, C.cast(arr, C.c_void_p), 16, C.byref(cnt)),
+          "mqr_profile_read")
+    out = {}
+    for i in range(min(cnt.value, 16)):
+        s = arr[i]
+        out[s.name.decode()] = {"launches": int(s.launches), "total_ms": float(s.total_ms),
+                                "flops": float(s.flops), "bytes": float(s.bytes)}
+    return out
+
+
+class MqrError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        self.status = status
+        super().__init__(f"{where} failed with status {status}: {strerror(status)}")
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    """Load libmatqr.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m mat_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def strerror(status: int) -> str:
+    return lib().mqr_strerror(status).decode()
+
+
+def check(status: int, where: str) -> None:
+    if status != MQR_SUCCESS:
+        raise MqrError(status, where)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    lib().mqr_get_device_count(C.byref(n))
+    return n.value
+
+
+class Handle:
+    """mqr_handle_t: device, stream (torch's current stream by default) and
+    workspace."""
+
+    def __init__(self, device: int = 0, use_torch_stream: bool = True, nb: Optional[int] = None):
+        self.h = C.c_void_p()
+        check(lib().mqr_create(C.byref(self.h), device), "mqr_create")
+        self.device = device
+        if use_torch_stream:
+            import torch
+            s = torch.cuda.current_stream(device)
+            check(lib().mqr_set_stream(self.h, C.c_void_p(s.cuda_stream)), "mqr_set_stream")
+        if nb is not None:
+            check(lib().mqr_set_block_size(self.h, nb), "mqr_set_block_size")
+
+    @property
+    def nb(self) -> int:
+        v = C.c_int(0)
+        check(lib().mqr_get_block_size(self.h, C.byref(v)), "mqr_get_block_size")
+        return v.value
+
+    def synchronize(self) -> None:
+        check(lib().mqr_synchronize(self.h), "mqr_synchronize")
+
+    def close(self) -> None:
+        if self.h:
+            check(lib().mqr_destroy(self.h), "mqr_destroy")
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# column-major device buffers in torch tensors
+# ---------------------------------------------------------------------------
+
+def colmajor(x, device="cuda", dtype=None):
+    """A (m, n) torch tensor with strides (1, m) holding x (numpy or torch)."""
+    import numpy as np
+    import torch
+    if isinstance(x, np.ndarray):
+        t = torch.from_numpy(np.ascontiguousarray(x.T))
+    else:
+        t = x.T.contiguous()
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to(device).T
+
+
+def empty_colmajor(m: int, n: int, device="cuda", dtype=None):
+    import torch
+    return torch.empty((n, m), device=device, dtype=dtype or torch.float64).T
+
+
+def ld(t) -> int:
+    """Leading dimension of a column-major (m, n) tensor."""
+    assert t.dim() == 2 and t.stride(0) == 1, "expected a column-major tensor"
+    return max(t.stride(1), t.shape[0], 1)
+
+
+def _ptr(t) -> C.c_void_p:
+    return C.c_void_p(t.data_ptr() if t is not None else 0)
+
+
+# ---------------------------------------------------------------------------
+# device-level entry points
+# ---------------------------------------------------------------------------
+
+def gemm(h: Handle, transa: str, transb: str, alpha: float, A, B, beta: float, Cm):
+    """C = alpha op(A) op(B) + beta C on column-major float64 device tensors."""
+    m, n = Cm.shape
+    k = A.shape[0] if transa == "T" else A.shape[1]
+    check(lib().mqr_dgemm(h.h, transa.encode(), transb.encode(), m, n, k, alpha, _ptr(A), ld(A),
+                          _ptr(B), ld(B), beta, _ptr(Cm), ld(Cm)), "mqr_dgemm")
+    return Cm
+
+
+def geqrf(h: Handle, A):
+    """Factor the column-major float64 device matrix A in place; returns
+    (tau, T) with T of shape (nb, k) in the dgeqrt layout."""
+    import torch
+    m, n = A.shape
+    k = min(m, n)
+    tau = torch.zeros(max(k, 1), device=A.device, dtype=torch.float64)
+    nb = h.nb
+    T = empty_colmajor(nb, max(k, 1), device=A.device)
+    T.zero_()
+    check(lib().mqr_dgeqrf(h.h, m, n, _ptr(A), ld(A), _ptr(tau), _ptr(T), nb), "mqr_dgeqrf")
+    return tau[:k], T
+
+
+def orgqr(h: Handle, VR, T, k: int, nq: Optional[int] = None):
+    m = VR.shape[0]
+    nq = m if nq is None else nq
+    Q = empty_colmajor(m, nq, device=VR.device)
+    check(lib().mqr_dorgqr(h.h, m, nq, k, _ptr(VR), ld(VR), _ptr(T), ld(T), _ptr(Q), ld(Q)),
+          "mqr_dorgqr")
+    return Q
+
+
+def ormqr(h: Handle, trans: str, VR, T, k: int, Cm):
+    m, nrhs = Cm.shape
+    check(lib().mqr_dormqr(h.h, trans.encode(), m, nrhs, k, _ptr(VR), ld(VR), _ptr(T), ld(T),
+                           _ptr(Cm), ld(Cm)), "mqr_dormqr")
+    return Cm
+
+
+def trsm_upper(h: Handle, trans: str, R, B):
+    n = R.shape[1]
+    check(lib().mqr_dtrsm_upper(h.h, trans.encode(), n, B.shape[1], _ptr(R), ld(R), _ptr(B), ld(B)),
+          "mqr_dtrsm_upper")
+    return B
+
+
+def gels(h: Handle, A, B):
+    """Plain FP64 least squares; A and B are overwritten, returns X = B[:n]."""
+    m, n = A.shape
+    check(lib().mqr_dgels(h.h, m, n, B.shape[1], _ptr(A), ld(A), _ptr(B), ld(B)), "mqr_dgels")
+    return B[:n]
+
+
+def gels_mixed(h: Handle, prec: str, A, b, max_iter: int = 30, tol: float = 0.0,
+               fallback: bool = True):
+    """Mixed-precision least squares with refinement; returns (x, IRInfo)."""
+    import torch
+    m, n = A.shape
+    x = torch.zeros(n, device=A.device, dtype=torch.float64)
+    opts = IROpts(max_iter, tol, 1 if fallback else 0)
+    info = IRInfo()
+    check(lib().mqr_gels_mixed(h.h, PREC[prec], m, n, _ptr(A), ld(A), _ptr(b), _ptr(x),
+                               C.byref(opts), C.byref(info)), "mqr_gels_mixed")
+    return x, info
+
+
+# ---------------------------------------------------------------------------
+# host-level calls with the reference's signatures
+# ---------------------------------------------------------------------------
+
+_default_handle: Optional[Handle] = None
+
+
+def default_handle() -> Handle:
+    global _default_handle
+    if _default_handle is None:
+        _default_handle = Handle(0, use_torch_stream=False)
+    return _default_handle
+
+
+def QR(A, Q, R, qr_type: int = QR_Householder, handle: Optional[Handle] = None) -> None:
+    """QR(A, Q, R, QR_Householder) on host float64 Fortran-ordered numpy
+    arrays, like QR/QR.h:7-24: Q (m x m) and R (m x n, zero below the
+    diagonal) are written in place.  Only QR_Householder runs on the GPU."""
+    import numpy as np
+    if qr_type != QR_Householder:
+        raise MqrError(MQR_ERR_UNSUPPORTED, "QR")
+    h = handle or default_handle()
+    A = np.asfortranarray(A, dtype=np.float64)
+    m, n = A.shape
+    assert Q.shape == (m, m) and R.shape == (m, n)
+    assert Q.flags.f_contiguous and R.flags.f_contiguous
+    assert Q.dtype == np.float64 and R.dtype == np.float64
+    check(lib().mqr_qr_householder_host(h.h, m, n, A.ctypes.data_as(C.c_void_p),
+                                        Q.ctypes.data_as(C.c_void_p), R.ctypes.data_as(C.c_void_p)),
+          "mqr_qr_householder_host")
+
+
+def LeastSquares(A, b, prec: str = "fp64", handle: Optional[Handle] = None):
+    """x = argmin ||A x - b|| on host arrays; returns (x, IRInfo)."""
+    import numpy as np
+    h = handle or default_handle()
+    A = np.asfortranarray(A, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    m, n = A.shape
+    x = np.zeros(n)
+    info = IRInfo()
+    check(lib().mqr_gels_host(h.h, PREC[prec], m, n, A.ctypes.data_as(C.c_void_p),
+                              b.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p),
+                              C.byref(info) if prec != "fp64" else None), "mqr_gels_host")
+    return x, info

@@ -1,0 +1,443 @@
+// C ABI entry points (include/mat_qr.h): handle, workspace, argument checks,
+// and the host-memory conveniences the header-only MAT<T> API calls.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "mqr_internal.h"
+
+namespace mqr {
+
+int geqrf_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, double *tau,
+              double *T, int64_t ldt);
+int ormqr_f64(mqr_handle_t hd, bool trans, int64_t m, int64_t nrhs, int64_t k, const double *VR,
+              int64_t ldvr, const double *T, int64_t ldt, double *C, int64_t ldc);
+int orgqr_f64(mqr_handle_t hd, int64_t m, int64_t nq, int64_t k, const double *VR, int64_t ldvr,
+              const double *T, int64_t ldt, double *Q, int64_t ldq);
+int trsm_upper_f64(mqr_handle_t hd, bool trans, int64_t n, int64_t nrhs, const double *R,
+                   int64_t ldr, double *B, int64_t ldb);
+int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, const double *A,
+               int64_t lda, const double *b, double *x, const mqr_ir_opts_t *opts,
+               mqr_ir_info_t *info);
+
+namespace {
+thread_local char g_last_hip[256] = {0};
+
+// Makes the handle's device current for the duration of a call.
+struct DeviceGuard {
+    int prev = -1;
+    bool ok = true;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) ok = hipSetDevice(dev) == hipSuccess;
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+}  // namespace
+
+void set_last_hip_error(hipError_t e, const char *where) {
+    std::snprintf(g_last_hip, sizeof(g_last_hip), "%s: %s", where, hipGetErrorString(e));
+}
+
+int ws_reserve(mqr_handle_t h, Workspace &ws, size_t bytes) {
+    if (ws.bytes >= bytes) return MQR_SUCCESS;
+    MQR_HIP(hipStreamSynchronize(h->stream));
+    if (ws.ptr) {
+        MQR_HIP(hipFree(ws.ptr));
+        ws.ptr = nullptr;
+        ws.bytes = 0;
+    }
+    const size_t want = ((bytes + bytes / 4) + (2u << 20) - 1) & ~(size_t)((2u << 20) - 1);
+    if (hipMalloc(&ws.ptr, want) != hipSuccess) {
+        ws.ptr = nullptr;
+        (void)hipGetLastError();
+        return MQR_ERR_ALLOC;
+    }
+    ws.bytes = want;
+    return MQR_SUCCESS;
+}
+
+}  // namespace mqr
+
+using namespace mqr;
+
+#define MQR_GUARD(h)                                  \
+    if (!(h)) return -1;                              \
+    DeviceGuard guard_((h)->device);                  \
+    if (!guard_.ok) return MQR_ERR_NO_DEVICE;         \
+    ::mqr::t_prof = (h)->prof.on ? &(h)->prof : nullptr
+
+extern "C" {
+
+const char *mqr_strerror(int status) {
+    if (status == MQR_SUCCESS) return "success";
+    if (status < 0) return "invalid argument (the status is minus its 1-based position)";
+    switch (status) {
+        case MQR_ERR_NO_DEVICE: return "no HIP device available (or device index out of range)";
+        case MQR_ERR_HIP: return g_last_hip[0] ? g_last_hip : "HIP runtime error";
+        case MQR_ERR_ALLOC: return "device memory allocation failed";
+        case MQR_ERR_NOT_CONVERGED: return "iterative refinement did not converge";
+        case MQR_ERR_UNSUPPORTED: return "request not supported by this build";
+        case MQR_ERR_NOT_INITIALIZED: return "handle not initialised";
+        case MQR_ERR_INTERNAL: return "internal error";
+        default: return "unknown status";
+    }
+}
+
+int mqr_get_device_count(int *count) {
+    if (!count) return -1;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) {
+        (void)hipGetLastError();
+        *count = 0;
+        return MQR_ERR_NO_DEVICE;
+    }
+    *count = c;
+    return c > 0 ? MQR_SUCCESS : MQR_ERR_NO_DEVICE;
+}
+
+int mqr_create(mqr_handle_t *handle, int device) {
+    if (!handle) return -1;
+    *handle = nullptr;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess || c <= 0) {
+        (void)hipGetLastError();
+        return MQR_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= c) return -2;
+    DeviceGuard guard(device);
+    if (!guard.ok) return MQR_ERR_NO_DEVICE;
+    mqr_handle_st *h = new (std::nothrow) mqr_handle_st();
+    if (!h) return MQR_ERR_ALLOC;
+    h->device = device;
+    if (hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return MQR_ERR_HIP;
+    }
+    h->stream = h->own_stream;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+        cus > 0)
+        h->cu_count = cus;
+    if (hipHostMalloc((void **)&h->host_pinned, 64 * sizeof(double), hipHostMallocDefault) !=
+        hipSuccess) {
+        (void)hipStreamDestroy(h->own_stream);
+        delete h;
+        return MQR_ERR_ALLOC;
+    }
+    *handle = h;
+    return MQR_SUCCESS;
+}
+
+int mqr_destroy(mqr_handle_t h) {
+    MQR_GUARD(h);
+    (void)hipStreamSynchronize(h->stream);
+    if (h->ws.ptr) (void)hipFree(h->ws.ptr);
+    if (h->ws_ir.ptr) (void)hipFree(h->ws_ir.ptr);
+    if (h->host_pinned) (void)hipHostFree(h->host_pinned);
+    if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+    delete h;
+    return MQR_SUCCESS;
+}
+
+int mqr_set_stream(mqr_handle_t h, void *stream) {
+    if (!h) return -1;
+    h->stream = stream ? (hipStream_t)stream : h->own_stream;
+    return MQR_SUCCESS;
+}
+
+int mqr_get_stream(mqr_handle_t h, void **stream) {
+    if (!h) return -1;
+    if (!stream) return -2;
+    *stream = (void *)h->stream;
+    return MQR_SUCCESS;
+}
+
+int mqr_set_block_size(mqr_handle_t h, int nb) {
+    if (!h) return -1;
+    if (nb < 16 || nb > kMaxNb || nb % 16 != 0) return -2;
+    h->nb = nb;
+    return MQR_SUCCESS;
+}
+
+int mqr_get_block_size(mqr_handle_t h, int *nb) {
+    if (!h) return -1;
+    if (!nb) return -2;
+    *nb = h->nb;
+    return MQR_SUCCESS;
+}
+
+int mqr_synchronize(mqr_handle_t h) {
+    MQR_GUARD(h);
+    MQR_HIP(hipStreamSynchronize(h->stream));
+    return MQR_SUCCESS;
+}
+
+int mqr_malloc(mqr_handle_t h, void **ptr, size_t bytes) {
+    MQR_GUARD(h);
+    if (!ptr) return -2;
+    *ptr = nullptr;
+    if (bytes == 0) return MQR_SUCCESS;
+    if (hipMalloc(ptr, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        *ptr = nullptr;
+        return MQR_ERR_ALLOC;
+    }
+    return MQR_SUCCESS;
+}
+
+int mqr_free(mqr_handle_t h, void *ptr) {
+    MQR_GUARD(h);
+    if (ptr) MQR_HIP(hipFree(ptr));
+    return MQR_SUCCESS;
+}
+
+int mqr_memcpy_htod(mqr_handle_t h, void *dst, const void *src, size_t bytes) {
+    MQR_GUARD(h);
+    if (bytes && !dst) return -2;
+    if (bytes && !src) return -3;
+    if (bytes) {
+        MQR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
+        MQR_HIP(hipStreamSynchronize(h->stream));
+    }
+    return MQR_SUCCESS;
+}
+
+int mqr_memcpy_dtoh(mqr_handle_t h, void *dst, const void *src, size_t bytes) {
+    MQR_GUARD(h);
+    if (bytes && !dst) return -2;
+    if (bytes && !src) return -3;
+    if (bytes) {
+        MQR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream));
+        MQR_HIP(hipStreamSynchronize(h->stream));
+    }
+    return MQR_SUCCESS;
+}
+
+int mqr_dgemm(mqr_handle_t h, char transa, char transb, int64_t m, int64_t n, int64_t k,
+              double alpha, const double *A, int64_t lda, const double *B, int64_t ldb,
+              double beta, double *C, int64_t ldc) {
+    if (!h) return -1;
+    const bool ta = transa == 'T' || transa == 't', tb = transb == 'T' || transb == 't';
+    if (!ta && transa != 'N' && transa != 'n') return -2;
+    if (!tb && transb != 'N' && transb != 'n') return -3;
+    if (m < 0 || m > INT32_MAX) return -4;
+    if (n < 0 || n > INT32_MAX) return -5;
+    if (k < 0 || k > INT32_MAX) return -6;
+    if (m * k > 0 && !A) return -8;
+    if (lda < std::max<int64_t>(1, ta ? k : m)) return -9;
+    if (k * n > 0 && !B) return -10;
+    if (ldb < std::max<int64_t>(1, tb ? n : k)) return -11;
+    if (m * n > 0 && !C) return -13;
+    if (ldc < std::max<int64_t>(1, m)) return -14;
+    if (m == 0 || n == 0) return MQR_SUCCESS;
+    MQR_GUARD(h);
+    if (k == 0 || alpha == 0.0) return scale_f64(m, n, beta, C, ldc, h->stream);
+    GemmF64 g;
+    g.M = (int)m; g.N = (int)n; g.K = (int)k;
+    g.ta = ta; g.tb = tb;
+    g.A = A; g.lda = lda; g.B = B; g.ldb = ldb;
+    g.C = C; g.ldc = ldc; g.alpha = alpha; g.beta = beta;
+    return gemm_f64(g, h->stream);
+}
+
+int mqr_dgeqrf(mqr_handle_t h, int64_t m, int64_t n, double *A, int64_t lda, double *tau,
+               double *T, int64_t ldt) {
+    if (!h) return -1;
+    if (m < 0) return -2;
+    if (n < 0) return -3;
+    const int64_t k = std::min(m, n);
+    if (m * n > 0 && !A) return -4;
+    if (lda < std::max<int64_t>(1, m)) return -5;
+    if (k > 0 && !tau) return -6;
+    if (k > 0 && !T) return -7;
+    if (ldt < std::min<int64_t>(h->nb, std::max<int64_t>(k, 1))) return -8;
+    if (k == 0) return MQR_SUCCESS;
+    if (m > INT32_MAX || n > INT32_MAX) return MQR_ERR_UNSUPPORTED;
+    MQR_GUARD(h);
+    return geqrf_f64(h, m, n, A, lda, tau, T, ldt);
+}
+
+int mqr_dorgqr(mqr_handle_t h, int64_t m, int64_t nq, int64_t k, const double *VR, int64_t ldvr,
+               const double *T, int64_t ldt, double *Q, int64_t ldq) {
+    if (!h) return -1;
+    if (m < 0) return -2;
+    if (nq < 0 || nq > m) return -3;
+    if (k < 0 || k > m) return -4;
+    if (k > 0 && !VR) return -5;
+    if (ldvr < std::max<int64_t>(1, m)) return -6;
+    if (k > 0 && !T) return -7;
+    if (ldt < std::min<int64_t>(h->nb, std::max<int64_t>(k, 1))) return -8;
+    if (m * nq > 0 && !Q) return -9;
+    if (ldq < std::max<int64_t>(1, m)) return -10;
+    if (m * nq == 0) return MQR_SUCCESS;
+    MQR_GUARD(h);
+    return orgqr_f64(h, m, nq, k, VR, ldvr, T, ldt, Q, ldq);
+}
+
+int mqr_dormqr(mqr_handle_t h, char trans, int64_t m, int64_t nrhs, int64_t k, const double *VR,
+               int64_t ldvr, const double *T, int64_t ldt, double *C, int64_t ldc) {
+    if (!h) return -1;
+    const bool tr = trans == 'T' || trans == 't';
+    if (!tr && trans != 'N' && trans != 'n') return -2;
+    if (m < 0) return -3;
+    if (nrhs < 0) return -4;
+    if (k < 0 || k > m) return -5;
+    if (k > 0 && !VR) return -6;
+    if (ldvr < std::max<int64_t>(1, m)) return -7;
+    if (k > 0 && !T) return -8;
+    if (ldt < std::min<int64_t>(h->nb, std::max<int64_t>(k, 1))) return -9;
+    if (m * nrhs > 0 && !C) return -10;
+    if (ldc < std::max<int64_t>(1, m)) return -11;
+    if (m * nrhs == 0 || k == 0) return MQR_SUCCESS;
+    MQR_GUARD(h);
+    return ormqr_f64(h, tr, m, nrhs, k, VR, ldvr, T, ldt, C, ldc);
+}
+
+int mqr_dtrsm_upper(mqr_handle_t h, char trans, int64_t n, int64_t nrhs, const double *R,
+                    int64_t ldr, double *B, int64_t ldb) {
+    if (!h) return -1;
+    const bool tr = trans == 'T' || trans == 't';
+    if (!tr && trans != 'N' && trans != 'n') return -2;
+    if (n < 0) return -3;
+    if (nrhs < 0) return -4;
+    if (n > 0 && !R) return -5;
+    if (ldr < std::max<int64_t>(1, n)) return -6;
+    if (n * nrhs > 0 && !B) return -7;
+    if (ldb < std::max<int64_t>(1, n)) return -8;
+    if (n * nrhs == 0) return MQR_SUCCESS;
+    MQR_GUARD(h);
+    return trsm_upper_f64(h, tr, n, nrhs, R, ldr, B, ldb);
+}
+
+int mqr_dgels(mqr_handle_t h, int64_t m, int64_t n, int64_t nrhs, double *A, int64_t lda,
+              double *B, int64_t ldb) {
+    if (!h) return -1;
+    if (m < 0) return -2;
+    if (n < 0 || n > m) return -3;
+    if (nrhs < 0) return -4;
+    if (m * n > 0 && !A) return -5;
+    if (lda < std::max<int64_t>(1, m)) return -6;
+    if (m * nrhs > 0 && !B) return -7;
+    if (ldb < std::max<int64_t>(1, m)) return -8;
+    if (n == 0 || nrhs == 0) return MQR_SUCCESS;
+    MQR_GUARD(h);
+    const int nb = h->nb;
+    double *tauT = nullptr;
+    // tau (n) and T (nb x n) live in the refinement workspace
+    MQR_CHECK(ws_reserve(h, h->ws_ir, sizeof(double) * ((size_t)n * (nb + 1) + 64)));
+    tauT = (double *)h->ws_ir.ptr;
+    double *T = tauT + n;
+    MQR_CHECK(geqrf_f64(h, m, n, A, lda, tauT, T, nb));
+    MQR_CHECK(ormqr_f64(h, true, m, nrhs, n, A, lda, T, nb, B, ldb));
+    return trsm_upper_f64(h, false, n, nrhs, A, lda, B, ldb);
+}
+
+int mqr_gels_mixed(mqr_handle_t h, mqr_precision_t prec, int64_t m, int64_t n, const double *A,
+                   int64_t lda, const double *b, double *x, const mqr_ir_opts_t *opts,
+                   mqr_ir_info_t *info) {
+    if (!h) return -1;
+    if (prec < MQR_PREC_FP64 || prec > MQR_PREC_BF16) return -2;
+    if (m < 0) return -3;
+    if (n < 0 || n > m) return -4;
+    if (m * n > 0 && !A) return -5;
+    if (lda < std::max<int64_t>(1, m)) return -6;
+    if (m > 0 && !b) return -7;
+    if (n > 0 && !x) return -8;
+    if (info) std::memset(info, 0, sizeof(*info));
+    if (n == 0) return MQR_SUCCESS;
+    MQR_GUARD(h);
+    return gels_mixed(h, prec, m, n, A, lda, b, x, opts, info);
+}
+
+int mqr_qr_householder_host(mqr_handle_t h, int64_t m, int64_t n, const double *A, double *Q,
+                            double *R) {
+    if (!h) return -1;
+    if (m < 0) return -2;
+    if (n < 0) return -3;
+    if (m * n > 0 && !A) return -4;
+    if (m > 0 && !Q) return -5;
+    if (m * n > 0 && !R) return -6;
+    if (m == 0) return MQR_SUCCESS;
+    MQR_GUARD(h);
+    const int64_t k = std::min(m, n);
+    const int nb = h->nb;
+    double *dA = nullptr, *dQ = nullptr, *dTT = nullptr;
+    int st = MQR_SUCCESS;
+    const size_t szA = sizeof(double) * (size_t)(m * std::max<int64_t>(n, 1));
+    const size_t szQ = sizeof(double) * (size_t)(m * m);
+    const size_t szT = sizeof(double) * (size_t)(std::max<int64_t>(k, 1) * (nb + 1));
+    if (hipMalloc(&dA, szA) != hipSuccess || hipMalloc(&dQ, szQ) != hipSuccess ||
+        hipMalloc(&dTT, szT) != hipSuccess) {
+        (void)hipGetLastError();
+        st = MQR_ERR_ALLOC;
+    }
+    if (st == MQR_SUCCESS && n > 0 &&
+        hipMemcpyAsync(dA, A, sizeof(double) * m * n, hipMemcpyHostToDevice, h->stream) != hipSuccess)
+        st = MQR_ERR_HIP;
+    double *dtau = dTT, *dT = dTT + std::max<int64_t>(k, 1);
+    if (st == MQR_SUCCESS && k > 0) st = geqrf_f64(h, m, n, dA, m, dtau, dT, nb);
+    if (st == MQR_SUCCESS) st = orgqr_f64(h, m, m, k, dA, m, dT, nb, dQ, m);
+    if (st == MQR_SUCCESS &&
+        (hipMemcpyAsync(Q, dQ, szQ, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+         (n > 0 && hipMemcpyAsync(R, dA, sizeof(double) * m * n, hipMemcpyDeviceToHost, h->stream) !=
+                       hipSuccess) ||
+         hipStreamSynchronize(h->stream) != hipSuccess))
+        st = MQR_ERR_HIP;
+    if (st == MQR_SUCCESS)
+        for (int64_t c = 0; c < n; ++c)
+            for (int64_t r = c + 1; r < m; ++r) R[r + c * m] = 0.0;  // QR/__Householder_Blocked.h:346-352
+    if (dA) (void)hipFree(dA);
+    if (dQ) (void)hipFree(dQ);
+    if (dTT) (void)hipFree(dTT);
+    return st;
+}
+
+int mqr_gels_host(mqr_handle_t h, mqr_precision_t prec, int64_t m, int64_t n, const double *A,
+                  const double *b, double *x, mqr_ir_info_t *info) {
+    if (!h) return -1;
+    if (prec < MQR_PREC_FP64 || prec > MQR_PREC_BF16) return -2;
+    if (m < 0) return -3;
+    if (n < 0 || n > m) return -4;
+    if (m * n > 0 && !A) return -5;
+    if (m > 0 && !b) return -6;
+    if (n > 0 && !x) return -7;
+    if (info) std::memset(info, 0, sizeof(*info));
+    if (n == 0) return MQR_SUCCESS;
+    MQR_GUARD(h);
+    double *dA = nullptr, *db = nullptr, *dx = nullptr;
+    int st = MQR_SUCCESS;
+    if (hipMalloc(&dA, sizeof(double) * m * n) != hipSuccess ||
+        hipMalloc(&db, sizeof(double) * m) != hipSuccess ||
+        hipMalloc(&dx, sizeof(double) * std::max<int64_t>(n, 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        st = MQR_ERR_ALLOC;
+    }
+    if (st == MQR_SUCCESS &&
+        (hipMemcpyAsync(dA, A, sizeof(double) * m * n, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+         hipMemcpyAsync(db, b, sizeof(double) * m, hipMemcpyHostToDevice, h->stream) != hipSuccess))
+        st = MQR_ERR_HIP;
+    if (st == MQR_SUCCESS) {
+        if (prec == MQR_PREC_FP64 && !info) {
+            st = mqr_dgels(h, m, n, 1, dA, m, db, m);
+            if (st == MQR_SUCCESS &&
+                hipMemcpyAsync(dx, db, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream) != hipSuccess)
+                st = MQR_ERR_HIP;
+        } else {
+            st = gels_mixed(h, prec, m, n, dA, m, db, dx, nullptr, info);
+        }
+    }
+    if (st == MQR_SUCCESS &&
+        (hipMemcpyAsync(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+         hipStreamSynchronize(h->stream) != hipSuccess))
+        st = MQR_ERR_HIP;
+    if (dA) (void)hipFree(dA);
+    if (db) (void)hipFree(db);
+    if (dx) (void)hipFree(dx);
+    return st;
+}
+
+}  // extern "C"

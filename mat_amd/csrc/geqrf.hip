@@ -1,0 +1,228 @@
+// FP64 blocked Householder QR drivers: factorisation, Q formation, Q / Q^T
+// application, triangular solve and the plain least-squares solve.
+//
+// The factor loop follows QR/__Householder_Blocked.h:214-245: for each block
+// of nb columns, the panel reflectors (:62-84), the T factor (:86-103) and the
+// trailing update C <- (I - V T V^T)^T C (:143-212).  On the device the update
+// is three launches: one MFMA GEMM computing V^T [V | C] in split-K partials
+// (its first b columns are the Gram matrix V^T V the T recurrence needs),
+// W2 = T^T (sum of partials), and the MFMA GEMM C -= V W2.
+#include <algorithm>
+
+#include "mqr_internal.h"
+
+namespace mqr {
+namespace {
+
+__global__ void k_set_identity(int64_t m, int64_t n, double *Q, int64_t ldq) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= m * n) return;
+    const int64_t r = idx % m, c = idx / m;
+    Q[r + c * ldq] = r == c ? 1.0 : 0.0;
+}
+
+constexpr int TRB = 64;  // diagonal block of the triangular solve
+
+// Solve the TRB x TRB (or smaller) diagonal block in place for up to 64
+// right-hand sides per workgroup: upper (R x = b, backward) or its transpose
+// (R^T x = b, forward).
+__global__ __launch_bounds__(256) void k_trsm_diag(int nbk, int64_t nrhs, const double *R,
+                                                   int64_t ldr, double *B, int64_t ldb,
+                                                   int trans) {
+    __shared__ double Rs[TRB][TRB + 1];
+    __shared__ double Xs[TRB][65];
+    const int tid = threadIdx.x;
+    const int64_t c0 = (int64_t)blockIdx.x * 64;
+    const int ncol = (int)std::min<int64_t>(64, nrhs - c0);
+    for (int e = tid; e < nbk * nbk; e += 256) {
+        const int r = e % nbk, c = e / nbk;
+        Rs[r][c] = R[r + (int64_t)c * ldr];
+    }
+    for (int e = tid; e < nbk * ncol; e += 256) {
+        const int r = e % nbk, c = e / nbk;
+        Xs[r][c] = B[r + (c0 + c) * ldb];
+    }
+    __syncthreads();
+    if (!trans) {
+        for (int j = nbk - 1; j >= 0; --j) {
+            if (tid < ncol) Xs[j][tid] = Xs[j][tid] / Rs[j][j];
+            __syncthreads();
+            for (int e = tid; e < j * ncol; e += 256) {
+                const int i = e % j, c = e / j;
+                Xs[i][c] = fma(-Rs[i][j], Xs[j][c], Xs[i][c]);
+            }
+            __syncthreads();
+        }
+    } else {
+        for (int j = 0; j < nbk; ++j) {
+            if (tid < ncol) Xs[j][tid] = Xs[j][tid] / Rs[j][j];
+            __syncthreads();
+            const int rest = nbk - j - 1;
+            for (int e = tid; e < rest * ncol; e += 256) {
+                const int i = j + 1 + e % rest, c = e / rest;
+                Xs[i][c] = fma(-Rs[j][i], Xs[j][c], Xs[i][c]);
+            }
+            __syncthreads();
+        }
+    }
+    for (int e = tid; e < nbk * ncol; e += 256) {
+        const int r = e % nbk, c = e / nbk;
+        B[r + (c0 + c) * ldb] = Xs[r][c];
+    }
+}
+
+int target_blocks(mqr_handle_t h) { return 4 * h->cu_count; }
+
+// Apply one block reflector (V: h x b at Vp, T at Tp) to the h x nc matrix C:
+// trans: C <- (I - V T V^T)^T C; otherwise C <- (I - V T V^T) C.
+int apply_block(mqr_handle_t hd, int64_t h, int b, const double *Vp, int64_t ldv, const double *Tp,
+                int64_t ldt, double *C, int64_t ldc, int64_t nc, bool trans, double *P,
+                int64_t pcap, double *W2) {
+    if (nc <= 0) return MQR_SUCCESS;
+    int splits = gemm_pick_splits(b, (int)nc, (int)h, target_blocks(hd));
+    while (splits > 1 && (int64_t)splits * b * nc > pcap) --splits;
+    GemmF64 g;
+    g.M = b; g.N = (int)nc; g.K = (int)h;
+    g.ta = true; g.amask = MASK_VLOWER;
+    g.A = Vp; g.lda = ldv; g.B = C; g.ldb = ldc;
+    g.C = P; g.ldc = b; g.alpha = 1.0; g.beta = 0.0;
+    g.splits = splits; g.split_stride = (int64_t)b * nc; g.kind = K_VTA;
+    MQR_CHECK(gemm_f64(g, hd->stream));
+    MQR_CHECK(block_w2(b, nc, Tp, ldt, trans, P, splits, (int64_t)b * nc, W2, hd->stream));
+    GemmF64 u;
+    u.M = (int)h; u.N = (int)nc; u.K = b;
+    u.amask = MASK_VLOWER;
+    u.A = Vp; u.lda = ldv; u.B = W2; u.ldb = b;
+    u.C = C; u.ldc = ldc; u.alpha = -1.0; u.beta = 1.0; u.kind = K_UPDATE;
+    return gemm_f64(u, hd->stream);
+}
+
+}  // namespace
+
+// Workspace (doubles) the factorisation of an m x n matrix uses.
+static int64_t geqrf_ws_doubles(int64_t m, int64_t n, int nb) {
+    (void)m;
+    return (int64_t)16 * nb * (n + nb) + (int64_t)nb * (n + nb) + 1024;
+}
+
+int geqrf_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, double *tau,
+              double *T, int64_t ldt) {
+    const int64_t k = std::min(m, n);
+    const int nb = hd->nb;
+    const int64_t pcap = (int64_t)16 * nb * (n + nb);
+    MQR_CHECK(ws_reserve(hd, hd->ws, sizeof(double) * geqrf_ws_doubles(m, n, nb)));
+    double *P = (double *)hd->ws.ptr;
+    double *W2 = P + pcap;
+    for (int64_t j = 0; j < k; j += nb) {
+        const int b = (int)std::min<int64_t>(nb, k - j);
+        const int64_t h = m - j;
+        double *V = A + j + j * lda;
+        MQR_CHECK(panel_householder(h, b, V, lda, tau + j, hd->stream));
+        // V^T [V | C] with C = A[j:m, j+b:n]: Gram of V in the first b columns
+        const int64_t ncols = n - j;
+        int splits = gemm_pick_splits(b, (int)ncols, (int)h, target_blocks(hd));
+        while (splits > 1 && (int64_t)splits * b * ncols > pcap) --splits;
+        GemmF64 g;
+        g.M = b; g.N = (int)ncols; g.K = (int)h;
+        g.ta = true; g.amask = MASK_VLOWER; g.bmask = MASK_VLOWER; g.bmask_cols = b;
+        g.A = V; g.lda = lda; g.B = V; g.ldb = lda;
+        g.C = P; g.ldc = b; g.splits = splits; g.split_stride = (int64_t)b * ncols;
+        g.kind = K_VTA;
+        MQR_CHECK(gemm_f64(g, hd->stream));
+        double *Tj = T + j * ldt;
+        MQR_CHECK(larft_from_gram(b, P, splits, (int64_t)b * ncols, tau + j, Tj, ldt, hd->stream));
+        const int64_t nt = n - j - b;
+        if (nt > 0) {
+            MQR_CHECK(block_w2(b, nt, Tj, ldt, true, P + (int64_t)b * b, splits,
+                               (int64_t)b * ncols, W2, hd->stream));
+            GemmF64 u;
+            u.M = (int)h; u.N = (int)nt; u.K = b;
+            u.amask = MASK_VLOWER;
+            u.A = V; u.lda = lda; u.B = W2; u.ldb = b;
+            u.C = A + j + (j + b) * lda; u.ldc = lda; u.alpha = -1.0; u.beta = 1.0;
+            u.kind = K_UPDATE;
+            MQR_CHECK(gemm_f64(u, hd->stream));
+        }
+    }
+    return MQR_SUCCESS;
+}
+
+int ormqr_f64(mqr_handle_t hd, bool trans, int64_t m, int64_t nrhs, int64_t k, const double *VR,
+              int64_t ldvr, const double *T, int64_t ldt, double *C, int64_t ldc) {
+    const int nb = hd->nb;
+    const int64_t pcap = (int64_t)16 * nb * (nrhs + nb);
+    MQR_CHECK(ws_reserve(hd, hd->ws, sizeof(double) * (pcap + (int64_t)nb * (nrhs + nb) + 1024)));
+    double *P = (double *)hd->ws.ptr;
+    double *W2 = P + pcap;
+    const int64_t nblk = (k + nb - 1) / nb;
+    for (int64_t q = 0; q < nblk; ++q) {
+        const int64_t j = trans ? q * nb : (nblk - 1 - q) * nb;
+        const int b = (int)std::min<int64_t>(nb, k - j);
+        MQR_CHECK(apply_block(hd, m - j, b, VR + j + j * ldvr, ldvr, T + j * ldt, ldt, C + j, ldc,
+                              nrhs, trans, P, pcap, W2));
+    }
+    return MQR_SUCCESS;
+}
+
+int orgqr_f64(mqr_handle_t hd, int64_t m, int64_t nq, int64_t k, const double *VR, int64_t ldvr,
+              const double *T, int64_t ldt, double *Q, int64_t ldq) {
+    const int nb = hd->nb;
+    const int64_t total = m * nq;
+    if (total > 0) {
+        hipLaunchKernelGGL(k_set_identity, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                           hd->stream, m, nq, Q, ldq);
+        MQR_LAUNCH_CHECK();
+    }
+    const int64_t pcap = (int64_t)16 * nb * (nq + nb);
+    MQR_CHECK(ws_reserve(hd, hd->ws, sizeof(double) * (pcap + (int64_t)nb * (nq + nb) + 1024)));
+    double *P = (double *)hd->ws.ptr;
+    double *W2 = P + pcap;
+    const int64_t kk = std::min(k, nq);
+    const int64_t nblk = (kk + nb - 1) / nb;
+    for (int64_t q = nblk - 1; q >= 0; --q) {
+        const int64_t j = q * nb;
+        const int b = (int)std::min<int64_t>(nb, kk - j);
+        // columns < j of Q[j:m, :] are still zero: skip them
+        MQR_CHECK(apply_block(hd, m - j, b, VR + j + j * ldvr, ldvr, T + j * ldt, ldt,
+                              Q + j + j * ldq, ldq, nq - j, false, P, pcap, W2));
+    }
+    return MQR_SUCCESS;
+}
+
+int trsm_upper_f64(mqr_handle_t hd, bool trans, int64_t n, int64_t nrhs, const double *R,
+                   int64_t ldr, double *B, int64_t ldb) {
+    if (n <= 0 || nrhs <= 0) return MQR_SUCCESS;
+    const int64_t nblk = (n + TRB - 1) / TRB;
+    const unsigned cgrid = (unsigned)((nrhs + 63) / 64);
+    for (int64_t q = 0; q < nblk; ++q) {
+        const int64_t i0 = trans ? q * TRB : (nblk - 1 - q) * TRB;
+        const int bs = (int)std::min<int64_t>(TRB, n - i0);
+        {
+            ProfScope ps(K_TRSM, (double)bs * bs * nrhs, 8.0 * (bs * bs + 2.0 * bs * nrhs),
+                         hd->stream);
+            hipLaunchKernelGGL(k_trsm_diag, dim3(cgrid), dim3(256), 0, hd->stream, bs, nrhs,
+                               R + i0 + i0 * ldr, ldr, B + i0, ldb, trans ? 1 : 0);
+            MQR_LAUNCH_CHECK();
+        }
+        GemmF64 g;
+        if (!trans) {
+            if (i0 == 0) continue;
+            // B[0:i0] -= R[0:i0, i0:i0+bs] X
+            g.M = (int)i0; g.N = (int)nrhs; g.K = bs;
+            g.A = R + i0 * ldr; g.lda = ldr; g.B = B + i0; g.ldb = ldb;
+            g.C = B; g.ldc = ldb;
+        } else {
+            const int64_t rest = n - i0 - bs;
+            if (rest <= 0) continue;
+            // B[i0+bs:] -= R[i0:i0+bs, i0+bs:]^T X
+            g.M = (int)rest; g.N = (int)nrhs; g.K = bs; g.ta = true;
+            g.A = R + i0 + (i0 + bs) * ldr; g.lda = ldr; g.B = B + i0; g.ldb = ldb;
+            g.C = B + i0 + bs; g.ldc = ldb;
+        }
+        g.alpha = -1.0; g.beta = 1.0; g.kind = K_TRSM;
+        MQR_CHECK(gemm_f64(g, hd->stream));
+    }
+    return MQR_SUCCESS;
+}
+
+}  // namespace mqr

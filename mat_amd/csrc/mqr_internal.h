@@ -1,0 +1,144 @@
+// Internal declarations shared by the HIP translation units of libmatqr.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/mat_qr.h"
+
+namespace mqr {
+
+// Status helpers --------------------------------------------------------------
+void set_last_hip_error(hipError_t e, const char *where);
+
+#define MQR_HIP(call)                                        \
+    do {                                                     \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) {                              \
+            ::mqr::set_last_hip_error(e_, #call);            \
+            return MQR_ERR_HIP;                              \
+        }                                                    \
+    } while (0)
+
+#define MQR_CHECK(expr)                 \
+    do {                                \
+        int s_ = (expr);                \
+        if (s_ != MQR_SUCCESS) return s_; \
+    } while (0)
+
+#define MQR_LAUNCH_CHECK() MQR_HIP(hipGetLastError())
+
+// Device workspace owned by a handle: grows, never shrinks until destroy.
+struct Workspace {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+// Per-kernel-kind timing with HIP event pairs (enabled per handle).
+enum KernelKind {
+    K_PANEL = 0,      // panel reflectors
+    K_VTA = 1,        // split-K V^T [V | C] (Gram + W partials)
+    K_LARFT = 2,      // T recurrence
+    K_W2 = 3,         // W2 = op(T) W
+    K_UPDATE = 4,     // C -= V W2 (trailing update / Q formation)
+    K_TRSM = 5,       // triangular solves
+    K_GEMM = 6,       // plain GEMM calls
+    K_RESID = 7,      // refinement residual passes
+    K_CAST = 8,       // precision conversions
+    K_OTHER = 9,
+    K_COUNT = 10
+};
+const char *kernel_kind_name(int k);
+
+struct Profiler {
+    bool on = false;
+    struct Rec {
+        int kind;
+        double flops, bytes;
+        hipEvent_t a, b;
+    };
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<Rec> recs;
+};
+
+// Active profiler of the calling thread (null when profiling is off).
+extern thread_local Profiler *t_prof;
+
+// Records an event pair around the launches made while it is alive.
+struct ProfScope {
+    Profiler *p;
+    size_t idx;
+    hipStream_t s;
+    ProfScope(int kind, double flops, double bytes, hipStream_t stream);
+    ~ProfScope();
+};
+
+}  // namespace mqr
+
+struct mqr_handle_st {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;  // current stream (own or caller's)
+    int nb = 64;                   // factorisation block size
+    int cu_count = 256;
+    mqr::Workspace ws;             // general scratch
+    mqr::Workspace ws_ir;          // mixed-precision / refinement scratch
+    double *host_pinned = nullptr; // small pinned buffer for device->host scalars
+    mqr::Profiler prof;
+};
+
+namespace mqr {
+
+// Ensure ws holds at least `bytes`; may synchronise the stream.
+int ws_reserve(mqr_handle_t h, Workspace &ws, size_t bytes);
+
+// ---- FP64 MFMA GEMM -------------------------------------------------------
+// C[M x N] = alpha * op(A) * op(B) + beta * C, column-major.
+// Mask modes apply to the *stored* operand before op():
+//   MASK_NONE, MASK_VLOWER (unit lower trapezoid: zero above the diagonal,
+//   one on it) for stored columns < mask_cols.
+enum { MASK_NONE = 0, MASK_VLOWER = 1 };
+
+struct GemmF64 {
+    int M = 0, N = 0, K = 0;
+    bool ta = false, tb = false;
+    int amask = MASK_NONE, bmask = MASK_NONE;
+    int bmask_cols = 0;  // B mask applies to stored columns < bmask_cols
+    const double *A = nullptr;
+    int64_t lda = 0;
+    const double *B = nullptr;
+    int64_t ldb = 0;
+    double *C = nullptr;
+    int64_t ldc = 0;
+    double alpha = 1.0, beta = 0.0;
+    // split-K: when splits > 1, partial products are written (not accumulated)
+    // to C + s * split_stride for split s; beta is ignored.
+    int splits = 1;
+    int64_t split_stride = 0;
+    int kind = K_GEMM;  // profiler bucket
+};
+int gemm_f64(const GemmF64 &g, hipStream_t s);
+// C = beta * C (beta == 0 writes zeros).
+int scale_f64(int64_t m, int64_t n, double beta, double *C, int64_t ldc, hipStream_t s);
+// Number of K splits giving roughly `target_blocks` workgroups.
+int gemm_pick_splits(int M, int N, int K, int target_blocks);
+
+// ---- panel / T / block reflector ------------------------------------------
+// Householder panel on h x b at P (ld), single workgroup, in place.
+int panel_householder(int64_t h, int b, double *P, int64_t ld, double *tau, hipStream_t s);
+// Largest block size the LDS-resident T kernels accept.
+constexpr int kMaxNb = 64;
+// T from partial Gram matrices G_s (b x b each, ld = b, `splits` of them at
+// stride pstride) of V and tau: T(0:b, 0:b) with leading dimension ldt.
+int larft_from_gram(int b, const double *Gpart, int splits, int64_t pstride, const double *tau,
+                    double *T, int64_t ldt, hipStream_t s);
+// W2 = op(T) * sum_s P_s  (P_s: b x nc, ld = b, stride pstride), trans: T^T.
+int block_w2(int b, int64_t nc, const double *T, int64_t ldt, bool trans, const double *P,
+             int splits, int64_t pstride, double *W2, hipStream_t s);
+
+}  // namespace mqr

@@ -1,0 +1,249 @@
+"""GPU parity of the FP64 path: every call goes through the C ABI
+(libmatqr.so) and is compared with the CPU oracle (a restatement of the
+reference's blocked Householder QR), LAPACK vectors, brute-force products and
+size-independent invariants.
+
+Tolerances: the reference's results are FP64 with O(n u) rounding; the GPU
+accumulates in a different order (MFMA tiles, split-K partials), so elementwise
+agreement is asked to c * max(m, n) * u * ||A|| with c = 64 (R, V, tau, Q) and
+exact equality where all arithmetic is exact (integer-valued GEMM inputs,
+identity input)."""
+import numpy as np
+import pytest
+
+import mat_amd as M
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+EPS = np.finfo(np.float64).eps
+
+
+def note(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+@pytest.fixture(scope="module")
+def h(torch_mod):
+    hd = M.Handle(0)
+    yield hd
+    hd.close()
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle("avx2")
+
+
+def host(t):
+    """Column-major device tensor -> Fortran-ordered numpy array."""
+    return np.asfortranarray(t.detach().cpu().numpy())
+
+
+def unit_lower(VR, b):
+    V = np.tril(VR[:, :b], -1)
+    V[np.arange(b), np.arange(b)] = 1.0
+    return V
+
+
+def gpu_factor(h, A):
+    dA = M.colmajor(A)
+    tau, T = M.geqrf(h, dA)
+    h.synchronize()
+    return dA, tau, T
+
+
+# ---------------------------------------------------------------------------
+# MFMA GEMM tile
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+def test_gemm_exact_on_integers(h, orc, ta, tb):
+    rng = np.random.default_rng(1)
+    for (m, n, k) in [(1, 1, 1), (16, 16, 4), (64, 64, 16), (37, 29, 41), (200, 150, 90),
+                      (1000, 70, 300)]:
+        note(f"gemm {ta}{tb} m={m} n={n} k={k}")
+        A = rng.integers(-8, 9, (k, m) if ta == "T" else (m, k)).astype(np.float64)
+        B = rng.integers(-8, 9, (n, k) if tb == "T" else (k, n)).astype(np.float64)
+        Cm = rng.integers(-8, 9, (m, n)).astype(np.float64)
+        opA = A.T if ta == "T" else A
+        opB = B.T if tb == "T" else B
+        exact = Cm.astype(np.int64) - opA.astype(np.int64) @ opB.astype(np.int64)
+        dC = M.colmajor(Cm)
+        M.gemm(h, ta, tb, -1.0, M.colmajor(A), M.colmajor(B), 1.0, dC)
+        got = host(dC)
+        assert np.array_equal(got, exact.astype(np.float64)), (m, n, k)
+        if ta == "N" and tb == "N":
+            assert np.array_equal(got, orc.mulAB_packed_n(opA, opB, Cm))
+
+
+def test_gemm_float_against_oracle(h, orc):
+    rng = np.random.default_rng(2)
+    A = rng.standard_normal((300, 200))
+    B = rng.standard_normal((200, 170))
+    dC = M.colmajor(np.zeros((300, 170)))
+    M.gemm(h, "N", "N", 1.0, M.colmajor(A), M.colmajor(B), 0.0, dC)
+    np.testing.assert_allclose(host(dC), orc.gemm(A, B), rtol=0, atol=200 * 4 * EPS * 20)
+
+
+# ---------------------------------------------------------------------------
+# factorisation parity
+# ---------------------------------------------------------------------------
+
+def test_geqrf_matches_lapack_vectors(h, orc, golden):
+    nb = h.nb
+    for key in golden.files:
+        if not key.startswith("A_"):
+            continue
+        m, n = (int(x) for x in key[2:].split("x"))
+        note(f"geqrf fixture {m}x{n}")
+        A = golden[key]
+        dA, tau, T = gpu_factor(h, A)
+        VR, tau, T = host(dA), tau.cpu().numpy(), host(T)
+        VRl, taul = golden[f"VR_{m}x{n}"], golden[f"tau_{m}x{n}"]
+        k = min(m, n)
+        tol = 64 * max(m, n) * EPS * max(1.0, np.abs(A).max())
+        kk = k - 1 if m <= n else k
+        np.testing.assert_allclose(tau[:kk], taul[:kk], rtol=0, atol=tol)
+        np.testing.assert_allclose(np.triu(VR)[:kk], np.triu(VRl)[:kk], rtol=0, atol=tol)
+        np.testing.assert_allclose(np.tril(VR, -1), np.tril(VRl, -1), rtol=0, atol=tol)
+        if m <= n:
+            assert tau[k - 1] == 2.0
+            np.testing.assert_allclose(VR[k - 1, k - 1:], -VRl[k - 1, k - 1:], rtol=0, atol=tol)
+        # every T block against the oracle's recurrence on the same reflectors
+        for j in range(0, k, nb):
+            b = min(nb, k - j)
+            V = unit_lower(VR[j:, j:j + b], b)
+            Tref = orc.larft(V, tau[j:j + b])
+            np.testing.assert_allclose(np.triu(T[:b, j:j + b]), Tref, rtol=0,
+                                       atol=64 * m * EPS * max(1.0, np.abs(Tref).max()))
+
+
+@pytest.mark.parametrize("m,n", [(100, 64), (300, 250), (500, 500), (1024, 1024), (777, 333),
+                                 (250, 400), (2048, 96), (65, 130)])
+def test_qr_matches_oracle(h, orc, m, n):
+    note(f"qr vs oracle {m}x{n}")
+    rng = np.random.default_rng(m * 7 + n)
+    A = rng.standard_normal((m, n))
+    dA, tau, T = gpu_factor(h, A)
+    k = min(m, n)
+    dQ = M.orgqr(h, dA, T, k, nq=m)
+    h.synchronize()
+    VR, Q = host(dA), host(dQ)
+    Qo, Ro, _ = orc.qr(A)
+    nA = np.linalg.norm(A)
+    tol = 64 * max(m, n) * EPS
+    np.testing.assert_allclose(np.triu(VR), Ro, rtol=0, atol=tol * np.abs(A).max() * 4)
+    np.testing.assert_allclose(Q, Qo, rtol=0, atol=tol)
+    R = np.triu(VR)
+    assert np.linalg.norm(Q @ R - A) / nA < 16 * max(m, n) * EPS
+    assert np.linalg.norm(Q.T @ Q - np.eye(m)) < 16 * m * EPS * np.sqrt(m)
+
+
+def test_thin_q_and_apply(h, orc):
+    rng = np.random.default_rng(5)
+    m, n = 900, 300
+    A = rng.standard_normal((m, n))
+    dA, tau, T = gpu_factor(h, A)
+    dQt = M.orgqr(h, dA, T, n, nq=n)
+    dQf = M.orgqr(h, dA, T, n, nq=m)
+    Cm = rng.standard_normal((m, 5))
+    dC = M.colmajor(Cm)
+    M.ormqr(h, "T", dA, T, n, dC)
+    QtC = host(dC)
+    M.ormqr(h, "N", dA, T, n, dC)
+    back = host(dC)
+    h.synchronize()
+    Qf = host(dQf)
+    np.testing.assert_allclose(host(dQt), Qf[:, :n], rtol=0, atol=0)
+    np.testing.assert_allclose(QtC, Qf.T @ Cm, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(back, Cm, rtol=0, atol=1e-12)
+
+
+def test_trsm_upper(h):
+    rng = np.random.default_rng(6)
+    n = 333
+    R = np.triu(rng.standard_normal((n, n))) + 4 * np.eye(n) * np.sign(rng.standard_normal(n))
+    B = rng.standard_normal((n, 3))
+    import scipy.linalg as sla
+    for trans in ["N", "T"]:
+        dB = M.colmajor(B)
+        M.trsm_upper(h, trans, M.colmajor(R), dB)
+        ref = sla.solve_triangular(R, B, trans=0 if trans == "N" else 1, lower=False)
+        np.testing.assert_allclose(host(dB), ref, rtol=0, atol=1e-11)
+
+
+def test_dgels_matches_oracle(h, orc):
+    rng = np.random.default_rng(7)
+    for (m, n, nrhs) in [(600, 200, 1), (1000, 1000, 2), (4000, 130, 3)]:
+        note(f"dgels {m}x{n} nrhs={nrhs}")
+        A = rng.standard_normal((m, n))
+        B = rng.standard_normal((m, nrhs))
+        dA, dB = M.colmajor(A), M.colmajor(B)
+        X = host(M.gels(h, dA, dB))
+        for c in range(nrhs):
+            xo = orc.gels(A, B[:, c])
+            np.testing.assert_allclose(X[:, c], xo, rtol=0, atol=1e-9 * max(1, np.abs(xo).max()))
+
+
+# ---------------------------------------------------------------------------
+# special cases (closed forms) and the host-level drop-in
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("n", [1, 5, 64, 200])
+def test_identity_gives_minus_identity_exactly(h, n):
+    dA, tau, T = gpu_factor(h, np.eye(n))
+    dQ = M.orgqr(h, dA, T, n, nq=n)
+    h.synchronize()
+    assert np.array_equal(np.triu(host(dA)), -np.eye(n))
+    assert np.array_equal(host(dQ), -np.eye(n))
+    assert np.all(tau.cpu().numpy() == 2.0)
+
+
+def test_zero_and_dependent_columns(h, orc):
+    rng = np.random.default_rng(8)
+    for (m, n, zc) in [(8, 5, 0), (400, 250, 0), (400, 250, 130), (300, 300, 299)]:
+        A = rng.standard_normal((m, n))
+        A[:, zc] = 0.0
+        dA, tau, T = gpu_factor(h, A)
+        tau = tau.cpu().numpy()
+        if zc == 0:
+            assert tau[0] == 0.0
+        VR = host(dA)
+        dQ = M.orgqr(h, dA, T, min(m, n), nq=m)
+        Q = host(dQ)
+        assert np.linalg.norm(Q @ np.triu(VR) - A) < 1e-12 * m
+        assert np.linalg.norm(Q.T @ Q - np.eye(m)) < 1e-12 * m
+
+
+def test_host_qr_api_matches_oracle(orc):
+    rng = np.random.default_rng(9)
+    for (m, n) in [(7, 4), (300, 200), (257, 300)]:
+        A = np.asfortranarray(rng.standard_normal((m, n)))
+        Q = np.zeros((m, m), order="F")
+        R = np.zeros((m, n), order="F")
+        M.QR(A, Q, R, M.QR_Householder)
+        Qo, Ro, _ = orc.qr(A)
+        assert np.all(np.tril(R, -1) == 0.0)
+        np.testing.assert_allclose(R, np.triu(Ro), rtol=0, atol=1e-12 * max(m, n))
+        np.testing.assert_allclose(Q, Qo, rtol=0, atol=1e-12 * m)
+
+
+def test_argument_errors_are_lapack_style(h):
+    L = M.lib()
+    assert L.mqr_dgeqrf(h.h, -1, 4, None, 4, None, None, 64) == -2
+    assert L.mqr_dgeqrf(h.h, 4, -1, None, 4, None, None, 64) == -3
+    assert L.mqr_dgeqrf(h.h, 4, 4, None, 4, None, None, 64) == -4
+    import torch
+    A = torch.zeros(16, device="cuda", dtype=torch.float64)
+    assert L.mqr_dgeqrf(h.h, 4, 4, M._ptr(A), 3, None, None, 64) == -5
+    assert L.mqr_dgeqrf(h.h, 4, 4, M._ptr(A), 4, None, None, 64) == -6
+    assert L.mqr_set_block_size(h.h, 24) == -2
+    assert L.mqr_dorgqr(h.h, 4, 5, 4, M._ptr(A), 4, M._ptr(A), 64, M._ptr(A), 4) == -3
+    assert L.mqr_dgels(h.h, 3, 4, 1, M._ptr(A), 3, M._ptr(A), 3) == -3
