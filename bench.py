@@ -1,0 +1,268 @@
+"""Benchmark: QR least squares at 16384 x 4096 on one MI355X.
+
+Workload (the headline configuration): min ||A x - b|| with
+A = U diag(s) V^T, s = logspace(0, -6, n), U (m x n) and V (n x n) Haar
+orthogonal (Q factor of a seeded Gaussian with diag(R) signs removed),
+x0 ~ N(0, 1), b = A x0 + 1e-10 N(0, 1).  Inputs are synthetic, generated on the
+device from numpy seeds, and resident in HBM before timing.
+
+A step is one complete solve through the C ABI:
+  plain : device copy of A, FP64 blocked Householder QR, Q^T b, R x = c
+  mixed : FP32 MFMA trailing updates, FP64 panels / T / residuals, refinement
+          on the augmented system to FP64 backward error (A is not modified)
+TFLOP/s uses the LAPACK count 2mn^2 - 2n^3/3 + 4mn - n^2 (refinement work is
+not counted).  The JSON line reports the mixed path as `value` when it is
+enabled (default), with the plain FP64 path beside it at its own error.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "QR TFLOP/s at 16384x4096: mixed+IR vs FP64 plain at equal error; % MFMA peak"
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
+
+
+def ls_flops(m: int, n: int) -> float:
+    return 2.0 * m * n * n - 2.0 * n ** 3 / 3.0 + 4.0 * m * n - 1.0 * n * n
+
+
+def haar(M, h, rows: int, cols: int, seed: int):
+    """Haar-distributed orthonormal columns: Q diag(sign(diag R)) of a seeded
+    Gaussian, factored with this library's own QR."""
+    import torch
+    G = np.random.default_rng(seed).standard_normal((rows, cols))
+    dA = M.colmajor(G)
+    tau, T = M.geqrf(h, dA)
+    Q = M.orgqr(h, dA, T, cols, nq=cols)
+    d = torch.sign(torch.diagonal(dA[:cols, :cols]))
+    d[d == 0] = 1.0
+    Q = Q * d[None, :]
+    return Q
+
+
+def make_problem(M, h, m: int, n: int, seed: int = 1234, smin: float = 1e-6):
+    import torch
+    U = haar(M, h, m, n, seed)
+    V = haar(M, h, n, n, seed + 1)
+    s = torch.logspace(0.0, float(np.log10(smin)), n, device="cuda", dtype=torch.float64)
+    Us = M.colmajor((U * s[None, :]).contiguous())
+    A = M.empty_colmajor(m, n)
+    M.gemm(h, "N", "T", 1.0, Us, M.colmajor(V.contiguous()), 0.0, A)
+    rng = np.random.default_rng(seed + 2)
+    x0 = torch.from_numpy(rng.standard_normal(n)).cuda()
+    noise = torch.from_numpy(rng.standard_normal(m)).cuda()
+    b = (A @ x0) + 1e-10 * noise
+    h.synchronize()
+    del Us
+    return {"A": A, "b": b.contiguous(), "x0": x0, "U": U, "V": V, "s": s}
+
+
+def backward_errors(P, x):
+    """cheap: ||A^T r|| / (||A||_F (||A||_F ||x|| + ||b||));
+    kw: Karlson-Walden estimate ||(A^T A + phi^2 I)^{-1/2} A^T r|| / ||x|| / ||A||_F,
+    exact through the known SVD A = U diag(s) V^T."""
+    import torch
+    A, b, U, s = P["A"], P["b"], P["U"], P["s"]
+    r = b - A @ x
+    Atr = A.T @ r
+    nA = torch.linalg.norm(A)
+    nx, nr, nb = torch.linalg.norm(x), torch.linalg.norm(r), torch.linalg.norm(b)
+    cheap = torch.linalg.norm(Atr) / (nA * (nA * nx + nb))
+    phi = nr / nx
+    y = s * (U.T @ r)
+    kw = torch.linalg.norm(y / torch.sqrt(s * s + phi * phi)) / nx / nA
+    return {"cheap": float(cheap), "kw": float(kw), "resid": float(nr)}
+
+
+def run_plain(M, h, P, steps: int, warmup: int):
+    import torch
+    A0, b0 = P["A"], P["b"]
+    m, n = A0.shape
+    Aw = M.empty_colmajor(m, n)
+    bw = M.empty_colmajor(m, 1)
+
+    def step():
+        Aw.copy_(A0)
+        bw[:, 0].copy_(b0)
+        M.gels(h, Aw, bw)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    M.profile_enable(h, True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    prof = M.profile_read(h)
+    M.profile_enable(h, False)
+    x = bw[:n, 0].clone()
+    return dt, x, prof
+
+
+def run_mixed(M, h, P, prec: str, steps: int, warmup: int):
+    import torch
+    A0, b0 = P["A"], P["b"]
+    infos = []
+    for _ in range(warmup):
+        M.gels_mixed(h, prec, A0, b0)
+    torch.cuda.synchronize()
+    M.profile_enable(h, True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        x, info = M.gels_mixed(h, prec, A0, b0)
+        infos.append(info)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    prof = M.profile_read(h)
+    M.profile_enable(h, False)
+    last = infos[-1]
+    return dt, x, prof, {"iterations": last.iterations, "converged": last.converged,
+                         "fell_back": last.fell_back, "backward_error_est": last.backward_error}
+
+
+def roofline_from(prof: dict, steps: int, prec: str) -> dict:
+    """Dominant kernel = the kind with the largest summed duration."""
+    if not prof:
+        return None
+    kind = max(prof, key=lambda k: prof[k]["total_ms"])
+    p = prof[kind]
+    avg_ms = p["total_ms"] / max(p["launches"], 1)
+    flops_per_launch = p["flops"] / max(p["launches"], 1)
+    mfma = kind in ("update_gemm", "vta_gemm", "gemm")
+    if mfma:
+        achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+        peak = PEAK_TFLOPS["fp64" if prec == "fp64" else "fp32"]
+        unit, bound = "TFLOP/s", "mfma"
+    else:
+        achieved = (p["bytes"] / max(p["launches"], 1)) / (avg_ms * 1e-3) / 1e9
+        peak, unit, bound = 8000.0, "GB/s", "hbm"
+    return {"kernel": kind, "bound": bound, "achieved": round(achieved, 3), "peak": peak,
+            "unit": unit, "frac": round(achieved / peak, 4), "traffic": None,
+            "avg_launch_ms": round(avg_ms, 5), "launches_per_step": p["launches"] // max(steps, 1),
+            "flops_per_launch": flops_per_launch,
+            "kernels": {k: {"ms_per_step": round(v["total_ms"] / steps, 4),
+                            "launches_per_step": v["launches"] // max(steps, 1),
+                            "tflops": round(v["flops"] / max(v["total_ms"], 1e-9) / 1e9, 3)}
+                        for k, v in prof.items()}}
+
+
+def cpu_baseline(seconds: float = 20.0) -> dict:
+    """The oracle's blocked Householder least squares (single thread, the
+    reference's AVX-512 or AVX2 blocking), timed on a bounded sample."""
+    from oracle.oracle import Oracle, cpu_has_avx512
+    o = Oracle("auto")
+    m, n = 8192, 2048
+    rng = np.random.default_rng(7)
+    A = np.asfortranarray(rng.standard_normal((m, n)))
+    b = rng.standard_normal(m)
+    times = []
+    t_end = time.perf_counter() + seconds
+    while True:
+        t0 = time.perf_counter()
+        o.gels(A, b)
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() > t_end or len(times) >= 5:
+            break
+    t = float(np.median(times))
+    return {"value": round(ls_flops(m, n) / t / 1e12, 5), "unit": "TFLOP/s", "cores": 1,
+            "kind": "port",
+            "sample": f"oracle blocked Householder LS {m}x{n} iid N(0,1), FP64, 1 thread, "
+                      f"{'AVX-512 blocking (BS 48, 16x12)' if o.variant == 'avx512' else 'AVX2 blocking (BS 96, 8x6)'}, "
+                      f"median of {len(times)} runs ({t:.2f} s each)",
+            "cpu_avx512": bool(cpu_has_avx512())}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--m", type=int, default=16384)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--nb", type=int, default=None)
+    ap.add_argument("--prec", default="fp32", choices=["fp32", "bf16x3", "bf16", "none"],
+                    help="mixed-precision trailing updates ('none': plain FP64 only)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
+              "(the path runs on a single MI355X; see the sharding design notes)", file=sys.stderr)
+        return 1
+
+    import torch
+    import mat_amd as M
+    assert torch.cuda.is_available(), "no GPU"
+    h = M.Handle(0, nb=args.nb)
+    m, n = args.m, args.n
+    print(f"[bench] generating {m}x{n} graded problem", file=sys.stderr, flush=True)
+    P = make_problem(M, h, m, n)
+    flops = ls_flops(m, n)
+
+    print("[bench] plain FP64", file=sys.stderr, flush=True)
+    dt_p, x_p, prof_p = run_plain(M, h, P, args.steps, args.warmup)
+    be_p = backward_errors(P, x_p)
+    plain = {"tflops": round(flops / dt_p / 1e12, 4), "ms": round(dt_p * 1e3, 3),
+             "pct_fp64_peak": round(100 * flops / dt_p / 1e12 / PEAK_TFLOPS["fp64"], 2),
+             "backward_error": be_p, "roofline": roofline_from(prof_p, args.steps, "fp64")}
+    print(f"[bench] plain: {plain['tflops']} TFLOP/s, {plain['ms']} ms, be={be_p}",
+          file=sys.stderr, flush=True)
+
+    mixed = None
+    if args.prec != "none":
+        print(f"[bench] mixed {args.prec} + refinement", file=sys.stderr, flush=True)
+        try:
+            dt_m, x_m, prof_m, ir = run_mixed(M, h, P, args.prec, args.steps, args.warmup)
+            be_m = backward_errors(P, x_m)
+            mixed = {"prec": args.prec, "tflops": round(flops / dt_m / 1e12, 4),
+                     "ms": round(dt_m * 1e3, 3), "backward_error": be_m, "ir": ir,
+                     "roofline": roofline_from(prof_m, args.steps, args.prec)}
+            print(f"[bench] mixed: {mixed['tflops']} TFLOP/s, {mixed['ms']} ms, be={be_m}, ir={ir}",
+                  file=sys.stderr, flush=True)
+        except M.MqrError as e:
+            print(f"[bench] mixed path unavailable: {e}", file=sys.stderr, flush=True)
+
+    head = mixed if mixed is not None else plain
+    dt_head = head["ms"] / 1e3
+    cpu = None if args.no_cpu else cpu_baseline()
+    rec = {
+        "metric": METRIC,
+        "value": head["tflops"],
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if mixed is None else f"{args.prec}+f64",
+        "data": "synthetic",
+        "config": {"workload": f"least squares {m}x{n}, A = U diag(logspace(0,-6)) V^T (Haar U, V), "
+                               f"b = A x0 + 1e-10 noise; one full solve per step",
+                   "m": m, "n": n, "nb": h.nb, "mixed_prec": None if mixed is None else args.prec,
+                   "seed": 1234},
+        "pct_mfma_peak": round(100 * head["tflops"] / PEAK_TFLOPS["fp64" if mixed is None else "fp32"], 2),
+        "speedup_vs_fp64_plain": None if mixed is None else round(plain["ms"] / mixed["ms"], 3),
+        "fp64_plain": plain,
+        "mixed": mixed,
+        "roofline": head["roofline"],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec))
+    h.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

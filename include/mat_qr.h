@@ -68,7 +68,8 @@ const char *mqr_strerror(int status);
 int mqr_get_device_count(int *count);
 int mqr_create(mqr_handle_t *handle, int device);
 int mqr_destroy(mqr_handle_t handle);
-/* stream: a hipStream_t, or NULL for the handle's own stream */
+/* stream: a hipStream_t; NULL selects the default (null) stream.  A new
+ * handle starts on a non-blocking stream of its own (mqr_get_stream). */
 int mqr_set_stream(mqr_handle_t handle, void *stream);
 int mqr_get_stream(mqr_handle_t handle, void **stream);
 /* Block size nb of the blocked factorisation (the reference's __BlockSize_QR,

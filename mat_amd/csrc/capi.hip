@@ -145,7 +145,7 @@ int mqr_destroy(mqr_handle_t h) {
 
 int mqr_set_stream(mqr_handle_t h, void *stream) {
     if (!h) return -1;
-    h->stream = stream ? (hipStream_t)stream : h->own_stream;
+    h->stream = (hipStream_t)stream;
     return MQR_SUCCESS;
 }
 

@@ -99,8 +99,10 @@ __global__ __launch_bounds__(NT) void k_gemm_f64(GemmF64 g, int kchunk) {
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
+                // v_mfma_f64_16x16x4_f64: lane l, accumulator r holds output
+                // row (l >> 4) + 4 r, column l & 15 (of the swapped product)
                 const int gm = m0 + wm * 32 + i * 16 + (lane & 15);
-                const int gn = n0 + wn * 32 + j * 16 + 4 * (lane >> 4) + r;
+                const int gn = n0 + wn * 32 + j * 16 + (lane >> 4) + 4 * r;
                 if (gm < g.M && gn < g.N) {
                     const int64_t idx = (int64_t)gm + (int64_t)gn * g.ldc;
                     const double v = g.alpha * acc[i][j][r];

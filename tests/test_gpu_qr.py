@@ -176,7 +176,8 @@ def test_trsm_upper(h):
         dB = M.colmajor(B)
         M.trsm_upper(h, trans, M.colmajor(R), dB)
         ref = sla.solve_triangular(R, B, trans=0 if trans == "N" else 1, lower=False)
-        np.testing.assert_allclose(host(dB), ref, rtol=0, atol=1e-11)
+        # random triangular R is ill-conditioned: bound relative to the solution size
+        np.testing.assert_allclose(host(dB), ref, rtol=0, atol=1e-12 * n * np.abs(ref).max())
 
 
 def test_dgels_matches_oracle(h, orc):
