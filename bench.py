@@ -183,6 +183,36 @@ def cpu_baseline(seconds: float = 20.0) -> dict:
             "cpu_avx512": bool(cpu_has_avx512())}
 
 
+def gemm_rates(M, h) -> list:
+    """FP64 MFMA GEMM tile (mqr_dgemm) at the factorisation's shapes, timed
+    with events on the handle's stream (torch's current stream)."""
+    import torch
+    out = []
+    shapes = [("square", "N", "N", 4096, 4096, 4096), ("update_k64", "N", "N", 16384, 4096, 64),
+              ("update_k256", "N", "N", 16384, 4096, 256), ("vtc_k16384", "T", "N", 256, 4096, 16384)]
+    for name, ta, tb, m, n, k in shapes:
+        A = M.colmajor(torch.randn(k, m, dtype=torch.float64, device="cuda") if ta == "T"
+                       else torch.randn(m, k, dtype=torch.float64, device="cuda"))
+        B = M.colmajor(torch.randn(k, n, dtype=torch.float64, device="cuda"))
+        Cm = M.colmajor(torch.zeros(m, n, dtype=torch.float64, device="cuda"))
+        for _ in range(2):
+            M.gemm(h, ta, tb, 1.0, A, B, 1.0, Cm)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 5
+        e0.record()
+        for _ in range(reps):
+            M.gemm(h, ta, tb, 1.0, A, B, 1.0, Cm)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        tf = 2.0 * m * n * k / (ms * 1e-3) / 1e12
+        out.append({"shape": name, "m": m, "n": n, "k": k, "ms": round(ms, 4), "tflops": round(tf, 2),
+                    "frac_fp64_peak": round(tf / PEAK_TFLOPS["fp64"], 3)})
+        print(f"[bench] dgemm {name} {m}x{n}x{k}: {ms:.3f} ms {tf:.2f} TFLOP/s", file=sys.stderr,
+              flush=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,7 +224,14 @@ def main() -> int:
     ap.add_argument("--prec", default="fp32", choices=["fp32", "bf16x3", "bf16", "none"],
                     help="mixed-precision trailing updates ('none': plain FP64 only)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel", choices=["gemm"], default=None,
+                    help="time one kernel family instead of the solve")
     args = ap.parse_args()
+    if args.kernel == "gemm":
+        import mat_amd as M
+        h = M.Handle(0)
+        print(json.dumps({"dgemm": gemm_rates(M, h)}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
               "(the path runs on a single MI355X; see the sharding design notes)", file=sys.stderr)

@@ -113,6 +113,13 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m mat_amd.build`")
+        # torch ships its own HIP runtime under the same SONAME
+        # (libamdhip64.so.7).  Loading torch first makes libmatqr bind to that
+        # already-loaded runtime, so one process never holds two of them.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

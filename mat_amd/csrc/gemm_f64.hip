@@ -1,13 +1,17 @@
-// FP64 MFMA GEMM tile for gfx950: the device counterpart of the reference's
+// FP64 MFMA GEMM tiles for gfx950: the device counterpart of the reference's
 // packed MR x NR micro-kernel and its macro-kernel drivers
 // (Mul/__micro_kernel_packed_double.h:301-397,763-817; Mul/Mul.h:239-322).
 //
 // Instead of packing into MR/NR panels in a separate pass
-// (Mul/__pack_double.h), each workgroup stages a 64 x 16 slice of op(A) and a
-// 16 x 64 slice of op(B) in LDS, applying the unit-lower-trapezoid mask of a
-// reflector block V on the fly (the reference's __pack_L_block /
-// __pack_LT_block, Mul/__pack.h:108-189).  Four waves each own a 32 x 32
-// quarter of the 64 x 64 output tile as 2 x 2 v_mfma_f64_16x16x4_f64 blocks.
+// (Mul/__pack_double.h), each workgroup stages slices of op(A) and op(B) in
+// LDS, applying the unit-lower-trapezoid mask of a reflector block V on the
+// fly (the reference's __pack_L_block / __pack_LT_block, Mul/__pack.h:108-189).
+// Loads are branch-free (clamped in-bounds addresses, out-of-range values
+// selected to zero) so that a whole slice is in flight at once.
+//
+// Two tiles: 64 x 64 (four waves, 2 x 2 MFMA blocks each) for small or thin
+// products, and 128 x 128 with a register-prefetched double-buffered LDS
+// pipeline (four waves, 4 x 4 MFMA blocks each) for the large ones.
 #include <climits>
 
 #include "mqr_internal.h"
@@ -17,106 +21,188 @@ namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-constexpr int TBM = 64, TBN = 64, TBK = 16, NT = 256;
+constexpr int NT = 256;
 
-template <int MASK>
-__device__ __forceinline__ double ld_masked(const double *A, int64_t ld, int r, int c,
-                                            int mask_cols) {
-    if (MASK == MASK_VLOWER && c < mask_cols) {
-        if (r < c) return 0.0;
-        if (r == c) return 1.0;
-    }
-    return A[(int64_t)r + (int64_t)c * ld];
-}
-
-template <bool TA, bool TB, int AM, int BMK>
-__global__ __launch_bounds__(NT) void k_gemm_f64(GemmF64 g, int kchunk) {
-    __shared__ double As[TBK][TBM + 1];
-    __shared__ double Bs[TBK][TBN + 1];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
-    const int m0 = blockIdx.x * TBM, n0 = blockIdx.y * TBN;
-    const int kb = blockIdx.z * kchunk;
-    const int ke = min(g.K, kb + kchunk);
-
-    v4d acc[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
-
-    for (int k0 = kb; k0 < ke; k0 += TBK) {
-#pragma unroll
-        for (int t = 0; t < (TBM * TBK) / NT; ++t) {
-            const int e = tid + t * NT;
-            int mm, kk;
-            if (TA) { kk = e % TBK; mm = e / TBK; }
-            else { mm = e % TBM; kk = e / TBM; }
-            const int gm = m0 + mm, gk = k0 + kk;
-            double v = 0.0;
-            if (gm < g.M && gk < ke)
-                v = TA ? ld_masked<AM>(g.A, g.lda, gk, gm, INT_MAX)
-                       : ld_masked<AM>(g.A, g.lda, gm, gk, INT_MAX);
-            As[kk][mm] = v;
-        }
-#pragma unroll
-        for (int t = 0; t < (TBN * TBK) / NT; ++t) {
-            const int e = tid + t * NT;
-            int nn, kk;
-            if (!TB) { kk = e % TBK; nn = e / TBK; }
-            else { nn = e % TBN; kk = e / TBN; }
-            const int gn = n0 + nn, gk = k0 + kk;
-            double v = 0.0;
-            if (gn < g.N && gk < ke)
-                v = !TB ? ld_masked<BMK>(g.B, g.ldb, gk, gn, g.bmask_cols)
-                        : ld_masked<BMK>(g.B, g.ldb, gn, gk, g.bmask_cols);
-            Bs[kk][nn] = v;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < TBK; kk += 4) {
-            double a[2], b[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) a[i] = As[kk + (lane >> 4)][wm * 32 + i * 16 + (lane & 15)];
-#pragma unroll
-            for (int j = 0; j < 2; ++j) b[j] = Bs[kk + (lane >> 4)][wn * 32 + j * 16 + (lane & 15)];
-            // operands swapped so that the 16 lanes of a row group hold 16
-            // consecutive rows of one output column (coalesced column-major stores)
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
-        }
-        __syncthreads();
-    }
-
+// Epilogue shared by the tiles: acc element (i, j, r) of a wave's
+// (BI*16) x (BJ*16) block goes to row rowbase + i*16 + (lane & 15), column
+// colbase + j*16 + (lane >> 4) + 4 r.  v_mfma_f64_16x16x4_f64 with swapped
+// operands leaves output row (l >> 4) + 4 r, column l & 15 of the transposed
+// product in lane l, register r: 16 consecutive rows of one column per
+// row group, i.e. coalesced column-major stores.
+template <int BI, int BJ>
+__device__ __forceinline__ void epilogue(const GemmF64 &g, v4d (&acc)[BI][BJ], int rowbase,
+                                         int colbase, bool interior) {
+    const int lane = threadIdx.x & 63;
     double *Cout = g.C + (g.splits > 1 ? (int64_t)blockIdx.z * g.split_stride : 0);
     const bool direct = g.splits > 1 || g.beta == 0.0;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < BI; ++i) {
+        const int gm = rowbase + i * 16 + (lane & 15);
+        double cv[BJ][4];
+        if (!direct) {
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+            for (int j = 0; j < BJ; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int gn = colbase + j * 16 + (lane >> 4) + 4 * r;
+                    const int cm = gm < g.M ? gm : g.M - 1, cn = gn < g.N ? gn : g.N - 1;
+                    cv[j][r] = Cout[(int64_t)cm + (int64_t)cn * g.ldc];
+                }
+        }
+#pragma unroll
+        for (int j = 0; j < BJ; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                // v_mfma_f64_16x16x4_f64: lane l, accumulator r holds output
-                // row (l >> 4) + 4 r, column l & 15 (of the swapped product)
-                const int gm = m0 + wm * 32 + i * 16 + (lane & 15);
-                const int gn = n0 + wn * 32 + j * 16 + (lane >> 4) + 4 * r;
-                if (gm < g.M && gn < g.N) {
-                    const int64_t idx = (int64_t)gm + (int64_t)gn * g.ldc;
-                    const double v = g.alpha * acc[i][j][r];
-                    Cout[idx] = direct ? v : g.beta * Cout[idx] + v;
-                }
+                const int gn = colbase + j * 16 + (lane >> 4) + 4 * r;
+                double v = g.alpha * acc[i][j][r];
+                if (!direct) v = fma(g.beta, cv[j][r], v);
+                if (interior || (gm < g.M && gn < g.N))
+                    Cout[(int64_t)gm + (int64_t)gn * g.ldc] = v;
             }
+    }
 }
+
+// ---------------------------------------------------------------------------
+// TILE x TILE output tile (TILE = 64 or 128), 16-deep K slices, 256 threads.
+// The next slices of op(A) and op(B) are loaded into registers while the
+// MFMAs consume the current LDS buffers (double buffering, one barrier per
+// slice).  Each wave owns a (TILE/2) x (TILE/2) quarter as (TILE/32)^2
+// 16 x 16 MFMA blocks.
+// ---------------------------------------------------------------------------
+constexpr int BK = 16;
+
+// Position (r, k) of element q of a thread's share of a TILE x 16 slice of
+// op(X).  KMAJOR: the stored matrix is k x r (consecutive k contiguous);
+// otherwise r x k (consecutive r contiguous).
+template <int TILE, bool KMAJOR>
+__device__ __forceinline__ void slot(int tid, int q, int &r, int &k) {
+    constexpr int NV = TILE * BK / NT;  // values per thread: 4 or 8
+    if (!KMAJOR) {
+        constexpr int RT = TILE / 2;     // threads along r (two values each)
+        r = (tid % RT) * 2 + (q & 1);
+        k = (tid / RT) + (NT / RT) * (q >> 1);
+    } else {
+        constexpr int KT = BK / NV;      // threads along k
+        r = tid / KT;
+        k = (tid % KT) * NV + q;
+    }
+}
+
+// Raw loads from clamped, always in-bounds addresses.
+template <int TILE, bool KMAJOR>
+__device__ __forceinline__ void fetch(const double *__restrict__ X, int64_t ld, int r0, int k0,
+                                      int R, int kend, int tid, double *v) {
+    constexpr int NV = TILE * BK / NT;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        int r, k;
+        slot<TILE, KMAJOR>(tid, q, r, k);
+        r += r0;
+        k += k0;
+        const int rr = r < R ? r : R - 1, kc = k < kend ? k : kend - 1;
+        v[q] = KMAJOR ? X[(int64_t)kc + (int64_t)rr * ld] : X[(int64_t)rr + (int64_t)kc * ld];
+    }
+}
+
+// Bounds and the reflector mask applied at LDS-store time, after the MFMAs
+// of the current slice have been issued.  Stored coordinates (row, col) are
+// (k, r) when KMAJOR, else (r, k).
+template <int TILE, bool KMAJOR, int MASK>
+__device__ __forceinline__ void stash(double (*S)[TILE + 2], int tid, const double *v, int r0,
+                                      int k0, int R, int kend, int mask_cols) {
+    constexpr int NV = TILE * BK / NT;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        int r, k;
+        slot<TILE, KMAJOR>(tid, q, r, k);
+        const int gr = r0 + r, gk = k0 + k;
+        const bool in = gr < R && gk < kend;
+        const int srow = KMAJOR ? gk : gr, scol = KMAJOR ? gr : gk;
+        bool keep = in, one = false;
+        if (MASK == MASK_VLOWER) {
+            const bool m = (scol < mask_cols) && (srow <= scol);
+            keep = in && !m;
+            one = in && m && (srow == scol);
+        }
+        const unsigned long long bits = (__double_as_longlong(v[q]) & (keep ? ~0ull : 0ull)) |
+                                        (one ? 0x3FF0000000000000ull : 0ull);
+        S[k][r] = __longlong_as_double(bits);
+    }
+}
+
+template <int TILE, bool TA, bool TB, int AM, int BMK>
+__global__ __launch_bounds__(NT) void k_gemm_f64(GemmF64 g, int kchunk) {
+    constexpr int NV = TILE * BK / NT, BW = TILE / 32, HALF = TILE / 2;
+    __shared__ double As[2][BK][TILE + 2];
+    __shared__ double Bs[2][BK][TILE + 2];
+    if (g.skip_if && *g.skip_if) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int m0 = blockIdx.x * TILE, n0 = blockIdx.y * TILE;
+    const int kb = blockIdx.z * kchunk;
+    const int ke = min(g.K, kb + kchunk);
+
+    v4d acc[BW][BW];
+#pragma unroll
+    for (int i = 0; i < BW; ++i)
+#pragma unroll
+        for (int j = 0; j < BW; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+
+    if (kb < ke) {
+        double va[NV], vb[NV];
+        // op(A) is stored k-major when transposed; op(B)^T likewise when not
+        fetch<TILE, TA>(g.A, g.lda, m0, kb, g.M, ke, tid, va);
+        fetch<TILE, !TB>(g.B, g.ldb, n0, kb, g.N, ke, tid, vb);
+        stash<TILE, TA, AM>(As[0], tid, va, m0, kb, g.M, ke, INT_MAX);
+        stash<TILE, !TB, BMK>(Bs[0], tid, vb, n0, kb, g.N, ke, g.bmask_cols);
+        __syncthreads();
+        int buf = 0;
+        for (int k0 = kb; k0 < ke; k0 += BK) {
+            const bool more = k0 + BK < ke;
+            if (more) {
+                fetch<TILE, TA>(g.A, g.lda, m0, k0 + BK, g.M, ke, tid, va);
+                fetch<TILE, !TB>(g.B, g.ldb, n0, k0 + BK, g.N, ke, tid, vb);
+            }
+#pragma unroll
+            for (int kk = 0; kk < BK; kk += 4) {
+                double a[BW], b[BW];
+#pragma unroll
+                for (int i = 0; i < BW; ++i)
+                    a[i] = As[buf][kk + (lane >> 4)][wm * HALF + i * 16 + (lane & 15)];
+#pragma unroll
+                for (int j = 0; j < BW; ++j)
+                    b[j] = Bs[buf][kk + (lane >> 4)][wn * HALF + j * 16 + (lane & 15)];
+#pragma unroll
+                for (int i = 0; i < BW; ++i)
+#pragma unroll
+                    for (int j = 0; j < BW; ++j)
+                        acc[i][j] =
+                            __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
+            }
+            if (more) {
+                stash<TILE, TA, AM>(As[buf ^ 1], tid, va, m0, k0 + BK, g.M, ke, INT_MAX);
+                stash<TILE, !TB, BMK>(Bs[buf ^ 1], tid, vb, n0, k0 + BK, g.N, ke, g.bmask_cols);
+            }
+            __syncthreads();
+            buf ^= 1;
+        }
+    }
+    const bool interior = (m0 + TILE <= g.M) && (n0 + TILE <= g.N);
+    epilogue<BW, BW>(g, acc, m0 + wm * HALF, n0 + wn * HALF, interior);
+}
+
+constexpr int BIG = 128, SMALL = 64;
 
 template <bool TA, bool TB, int AM, int BMK>
 void launch(const GemmF64 &g, hipStream_t s) {
+    const int tile = (g.M >= BIG && g.N >= BIG) ? BIG : SMALL;
     int kchunk = (g.K + g.splits - 1) / g.splits;
-    kchunk = ((kchunk + TBK - 1) / TBK) * TBK;
-    dim3 grid((g.M + TBM - 1) / TBM, (g.N + TBN - 1) / TBN, g.splits);
-    hipLaunchKernelGGL((k_gemm_f64<TA, TB, AM, BMK>), grid, dim3(NT), 0, s, g, kchunk);
+    kchunk = ((kchunk + BK - 1) / BK) * BK;
+    dim3 grid((g.M + tile - 1) / tile, (g.N + tile - 1) / tile, g.splits);
+    if (tile == BIG)
+        hipLaunchKernelGGL((k_gemm_f64<BIG, TA, TB, AM, BMK>), grid, dim3(NT), 0, s, g, kchunk);
+    else
+        hipLaunchKernelGGL((k_gemm_f64<SMALL, TA, TB, AM, BMK>), grid, dim3(NT), 0, s, g, kchunk);
 }
 
 __global__ void k_scale(int64_t m, int64_t n, double beta, double *C, int64_t ldc) {
@@ -137,7 +223,8 @@ int scale_f64(int64_t m, int64_t n, double beta, double *C, int64_t ldc, hipStre
 }
 
 int gemm_pick_splits(int M, int N, int K, int target_blocks) {
-    const int tiles = ((M + TBM - 1) / TBM) * ((N + TBN - 1) / TBN);
+    const int t = (M >= BIG && N >= BIG) ? BIG : SMALL;
+    const int tiles = ((M + t - 1) / t) * ((N + t - 1) / t);
     if (tiles >= target_blocks || K <= 512) return 1;
     int s = (target_blocks + tiles - 1) / tiles;
     const int maxs = (K + 255) / 256;

@@ -88,7 +88,8 @@ int apply_block(mqr_handle_t hd, int64_t h, int b, const double *Vp, int64_t ldv
     g.C = P; g.ldc = b; g.alpha = 1.0; g.beta = 0.0;
     g.splits = splits; g.split_stride = (int64_t)b * nc; g.kind = K_VTA;
     MQR_CHECK(gemm_f64(g, hd->stream));
-    MQR_CHECK(block_w2(b, nc, Tp, ldt, trans, P, splits, (int64_t)b * nc, W2, hd->stream));
+    MQR_CHECK(reduce_partials((int64_t)b * nc, P, splits, (int64_t)b * nc, hd->stream));
+    MQR_CHECK(block_w2(b, nc, Tp, ldt, trans, P, 1, (int64_t)b * nc, W2, hd->stream));
     GemmF64 u;
     u.M = (int)h; u.N = (int)nc; u.K = b;
     u.amask = MASK_VLOWER;
@@ -99,10 +100,17 @@ int apply_block(mqr_handle_t hd, int64_t h, int b, const double *Vp, int64_t ldv
 
 }  // namespace
 
-// Workspace (doubles) the factorisation of an m x n matrix uses.
+// Workspace (doubles) the factorisation of an m x n matrix uses: split-K
+// partials, W2, and the tall-panel scratch.
 static int64_t geqrf_ws_doubles(int64_t m, int64_t n, int nb) {
-    (void)m;
-    return (int64_t)16 * nb * (n + nb) + (int64_t)nb * (n + nb) + 1024;
+    return (int64_t)16 * nb * (n + nb) + (int64_t)nb * (n + nb) + 1024 +
+           (int64_t)(panel_cholqr_ws_bytes(m) / sizeof(double)) + 64;
+}
+
+int panel_factor(mqr_handle_t hd, int64_t h, int b, double *V, int64_t lda, double *tau,
+                 void *pws) {
+    if (h >= kCholqrMinRows) return panel_cholqr(hd, h, b, V, lda, tau, pws);
+    return panel_householder(h, b, V, lda, tau, hd->stream);
 }
 
 int geqrf_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, double *tau,
@@ -113,11 +121,12 @@ int geqrf_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, dou
     MQR_CHECK(ws_reserve(hd, hd->ws, sizeof(double) * geqrf_ws_doubles(m, n, nb)));
     double *P = (double *)hd->ws.ptr;
     double *W2 = P + pcap;
+    void *pws = (void *)(((uintptr_t)(W2 + (int64_t)nb * (n + nb) + 1024) + 255) & ~(uintptr_t)255);
     for (int64_t j = 0; j < k; j += nb) {
         const int b = (int)std::min<int64_t>(nb, k - j);
         const int64_t h = m - j;
         double *V = A + j + j * lda;
-        MQR_CHECK(panel_householder(h, b, V, lda, tau + j, hd->stream));
+        MQR_CHECK(panel_factor(hd, h, b, V, lda, tau + j, pws));
         // V^T [V | C] with C = A[j:m, j+b:n]: Gram of V in the first b columns
         const int64_t ncols = n - j;
         int splits = gemm_pick_splits(b, (int)ncols, (int)h, target_blocks(hd));
@@ -130,11 +139,12 @@ int geqrf_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, dou
         g.kind = K_VTA;
         MQR_CHECK(gemm_f64(g, hd->stream));
         double *Tj = T + j * ldt;
-        MQR_CHECK(larft_from_gram(b, P, splits, (int64_t)b * ncols, tau + j, Tj, ldt, hd->stream));
+        MQR_CHECK(reduce_partials((int64_t)b * ncols, P, splits, (int64_t)b * ncols, hd->stream));
+        MQR_CHECK(larft_from_gram(b, P, 1, (int64_t)b * ncols, tau + j, Tj, ldt, hd->stream));
         const int64_t nt = n - j - b;
         if (nt > 0) {
-            MQR_CHECK(block_w2(b, nt, Tj, ldt, true, P + (int64_t)b * b, splits,
-                               (int64_t)b * ncols, W2, hd->stream));
+            MQR_CHECK(block_w2(b, nt, Tj, ldt, true, P + (int64_t)b * b, 1, (int64_t)b * ncols, W2,
+                               hd->stream));
             GemmF64 u;
             u.M = (int)h; u.N = (int)nt; u.K = b;
             u.amask = MASK_VLOWER;

@@ -121,6 +121,8 @@ struct GemmF64 {
     int splits = 1;
     int64_t split_stride = 0;
     int kind = K_GEMM;  // profiler bucket
+    // when non-null and *skip_if != 0 at run time, the launch does nothing
+    const int *skip_if = nullptr;
 };
 int gemm_f64(const GemmF64 &g, hipStream_t s);
 // C = beta * C (beta == 0 writes zeros).
@@ -131,6 +133,17 @@ int gemm_pick_splits(int M, int N, int K, int target_blocks);
 // ---- panel / T / block reflector ------------------------------------------
 // Householder panel on h x b at P (ld), single workgroup, in place.
 int panel_householder(int64_t h, int b, double *P, int64_t ld, double *tau, hipStream_t s);
+// Same, launched unconditionally but running only when *gate != 0.
+int panel_householder_gated(int64_t h, int b, double *P, int64_t ld, double *tau, const int *gate,
+                            hipStream_t s);
+// Shifted CholeskyQR3 + Householder reconstruction panel (falls back to the
+// gated Householder kernel on breakdown).  ws: panel_cholqr_ws_bytes(h).
+size_t panel_cholqr_ws_bytes(int64_t h);
+int panel_cholqr(mqr_handle_t hd, int64_t h, int b, double *X, int64_t ld, double *tau, void *ws);
+// Panels at least this tall use panel_cholqr.
+constexpr int64_t kCholqrMinRows = 384;
+// P[0:count] += P[s*stride + 0:count] for s = 1..splits-1.
+int reduce_partials(int64_t count, double *P, int splits, int64_t stride, hipStream_t s);
 // Largest block size the LDS-resident T kernels accept.
 constexpr int kMaxNb = 64;
 // T from partial Gram matrices G_s (b x b each, ld = b, `splits` of them at

@@ -2,14 +2,14 @@
 //
 //  k_panel_hh  : QR/__Householder_Blocked.h:62-84 -- b Householder reflectors
 //                of an h x b panel, one workgroup, wavefront reductions.
-//  k_larft     : QR/__Householder_Blocked.h:86-103 -- the forward columnwise
-//                T recurrence T(0:i,i) = -tau_i T(0:i,0:i) s, with
-//                s_l = V(:,l)^T V(:,i) taken from the Gram matrix V^T V that
-//                the MFMA GEMM produced.
+//  k_larft     : QR/__Householder_Blocked.h:86-103 -- the compact-WY T of
+//                the panel's reflectors from the Gram matrix V^T V that the
+//                MFMA GEMM produced.
 //  k_block_w2  : the middle product of the trailing update,
 //                W2 = T^T W (QR/__Householder_Blocked.h:180-190) or T W
 //                (Q formation, :299-308), with W summed from split-K partials.
 #include "mqr_internal.h"
+#include "small_dense.h"
 
 namespace mqr {
 namespace {
@@ -23,9 +23,13 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// gate: when non-null, the kernel only runs if *gate is nonzero (the fast
+// panel raised its fallback flag).
 __global__ __launch_bounds__(PNT) void k_panel_hh(int64_t h, int b, double *__restrict__ P,
-                                                  int64_t ld, double *__restrict__ tau) {
+                                                  int64_t ld, double *__restrict__ tau,
+                                                  const int *__restrict__ gate) {
     __shared__ double red[PNT / 64][CH];
+    if (gate && *gate == 0) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int r = 0; r < b; ++r) {
         double *w = P + (int64_t)r * ld + r;
@@ -98,37 +102,45 @@ __global__ __launch_bounds__(PNT) void k_panel_hh(int64_t h, int b, double *__re
     }
 }
 
-// T from the split-K partial Gram matrices of V (slot s at Gp + s * pstride,
-// ld = b).  Dynamic LDS: 2 * b * b doubles (b <= 64).
-__global__ __launch_bounds__(256) void k_larft(int b, const double *__restrict__ Gp, int splits,
-                                               int64_t pstride, const double *__restrict__ tau,
-                                               double *__restrict__ T, int64_t ldt) {
-    extern __shared__ double sm[];
-    double *G = sm;
-    double *Tl = sm + b * b;
-    const int tid = threadIdx.x;
-    for (int e = tid; e < b * b; e += blockDim.x) {
-        double acc = 0.0;
-        for (int s = 0; s < splits; ++s) acc += Gp[(int64_t)s * pstride + e];
-        G[e] = acc;
-        Tl[e] = 0.0;
-    }
-    __syncthreads();
-    if (tid == 0) Tl[0] = tau[0];
-    __syncthreads();
-    for (int i = 1; i < b; ++i) {
-        const double ti = tau[i];
-        if (tid < i) {
-            double acc = 0.0;
-            for (int c = tid; c < i; ++c) acc = fma(Tl[tid + c * b], G[c + i * b], acc);
-            Tl[tid + i * b] = -ti * acc;
+// T from the Gram matrix G = V^T V (summed split-K partials, slot s at
+// Gp + s * pstride, ld = b) and tau.  The forward columnwise recurrence
+// T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i), T(i, i) = tau_i
+// (QR/__Householder_Blocked.h:86-103) has the closed form
+// T^{-1} = strict_upper(G) + diag(1 / tau); it is evaluated here as one
+// register-resident triangular inverse (small_dense.h).  A reflector with
+// tau = 0 is the identity: its row and column are removed from the inverse
+// and left zero in T, which is what the recurrence produces.
+__global__ __launch_bounds__(64) void k_larft(int b, const double *__restrict__ Gp, int splits,
+                                              int64_t pstride, const double *__restrict__ tau,
+                                              double *__restrict__ T, int64_t ldt) {
+    __shared__ double S[small::N][small::N + 1];
+    __shared__ double rdiag[small::N];
+    const int c = small::lane_id();
+    const double tc = c < b ? tau[c] : 1.0;
+    double m[small::N], x[small::N];
+#pragma unroll
+    for (int r = 0; r < small::N; ++r) {
+        double v = 0.0;
+        if (r < b && c < b && r < c) {
+            for (int s = 0; s < splits; ++s) v += Gp[(int64_t)s * pstride + r + (int64_t)c * b];
+            if (tau[r] == 0.0 || tc == 0.0) v = 0.0;
+        } else if (r == c) {
+            v = (c < b && tc != 0.0) ? 1.0 / tc : 1.0;
         }
-        if (tid == 0) Tl[i + i * b] = ti;
-        __syncthreads();
+        m[r] = v;
     }
-    for (int e = tid; e < b * b; e += blockDim.x) {
-        const int r = e % b, c = e / b;
-        T[r + (int64_t)c * ldt] = r <= c ? Tl[e] : 0.0;
+    small::col_to_lds(m, S);
+    __syncthreads();
+    rdiag[c] = 1.0 / S[c][c];
+    __syncthreads();
+    small::inv_upper_lds(S, rdiag, x);
+#pragma unroll
+    for (int r = 0; r < small::N; ++r) {
+        if (r < b && c < b) {
+            double v = r <= c ? x[r] : 0.0;
+            if (r == c && tc == 0.0) v = 0.0;
+            T[r + (int64_t)c * ldt] = v;
+        }
     }
 }
 
@@ -175,8 +187,6 @@ bool g_lds_set = false;
 
 int ensure_lds_attrs() {
     if (g_lds_set) return MQR_SUCCESS;
-    MQR_HIP(hipFuncSetAttribute((const void *)k_larft, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                2 * kMaxNb * kMaxNb * 8));
     MQR_HIP(hipFuncSetAttribute((const void *)k_block_w2,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (kMaxNb * kMaxNb + kMaxNb * W2COLS) * 8));
@@ -189,18 +199,50 @@ int ensure_lds_attrs() {
 int panel_householder(int64_t h, int b, double *P, int64_t ld, double *tau, hipStream_t s) {
     const double hh = (double)h, bb = b;
     ProfScope ps(K_PANEL, 2.0 * hh * bb * bb - 2.0 * bb * bb * bb / 3.0, 16.0 * hh * bb, s);
-    hipLaunchKernelGGL(k_panel_hh, dim3(1), dim3(PNT), 0, s, h, b, P, ld, tau);
+    hipLaunchKernelGGL(k_panel_hh, dim3(1), dim3(PNT), 0, s, h, b, P, ld, tau,
+                       (const int *)nullptr);
+    MQR_LAUNCH_CHECK();
+    return MQR_SUCCESS;
+}
+
+int panel_householder_gated(int64_t h, int b, double *P, int64_t ld, double *tau, const int *gate,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_panel_hh, dim3(1), dim3(PNT), 0, s, h, b, P, ld, tau, gate);
+    MQR_LAUNCH_CHECK();
+    return MQR_SUCCESS;
+}
+
+// P[0:count] = sum over s of P[s * stride + 0:count]
+__global__ void k_reduce_partials(int64_t count, double *__restrict__ P, int splits,
+                                  int64_t stride) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    // four independent chains keep several loads in flight
+    double a0 = P[e], a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int s = 1;
+    for (; s + 3 < splits; s += 4) {
+        a1 += P[(int64_t)s * stride + e];
+        a2 += P[(int64_t)(s + 1) * stride + e];
+        a3 += P[(int64_t)(s + 2) * stride + e];
+        a0 += P[(int64_t)(s + 3) * stride + e];
+    }
+    for (; s < splits; ++s) a1 += P[(int64_t)s * stride + e];
+    P[e] = (a0 + a1) + (a2 + a3);
+}
+
+int reduce_partials(int64_t count, double *P, int splits, int64_t stride, hipStream_t s) {
+    if (splits <= 1 || count <= 0) return MQR_SUCCESS;
+    ProfScope ps(K_OTHER, (double)count * (splits - 1), 8.0 * count * (splits + 1), s);
+    hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
+                       count, P, splits, stride);
     MQR_LAUNCH_CHECK();
     return MQR_SUCCESS;
 }
 
 int larft_from_gram(int b, const double *Gpart, int splits, int64_t pstride, const double *tau,
                     double *T, int64_t ldt, hipStream_t s) {
-    MQR_CHECK(ensure_lds_attrs());
-    const size_t lds = (size_t)2 * b * b * sizeof(double);
     ProfScope ps(K_LARFT, (double)b * b * b / 3.0, 8.0 * b * b * (splits + 1), s);
-    hipLaunchKernelGGL(k_larft, dim3(1), dim3(256), lds, s, b, Gpart, splits, pstride, tau, T,
-                       ldt);
+    hipLaunchKernelGGL(k_larft, dim3(1), dim3(64), 0, s, b, Gpart, splits, pstride, tau, T, ldt);
     MQR_LAUNCH_CHECK();
     return MQR_SUCCESS;
 }

@@ -67,7 +67,7 @@ def gpu_factor(h, A):
 def test_gemm_exact_on_integers(h, orc, ta, tb):
     rng = np.random.default_rng(1)
     for (m, n, k) in [(1, 1, 1), (16, 16, 4), (64, 64, 16), (37, 29, 41), (200, 150, 90),
-                      (1000, 70, 300)]:
+                      (1000, 70, 300), (513, 384, 200), (128, 128, 16), (260, 1030, 33)]:
         note(f"gemm {ta}{tb} m={m} n={n} k={k}")
         A = rng.integers(-8, 9, (k, m) if ta == "T" else (m, k)).astype(np.float64)
         B = rng.integers(-8, 9, (n, k) if tb == "T" else (k, n)).astype(np.float64)
@@ -221,6 +221,57 @@ def test_zero_and_dependent_columns(h, orc):
         Q = host(dQ)
         assert np.linalg.norm(Q @ np.triu(VR) - A) < 1e-12 * m
         assert np.linalg.norm(Q.T @ Q - np.eye(m)) < 1e-12 * m
+
+
+def haar_np(rng, rows, cols):
+    Q, R = np.linalg.qr(rng.standard_normal((rows, cols)))
+    return Q * np.sign(np.diag(R))[None, :]
+
+
+@pytest.mark.parametrize("m,n,smin", [(1500, 1000, 1e-8), (3000, 700, 1e-12), (1200, 1200, 1e-14)])
+def test_graded_matrices_match_oracle(h, orc, m, n, smin):
+    """Tall panels (CholeskyQR-based) on ill-conditioned inputs: same R, V,
+    tau as the oracle to backward-error level, orthogonal Q."""
+    note(f"graded {m}x{n} smin={smin}")
+    rng = np.random.default_rng(int(m + n))
+    A = (haar_np(rng, m, n) * np.logspace(0, np.log10(smin), n)) @ haar_np(rng, n, n).T
+    dA, tau, T = gpu_factor(h, A)
+    dQ = M.orgqr(h, dA, T, n, nq=n)
+    VR, Q = host(dA), host(dQ)
+    R = np.triu(VR)[:n]
+    VRo, _, tau_o = orc.geqrf_blocked(A)
+    nA = np.linalg.norm(A)
+    assert np.linalg.norm(Q @ R - A) / nA < 8 * m * EPS
+    assert np.linalg.norm(Q.T @ Q - np.eye(n)) < 8 * m * EPS
+    # R agrees entrywise with the oracle relative to ||A|| (same sign convention)
+    assert np.abs(R - np.triu(VRo)[:n]).max() / nA < 1e-12
+    # reflectors agree where the columns are well determined (leading block)
+    lead = n // 4
+    np.testing.assert_allclose(tau[:lead].cpu().numpy(), tau_o[:lead], rtol=0, atol=1e-10)
+
+
+def test_tall_panel_fallback_on_dependent_columns(h, orc):
+    """Zero and duplicated columns inside tall panels: the fast panel hands
+    over to the column-by-column kernel, which keeps tau = 0 for a column that
+    is exactly zero, as the reference does."""
+    rng = np.random.default_rng(11)
+    m, n = 2000, 300
+    A = rng.standard_normal((m, n))
+    A[:, 5] = 0.0                 # zero column inside the first panel
+    A[:, 70] = A[:, 66]           # duplicate inside the second panel
+    A[:, 200] = 2.0 * A[:, 3]     # dependent on an earlier panel
+    dA, tau, T = gpu_factor(h, A)
+    tau = tau.cpu().numpy()
+    assert tau[5] == 0.0
+    VR = host(dA)
+    assert np.all(VR[5:, 5] == 0.0)
+    dQ = M.orgqr(h, dA, T, n, nq=m)
+    Q = host(dQ)
+    assert np.linalg.norm(Q @ np.triu(VR) - A) / np.linalg.norm(A) < 1e-14 * m
+    assert np.linalg.norm(Q.T @ Q - np.eye(m)) < 1e-13 * m
+    VRo, _, tau_o = orc.geqrf_blocked(A)
+    assert tau_o[5] == 0.0
+    assert abs(VR[70, 70]) < 1e-12 * np.linalg.norm(A) and abs(VR[200, 200]) < 1e-12 * np.linalg.norm(A)
 
 
 def test_host_qr_api_matches_oracle(orc):
