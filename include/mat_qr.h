@@ -109,6 +109,12 @@ int mqr_dgemm(mqr_handle_t handle, char transa, char transb, int64_t m, int64_t 
               double alpha, const double *A, int64_t lda, const double *B, int64_t ldb,
               double beta, double *C, int64_t ldc);
 
+/* Same for float (FP32 MFMA, FP32 accumulation): the counterpart of the
+ * generic packed micro-kernel for float (Mul/__micro_kernel_packed.h:1-307). */
+int mqr_sgemm(mqr_handle_t handle, char transa, char transb, int64_t m, int64_t n, int64_t k,
+              float alpha, const float *A, int64_t lda, const float *B, int64_t ldb, float beta,
+              float *C, int64_t ldc);
+
 /* ---- FP64 blocked Householder QR (device pointers) ------------------------ */
 /* Factor the m x n matrix A in place: R on and above the diagonal, the
  * reflectors below it, tau[min(m,n)], and the T blocks in T (ldt >= nb, ldt x
@@ -147,7 +153,7 @@ int mqr_dgels(mqr_handle_t handle, int64_t m, int64_t n, int64_t nrhs, double *A
 /* ---- mixed precision least squares with iterative refinement ------------ */
 typedef struct {
     int max_iter;    /* refinement steps allowed (default 30) */
-    double tol;      /* stop when the backward-error estimate <= tol (default 1e-15) */
+    double tol;      /* stop when the backward-error estimate <= tol (default 1e-14) */
     int fallback;    /* 1: on stagnation refactor in plain FP64 and solve (default 1) */
 } mqr_ir_opts_t;
 
@@ -155,14 +161,18 @@ typedef struct {
     int iterations;       /* refinement steps taken */
     int converged;        /* 1 when tol was met */
     int fell_back;        /* 1 when the FP64 fallback ran */
-    double backward_error;/* ||A^T r|| / (||A||_F (||A||_F ||x|| + ||b||)) at exit */
+    double backward_error;/* ||R^{-T} A^T r|| / (||A||_F ||x||) at exit: bounds the
+                             Karlson-Walden normwise backward error from above */
     double residual_norm; /* ||b - A x|| at exit */
 } mqr_ir_info_t;
 
 /* x = argmin ||A x - b|| for one right-hand side, m >= n.  A (m x n, FP64,
  * device) and b (m, device) are not modified; x (n, device) is written.
- * The factorisation runs its trailing updates in `prec`; refinement works on
- * the augmented system [I A; A^T 0][r; x] = [b; 0] with FP64 residuals.
+ * The factorisation runs its trailing updates in `prec` (panels, T factors
+ * and R stay FP64); refinement (corrected semi-normal equations with FP64
+ * residuals r = b - A x from the caller's A) converges to the FP64
+ * least-squares solution while the estimate keeps falling; on stagnation the
+ * plain FP64 solve runs instead when opts->fallback is set (info->fell_back).
  * Blocks until the solve is done (the stopping test is read on the host).
  * No reference counterpart. */
 int mqr_gels_mixed(mqr_handle_t handle, mqr_precision_t prec, int64_t m, int64_t n,

@@ -55,6 +55,8 @@ SIGNATURES = {
     "mqr_profile_read": (_i32, [_vp, _vp, _i32, _vp]),
     "mqr_dgemm": (_i32, [_vp, C.c_char, C.c_char, _i64, _i64, _i64, _dbl, _vp, _i64, _vp, _i64,
                          _dbl, _vp, _i64]),
+    "mqr_sgemm": (_i32, [_vp, C.c_char, C.c_char, _i64, _i64, _i64, C.c_float, _vp, _i64, _vp,
+                         _i64, C.c_float, _vp, _i64]),
     "mqr_dgeqrf": (_i32, [_vp, _i64, _i64, _vp, _i64, _vp, _vp, _i64]),
     "mqr_dorgqr": (_i32, [_vp, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64]),
     "mqr_dormqr": (_i32, [_vp, C.c_char, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64]),
@@ -223,9 +225,15 @@ def _ptr(t) -> C.c_void_p:
 # ---------------------------------------------------------------------------
 
 def gemm(h: Handle, transa: str, transb: str, alpha: float, A, B, beta: float, Cm):
-    """C = alpha op(A) op(B) + beta C on column-major float64 device tensors."""
+    """C = alpha op(A) op(B) + beta C on column-major device tensors (float64
+    through mqr_dgemm, float32 through mqr_sgemm)."""
+    import torch
     m, n = Cm.shape
     k = A.shape[0] if transa == "T" else A.shape[1]
+    if Cm.dtype == torch.float32:
+        check(lib().mqr_sgemm(h.h, transa.encode(), transb.encode(), m, n, k, alpha, _ptr(A),
+                              ld(A), _ptr(B), ld(B), beta, _ptr(Cm), ld(Cm)), "mqr_sgemm")
+        return Cm
     check(lib().mqr_dgemm(h.h, transa.encode(), transb.encode(), m, n, k, alpha, _ptr(A), ld(A),
                           _ptr(B), ld(B), beta, _ptr(Cm), ld(Cm)), "mqr_dgemm")
     return Cm

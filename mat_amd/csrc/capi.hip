@@ -244,6 +244,33 @@ int mqr_dgemm(mqr_handle_t h, char transa, char transb, int64_t m, int64_t n, in
     return gemm_f64(g, h->stream);
 }
 
+int mqr_sgemm(mqr_handle_t h, char transa, char transb, int64_t m, int64_t n, int64_t k,
+              float alpha, const float *A, int64_t lda, const float *B, int64_t ldb, float beta,
+              float *C, int64_t ldc) {
+    if (!h) return -1;
+    const bool ta = transa == 'T' || transa == 't', tb = transb == 'T' || transb == 't';
+    if (!ta && transa != 'N' && transa != 'n') return -2;
+    if (!tb && transb != 'N' && transb != 'n') return -3;
+    if (m < 0 || m > INT32_MAX) return -4;
+    if (n < 0 || n > INT32_MAX) return -5;
+    if (k < 0 || k > INT32_MAX) return -6;
+    if (m * k > 0 && !A) return -8;
+    if (lda < std::max<int64_t>(1, ta ? k : m)) return -9;
+    if (k * n > 0 && !B) return -10;
+    if (ldb < std::max<int64_t>(1, tb ? n : k)) return -11;
+    if (m * n > 0 && !C) return -13;
+    if (ldc < std::max<int64_t>(1, m)) return -14;
+    if (m == 0 || n == 0) return MQR_SUCCESS;
+    if (k == 0) return MQR_ERR_UNSUPPORTED;
+    MQR_GUARD(h);
+    GemmF32 g;
+    g.M = (int)m; g.N = (int)n; g.K = (int)k;
+    g.ta = ta; g.tb = tb;
+    g.A = A; g.lda = lda; g.B = B; g.ldb = ldb;
+    g.C = C; g.ldc = ldc; g.alpha = alpha; g.beta = beta;
+    return gemm_f32(g, h->stream);
+}
+
 int mqr_dgeqrf(mqr_handle_t h, int64_t m, int64_t n, double *A, int64_t lda, double *tau,
                double *T, int64_t ldt) {
     if (!h) return -1;

@@ -1,0 +1,167 @@
+// FP32 MFMA GEMM tiles for gfx950 (v_mfma_f32_16x16x4_f32, FP32
+// accumulation): the low-precision trailing update of the mixed-precision
+// factorisation and the FP32 counterpart of the reference's packed
+// micro-kernel behind operator* for float (Mul/__micro_kernel_packed.h:1-307).
+//
+// Same structure as the FP64 tiles: a TILE x TILE output tile per workgroup
+// of four waves, 16-deep K slices staged in LDS with register prefetch and
+// double buffering, branch-free clamped loads, split-K partial outputs.
+#include <climits>
+
+#include "mqr_internal.h"
+
+namespace mqr {
+namespace {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+constexpr int NT = 256, BK = 16;
+
+template <int TILE, bool KMAJOR>
+__device__ __forceinline__ void slot(int tid, int q, int &r, int &k) {
+    constexpr int NV = TILE * BK / NT;
+    if (!KMAJOR) {
+        constexpr int RT = TILE / 2;
+        r = (tid % RT) * 2 + (q & 1);
+        k = (tid / RT) + (NT / RT) * (q >> 1);
+    } else {
+        constexpr int KT = BK / NV;
+        r = tid / KT;
+        k = (tid % KT) * NV + q;
+    }
+}
+
+template <int TILE, bool KMAJOR>
+__device__ __forceinline__ void fetch(const float *__restrict__ X, int64_t ld, int r0, int k0,
+                                      int R, int kend, int tid, float *v) {
+    constexpr int NV = TILE * BK / NT;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        int r, k;
+        slot<TILE, KMAJOR>(tid, q, r, k);
+        r += r0;
+        k += k0;
+        const int rr = r < R ? r : R - 1, kc = k < kend ? k : kend - 1;
+        v[q] = KMAJOR ? X[(int64_t)kc + (int64_t)rr * ld] : X[(int64_t)rr + (int64_t)kc * ld];
+    }
+}
+
+template <int TILE, bool KMAJOR>
+__device__ __forceinline__ void stash(float (*S)[TILE + 4], int tid, const float *v, int r0, int k0,
+                                      int R, int kend) {
+    constexpr int NV = TILE * BK / NT;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        int r, k;
+        slot<TILE, KMAJOR>(tid, q, r, k);
+        const bool in = (r0 + r < R) && (k0 + k < kend);
+        S[k][r] = __int_as_float(__float_as_int(v[q]) & (in ? ~0 : 0));
+    }
+}
+
+template <int TILE, bool TA, bool TB>
+__global__ __launch_bounds__(NT) void k_gemm_f32(GemmF32 g, int kchunk) {
+    constexpr int NV = TILE * BK / NT, BW = TILE / 32, HALF = TILE / 2;
+    __shared__ float As[2][BK][TILE + 4];
+    __shared__ float Bs[2][BK][TILE + 4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int m0 = blockIdx.x * TILE, n0 = blockIdx.y * TILE;
+    const int kb = blockIdx.z * kchunk;
+    const int ke = min(g.K, kb + kchunk);
+
+    v4f acc[BW][BW];
+#pragma unroll
+    for (int i = 0; i < BW; ++i)
+#pragma unroll
+        for (int j = 0; j < BW; ++j) acc[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+
+    if (kb < ke) {
+        float va[NV], vb[NV];
+        fetch<TILE, TA>(g.A, g.lda, m0, kb, g.M, ke, tid, va);
+        fetch<TILE, !TB>(g.B, g.ldb, n0, kb, g.N, ke, tid, vb);
+        stash<TILE, TA>(As[0], tid, va, m0, kb, g.M, ke);
+        stash<TILE, !TB>(Bs[0], tid, vb, n0, kb, g.N, ke);
+        __syncthreads();
+        int buf = 0;
+        for (int k0 = kb; k0 < ke; k0 += BK) {
+            const bool more = k0 + BK < ke;
+            if (more) {
+                fetch<TILE, TA>(g.A, g.lda, m0, k0 + BK, g.M, ke, tid, va);
+                fetch<TILE, !TB>(g.B, g.ldb, n0, k0 + BK, g.N, ke, tid, vb);
+            }
+#pragma unroll
+            for (int kk = 0; kk < BK; kk += 4) {
+                float a[BW], b[BW];
+#pragma unroll
+                for (int i = 0; i < BW; ++i)
+                    a[i] = As[buf][kk + (lane >> 4)][wm * HALF + i * 16 + (lane & 15)];
+#pragma unroll
+                for (int j = 0; j < BW; ++j)
+                    b[j] = Bs[buf][kk + (lane >> 4)][wn * HALF + j * 16 + (lane & 15)];
+#pragma unroll
+                for (int i = 0; i < BW; ++i)
+#pragma unroll
+                    for (int j = 0; j < BW; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(b[j], a[i], acc[i][j], 0, 0, 0);
+            }
+            if (more) {
+                stash<TILE, TA>(As[buf ^ 1], tid, va, m0, k0 + BK, g.M, ke);
+                stash<TILE, !TB>(Bs[buf ^ 1], tid, vb, n0, k0 + BK, g.N, ke);
+            }
+            __syncthreads();
+            buf ^= 1;
+        }
+    }
+    // v_mfma_f32_16x16x4_f32 with swapped operands: lane l, register r holds
+    // output row (l >> 4) * 4 + r of the transposed product, column l & 15,
+    // i.e. C row = base + (l & 15), column = base + 4 (l >> 4) + r.
+    float *Cout = g.C + (g.splits > 1 ? (int64_t)blockIdx.z * g.split_stride : 0);
+    const bool direct = g.splits > 1 || g.beta == 0.f;
+    const bool interior = (m0 + TILE <= g.M) && (n0 + TILE <= g.N);
+#pragma unroll
+    for (int i = 0; i < BW; ++i) {
+        const int gm = m0 + wm * HALF + i * 16 + (lane & 15);
+#pragma unroll
+        for (int j = 0; j < BW; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gn = n0 + wn * HALF + j * 16 + (g.layout ? (lane >> 4) + 4 * r
+                                                                    : 4 * (lane >> 4) + r);
+                if (interior || (gm < g.M && gn < g.N)) {
+                    float v = g.alpha * acc[i][j][r];
+                    float *p = Cout + (int64_t)gm + (int64_t)gn * g.ldc;
+                    if (!direct) v = fmaf(g.beta, *p, v);
+                    *p = v;
+                }
+            }
+    }
+}
+
+template <bool TA, bool TB>
+void launch(const GemmF32 &g, hipStream_t s) {
+    const int tile = (g.M >= 128 && g.N >= 128) ? 128 : 64;
+    int kchunk = (g.K + g.splits - 1) / g.splits;
+    kchunk = ((kchunk + BK - 1) / BK) * BK;
+    dim3 grid((g.M + tile - 1) / tile, (g.N + tile - 1) / tile, g.splits);
+    if (tile == 128)
+        hipLaunchKernelGGL((k_gemm_f32<128, TA, TB>), grid, dim3(NT), 0, s, g, kchunk);
+    else
+        hipLaunchKernelGGL((k_gemm_f32<64, TA, TB>), grid, dim3(NT), 0, s, g, kchunk);
+}
+
+}  // namespace
+
+int gemm_f32(const GemmF32 &g, hipStream_t s) {
+    if (g.M <= 0 || g.N <= 0) return MQR_SUCCESS;
+    if (g.K <= 0) return MQR_ERR_INTERNAL;
+    const double M = g.M, N = g.N, K = g.K;
+    ProfScope ps(g.kind, 2.0 * M * N * K, 4.0 * (M * K + K * N + 2.0 * M * N * g.splits), s);
+    if (!g.ta && !g.tb) launch<false, false>(g, s);
+    else if (g.ta && !g.tb) launch<true, false>(g, s);
+    else if (!g.ta && g.tb) launch<false, true>(g, s);
+    else launch<true, true>(g, s);
+    MQR_LAUNCH_CHECK();
+    return MQR_SUCCESS;
+}
+
+}  // namespace mqr

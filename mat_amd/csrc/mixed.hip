@@ -1,15 +1,616 @@
-// Mixed-precision least squares with iterative refinement (placeholder until
-// the low-precision trailing update lands).
+// Mixed-precision least squares with iterative refinement (no reference
+// counterpart; built on the Householder path of
+// QR/__Householder_Blocked.h:214-245).
+//
+// Factorisation: A is copied to an FP32 working matrix; each panel is cast
+// back to FP64 and factored in FP64 (reflectors, tau, T and the diagonal
+// block of R stay FP64); the trailing update C -= V (T^T (V^T C)) runs as two
+// FP32 MFMA GEMMs with FP32 accumulation (the T product is formed in FP64).
+// The computed R is the exact R factor of a nearby matrix A + dA with
+// ||dA|| = O(u_32 ||A||).
+//
+// Refinement (corrected semi-normal equations): with X = R^{-1} formed once,
+//   r = b - A x,  z = A^T r  (FP64, from A itself),  y = X^T z,  x += X y.
+// The fixed point satisfies A^T (b - A x) = 0 to FP64 rounding, i.e. it is the
+// FP64 least-squares solution.  R is the exact factor of A + dA with
+// ||dA|| = O(u_32 ||A||); the iteration matrix R^{-1} [Q^T dA R^{-1} + (.)^T] R
+// is similar to one of norm O(u_32 kappa), which is the contraction per step.
+// The stopping test uses omega = ||y|| / (||A||_F ||x||), an upper estimate of
+// the normwise backward error.  Stagnation, divergence or a non-finite value
+// falls back to the plain FP64 solve.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+
 #include "mqr_internal.h"
+#include "small_dense.h"
 
 namespace mqr {
+
+int geqrf_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, double *tau,
+              double *T, int64_t ldt);
+int ormqr_f64(mqr_handle_t hd, bool trans, int64_t m, int64_t nrhs, int64_t k, const double *VR,
+              int64_t ldvr, const double *T, int64_t ldt, double *C, int64_t ldc);
+int trsm_upper_f64(mqr_handle_t hd, bool trans, int64_t n, int64_t nrhs, const double *R,
+                   int64_t ldr, double *B, int64_t ldb);
+int panel_factor(mqr_handle_t hd, int64_t h, int b, double *V, int64_t lda, double *tau, void *pws);
+
+namespace {
+
+__device__ __forceinline__ int64_t gidx() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+
+// B (f32, ldb) = A (f64, lda), m x n
+__global__ void k_cast_d2s(int64_t m, int64_t n, const double *__restrict__ A, int64_t lda,
+                           float *__restrict__ B, int64_t ldb) {
+    for (int64_t e = gidx(); e < m * n; e += (int64_t)gridDim.x * blockDim.x)
+        B[e % m + (e / m) * ldb] = (float)A[e % m + (e / m) * lda];
+}
+
+// B (f64, ldb) = A (f32, lda), m x n
+__global__ void k_cast_s2d(int64_t m, int64_t n, const float *__restrict__ A, int64_t lda,
+                           double *__restrict__ B, int64_t ldb) {
+    for (int64_t e = gidx(); e < m * n; e += (int64_t)gridDim.x * blockDim.x)
+        B[e % m + (e / m) * ldb] = (double)A[e % m + (e / m) * lda];
+}
+
+// V32 (h x b, ld h) = fp32 of the unit lower trapezoid stored in V (ld)
+__global__ void k_cast_v(int64_t h, int b, const double *__restrict__ V, int64_t ld,
+                         float *__restrict__ V32) {
+    for (int64_t e = gidx(); e < h * b; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = e % h, c = e / h;
+        V32[e] = r > c ? (float)V[r + c * ld] : (r == c ? 1.f : 0.f);
+    }
+}
+
+// P[0:count] += P[s * stride + 0:count] (FP32 partials, FP64 sum)
+__global__ void k_reduce_f32(int64_t count, float *__restrict__ P, int splits, int64_t stride) {
+    const int64_t e = gidx();
+    if (e >= count) return;
+    double acc = 0.0;
+    for (int s = 0; s < splits; ++s) acc += (double)P[(int64_t)s * stride + e];
+    P[e] = (float)acc;
+}
+
+// W2 (f32, b x nc, ld b) = T^T W with W (f32, ld b), products in FP64
+__global__ __launch_bounds__(256) void k_w2_f32(int b, int64_t nc, const double *__restrict__ T,
+                                                int64_t ldt, const float *__restrict__ W,
+                                                float *__restrict__ W2) {
+    __shared__ double Ts[64 * 64];
+    __shared__ double Ws[64 * 32];
+    const int tid = threadIdx.x;
+    const int64_t n0 = (int64_t)blockIdx.x * 32;
+    for (int e = tid; e < b * b; e += 256) {
+        const int r = e % b, c = e / b;
+        Ts[e] = r <= c ? T[r + (int64_t)c * ldt] : 0.0;
+    }
+    for (int e = tid; e < b * 32; e += 256) {
+        const int64_t n = n0 + e / b;
+        Ws[e] = n < nc ? (double)W[e % b + n * b] : 0.0;
+    }
+    __syncthreads();
+    for (int e = tid; e < b * 32; e += 256) {
+        const int r = e % b, col = e / b;
+        const int64_t n = n0 + col;
+        if (n >= nc) continue;
+        double acc = 0.0;
+        for (int l = 0; l <= r; ++l) acc = fma(Ts[l + r * b], Ws[l + col * b], acc);
+        W2[r + n * b] = (float)acc;
+    }
+}
+
+// Sum of squares of an m x n FP64 matrix into out[blockIdx.x] (partials).
+__global__ __launch_bounds__(256) void k_sumsq(int64_t m, int64_t n, const double *__restrict__ A,
+                                               int64_t lda, double *__restrict__ out) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (int64_t e = gidx(); e < m * n; e += (int64_t)gridDim.x * blockDim.x) {
+        const double v = A[e % m + (e / m) * lda];
+        s = fma(v, v, s);
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// out[slot] = sum of in[0:cnt] (one workgroup)
+__global__ __launch_bounds__(256) void k_sum(int cnt, const double *__restrict__ in,
+                                             double *__restrict__ out, int slot) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < cnt; i += 256) s += in[i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[slot] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+constexpr int GV_ROWS = 256, GV_COLS = 256;
+
+// part[chunk][row] = sum over the chunk's columns of A[row, c] x[c]
+__global__ __launch_bounds__(256) void k_gemv_n(int64_t m, int64_t n, const double *__restrict__ A,
+                                                int64_t lda, const double *__restrict__ x,
+                                                double *__restrict__ part) {
+    const int64_t row = (int64_t)blockIdx.x * GV_ROWS + threadIdx.x;
+    const int64_t c0 = (int64_t)blockIdx.y * GV_COLS;
+    const int64_t c1 = std::min<int64_t>(n, c0 + GV_COLS);
+    if (row >= m) return;
+    double s0 = 0.0, s1 = 0.0;
+    int64_t c = c0;
+    for (; c + 1 < c1; c += 2) {
+        s0 = fma(A[row + c * lda], x[c], s0);
+        s1 = fma(A[row + (c + 1) * lda], x[c + 1], s1);
+    }
+    if (c < c1) s0 = fma(A[row + c * lda], x[c], s0);
+    part[(int64_t)blockIdx.y * m + row] = s0 + s1;
+}
+
+// r = b - sum of parts; also per-block partial of ||r||^2
+__global__ __launch_bounds__(256) void k_resid(int64_t m, int nparts, const double *__restrict__ b,
+                                               const double *__restrict__ part,
+                                               double *__restrict__ r) {
+    const int64_t i = gidx();
+    if (i >= m) return;
+    double s = 0.0;
+    for (int p = 0; p < nparts; ++p) s += part[(int64_t)p * m + i];
+    r[i] = b[i] - s;
+}
+
+// z[c] = sum_{r < rows(c)} A[r, c] v[r], one wave per column; rows(c) = m, or
+// c + 1 when upper is set (triangular A).  Also writes z[c]^2 to zz[c].
+__global__ __launch_bounds__(256) void k_gemv_t(int64_t m, int64_t n, const double *__restrict__ A,
+                                                int64_t lda, const double *__restrict__ v,
+                                                double *__restrict__ z, double *__restrict__ zz,
+                                                int upper) {
+    const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (c >= n) return;
+    const int64_t rows = upper ? std::min<int64_t>(m, c + 1) : m;
+    const double *col = A + c * lda;
+    double s0 = 0.0, s1 = 0.0;
+    int64_t r = lane;
+    for (; r + 64 < rows; r += 128) {
+        s0 = fma(col[r], v[r], s0);
+        s1 = fma(col[r + 64], v[r + 64], s1);
+    }
+    if (r < rows) s0 = fma(col[r], v[r], s0);
+    double s = s0 + s1;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) {
+        z[c] = s;
+        if (zz) zz[c] = s * s;
+    }
+}
+
+// x[i] += sum_p part[p * n + i]; xx[i] = x[i]^2
+__global__ void k_update_x(int64_t n, int nparts, const double *__restrict__ part,
+                           double *__restrict__ x, double *__restrict__ xx) {
+    const int64_t i = gidx();
+    if (i >= n) return;
+    double s = 0.0;
+    for (int p = 0; p < nparts; ++p) s += part[(int64_t)p * n + i];
+    const double v = x[i] + s;
+    x[i] = v;
+    xx[i] = v * v;
+}
+
+__global__ void k_sq(int64_t n, const double *__restrict__ x, double *__restrict__ xx) {
+    const int64_t i = gidx();
+    if (i < n) xx[i] = x[i] * x[i];
+}
+
+__global__ void k_fill(int64_t n, double *__restrict__ x, double v) {
+    const int64_t i = gidx();
+    if (i < n) x[i] = v;
+}
+
+// X (ld ldx) = inverse of each 64 x 64 diagonal block of upper R (ld ldr)
+__global__ __launch_bounds__(64) void k_inv_diag_blocks(int64_t n, const double *__restrict__ R,
+                                                        int64_t ldr, double *__restrict__ X,
+                                                        int64_t ldx) {
+    __shared__ double S[small::N][small::N + 1];
+    __shared__ double rdiag[small::N];
+    const int64_t i0 = (int64_t)blockIdx.x * small::N;
+    const int b = (int)std::min<int64_t>(small::N, n - i0);
+    const int c = small::lane_id();
+    double a[small::N], x[small::N];
+#pragma unroll
+    for (int r = 0; r < small::N; ++r)
+        a[r] = (r < b && c < b) ? (r <= c ? R[i0 + r + (i0 + c) * ldr] : 0.0) : (r == c ? 1.0 : 0.0);
+    small::col_to_lds(a, S);
+    __syncthreads();
+    rdiag[c] = 1.0 / S[c][c];
+    __syncthreads();
+    small::inv_upper_lds(S, rdiag, x);
+#pragma unroll
+    for (int r = 0; r < small::N; ++r)
+        if (r < b && c < b) X[i0 + r + (i0 + c) * ldx] = x[r];
+}
+
+// R (ldr) rows [0, b), columns [0, nt) = Aw rows [0, b) (FP32 -> FP64)
+__global__ void k_copy_rrows(int b, int64_t nt, const float *__restrict__ Aw, int64_t ldw,
+                             double *__restrict__ R, int64_t ldr) {
+    for (int64_t e = gidx(); e < (int64_t)b * nt; e += (int64_t)gridDim.x * blockDim.x)
+        R[e % b + (e / b) * ldr] = (double)Aw[e % b + (e / b) * ldw];
+}
+
+// R (ldr) b x b = upper triangle of the factored panel P (ld), zero below
+__global__ void k_copy_rdiag(int b, const double *__restrict__ P, int64_t ld,
+                             double *__restrict__ R, int64_t ldr) {
+    const int e = threadIdx.x + blockIdx.x * blockDim.x;
+    if (e >= b * b) return;
+    const int r = e % b, c = e / b;
+    R[r + c * ldr] = r <= c ? P[r + c * ld] : 0.0;
+}
+
+unsigned grid_for(int64_t count, int per = 256, int64_t cap = 8192) {
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cap, (count + per - 1) / per));
+}
+
+// Device workspace of one mixed solve (carved from the handle's IR scratch).
+struct MixedWs {
+    float *Aw = nullptr;   // m x n FP32 working copy (FP32 mode)
+    double *F = nullptr;   // m x n FP64 factorisation (FP64 mode)
+    double *Pb = nullptr;  // m x nb FP64 panel (FP32 mode)
+    double *R = nullptr;   // R factor, ld ldr
+    int64_t ldr = 0;
+    float *V32 = nullptr, *P32 = nullptr, *W2 = nullptr;
+    double *X = nullptr, *T = nullptr, *tau = nullptr, *Gp = nullptr;
+    double *r = nullptr, *z = nullptr, *y = nullptr, *part = nullptr, *sc = nullptr;
+    double *xx = nullptr, *zz = nullptr, *tmp = nullptr;
+    void *pws = nullptr;
+    size_t bytes = 0;
+};
+
+constexpr int kMaxSplits = 64;
+
+MixedWs carve(char *base, bool fp64, int64_t m, int64_t n, int nb) {
+    MixedWs w;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char *q = base + off;
+        off += (bytes + 255) & ~(size_t)255;
+        return (void *)q;
+    };
+    const int64_t nchunks = (n + GV_COLS - 1) / GV_COLS;
+    if (fp64) {
+        w.F = (double *)take(sizeof(double) * m * n);
+        w.R = w.F;
+        w.ldr = m;
+    } else {
+        w.Aw = (float *)take(sizeof(float) * m * n);
+        w.Pb = (double *)take(sizeof(double) * m * nb);
+        w.R = (double *)take(sizeof(double) * n * n);
+        w.ldr = n;
+        w.V32 = (float *)take(sizeof(float) * m * nb);
+        w.P32 = (float *)take(sizeof(float) * kMaxSplits * nb * n);
+        w.W2 = (float *)take(sizeof(float) * nb * n);
+    }
+    w.X = (double *)take(sizeof(double) * n * n);
+    w.tmp = (double *)take(sizeof(double) * ((n + 1) / 2) * n);
+    w.T = (double *)take(sizeof(double) * nb * n);
+    w.tau = (double *)take(sizeof(double) * n);
+    w.Gp = (double *)take(sizeof(double) * kMaxSplits * nb * nb);
+    w.r = (double *)take(sizeof(double) * m);
+    w.z = (double *)take(sizeof(double) * n);
+    w.y = (double *)take(sizeof(double) * n);
+    w.part = (double *)take(sizeof(double) * std::max<int64_t>(nchunks * std::max(m, n), 2048));
+    w.sc = (double *)take(sizeof(double) * 64);
+    w.xx = (double *)take(sizeof(double) * n);
+    w.zz = (double *)take(sizeof(double) * n);
+    if (!fp64) w.pws = take(panel_cholqr_ws_bytes(m));
+    w.bytes = off;
+    return w;
+}
+
+// Factorisation with FP32 trailing updates: R (upper, ld ldr) is written.
+int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t lda,
+                MixedWs &w) {
+    hipStream_t s = hd->stream;
+    const int nb = hd->nb;
+    const int64_t k = std::min(m, n);
+    {
+        ProfScope ps(K_CAST, 0.0, 12.0 * m * n, s);
+        hipLaunchKernelGGL(k_cast_d2s, dim3(grid_for(m * n)), dim3(256), 0, s, m, n, A, lda, w.Aw,
+                           m);
+        MQR_LAUNCH_CHECK();
+    }
+    for (int64_t j = 0; j < k; j += nb) {
+        const int b = (int)std::min<int64_t>(nb, k - j);
+        const int64_t h = m - j;
+        double *V = w.Pb;  // ld m, h rows
+        {
+            ProfScope ps(K_CAST, 0.0, 12.0 * h * b, s);
+            hipLaunchKernelGGL(k_cast_s2d, dim3(grid_for(h * b)), dim3(256), 0, s, h, (int64_t)b,
+                               (const float *)(w.Aw + j + j * m), m, V, m);
+            MQR_LAUNCH_CHECK();
+        }
+        MQR_CHECK(panel_factor(hd, h, b, V, m, w.tau + j, w.pws));
+        {
+            ProfScope ps(K_CAST, 0.0, 16.0 * b * b, s);
+            hipLaunchKernelGGL(k_copy_rdiag, dim3((unsigned)((b * b + 255) / 256)), dim3(256), 0, s,
+                               b, (const double *)V, m, w.R + j + j * w.ldr, w.ldr);
+            MQR_LAUNCH_CHECK();
+        }
+        // T from the FP64 Gram matrix of V
+        int splits = std::min(kMaxSplits, gemm_pick_splits(b, b, (int)h, 4 * hd->cu_count));
+        GemmF64 g;
+        g.M = b; g.N = b; g.K = (int)h;
+        g.ta = true; g.amask = MASK_VLOWER; g.bmask = MASK_VLOWER; g.bmask_cols = b;
+        g.A = V; g.lda = m; g.B = V; g.ldb = m;
+        g.C = w.Gp; g.ldc = b; g.splits = splits; g.split_stride = (int64_t)b * b;
+        g.kind = K_VTA;
+        MQR_CHECK(gemm_f64(g, s));
+        MQR_CHECK(reduce_partials((int64_t)b * b, w.Gp, splits, (int64_t)b * b, s));
+        double *Tj = w.T + j * nb;
+        MQR_CHECK(larft_from_gram(b, w.Gp, 1, (int64_t)b * b, w.tau + j, Tj, nb, s));
+        const int64_t nt = n - j - b;
+        if (nt <= 0) continue;
+        {
+            ProfScope ps(K_CAST, 0.0, 12.0 * h * b, s);
+            hipLaunchKernelGGL(k_cast_v, dim3(grid_for(h * b)), dim3(256), 0, s, h, b,
+                               (const double *)V, m, w.V32);
+            MQR_LAUNCH_CHECK();
+        }
+        // C <- (I - V T V^T)^T C in FP32: W = V^T C, W2 = T^T W, C -= V W2
+        float *C = w.Aw + j + (j + b) * m;
+        const int sp = std::min(kMaxSplits, gemm_pick_splits(b, (int)nt, (int)h, 4 * hd->cu_count));
+        GemmF32 f;
+        f.M = b; f.N = (int)nt; f.K = (int)h; f.ta = true;
+        f.A = w.V32; f.lda = h; f.B = C; f.ldb = m;
+        f.C = w.P32; f.ldc = b; f.splits = sp; f.split_stride = (int64_t)b * nt;
+        f.kind = K_VTA;
+        MQR_CHECK(gemm_f32(f, s));
+        if (sp > 1) {
+            ProfScope ps(K_OTHER, 0.0, 4.0 * b * nt * (sp + 1), s);
+            hipLaunchKernelGGL(k_reduce_f32, dim3((unsigned)(((int64_t)b * nt + 255) / 256)),
+                               dim3(256), 0, s, (int64_t)b * nt, w.P32, sp, (int64_t)b * nt);
+            MQR_LAUNCH_CHECK();
+        }
+        {
+            ProfScope ps(K_W2, (double)b * b * nt, 8.0 * b * nt, s);
+            hipLaunchKernelGGL(k_w2_f32, dim3((unsigned)((nt + 31) / 32)), dim3(256), 0, s, b, nt,
+                               (const double *)Tj, (int64_t)nb, (const float *)w.P32, w.W2);
+            MQR_LAUNCH_CHECK();
+        }
+        GemmF32 u;
+        u.M = (int)h; u.N = (int)nt; u.K = b;
+        u.A = w.V32; u.lda = h; u.B = w.W2; u.ldb = b;
+        u.C = C; u.ldc = m; u.alpha = -1.f; u.beta = 1.f; u.kind = K_UPDATE;
+        MQR_CHECK(gemm_f32(u, s));
+        {
+            ProfScope ps(K_CAST, 0.0, 12.0 * b * nt, s);
+            hipLaunchKernelGGL(k_copy_rrows, dim3(grid_for((int64_t)b * nt)), dim3(256), 0, s, b,
+                               nt, (const float *)C, m, w.R + j + (j + b) * w.ldr, w.ldr);
+            MQR_LAUNCH_CHECK();
+        }
+    }
+    return MQR_SUCCESS;
+}
+
+// X (ld n) = R^{-1} for the upper n x n R (ld ldr): the diagonal 64-blocks are
+// inverted directly, then the off-diagonal blocks by recursive doubling,
+// inv([A B; 0 C]) = [A^{-1}, -A^{-1} B C^{-1}; 0, C^{-1}].  tmp: n/2 x n.
+int tri_inverse(mqr_handle_t hd, int64_t n, const double *R, int64_t ldr, double *X,
+                double *tmp) {
+    hipStream_t s = hd->stream;
+    {
+        ProfScope ps(K_TRSM, (double)n * 64 * 64 / 3.0, 16.0 * n * n, s);
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(n * n, 256, 1 << 22)), dim3(256), 0, s, n * n, X,
+                           0.0);
+        hipLaunchKernelGGL(k_inv_diag_blocks, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, n, R,
+                           ldr, X, n);
+        MQR_LAUNCH_CHECK();
+    }
+    for (int64_t sz = 64; sz < n; sz *= 2) {
+        for (int64_t i = 0; i + sz < n; i += 2 * sz) {
+            const int64_t sc = std::min<int64_t>(sz, n - i - sz);
+            // tmp (sz x sc) = R[i:i+sz, i+sz:i+sz+sc] * X_C
+            GemmF64 g;
+            g.M = (int)sz; g.N = (int)sc; g.K = (int)sc;
+            g.A = R + i + (i + sz) * ldr; g.lda = ldr;
+            g.B = X + (i + sz) + (i + sz) * n; g.ldb = n;
+            g.C = tmp; g.ldc = sz; g.kind = K_TRSM;
+            MQR_CHECK(gemm_f64(g, s));
+            // X[i:i+sz, i+sz:i+sz+sc] = -X_A * tmp
+            GemmF64 h2;
+            h2.M = (int)sz; h2.N = (int)sc; h2.K = (int)sz;
+            h2.A = X + i + i * n; h2.lda = n; h2.B = tmp; h2.ldb = sz;
+            h2.C = X + i + (i + sz) * n; h2.ldc = n; h2.alpha = -1.0; h2.kind = K_TRSM;
+            MQR_CHECK(gemm_f64(h2, s));
+        }
+    }
+    return MQR_SUCCESS;
+}
+
+int read_scalars(mqr_handle_t hd, const double *dev, int cnt, double *host) {
+    MQR_HIP(hipMemcpyAsync(hd->host_pinned, dev, sizeof(double) * cnt, hipMemcpyDeviceToHost,
+                           hd->stream));
+    MQR_HIP(hipStreamSynchronize(hd->stream));
+    for (int i = 0; i < cnt; ++i) host[i] = hd->host_pinned[i];
+    return MQR_SUCCESS;
+}
+
+// sc[slot] = sum of squares of v[0:len]
+int sumsq_vec(int64_t len, const double *v, double *partial, double *sc, int slot,
+              hipStream_t s) {
+    hipLaunchKernelGGL(k_sumsq, dim3(64), dim3(256), 0, s, len, (int64_t)1, v, len, partial);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, s, 64, (const double *)partial, sc, slot);
+    MQR_LAUNCH_CHECK();
+    return MQR_SUCCESS;
+}
+
+// r = b - A x; z = A^T r; sc[3] = ||r||^2, sc[4] = ||x||^2 (from xx).  Two
+// FP64 passes over A.
+int residual_pass(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t lda,
+                  const double *b, const double *x, MixedWs &w) {
+    hipStream_t s = hd->stream;
+    const int64_t nchunks = (n + GV_COLS - 1) / GV_COLS;
+    ProfScope ps(K_RESID, 4.0 * m * n, 16.0 * m * n, s);
+    hipLaunchKernelGGL(k_gemv_n, dim3((unsigned)((m + GV_ROWS - 1) / GV_ROWS), (unsigned)nchunks),
+                       dim3(256), 0, s, m, n, A, lda, x, w.part);
+    hipLaunchKernelGGL(k_resid, dim3(grid_for(m, 256, 1 << 22)), dim3(256), 0, s, m, (int)nchunks,
+                       b, (const double *)w.part, w.r);
+    hipLaunchKernelGGL(k_gemv_t, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, m, n, A, lda,
+                       (const double *)w.r, w.z, (double *)nullptr, 0);
+    MQR_LAUNCH_CHECK();
+    MQR_CHECK(sumsq_vec(m, w.r, w.part, w.sc, 3, s));
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, s, (int)n, (const double *)w.xx, w.sc, 4);
+    MQR_LAUNCH_CHECK();
+    return MQR_SUCCESS;
+}
+
+// y = X^T z = R^{-T} A^T r; sc[5] = ||y||^2
+int precondition(mqr_handle_t hd, int64_t n, MixedWs &w) {
+    hipStream_t s = hd->stream;
+    ProfScope ps(K_TRSM, (double)n * n, 4.0 * n * n, s);
+    hipLaunchKernelGGL(k_gemv_t, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, n,
+                       (const double *)w.X, n, (const double *)w.z, w.y, w.zz, 1);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, s, (int)n, (const double *)w.zz, w.sc, 5);
+    MQR_LAUNCH_CHECK();
+    return MQR_SUCCESS;
+}
+
+// x += X y, xx = x^2
+int correction(mqr_handle_t hd, int64_t n, double *x, MixedWs &w) {
+    hipStream_t s = hd->stream;
+    const int64_t nchunks = (n + GV_COLS - 1) / GV_COLS;
+    ProfScope ps(K_TRSM, (double)n * n, 8.0 * n * n, s);
+    hipLaunchKernelGGL(k_gemv_n, dim3((unsigned)((n + GV_ROWS - 1) / GV_ROWS), (unsigned)nchunks),
+                       dim3(256), 0, s, n, n, (const double *)w.X, n, (const double *)w.y, w.part);
+    hipLaunchKernelGGL(k_update_x, dim3(grid_for(n, 256, 1 << 22)), dim3(256), 0, s, n,
+                       (int)nchunks, (const double *)w.part, x, w.xx);
+    MQR_LAUNCH_CHECK();
+    return MQR_SUCCESS;
+}
+
+// Backward-error estimate of the current x: residual, preconditioned normal
+// residual, omega = ||R^{-T} A^T r|| / (||A||_F ||x||).  Since
+// (A^T A + phi^2 I)^{-1/2} <= (A^T A)^{-1/2} = R^{-1} R^{-T} in the Loewner
+// order, omega bounds the Karlson-Walden estimate of the normwise least-squares
+// backward error from above (up to the accuracy of R).
+int estimate(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t lda, const double *b,
+             const double *x, double nA, MixedWs &w, double *omega, double *rnorm,
+             double *ynorm) {
+    MQR_CHECK(residual_pass(hd, m, n, A, lda, b, x, w));
+    MQR_CHECK(precondition(hd, n, w));
+    double hs[3];
+    MQR_CHECK(read_scalars(hd, w.sc + 3, 3, hs));
+    *rnorm = std::sqrt(hs[0]);
+    const double nx = std::sqrt(hs[1]), ny = std::sqrt(hs[2]);
+    *ynorm = ny;
+    if (ny == 0.0)
+        *omega = 0.0;
+    else
+        *omega = (nx > 0.0 && nA > 0.0) ? ny / (nA * nx) : INFINITY;
+    return MQR_SUCCESS;
+}
+
+}  // namespace
 
 int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, const double *A,
                int64_t lda, const double *b, double *x, const mqr_ir_opts_t *opts,
                mqr_ir_info_t *info) {
-    (void)hd; (void)prec; (void)m; (void)n; (void)A; (void)lda; (void)b; (void)x; (void)opts;
-    (void)info;
-    return MQR_ERR_UNSUPPORTED;
+    if (prec != MQR_PREC_FP64 && prec != MQR_PREC_FP32) return MQR_ERR_UNSUPPORTED;
+    if (m > INT32_MAX / 2 || n > INT32_MAX / 2) return MQR_ERR_UNSUPPORTED;
+    const int max_iter = (opts && opts->max_iter > 0) ? opts->max_iter : 30;
+    const double tol = (opts && opts->tol > 0.0) ? opts->tol : 1e-14;
+    const bool fallback = opts ? opts->fallback != 0 : true;
+    hipStream_t s = hd->stream;
+    const int nb = hd->nb;
+    const bool fp64 = prec == MQR_PREC_FP64;
+    MixedWs w = carve(nullptr, fp64, m, n, nb);
+    MQR_CHECK(ws_reserve(hd, hd->ws_ir, w.bytes + 4096));
+    w = carve((char *)hd->ws_ir.ptr, fp64, m, n, nb);
+
+    // ||A||_F^2 -> sc[0]
+    {
+        ProfScope ps(K_RESID, 2.0 * m * n, 8.0 * m * n, s);
+        hipLaunchKernelGGL(k_sumsq, dim3(1024), dim3(256), 0, s, m, n, A, lda, w.part);
+        hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, s, 1024, (const double *)w.part, w.sc, 0);
+        MQR_LAUNCH_CHECK();
+    }
+    // factorisation, then X = R^{-1}
+    if (fp64) {
+        ProfScope ps(K_OTHER, 0.0, 16.0 * m * n, s);
+        MQR_HIP(hipMemcpy2DAsync(w.F, sizeof(double) * m, A, sizeof(double) * lda,
+                                 sizeof(double) * m, n, hipMemcpyDeviceToDevice, s));
+    }
+    if (fp64)
+        MQR_CHECK(geqrf_f64(hd, m, n, w.F, m, w.tau, w.T, nb));
+    else
+        MQR_CHECK(factor_fp32(hd, m, n, A, lda, w));
+    MQR_CHECK(tri_inverse(hd, n, w.R, w.ldr, w.X, w.tmp));
+
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, s, n, x, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, s, n, w.xx, 0.0);
+    MQR_LAUNCH_CHECK();
+    double hA;
+    MQR_CHECK(read_scalars(hd, w.sc, 1, &hA));
+    const double nA = std::sqrt(hA);
+
+    // MQR_IR_TRACE=1 prints the stopping-test quantities of every step
+    static const bool trace = [] {
+        const char *e = std::getenv("MQR_IR_TRACE");
+        return e && e[0] == '1';
+    }();
+    // Progress is judged on ||y|| = ||R^{-T} A^T r||, which falls by the
+    // contraction factor every step; omega also depends on ||x||, which is
+    // far off in the first steps when kappa is large.
+    double omega = INFINITY, best = INFINITY, rnorm = 0.0, ynorm = 0.0;
+    int it = 0, stall = 0;
+    bool converged = false;
+    for (;;) {
+        MQR_CHECK(estimate(hd, m, n, A, lda, b, x, nA, w, &omega, &rnorm, &ynorm));
+        if (trace)
+            std::fprintf(stderr, "[mqr ir] step %d omega %.3e |y| %.3e |r| %.9e\n", it, omega,
+                         ynorm, rnorm);
+        if (omega <= tol) {
+            converged = true;
+            break;
+        }
+        // x = 0 at the first test: omega is infinite unless A^T b = 0
+        if (!std::isfinite(ynorm) || (it > 0 && !std::isfinite(omega))) break;
+        if (ynorm < 0.5 * best) {
+            best = ynorm;
+            stall = 0;
+        } else if (++stall >= 3 || ynorm > 1e3 * best) {
+            break;
+        }
+        if (it >= max_iter) break;
+        MQR_CHECK(correction(hd, n, x, w));
+        ++it;
+    }
+    bool fell_back = false;
+    if (!converged && fallback) {
+        // plain FP64 solve: factor a copy of A, apply Q^T to b, back substitute
+        fell_back = true;
+        MixedWs f = carve(nullptr, true, m, n, nb);
+        MQR_CHECK(ws_reserve(hd, hd->ws_ir, f.bytes + 4096));
+        f = carve((char *)hd->ws_ir.ptr, true, m, n, nb);
+        MQR_HIP(hipMemcpy2DAsync(f.F, sizeof(double) * m, A, sizeof(double) * lda,
+                                 sizeof(double) * m, n, hipMemcpyDeviceToDevice, s));
+        MQR_HIP(hipMemcpyAsync(f.r, b, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
+        MQR_CHECK(geqrf_f64(hd, m, n, f.F, m, f.tau, f.T, nb));
+        MQR_CHECK(ormqr_f64(hd, true, m, 1, n, f.F, m, f.T, nb, f.r, m));
+        MQR_CHECK(trsm_upper_f64(hd, false, n, 1, f.F, m, f.r, m));
+        MQR_HIP(hipMemcpyAsync(x, f.r, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_sq, dim3(grid_for(n)), dim3(256), 0, s, n, (const double *)x, f.xx);
+        MQR_LAUNCH_CHECK();
+        MQR_CHECK(tri_inverse(hd, n, f.F, m, f.X, f.tmp));
+        MQR_CHECK(estimate(hd, m, n, A, lda, b, x, nA, f, &omega, &rnorm, &ynorm));
+        if (trace)
+            std::fprintf(stderr, "[mqr ir] fp64 fallback omega %.3e |r| %.9e\n", omega, rnorm);
+    }
+    if (info) {
+        info->iterations = it;
+        info->converged = converged ? 1 : 0;
+        info->fell_back = fell_back ? 1 : 0;
+        info->backward_error = omega;
+        info->residual_norm = rnorm;
+    }
+    return (converged || fell_back) ? MQR_SUCCESS : MQR_ERR_NOT_CONVERGED;
 }
 
 }  // namespace mqr

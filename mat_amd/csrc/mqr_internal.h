@@ -125,6 +125,24 @@ struct GemmF64 {
     const int *skip_if = nullptr;
 };
 int gemm_f64(const GemmF64 &g, hipStream_t s);
+
+// FP32 MFMA GEMM (no masks): same conventions as GemmF64.
+struct GemmF32 {
+    int M = 0, N = 0, K = 0;
+    bool ta = false, tb = false;
+    const float *A = nullptr;
+    int64_t lda = 0;
+    const float *B = nullptr;
+    int64_t ldb = 0;
+    float *C = nullptr;
+    int64_t ldc = 0;
+    float alpha = 1.f, beta = 0.f;
+    int splits = 1;
+    int64_t split_stride = 0;
+    int kind = K_GEMM;
+    int layout = 0;  // accumulator layout selector (0 is the gfx950 one)
+};
+int gemm_f32(const GemmF32 &g, hipStream_t s);
 // C = beta * C (beta == 0 writes zeros).
 int scale_f64(int64_t m, int64_t n, double beta, double *C, int64_t ldc, hipStream_t s);
 // Number of K splits giving roughly `target_blocks` workgroups.

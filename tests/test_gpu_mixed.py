@@ -1,0 +1,195 @@
+"""GPU parity of the FP32 GEMM tile and of the mixed-precision least-squares
+solve (FP32 trailing updates, FP64 panels, FP64 refinement), through the C ABI.
+
+The solve has no counterpart in the reference; it is checked against the
+oracle's FP64 least-squares solution (a restatement of the reference's blocked
+Householder QR followed by Q^T b and back substitution) and against the exact
+normwise backward error computed from the SVD (Karlson-Walden formula,
+oracle.ls_backward_error): the target is the FP64 level, backward error
+<= 1e-14 after refinement."""
+import numpy as np
+import pytest
+
+import mat_amd as M
+from oracle.oracle import Oracle, ls_backward_error
+
+pytestmark = pytest.mark.gpu
+EPS = np.finfo(np.float64).eps
+
+
+def note(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+@pytest.fixture(scope="module")
+def h(torch_mod):
+    hd = M.Handle(0)
+    yield hd
+    hd.close()
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle("avx2")
+
+
+def host(t):
+    return np.asfortranarray(t.detach().cpu().numpy())
+
+
+def ls_problem(m, n, cond, seed, resid=1e-10):
+    """A = U diag(logspace(0, -log10 cond)) V^T with random orthogonal U, V;
+    b = A x0 + resid * (unit-norm noise) * ||A x0||."""
+    rng = np.random.default_rng(seed)
+    U, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s = np.logspace(0, -np.log10(cond), n)
+    A = np.asfortranarray((U * s) @ V.T)
+    x0 = rng.standard_normal(n)
+    b0 = A @ x0
+    e = rng.standard_normal(m)
+    b = b0 + resid * np.linalg.norm(b0) * e / np.linalg.norm(e)
+    return A, b
+
+
+# ---------------------------------------------------------------------------
+# FP32 MFMA GEMM tile
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+def test_sgemm_exact_on_integers(h, torch_mod, ta, tb):
+    """Integer inputs with |sums| < 2^24: every FP32 product and sum is exact,
+    so the result must equal the integer product exactly (this pins the MFMA
+    accumulator layout and every edge mask)."""
+    rng = np.random.default_rng(7)
+    for (m, n, k) in [(1, 1, 1), (16, 16, 4), (64, 64, 16), (37, 29, 41), (200, 150, 90),
+                      (1000, 70, 300), (513, 384, 200), (128, 128, 16), (260, 1030, 33)]:
+        note(f"sgemm {ta}{tb} m={m} n={n} k={k}")
+        A = rng.integers(-8, 9, (k, m) if ta == "T" else (m, k)).astype(np.float32)
+        B = rng.integers(-8, 9, (n, k) if tb == "T" else (k, n)).astype(np.float32)
+        Cm = rng.integers(-8, 9, (m, n)).astype(np.float32)
+        opA = A.T if ta == "T" else A
+        opB = B.T if tb == "T" else B
+        exact = 2 * Cm.astype(np.int64) - opA.astype(np.int64) @ opB.astype(np.int64)
+        f32 = torch_mod.float32
+        dC = M.colmajor(Cm, dtype=f32)
+        M.gemm(h, ta, tb, -1.0, M.colmajor(A, dtype=f32), M.colmajor(B, dtype=f32), 2.0, dC)
+        got = host(dC)
+        assert np.array_equal(got.astype(np.int64), exact), (ta, tb, m, n, k)
+
+
+def test_sgemm_random_against_fp64(h, torch_mod):
+    rng = np.random.default_rng(8)
+    m, n, k = 700, 900, 1500
+    A = rng.standard_normal((m, k)).astype(np.float32)
+    B = rng.standard_normal((k, n)).astype(np.float32)
+    ref = A.astype(np.float64) @ B.astype(np.float64)
+    f32 = torch_mod.float32
+    dC = M.empty_colmajor(m, n, dtype=f32)
+    M.gemm(h, "N", "N", 1.0, M.colmajor(A, dtype=f32), M.colmajor(B, dtype=f32), 0.0, dC)
+    got = host(dC).astype(np.float64)
+    # FP32 accumulation over k terms: |err| <= k u32 sum |a||b|
+    bound = k * np.finfo(np.float32).eps * (np.abs(A).astype(np.float64) @ np.abs(B))
+    assert np.all(np.abs(got - ref) <= bound)
+
+
+# ---------------------------------------------------------------------------
+# mixed-precision least squares
+# ---------------------------------------------------------------------------
+
+def run_mixed(h, A, b, prec="fp32", **kw):
+    dA = M.colmajor(A)
+    import torch
+    db = torch.from_numpy(b).cuda()
+    x, info = M.gels_mixed(h, prec, dA, db, **kw)
+    h.synchronize()
+    # A and b are inputs only
+    assert np.array_equal(host(dA), A)
+    assert np.array_equal(db.cpu().numpy(), b)
+    return x.cpu().numpy(), info
+
+
+@pytest.mark.parametrize("m,n,cond,resid", [(2000, 500, 1e6, 1e-10), (1000, 333, 1e2, 1e-10),
+                                            (1500, 300, 1e4, 1.0), (700, 700, 1e3, 0.0),
+                                            (4100, 130, 1e5, 1e-6)])
+def test_mixed_fp32_reaches_fp64_backward_error(h, orc, m, n, cond, resid):
+    note(f"gels_mixed fp32 m={m} n={n} cond={cond:g} resid={resid:g}")
+    A, b = ls_problem(m, n, cond, seed=m + n, resid=resid)
+    x, info = run_mixed(h, A, b)
+    note(f"  iterations={info.iterations} converged={info.converged} "
+         f"fell_back={info.fell_back} omega={info.backward_error:.3e}")
+    assert info.converged == 1 and info.fell_back == 0
+    be = ls_backward_error(A, b, x)
+    assert be["kw"] <= 1e-14, be
+    # the reported estimate bounds the exact backward error from above
+    assert info.backward_error <= 1e-14
+    assert be["kw"] <= 1.5 * info.backward_error + 1e-17
+    assert abs(info.residual_norm - be["resid"]) <= 1e-6 * max(be["resid"], 1e-300) + 1e-12
+    xo = orc.gels(A, b)
+    tol = 50 * np.sqrt(n) * cond * EPS * (1.0 + cond * resid)
+    assert np.linalg.norm(x - xo) <= tol * np.linalg.norm(xo)
+
+
+def test_mixed_fp64_refinement(h, orc):
+    A, b = ls_problem(1200, 400, 1e8, seed=3, resid=1e-8)
+    x, info = run_mixed(h, A, b, prec="fp64")
+    assert info.converged == 1 and info.iterations <= 3
+    be = ls_backward_error(A, b, x)
+    assert be["kw"] <= 1e-14, be
+
+
+def test_mixed_ill_conditioned_falls_back(h, orc):
+    """kappa = 1e10 is beyond what an FP32 factorisation can precondition:
+    refinement stagnates or diverges and the FP64 solve takes over."""
+    A, b = ls_problem(1500, 400, 1e10, seed=5, resid=1e-12)
+    x, info = run_mixed(h, A, b)
+    assert info.fell_back == 1 and info.converged == 0
+    be = ls_backward_error(A, b, x)
+    assert be["kw"] <= 1e-14, be
+    xo = orc.gels(A, b)
+    assert np.linalg.norm(x - xo) <= 1e-3 * np.linalg.norm(xo)
+
+
+def test_mixed_without_fallback_reports_non_convergence(h):
+    A, b = ls_problem(1500, 400, 1e10, seed=5, resid=1e-12)
+    with pytest.raises(M.MqrError) as e:
+        run_mixed(h, A, b, fallback=False)
+    assert e.value.status == M.MQR_ERR_NOT_CONVERGED
+
+
+def test_mixed_zero_rhs_and_single_column(h):
+    A, _ = ls_problem(500, 64, 10.0, seed=9)
+    x, info = run_mixed(h, A, np.zeros(500))
+    assert info.converged == 1 and np.all(x == 0.0)
+    rng = np.random.default_rng(2)
+    a = rng.standard_normal((300, 1))
+    b = rng.standard_normal(300)
+    x, info = run_mixed(h, np.asfortranarray(a), b)
+    assert info.converged == 1
+    exact = (a[:, 0] @ b) / (a[:, 0] @ a[:, 0])
+    assert abs(x[0] - exact) <= 1e-14 * abs(exact)
+
+
+def test_mixed_repeated_calls_reuse_workspace(h):
+    """Two solves in a row, the second smaller: the workspace carved for the
+    first must not leak state into the second."""
+    A1, b1 = ls_problem(3000, 600, 1e3, seed=11)
+    A2, b2 = ls_problem(900, 200, 1e3, seed=12)
+    x1, _ = run_mixed(h, A1, b1)
+    x2, _ = run_mixed(h, A2, b2)
+    assert ls_backward_error(A1, b1, x1)["kw"] <= 1e-14
+    assert ls_backward_error(A2, b2, x2)["kw"] <= 1e-14
+
+
+def test_host_least_squares_api(h):
+    A, b = ls_problem(800, 200, 1e4, seed=13)
+    x, info = M.LeastSquares(A, b, prec="fp32", handle=h)
+    assert info.converged == 1
+    assert ls_backward_error(A, b, x)["kw"] <= 1e-14
