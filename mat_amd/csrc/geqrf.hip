@@ -108,9 +108,9 @@ static int64_t geqrf_ws_doubles(int64_t m, int64_t n, int nb) {
 }
 
 int panel_factor(mqr_handle_t hd, int64_t h, int b, double *V, int64_t lda, double *tau,
-                 void *pws) {
-    if (h >= kCholqrMinRows) return panel_cholqr(hd, h, b, V, lda, tau, pws);
-    return panel_householder(h, b, V, lda, tau, hd->stream);
+                 double *T, int64_t ldt, void *pws) {
+    if (h >= kCholqrMinRows) return panel_cholqr(hd, h, b, V, lda, tau, T, ldt, pws);
+    return panel_householder(h, b, V, lda, tau, T, ldt, hd->stream);
 }
 
 int geqrf_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, double *tau,
@@ -126,33 +126,12 @@ int geqrf_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, dou
         const int b = (int)std::min<int64_t>(nb, k - j);
         const int64_t h = m - j;
         double *V = A + j + j * lda;
-        MQR_CHECK(panel_factor(hd, h, b, V, lda, tau + j, pws));
-        // V^T [V | C] with C = A[j:m, j+b:n]: Gram of V in the first b columns
-        const int64_t ncols = n - j;
-        int splits = gemm_pick_splits(b, (int)ncols, (int)h, target_blocks(hd));
-        while (splits > 1 && (int64_t)splits * b * ncols > pcap) --splits;
-        GemmF64 g;
-        g.M = b; g.N = (int)ncols; g.K = (int)h;
-        g.ta = true; g.amask = MASK_VLOWER; g.bmask = MASK_VLOWER; g.bmask_cols = b;
-        g.A = V; g.lda = lda; g.B = V; g.ldb = lda;
-        g.C = P; g.ldc = b; g.splits = splits; g.split_stride = (int64_t)b * ncols;
-        g.kind = K_VTA;
-        MQR_CHECK(gemm_f64(g, hd->stream));
         double *Tj = T + j * ldt;
-        MQR_CHECK(reduce_partials((int64_t)b * ncols, P, splits, (int64_t)b * ncols, hd->stream));
-        MQR_CHECK(larft_from_gram(b, P, 1, (int64_t)b * ncols, tau + j, Tj, ldt, hd->stream));
+        MQR_CHECK(panel_factor(hd, h, b, V, lda, tau + j, Tj, ldt, pws));
+        // C <- (I - V T V^T)^T C for C = A[j:m, j+b:n]
         const int64_t nt = n - j - b;
-        if (nt > 0) {
-            MQR_CHECK(block_w2(b, nt, Tj, ldt, true, P + (int64_t)b * b, 1, (int64_t)b * ncols, W2,
-                               hd->stream));
-            GemmF64 u;
-            u.M = (int)h; u.N = (int)nt; u.K = b;
-            u.amask = MASK_VLOWER;
-            u.A = V; u.lda = lda; u.B = W2; u.ldb = b;
-            u.C = A + j + (j + b) * lda; u.ldc = lda; u.alpha = -1.0; u.beta = 1.0;
-            u.kind = K_UPDATE;
-            MQR_CHECK(gemm_f64(u, hd->stream));
-        }
+        MQR_CHECK(apply_block(hd, h, b, V, lda, Tj, ldt, A + j + (j + b) * lda, lda, nt, true, P,
+                              pcap, W2));
     }
     return MQR_SUCCESS;
 }

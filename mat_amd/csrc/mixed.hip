@@ -34,7 +34,6 @@ int ormqr_f64(mqr_handle_t hd, bool trans, int64_t m, int64_t nrhs, int64_t k, c
               int64_t ldvr, const double *T, int64_t ldt, double *C, int64_t ldc);
 int trsm_upper_f64(mqr_handle_t hd, bool trans, int64_t n, int64_t nrhs, const double *R,
                    int64_t ldr, double *B, int64_t ldb);
-int panel_factor(mqr_handle_t hd, int64_t h, int b, double *V, int64_t lda, double *tau, void *pws);
 
 namespace {
 
@@ -206,26 +205,26 @@ __global__ void k_fill(int64_t n, double *__restrict__ x, double v) {
 }
 
 // X (ld ldx) = inverse of each 64 x 64 diagonal block of upper R (ld ldr)
-__global__ __launch_bounds__(64) void k_inv_diag_blocks(int64_t n, const double *__restrict__ R,
-                                                        int64_t ldr, double *__restrict__ X,
-                                                        int64_t ldx) {
-    __shared__ double S[small::N][small::N + 1];
-    __shared__ double rdiag[small::N];
+__global__ __launch_bounds__(small::NT) void k_inv_diag_blocks(int64_t n,
+                                                               const double *__restrict__ R,
+                                                               int64_t ldr, double *__restrict__ X,
+                                                               int64_t ldx) {
+    __shared__ small::Lds S[small::N];
+    __shared__ double rb[2][small::N];
     const int64_t i0 = (int64_t)blockIdx.x * small::N;
     const int b = (int)std::min<int64_t>(small::N, n - i0);
-    const int c = small::lane_id();
-    double a[small::N], x[small::N];
-#pragma unroll
-    for (int r = 0; r < small::N; ++r)
-        a[r] = (r < b && c < b) ? (r <= c ? R[i0 + r + (i0 + c) * ldr] : 0.0) : (r == c ? 1.0 : 0.0);
-    small::col_to_lds(a, S);
+    for (int e = threadIdx.x; e < small::N * small::N; e += small::NT) {
+        const int r = e & (small::N - 1), c = e >> 6;
+        S[r][c] = (r < b && c < b) ? (r <= c ? R[i0 + r + (i0 + c) * ldr] : 0.0)
+                                   : (r == c ? 1.0 : 0.0);
+    }
     __syncthreads();
-    rdiag[c] = 1.0 / S[c][c];
+    double x[small::RK];
+    small::inv_upper(S, rb, x);
     __syncthreads();
-    small::inv_upper_lds(S, rdiag, x);
-#pragma unroll
-    for (int r = 0; r < small::N; ++r)
-        if (r < b && c < b) X[i0 + r + (i0 + c) * ldx] = x[r];
+    small::to_lds(x, S);
+    __syncthreads();
+    small::lds_store(S, X + i0 + i0 * ldx, ldx, b, b);
 }
 
 // R (ldr) rows [0, b), columns [0, nt) = Aw rows [0, b) (FP32 -> FP64)
@@ -256,7 +255,7 @@ struct MixedWs {
     double *R = nullptr;   // R factor, ld ldr
     int64_t ldr = 0;
     float *V32 = nullptr, *P32 = nullptr, *W2 = nullptr;
-    double *X = nullptr, *T = nullptr, *tau = nullptr, *Gp = nullptr;
+    double *X = nullptr, *T = nullptr, *tau = nullptr;
     double *r = nullptr, *z = nullptr, *y = nullptr, *part = nullptr, *sc = nullptr;
     double *xx = nullptr, *zz = nullptr, *tmp = nullptr;
     void *pws = nullptr;
@@ -291,7 +290,6 @@ MixedWs carve(char *base, bool fp64, int64_t m, int64_t n, int nb) {
     w.tmp = (double *)take(sizeof(double) * ((n + 1) / 2) * n);
     w.T = (double *)take(sizeof(double) * nb * n);
     w.tau = (double *)take(sizeof(double) * n);
-    w.Gp = (double *)take(sizeof(double) * kMaxSplits * nb * nb);
     w.r = (double *)take(sizeof(double) * m);
     w.z = (double *)take(sizeof(double) * n);
     w.y = (double *)take(sizeof(double) * n);
@@ -326,25 +324,14 @@ int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t 
                                (const float *)(w.Aw + j + j * m), m, V, m);
             MQR_LAUNCH_CHECK();
         }
-        MQR_CHECK(panel_factor(hd, h, b, V, m, w.tau + j, w.pws));
+        double *Tj = w.T + j * nb;
+        MQR_CHECK(panel_factor(hd, h, b, V, m, w.tau + j, Tj, nb, w.pws));
         {
             ProfScope ps(K_CAST, 0.0, 16.0 * b * b, s);
             hipLaunchKernelGGL(k_copy_rdiag, dim3((unsigned)((b * b + 255) / 256)), dim3(256), 0, s,
                                b, (const double *)V, m, w.R + j + j * w.ldr, w.ldr);
             MQR_LAUNCH_CHECK();
         }
-        // T from the FP64 Gram matrix of V
-        int splits = std::min(kMaxSplits, gemm_pick_splits(b, b, (int)h, 4 * hd->cu_count));
-        GemmF64 g;
-        g.M = b; g.N = b; g.K = (int)h;
-        g.ta = true; g.amask = MASK_VLOWER; g.bmask = MASK_VLOWER; g.bmask_cols = b;
-        g.A = V; g.lda = m; g.B = V; g.ldb = m;
-        g.C = w.Gp; g.ldc = b; g.splits = splits; g.split_stride = (int64_t)b * b;
-        g.kind = K_VTA;
-        MQR_CHECK(gemm_f64(g, s));
-        MQR_CHECK(reduce_partials((int64_t)b * b, w.Gp, splits, (int64_t)b * b, s));
-        double *Tj = w.T + j * nb;
-        MQR_CHECK(larft_from_gram(b, w.Gp, 1, (int64_t)b * b, w.tau + j, Tj, nb, s));
         const int64_t nt = n - j - b;
         if (nt <= 0) continue;
         {
@@ -399,7 +386,7 @@ int tri_inverse(mqr_handle_t hd, int64_t n, const double *R, int64_t ldr, double
         ProfScope ps(K_TRSM, (double)n * 64 * 64 / 3.0, 16.0 * n * n, s);
         hipLaunchKernelGGL(k_fill, dim3(grid_for(n * n, 256, 1 << 22)), dim3(256), 0, s, n * n, X,
                            0.0);
-        hipLaunchKernelGGL(k_inv_diag_blocks, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, n, R,
+        hipLaunchKernelGGL(k_inv_diag_blocks, dim3((unsigned)((n + 63) / 64)), dim3(small::NT), 0, s, n, R,
                            ldr, X, n);
         MQR_LAUNCH_CHECK();
     }

@@ -149,25 +149,29 @@ int scale_f64(int64_t m, int64_t n, double beta, double *C, int64_t ldc, hipStre
 int gemm_pick_splits(int M, int N, int K, int target_blocks);
 
 // ---- panel / T / block reflector ------------------------------------------
-// Householder panel on h x b at P (ld), single workgroup, in place.
-int panel_householder(int64_t h, int b, double *P, int64_t ld, double *tau, hipStream_t s);
+// Every panel routine factors the h x b panel P (ld) in place (R on and above
+// the diagonal, reflectors below), and writes tau[0:b] and the b x b
+// compact-WY factor T (ld ldt, zero below the diagonal).
+// Householder panel, single workgroup.
+int panel_householder(int64_t h, int b, double *P, int64_t ld, double *tau, double *T,
+                      int64_t ldt, hipStream_t s);
 // Same, launched unconditionally but running only when *gate != 0.
-int panel_householder_gated(int64_t h, int b, double *P, int64_t ld, double *tau, const int *gate,
-                            hipStream_t s);
+int panel_householder_gated(int64_t h, int b, double *P, int64_t ld, double *tau, double *T,
+                            int64_t ldt, const int *gate, hipStream_t s);
 // Shifted CholeskyQR3 + Householder reconstruction panel (falls back to the
 // gated Householder kernel on breakdown).  ws: panel_cholqr_ws_bytes(h).
 size_t panel_cholqr_ws_bytes(int64_t h);
-int panel_cholqr(mqr_handle_t hd, int64_t h, int b, double *X, int64_t ld, double *tau, void *ws);
+int panel_cholqr(mqr_handle_t hd, int64_t h, int b, double *X, int64_t ld, double *tau,
+                 double *T, int64_t ldt, void *ws);
 // Panels at least this tall use panel_cholqr.
 constexpr int64_t kCholqrMinRows = 384;
+// Either of the above by height.
+int panel_factor(mqr_handle_t hd, int64_t h, int b, double *V, int64_t lda, double *tau,
+                 double *T, int64_t ldt, void *ws);
 // P[0:count] += P[s*stride + 0:count] for s = 1..splits-1.
 int reduce_partials(int64_t count, double *P, int splits, int64_t stride, hipStream_t s);
-// Largest block size the LDS-resident T kernels accept.
+// Largest block size the LDS-resident small kernels accept.
 constexpr int kMaxNb = 64;
-// T from partial Gram matrices G_s (b x b each, ld = b, `splits` of them at
-// stride pstride) of V and tau: T(0:b, 0:b) with leading dimension ldt.
-int larft_from_gram(int b, const double *Gpart, int splits, int64_t pstride, const double *tau,
-                    double *T, int64_t ldt, hipStream_t s);
 // W2 = op(T) * sum_s P_s  (P_s: b x nc, ld = b, stride pstride), trans: T^T.
 int block_w2(int b, int64_t nc, const double *T, int64_t ldt, bool trans, const double *P,
              int splits, int64_t pstride, double *W2, hipStream_t s);

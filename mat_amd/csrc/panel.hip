@@ -1,10 +1,9 @@
 // Panel reflectors, compact-WY T and the T-product of the block reflector.
 //
 //  k_panel_hh  : QR/__Householder_Blocked.h:62-84 -- b Householder reflectors
-//                of an h x b panel, one workgroup, wavefront reductions.
-//  k_larft     : QR/__Householder_Blocked.h:86-103 -- the compact-WY T of
-//                the panel's reflectors from the Gram matrix V^T V that the
-//                MFMA GEMM produced.
+//                of an h x b panel, one workgroup, wavefront reductions,
+//                followed by their compact-WY T (:86-103) from the Gram
+//                matrix V^T V.
 //  k_block_w2  : the middle product of the trailing update,
 //                W2 = T^T W (QR/__Householder_Blocked.h:180-190) or T W
 //                (Q formation, :299-308), with W summed from split-K partials.
@@ -27,8 +26,12 @@ __device__ __forceinline__ double wave_sum(double v) {
 // panel raised its fallback flag).
 __global__ __launch_bounds__(PNT) void k_panel_hh(int64_t h, int b, double *__restrict__ P,
                                                   int64_t ld, double *__restrict__ tau,
+                                                  double *__restrict__ T, int64_t ldt,
                                                   const int *__restrict__ gate) {
     __shared__ double red[PNT / 64][CH];
+    __shared__ double tau_s[small::N];
+    __shared__ small::Lds S[small::N];
+    __shared__ double rbuf[2][small::N];
     if (gate && *gate == 0) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int r = 0; r < b; ++r) {
@@ -46,7 +49,7 @@ __global__ __launch_bounds__(PNT) void k_panel_hh(int64_t h, int b, double *__re
         const double alpha = w[0];
         __syncthreads();  // red[] and w[0] read by everyone
         if (cr == 0.0) {
-            if (tid == 0) tau[r] = 0.0;
+            if (tid == 0) tau[r] = tau_s[r] = 0.0;
             continue;
         }
         if (alpha < 0.0) cr = -cr;
@@ -56,7 +59,7 @@ __global__ __launch_bounds__(PNT) void k_panel_hh(int64_t h, int b, double *__re
         for (int64_t i = 1 + tid; i < len; i += PNT) w[i] = w[i] * hr;
         if (tid == 0) {
             w[0] = -cr;
-            tau[r] = t;
+            tau[r] = tau_s[r] = t;
         }
         __syncthreads();
         for (int c0 = r + 1; c0 < b; c0 += CH) {
@@ -100,44 +103,51 @@ __global__ __launch_bounds__(PNT) void k_panel_hh(int64_t h, int b, double *__re
         }
         __syncthreads();
     }
-}
-
-// T from the Gram matrix G = V^T V (summed split-K partials, slot s at
-// Gp + s * pstride, ld = b) and tau.  The forward columnwise recurrence
-// T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i), T(i, i) = tau_i
-// (QR/__Householder_Blocked.h:86-103) has the closed form
-// T^{-1} = strict_upper(G) + diag(1 / tau); it is evaluated here as one
-// register-resident triangular inverse (small_dense.h).  A reflector with
-// tau = 0 is the identity: its row and column are removed from the inverse
-// and left zero in T, which is what the recurrence produces.
-__global__ __launch_bounds__(64) void k_larft(int b, const double *__restrict__ Gp, int splits,
-                                              int64_t pstride, const double *__restrict__ tau,
-                                              double *__restrict__ T, int64_t ldt) {
-    __shared__ double S[small::N][small::N + 1];
-    __shared__ double rdiag[small::N];
-    const int c = small::lane_id();
-    const double tc = c < b ? tau[c] : 1.0;
-    double m[small::N], x[small::N];
+    // T: the forward columnwise recurrence T(0:i, i) = -tau_i T(0:i, 0:i)
+    // V(:, 0:i)^T v_i, T(i, i) = tau_i (QR/__Householder_Blocked.h:86-103) has
+    // the closed form T^{-1} = strict_upper(V^T V) + diag(1 / tau).  G = V^T V
+    // is accumulated over 64-row slices of V staged in LDS; a reflector with
+    // tau = 0 is the identity, its row and column are removed from the
+    // inverse and left zero in T, which is what the recurrence produces.
+    const int c = lane, q = wave;
+    double g[small::RK];
 #pragma unroll
-    for (int r = 0; r < small::N; ++r) {
-        double v = 0.0;
-        if (r < b && c < b && r < c) {
-            for (int s = 0; s < splits; ++s) v += Gp[(int64_t)s * pstride + r + (int64_t)c * b];
-            if (tau[r] == 0.0 || tc == 0.0) v = 0.0;
-        } else if (r == c) {
-            v = (c < b && tc != 0.0) ? 1.0 / tc : 1.0;
+    for (int k = 0; k < small::RK; ++k) g[k] = 0.0;
+    for (int64_t r0 = 0; r0 < h; r0 += small::N) {
+        __syncthreads();
+        for (int e = tid; e < small::N * small::N; e += PNT) {
+            const int rr = e & (small::N - 1), cc = e >> 6;
+            const int64_t r = r0 + rr;
+            double v = 0.0;
+            if (cc < b && r < h) v = r > cc ? P[r + (int64_t)cc * ld] : (r == cc ? 1.0 : 0.0);
+            S[rr][cc] = v;
         }
-        m[r] = v;
-    }
-    small::col_to_lds(m, S);
-    __syncthreads();
-    rdiag[c] = 1.0 / S[c][c];
-    __syncthreads();
-    small::inv_upper_lds(S, rdiag, x);
+        __syncthreads();
+        for (int rr = 0; rr < small::N; ++rr) {
+            const double vc = S[rr][c];
 #pragma unroll
-    for (int r = 0; r < small::N; ++r) {
+            for (int k = 0; k < small::RK; ++k) g[k] = fma(S[rr][4 * k + q], vc, g[k]);
+        }
+    }
+    __syncthreads();
+    const double tc = c < b ? tau_s[c] : 1.0;
+#pragma unroll
+    for (int k = 0; k < small::RK; ++k) {
+        const int r = 4 * k + q;
+        const double tr = r < b ? tau_s[r] : 1.0;
+        double v = 0.0;
+        if (r < c && c < b) v = (tr == 0.0 || tc == 0.0) ? 0.0 : g[k];
+        else if (r == c) v = tc != 0.0 ? 1.0 / tc : 1.0;
+        g[k] = v;
+    }
+    small::to_lds(g, S);
+    __syncthreads();
+    small::inv_upper(S, rbuf, g);
+#pragma unroll
+    for (int k = 0; k < small::RK; ++k) {
+        const int r = 4 * k + q;
         if (r < b && c < b) {
-            double v = r <= c ? x[r] : 0.0;
+            double v = r <= c ? g[k] : 0.0;
             if (r == c && tc == 0.0) v = 0.0;
             T[r + (int64_t)c * ldt] = v;
         }
@@ -196,18 +206,19 @@ int ensure_lds_attrs() {
 
 }  // namespace
 
-int panel_householder(int64_t h, int b, double *P, int64_t ld, double *tau, hipStream_t s) {
+int panel_householder(int64_t h, int b, double *P, int64_t ld, double *tau, double *T,
+                      int64_t ldt, hipStream_t s) {
     const double hh = (double)h, bb = b;
-    ProfScope ps(K_PANEL, 2.0 * hh * bb * bb - 2.0 * bb * bb * bb / 3.0, 16.0 * hh * bb, s);
-    hipLaunchKernelGGL(k_panel_hh, dim3(1), dim3(PNT), 0, s, h, b, P, ld, tau,
+    ProfScope ps(K_PANEL, 3.0 * hh * bb * bb - 2.0 * bb * bb * bb / 3.0, 24.0 * hh * bb, s);
+    hipLaunchKernelGGL(k_panel_hh, dim3(1), dim3(PNT), 0, s, h, b, P, ld, tau, T, ldt,
                        (const int *)nullptr);
     MQR_LAUNCH_CHECK();
     return MQR_SUCCESS;
 }
 
-int panel_householder_gated(int64_t h, int b, double *P, int64_t ld, double *tau, const int *gate,
-                            hipStream_t s) {
-    hipLaunchKernelGGL(k_panel_hh, dim3(1), dim3(PNT), 0, s, h, b, P, ld, tau, gate);
+int panel_householder_gated(int64_t h, int b, double *P, int64_t ld, double *tau, double *T,
+                            int64_t ldt, const int *gate, hipStream_t s) {
+    hipLaunchKernelGGL(k_panel_hh, dim3(1), dim3(PNT), 0, s, h, b, P, ld, tau, T, ldt, gate);
     MQR_LAUNCH_CHECK();
     return MQR_SUCCESS;
 }
@@ -235,14 +246,6 @@ int reduce_partials(int64_t count, double *P, int splits, int64_t stride, hipStr
     ProfScope ps(K_OTHER, (double)count * (splits - 1), 8.0 * count * (splits + 1), s);
     hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
                        count, P, splits, stride);
-    MQR_LAUNCH_CHECK();
-    return MQR_SUCCESS;
-}
-
-int larft_from_gram(int b, const double *Gpart, int splits, int64_t pstride, const double *tau,
-                    double *T, int64_t ldt, hipStream_t s) {
-    ProfScope ps(K_LARFT, (double)b * b * b / 3.0, 8.0 * b * b * (splits + 1), s);
-    hipLaunchKernelGGL(k_larft, dim3(1), dim3(64), 0, s, b, Gpart, splits, pstride, tau, T, ldt);
     MQR_LAUNCH_CHECK();
     return MQR_SUCCESS;
 }

@@ -22,7 +22,8 @@
 // zero columns.
 //
 // The b x b factorisations run register-resident in one workgroup
-// (small_dense.h); every barrier is executed by all of its waves.
+// (small_dense.h); every barrier is executed by all of its waves.  The
+// compact-WY factor T comes out of the reconstruction as well.
 #include <climits>
 
 #include "mqr_internal.h"
@@ -34,141 +35,136 @@ namespace {
 constexpr int PB = kMaxNb;  // padded panel width (small::N)
 using SM = double[small::N + 1];
 
-// Cholesky + inverse of one pass (one wave).  G: reduced Gram (b x b, ld b).
-// Pass 1 adds the shift and resets the flag; later passes are skipped once
-// it is up.
-__global__ __launch_bounds__(64) void k_chol_inv(int b, int64_t h, const double *__restrict__ G,
-                                                 int pass, double *__restrict__ R,
-                                                 double *__restrict__ Rinv,
-                                                 int *__restrict__ flag) {
-    __shared__ double rowbuf[PB], rdiag[PB];
-    __shared__ SM S[PB];
+// Cholesky + inverse of one pass.  G: reduced Gram (b x b, ld b).  Pass 1
+// adds the shift and resets the flag; later passes are skipped once it is up.
+__global__ __launch_bounds__(small::NT) void k_chol_inv(int b, int64_t h,
+                                                        const double *__restrict__ G, int pass,
+                                                        double *__restrict__ R,
+                                                        double *__restrict__ Rinv,
+                                                        int *__restrict__ flag) {
+    __shared__ small::Lds S[PB];
+    __shared__ double rb[2][PB];
+    __shared__ double shift_s;
     __shared__ int fail;
     if (pass > 1 && *flag) return;
-    const int c = small::lane_id();
-    if (c == 0) fail = 0;
-    double shift = 0.0;
-    if (pass == 1) {
-        double tr = 0.0;
-        for (int i = 0; i < b; ++i) tr += G[i + (int64_t)i * b];
-        shift = 11.0 * ((double)h * b + (double)b * (b + 1)) * 1.1102230246251565e-16 * tr;
-    }
-    double a[small::N], x[small::N];
-    small::load_col(a, G, b, b);
-#pragma unroll
-    for (int r = 0; r < small::N; ++r)
-        if (r == c && c < b) a[r] += shift;
+    const int tid = threadIdx.x, c = small::lane_id(), q = small::wave_id();
+    if (tid == 0) fail = 0;
+    small::lds_load(S, G, b, b);
     __syncthreads();
+    if (q == 0) {
+        const double tr = small::wave_sum(c < b ? S[c][c] : 0.0);
+        if (c == 0)
+            shift_s = pass == 1
+                          ? 11.0 * ((double)h * b + (double)b * (b + 1)) * 1.1102230246251565e-16 * tr
+                          : 0.0;
+    }
+    __syncthreads();
+    const double shift = shift_s;
+    double a[small::RK];
+    small::from_lds(a, S);
+#pragma unroll
+    for (int k = 0; k < small::RK; ++k)
+        if (4 * k + q == c && c < b) a[k] += shift;
     // pass 1: a pivot the shift dominates marks a dependent column; later
     // passes see nearly orthonormal columns
-    small::chol_lds(a, rowbuf, b, pass == 1 ? 2.0 * shift : 1e-2, &fail, true);
-    small::store_col(a, R, PB);
-    small::col_to_lds(a, S);
+    small::chol(a, rb, b, pass == 1 ? 2.0 * shift : 1e-2, &fail);
+    small::to_lds(a, S);
     __syncthreads();
-    rdiag[c] = 1.0 / S[c][c];
+    small::lds_store(S, R, PB, PB, PB);
+    double x[small::RK];
+    small::inv_upper(S, rb, x);
     __syncthreads();
-    small::inv_upper_lds(S, rdiag, x);
-    small::store_col(x, Rinv, PB);
+    small::to_lds(x, S);
     __syncthreads();
-    if (c == 0) {
+    small::lds_store(S, Rinv, PB, PB, PB);
+    if (tid == 0) {
         if (pass == 1) *flag = fail;
         else if (fail) *flag = 1;
     }
 }
 
-// Final pass and reconstruction (256 threads; wave 0 runs the sequential
-// factorisations, all waves the products).  Inputs: reduced Gram G3 of Y
-// (b x b, ld b), the top b rows of Y (ld ldy), R1, R2 (padded, ld PB).
-// Outputs: the panel's top b x b block (R on/above the diagonal, V1 below),
-// tau, and Minv = R3^{-1} U^{-1} (padded) for V2 = Y(b:h) Minv.
-__global__ __launch_bounds__(256) void k_recon(int b, const double *__restrict__ G,
-                                               const double *__restrict__ Y, int64_t ldy,
-                                               const double *__restrict__ R1,
-                                               const double *__restrict__ R2,
-                                               double *__restrict__ A, int64_t lda,
-                                               double *__restrict__ tau,
-                                               double *__restrict__ Minv,
-                                               int *__restrict__ flag) {
-    __shared__ SM S0[PB], S1[PB], S2[PB], S3[PB];
-    __shared__ double sg[PB], rowbuf[PB], rdiag[PB];
+// Final pass and reconstruction.  Inputs: reduced Gram G3 of Z (b x b,
+// ld b), the top b rows of Z (ld ldz), R1, R2 (padded, ld PB).  Outputs: the
+// panel's top b x b block (R on/above the diagonal, V1 below), tau, the
+// compact-WY factor T (ld ldt) and Minv = R3^{-1} U^{-1} (padded) for
+// V2 = Z(b:h) Minv.
+//
+// With Q = Z R3^{-1}, Q1 = Q(0:b, :), and the LU factorisation
+// Q1 - S = L U (S the signs), the reflectors I - V T V^T whose first b
+// columns are Q S satisfy V1 = L, V2 = Q2 U^{-1} and T = -U S L^{-T}
+// (top block: Q1 S - I = -V1 T V1^T).  T^T = L^{-1} (-S U^T) is one
+// forward substitution.
+__global__ __launch_bounds__(small::NT) void k_recon(int b, const double *__restrict__ G,
+                                                     const double *__restrict__ Z, int64_t ldz,
+                                                     const double *__restrict__ R1,
+                                                     const double *__restrict__ R2,
+                                                     double *__restrict__ A, int64_t lda,
+                                                     double *__restrict__ tau,
+                                                     double *__restrict__ T, int64_t ldt,
+                                                     double *__restrict__ Minv,
+                                                     int *__restrict__ flag) {
+    __shared__ small::Lds S0[PB], S1[PB], S2[PB], S3[PB];
+    __shared__ double sg[PB], rb[2][PB];
     __shared__ int fail;
     if (*flag) return;
-    const int c = small::lane_id(), w = small::wave_id(), tid = threadIdx.x;
+    const int tid = threadIdx.x, c = small::lane_id(), q = small::wave_id();
     if (tid == 0) fail = 0;
-    for (int e = tid; e < PB * PB; e += 256) {
-        const int r = e % PB, cc = e / PB;
-        S0[r][cc] = (r < b && cc < b) ? Y[r + (int64_t)cc * ldy] : (r == cc ? 1.0 : 0.0);
-    }
+    small::lds_load(S0, Z, ldz, b);
+    small::lds_load(S1, G, b, b);
     __syncthreads();
+    double a[small::RK];
     // R3 = chol(G3) -> S2, R3^{-1} -> S1
-    {
-        double a[small::N], x[small::N];
-        if (w == 0) small::load_col(a, G, b, b);
-        small::chol_lds(a, rowbuf, b, 1e-2, &fail, w == 0);
-        if (w == 0) small::col_to_lds(a, S2);
-        __syncthreads();
-        if (tid < PB) rdiag[tid] = 1.0 / S2[tid][tid];
-        __syncthreads();
-        if (w == 0) {
-            small::inv_upper_lds(S2, rdiag, x);
-            small::col_to_lds(x, S1);
-        }
-    }
+    small::from_lds(a, S1);
+    small::chol(a, rb, b, 1e-2, &fail);
+    small::to_lds(a, S2);
     __syncthreads();
-    // Q1 = Y_top R3^{-1} -> S3
-    small::matmul_lds(S0, S1, S3, false);
+    small::inv_upper(S2, rb, a);
+    __syncthreads();  // S1 (G3) no longer read
+    small::to_lds(a, S1);
+    __syncthreads();
+    // Q1 = Z_top R3^{-1} -> S3
+    small::matmul(S0, S1, false, true, [&](int r, int cc, double v) { S3[r][cc] = v; });
     __syncthreads();
     // LU of Q1 - S -> S3 (L strictly below, U on and above); U -> S0
-    if (w == 0) {
-        double q[small::N];
-        small::lds_to_col(q, S3);
-        small::lu_sign(q, sg);
-        small::col_to_lds(q, S3);
+    small::from_lds(a, S3);
+    small::lu_sign(a, rb, sg);
+    small::to_lds(a, S3);
+    small::keep_upper(a);
+    small::to_lds(a, S0);
+    __syncthreads();
+    // U^{-1} -> S0
+    small::inv_upper(S0, rb, a);
+    __syncthreads();
+    small::to_lds(a, S0);
+    __syncthreads();
+    // Minv = R3^{-1} U^{-1} straight to global
+    small::matmul(S1, S0, true, true, [&](int r, int cc, double v) { Minv[r + cc * PB] = v; });
+    // T^T = L^{-1} (-S U^T): B(r, c) = -sg(r) U(c, r) for c <= r
 #pragma unroll
-        for (int r = 0; r < small::N; ++r) q[r] = (r <= c) ? q[r] : 0.0;
-        small::col_to_lds(q, S0);
+    for (int k = 0; k < small::RK; ++k) {
+        const int r = 4 * k + q;
+        a[k] = (c <= r) ? -sg[r] * S3[c][r] : 0.0;
+    }
+    small::solve_unit_lower(S3, rb, a);
+    __syncthreads();  // S0 (U^{-1}) and S1 (R3^{-1}) no longer read
+    small::to_lds(a, S0);
+    small::lds_load(S1, R2, PB, PB);
+    __syncthreads();
+    for (int e = tid; e < PB * PB; e += small::NT) {
+        const int r = e & (PB - 1), cc = e >> 6;
+        if (r < b && cc < b) T[r + (int64_t)cc * ldt] = r <= cc ? S0[cc][r] : 0.0;
     }
     __syncthreads();
-    if (tid < PB) rdiag[tid] = 1.0 / S0[tid][tid];
+    // W = R3 R2 -> S0
+    small::matmul(S2, S1, true, true, [&](int r, int cc, double v) { S0[r][cc] = v; });
     __syncthreads();
-    // U^{-1} -> S0 (read U from S0 first, then overwrite)
-    {
-        double z[small::N];
-        if (w == 0) small::inv_upper_lds(S0, rdiag, z);
-        __syncthreads();
-        if (w == 0) small::col_to_lds(z, S0);
-    }
+    small::lds_load(S1, R1, PB, PB);
     __syncthreads();
-    // Minv = R3^{-1} U^{-1} (upper) straight to global
-#pragma unroll 1
-    for (int q = 0; q < 16; ++q) {
-        const int r = w * 16 + q;
-        double acc = 0.0;
-        for (int k = r; k <= c; ++k) acc = fma(S1[r][k], S0[k][c], acc);
-        Minv[r + c * PB] = acc;
-    }
-    __syncthreads();
-    // W = R3 R2 -> S1 (S0 <- R2)
-    for (int e = tid; e < PB * PB; e += 256) S0[e % PB][e / PB] = R2[e];
-    __syncthreads();
-    small::matmul_lds(S2, S0, S1, true);
-    __syncthreads();
-    // R = S (W R1): write with V1 (strictly below, from the LU) into the panel
-    for (int e = tid; e < PB * PB; e += 256) S0[e % PB][e / PB] = R1[e];
-    __syncthreads();
-#pragma unroll 1
-    for (int q = 0; q < 16; ++q) {
-        const int r = w * 16 + q;
-        if (r >= b || c >= b) continue;
-        if (r <= c) {
-            double acc = 0.0;
-            for (int k = r; k <= c; ++k) acc = fma(S1[r][k], S0[k][c], acc);
-            A[r + (int64_t)c * lda] = sg[r] * acc;
-        } else {
-            A[r + (int64_t)c * lda] = S3[r][c];
-        }
-    }
-    for (int i = tid; i < b; i += 256) tau[i] = -sg[i] * S3[i][i];
+    // R = S (W R1) with V1 (strictly below, from the LU) into the panel
+    small::matmul(S0, S1, true, true, [&](int r, int cc, double v) {
+        if (r < b && cc < b) A[r + (int64_t)cc * lda] = r <= cc ? sg[r] * v : S3[r][cc];
+    });
+    for (int i = tid; i < b; i += small::NT) tau[i] = -sg[i] * S3[i][i];
     __syncthreads();
     if (tid == 0 && fail) *flag = 1;
 }
@@ -183,7 +179,7 @@ size_t panel_cholqr_ws_bytes(int64_t h) {
 
 // Factor the h x b panel X (ld) in place.
 int panel_cholqr(mqr_handle_t hd, int64_t h, int b, double *X, int64_t ld, double *tau,
-                 void *ws) {
+                 double *T, int64_t ldt, void *ws) {
     hipStream_t s = hd->stream;
     double *Y = (double *)ws;
     double *Z = Y + (size_t)h * PB;
@@ -222,27 +218,27 @@ int panel_cholqr(mqr_handle_t hd, int64_t h, int b, double *X, int64_t ld, doubl
     MQR_CHECK(gram(X, ld));
     {
         ProfScope ps(K_PANEL, small_flops, small_bytes, s);
-        hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(64), 0, s, b, h, Gp, 1, R1, I1, flag);
+        hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(small::NT), 0, s, b, h, Gp, 1, R1, I1, flag);
     }
     MQR_CHECK(apply(h, X, ld, I1, Y, h, nullptr));
     // pass 2
     MQR_CHECK(gram(Y, h));
     {
         ProfScope ps(K_PANEL, small_flops, small_bytes, s);
-        hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(64), 0, s, b, h, Gp, 2, R2, I2, flag);
+        hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(small::NT), 0, s, b, h, Gp, 2, R2, I2, flag);
     }
     MQR_CHECK(apply(h, Y, h, I2, Z, h, nullptr));
     // pass 3 + reconstruction
     MQR_CHECK(gram(Z, h));
     {
         ProfScope ps(K_PANEL, 4.0 * small_flops, small_bytes, s);
-        hipLaunchKernelGGL(k_recon, dim3(1), dim3(256), 0, s, b, Gp, Z, h, R1, R2, X, ld, tau,
-                           Mi, flag);
+        hipLaunchKernelGGL(k_recon, dim3(1), dim3(small::NT), 0, s, b, Gp, Z, h, R1, R2, X, ld,
+                           tau, T, ldt, Mi, flag);
     }
     MQR_LAUNCH_CHECK();
     if (h > b) MQR_CHECK(apply(h - b, Z + b, h, Mi, X + b, ld, flag));
     // fallback: the column-by-column kernel runs only when the flag is up
-    return panel_householder_gated(h, b, X, ld, tau, flag, s);
+    return panel_householder_gated(h, b, X, ld, tau, T, ldt, flag, s);
 }
 
 }  // namespace mqr
