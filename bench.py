@@ -84,6 +84,7 @@ def backward_errors(P, x):
 
 
 def run_plain(M, h, P, steps: int, warmup: int):
+    """Time `steps` plain FP64 solves (profiling off), then profile one more."""
     import torch
     A0, b0 = P["A"], P["b"]
     m, n = A0.shape
@@ -98,32 +99,35 @@ def run_plain(M, h, P, steps: int, warmup: int):
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
-    M.profile_enable(h, True)
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
+    x = bw[:n, 0].clone()
+    M.profile_enable(h, True)
+    step()
     prof = M.profile_read(h)
     M.profile_enable(h, False)
-    x = bw[:n, 0].clone()
     return dt, x, prof
 
 
 def run_mixed(M, h, P, prec: str, steps: int, warmup: int):
+    """Time `steps` mixed-precision solves (profiling off), then profile one."""
     import torch
     A0, b0 = P["A"], P["b"]
     infos = []
     for _ in range(warmup):
         M.gels_mixed(h, prec, A0, b0)
     torch.cuda.synchronize()
-    M.profile_enable(h, True)
     t0 = time.perf_counter()
     for _ in range(steps):
         x, info = M.gels_mixed(h, prec, A0, b0)
         infos.append(info)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
+    M.profile_enable(h, True)
+    M.gels_mixed(h, prec, A0, b0)
     prof = M.profile_read(h)
     M.profile_enable(h, False)
     last = infos[-1]
@@ -132,14 +136,15 @@ def run_mixed(M, h, P, prec: str, steps: int, warmup: int):
 
 
 def roofline_from(prof: dict, steps: int, prec: str) -> dict:
-    """Dominant kernel = the kind with the largest summed duration."""
+    """Dominant kernel = the kind with the largest summed duration over the
+    profiled solves (`steps` of them)."""
     if not prof:
         return None
     kind = max(prof, key=lambda k: prof[k]["total_ms"])
     p = prof[kind]
     avg_ms = p["total_ms"] / max(p["launches"], 1)
     flops_per_launch = p["flops"] / max(p["launches"], 1)
-    mfma = kind in ("update_gemm", "vta_gemm", "gemm")
+    mfma = kind in ("update_gemm", "vta_gemm", "gemm", "panel_gemm")
     if mfma:
         achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
         peak = PEAK_TFLOPS["fp64" if prec == "fp64" else "fp32"]
@@ -251,7 +256,7 @@ def main() -> int:
     be_p = backward_errors(P, x_p)
     plain = {"tflops": round(flops / dt_p / 1e12, 4), "ms": round(dt_p * 1e3, 3),
              "pct_fp64_peak": round(100 * flops / dt_p / 1e12 / PEAK_TFLOPS["fp64"], 2),
-             "backward_error": be_p, "roofline": roofline_from(prof_p, args.steps, "fp64")}
+             "backward_error": be_p, "roofline": roofline_from(prof_p, 1, "fp64")}
     print(f"[bench] plain: {plain['tflops']} TFLOP/s, {plain['ms']} ms, be={be_p}",
           file=sys.stderr, flush=True)
 
@@ -263,7 +268,7 @@ def main() -> int:
             be_m = backward_errors(P, x_m)
             mixed = {"prec": args.prec, "tflops": round(flops / dt_m / 1e12, 4),
                      "ms": round(dt_m * 1e3, 3), "backward_error": be_m, "ir": ir,
-                     "roofline": roofline_from(prof_m, args.steps, args.prec)}
+                     "roofline": roofline_from(prof_m, 1, args.prec)}
             print(f"[bench] mixed: {mixed['tflops']} TFLOP/s, {mixed['ms']} ms, be={be_m}, ir={ir}",
                   file=sys.stderr, flush=True)
         except M.MqrError as e:

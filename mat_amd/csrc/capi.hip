@@ -45,6 +45,7 @@ void set_last_hip_error(hipError_t e, const char *where) {
 int ws_reserve(mqr_handle_t h, Workspace &ws, size_t bytes) {
     if (ws.bytes >= bytes) return MQR_SUCCESS;
     MQR_HIP(hipStreamSynchronize(h->stream));
+    MQR_HIP(hipStreamSynchronize(h->side_stream));
     if (ws.ptr) {
         MQR_HIP(hipFree(ws.ptr));
         ws.ptr = nullptr;
@@ -69,6 +70,34 @@ using namespace mqr;
     DeviceGuard guard_((h)->device);                  \
     if (!guard_.ok) return MQR_ERR_NO_DEVICE;         \
     ::mqr::t_prof = (h)->prof.on ? &(h)->prof : nullptr
+
+// Orders a call's work (queued on the handle's own stream) after what the
+// caller queued on its stream before the call, and the caller's later work
+// after the call's: the caller sees ordinary stream-ordered semantics while
+// the kernels run on a non-blocking stream (dependent launches on the legacy
+// null stream cost about twice as much).
+struct StreamBridge {
+    mqr_handle_t h;
+    bool active;
+    int status = MQR_SUCCESS;
+    explicit StreamBridge(mqr_handle_t hd) : h(hd), active(hd->user_stream != hd->stream) {
+        if (!active) return;
+        if (hipEventRecord(h->ev_in, h->user_stream) != hipSuccess ||
+            hipStreamWaitEvent(h->stream, h->ev_in, 0) != hipSuccess) {
+            ::mqr::set_last_hip_error(hipGetLastError(), "stream bridge (in)");
+            status = MQR_ERR_HIP;
+        }
+    }
+    ~StreamBridge() {
+        if (!active) return;
+        (void)hipEventRecord(h->ev_out, h->stream);
+        (void)hipStreamWaitEvent(h->user_stream, h->ev_out, 0);
+    }
+};
+
+#define MQR_BRIDGE(h)              \
+    StreamBridge bridge_(h);       \
+    if (bridge_.status) return bridge_.status
 
 extern "C" {
 
@@ -118,12 +147,29 @@ int mqr_create(mqr_handle_t *handle, int device) {
         return MQR_ERR_HIP;
     }
     h->stream = h->own_stream;
+    // second stream and events of the look-ahead schedule (default priority)
+    h->user_stream = h->own_stream;
+    if (hipStreamCreateWithFlags(&h->side_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_in, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_out, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out})
+            if (e) (void)hipEventDestroy(e);
+        if (h->side_stream) (void)hipStreamDestroy(h->side_stream);
+        (void)hipStreamDestroy(h->own_stream);
+        delete h;
+        return MQR_ERR_HIP;
+    }
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
         cus > 0)
         h->cu_count = cus;
     if (hipHostMalloc((void **)&h->host_pinned, 64 * sizeof(double), hipHostMallocDefault) !=
         hipSuccess) {
+        for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out}) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(h->side_stream);
         (void)hipStreamDestroy(h->own_stream);
         delete h;
         return MQR_ERR_ALLOC;
@@ -135,9 +181,14 @@ int mqr_create(mqr_handle_t *handle, int device) {
 int mqr_destroy(mqr_handle_t h) {
     MQR_GUARD(h);
     (void)hipStreamSynchronize(h->stream);
+    (void)hipStreamSynchronize(h->side_stream);
+    if (h->user_stream != h->stream) (void)hipStreamSynchronize(h->user_stream);
     if (h->ws.ptr) (void)hipFree(h->ws.ptr);
     if (h->ws_ir.ptr) (void)hipFree(h->ws_ir.ptr);
     if (h->host_pinned) (void)hipHostFree(h->host_pinned);
+    for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out})
+        if (e) (void)hipEventDestroy(e);
+    if (h->side_stream) (void)hipStreamDestroy(h->side_stream);
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     delete h;
     return MQR_SUCCESS;
@@ -145,14 +196,14 @@ int mqr_destroy(mqr_handle_t h) {
 
 int mqr_set_stream(mqr_handle_t h, void *stream) {
     if (!h) return -1;
-    h->stream = (hipStream_t)stream;
+    h->user_stream = (hipStream_t)stream;
     return MQR_SUCCESS;
 }
 
 int mqr_get_stream(mqr_handle_t h, void **stream) {
     if (!h) return -1;
     if (!stream) return -2;
-    *stream = (void *)h->stream;
+    *stream = (void *)h->user_stream;
     return MQR_SUCCESS;
 }
 
@@ -173,6 +224,8 @@ int mqr_get_block_size(mqr_handle_t h, int *nb) {
 int mqr_synchronize(mqr_handle_t h) {
     MQR_GUARD(h);
     MQR_HIP(hipStreamSynchronize(h->stream));
+    MQR_HIP(hipStreamSynchronize(h->side_stream));
+    if (h->user_stream != h->stream) MQR_HIP(hipStreamSynchronize(h->user_stream));
     return MQR_SUCCESS;
 }
 
@@ -200,8 +253,8 @@ int mqr_memcpy_htod(mqr_handle_t h, void *dst, const void *src, size_t bytes) {
     if (bytes && !dst) return -2;
     if (bytes && !src) return -3;
     if (bytes) {
-        MQR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
-        MQR_HIP(hipStreamSynchronize(h->stream));
+        MQR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->user_stream));
+        MQR_HIP(hipStreamSynchronize(h->user_stream));
     }
     return MQR_SUCCESS;
 }
@@ -211,8 +264,8 @@ int mqr_memcpy_dtoh(mqr_handle_t h, void *dst, const void *src, size_t bytes) {
     if (bytes && !dst) return -2;
     if (bytes && !src) return -3;
     if (bytes) {
-        MQR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream));
-        MQR_HIP(hipStreamSynchronize(h->stream));
+        MQR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->user_stream));
+        MQR_HIP(hipStreamSynchronize(h->user_stream));
     }
     return MQR_SUCCESS;
 }
@@ -235,6 +288,7 @@ int mqr_dgemm(mqr_handle_t h, char transa, char transb, int64_t m, int64_t n, in
     if (ldc < std::max<int64_t>(1, m)) return -14;
     if (m == 0 || n == 0) return MQR_SUCCESS;
     MQR_GUARD(h);
+    MQR_BRIDGE(h);
     if (k == 0 || alpha == 0.0) return scale_f64(m, n, beta, C, ldc, h->stream);
     GemmF64 g;
     g.M = (int)m; g.N = (int)n; g.K = (int)k;
@@ -263,6 +317,7 @@ int mqr_sgemm(mqr_handle_t h, char transa, char transb, int64_t m, int64_t n, in
     if (m == 0 || n == 0) return MQR_SUCCESS;
     if (k == 0) return MQR_ERR_UNSUPPORTED;
     MQR_GUARD(h);
+    MQR_BRIDGE(h);
     GemmF32 g;
     g.M = (int)m; g.N = (int)n; g.K = (int)k;
     g.ta = ta; g.tb = tb;
@@ -285,6 +340,7 @@ int mqr_dgeqrf(mqr_handle_t h, int64_t m, int64_t n, double *A, int64_t lda, dou
     if (k == 0) return MQR_SUCCESS;
     if (m > INT32_MAX || n > INT32_MAX) return MQR_ERR_UNSUPPORTED;
     MQR_GUARD(h);
+    MQR_BRIDGE(h);
     return geqrf_f64(h, m, n, A, lda, tau, T, ldt);
 }
 
@@ -302,6 +358,7 @@ int mqr_dorgqr(mqr_handle_t h, int64_t m, int64_t nq, int64_t k, const double *V
     if (ldq < std::max<int64_t>(1, m)) return -10;
     if (m * nq == 0) return MQR_SUCCESS;
     MQR_GUARD(h);
+    MQR_BRIDGE(h);
     return orgqr_f64(h, m, nq, k, VR, ldvr, T, ldt, Q, ldq);
 }
 
@@ -321,6 +378,7 @@ int mqr_dormqr(mqr_handle_t h, char trans, int64_t m, int64_t nrhs, int64_t k, c
     if (ldc < std::max<int64_t>(1, m)) return -11;
     if (m * nrhs == 0 || k == 0) return MQR_SUCCESS;
     MQR_GUARD(h);
+    MQR_BRIDGE(h);
     return ormqr_f64(h, tr, m, nrhs, k, VR, ldvr, T, ldt, C, ldc);
 }
 
@@ -337,6 +395,7 @@ int mqr_dtrsm_upper(mqr_handle_t h, char trans, int64_t n, int64_t nrhs, const d
     if (ldb < std::max<int64_t>(1, n)) return -8;
     if (n * nrhs == 0) return MQR_SUCCESS;
     MQR_GUARD(h);
+    MQR_BRIDGE(h);
     return trsm_upper_f64(h, tr, n, nrhs, R, ldr, B, ldb);
 }
 
@@ -352,6 +411,7 @@ int mqr_dgels(mqr_handle_t h, int64_t m, int64_t n, int64_t nrhs, double *A, int
     if (ldb < std::max<int64_t>(1, m)) return -8;
     if (n == 0 || nrhs == 0) return MQR_SUCCESS;
     MQR_GUARD(h);
+    MQR_BRIDGE(h);
     const int nb = h->nb;
     double *tauT = nullptr;
     // tau (n) and T (nb x n) live in the refinement workspace
@@ -377,6 +437,7 @@ int mqr_gels_mixed(mqr_handle_t h, mqr_precision_t prec, int64_t m, int64_t n, c
     if (info) std::memset(info, 0, sizeof(*info));
     if (n == 0) return MQR_SUCCESS;
     MQR_GUARD(h);
+    MQR_BRIDGE(h);
     return gels_mixed(h, prec, m, n, A, lda, b, x, opts, info);
 }
 
@@ -390,6 +451,7 @@ int mqr_qr_householder_host(mqr_handle_t h, int64_t m, int64_t n, const double *
     if (m * n > 0 && !R) return -6;
     if (m == 0) return MQR_SUCCESS;
     MQR_GUARD(h);
+    MQR_BRIDGE(h);
     const int64_t k = std::min(m, n);
     const int nb = h->nb;
     double *dA = nullptr, *dQ = nullptr, *dTT = nullptr;
@@ -435,6 +497,7 @@ int mqr_gels_host(mqr_handle_t h, mqr_precision_t prec, int64_t m, int64_t n, co
     if (info) std::memset(info, 0, sizeof(*info));
     if (n == 0) return MQR_SUCCESS;
     MQR_GUARD(h);
+    MQR_BRIDGE(h);
     double *dA = nullptr, *db = nullptr, *dx = nullptr;
     int st = MQR_SUCCESS;
     if (hipMalloc(&dA, sizeof(double) * m * n) != hipSuccess ||

@@ -248,6 +248,7 @@ int gemm_f64(const GemmF64 &g, hipStream_t s) {
         case 11: launch<true, false, MASK_VLOWER, MASK_VLOWER>(g, s); break;
         case 4: launch<false, true, MASK_NONE, MASK_NONE>(g, s); break;
         case 12: launch<true, true, MASK_NONE, MASK_NONE>(g, s); break;
+        case 6: launch<false, true, MASK_VLOWER, MASK_NONE>(g, s); break;
         default: return MQR_ERR_UNSUPPORTED;
     }
     MQR_LAUNCH_CHECK();

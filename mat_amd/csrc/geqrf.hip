@@ -73,11 +73,24 @@ __global__ __launch_bounds__(256) void k_trsm_diag(int nbk, int64_t nrhs, const 
 
 int target_blocks(mqr_handle_t h) { return 4 * h->cu_count; }
 
-// Apply one block reflector (V: h x b at Vp, T at Tp) to the h x nc matrix C:
-// trans: C <- (I - V T V^T)^T C; otherwise C <- (I - V T V^T) C.
-int apply_block(mqr_handle_t hd, int64_t h, int b, const double *Vp, int64_t ldv, const double *Tp,
-                int64_t ldt, double *C, int64_t ldc, int64_t nc, bool trans, double *P,
-                int64_t pcap, double *W2) {
+// Y (h x b, ld h) = V op(T) on stream st, op(T) = T^T (trans) or T: the
+// left factor of the block reflector, I - V T V^T = I - (V T) V^T.
+int form_y(hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv, const double *Tp,
+           int64_t ldt, bool trans, double *Y) {
+    GemmF64 g;
+    g.M = (int)h; g.N = b; g.K = b;
+    g.tb = trans; g.amask = MASK_VLOWER;
+    g.A = Vp; g.lda = ldv; g.B = Tp; g.ldb = ldt;
+    g.C = Y; g.ldc = h; g.kind = K_W2;
+    return gemm_f64(g, st);
+}
+
+// C <- C - Y (V^T C) for the h x nc matrix C on stream st, Y from form_y:
+// applies (I - V T V^T)^T (Y = V T^T) or I - V T V^T (Y = V T) as in
+// QR/__Householder_Blocked.h:143-212.  P (pcap doubles) receives the split-K
+// partials of W = V^T C.
+int apply_block(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv,
+                const double *Y, double *C, int64_t ldc, int64_t nc, double *P, int64_t pcap) {
     if (nc <= 0) return MQR_SUCCESS;
     int splits = gemm_pick_splits(b, (int)nc, (int)h, target_blocks(hd));
     while (splits > 1 && (int64_t)splits * b * nc > pcap) --splits;
@@ -87,23 +100,22 @@ int apply_block(mqr_handle_t hd, int64_t h, int b, const double *Vp, int64_t ldv
     g.A = Vp; g.lda = ldv; g.B = C; g.ldb = ldc;
     g.C = P; g.ldc = b; g.alpha = 1.0; g.beta = 0.0;
     g.splits = splits; g.split_stride = (int64_t)b * nc; g.kind = K_VTA;
-    MQR_CHECK(gemm_f64(g, hd->stream));
-    MQR_CHECK(reduce_partials((int64_t)b * nc, P, splits, (int64_t)b * nc, hd->stream));
-    MQR_CHECK(block_w2(b, nc, Tp, ldt, trans, P, 1, (int64_t)b * nc, W2, hd->stream));
+    MQR_CHECK(gemm_f64(g, st));
+    MQR_CHECK(reduce_partials((int64_t)b * nc, P, splits, (int64_t)b * nc, st));
     GemmF64 u;
     u.M = (int)h; u.N = (int)nc; u.K = b;
-    u.amask = MASK_VLOWER;
-    u.A = Vp; u.lda = ldv; u.B = W2; u.ldb = b;
+    u.A = Y; u.lda = h; u.B = P; u.ldb = b;
     u.C = C; u.ldc = ldc; u.alpha = -1.0; u.beta = 1.0; u.kind = K_UPDATE;
-    return gemm_f64(u, hd->stream);
+    return gemm_f64(u, st);
 }
 
 }  // namespace
 
 // Workspace (doubles) the factorisation of an m x n matrix uses: split-K
-// partials, W2, and the tall-panel scratch.
+// partials of the trailing update and of the next-block update, two Y
+// factors, and the tall-panel scratch.
 static int64_t geqrf_ws_doubles(int64_t m, int64_t n, int nb) {
-    return (int64_t)16 * nb * (n + nb) + (int64_t)nb * (n + nb) + 1024 +
+    return (int64_t)16 * nb * (n + nb) + (int64_t)64 * nb * nb + (int64_t)2 * m * nb + 4096 +
            (int64_t)(panel_cholqr_ws_bytes(m) / sizeof(double)) + 64;
 }
 
@@ -113,26 +125,73 @@ int panel_factor(mqr_handle_t hd, int64_t h, int b, double *V, int64_t lda, doub
     return panel_householder(h, b, V, lda, tau, T, ldt, hd->stream);
 }
 
+// Blocked factorisation with a look-ahead of one block: after panel j, the
+// handle's stream applies block j to the next block's columns and factors
+// the next panel, while the side stream applies block j to the remaining
+// columns.  Events order the two: the side stream starts block j's update
+// once panel j is done, and the handle's stream waits for it before touching
+// the next block again.  Per step the cost is max(rest update, next update +
+// panel) instead of their sum.
+// Blocked factorisation with a look-ahead of one block: after panel j, the
+// handle's stream applies block j to the next block's columns and factors
+// the next panel, while the side stream applies block j to the remaining
+// columns.  Events order the two: the side stream starts block j's update
+// once panel j (and its Y) is done, and the handle's stream waits for it
+// before touching the next block again.  Per step the cost is
+// max(rest update, next update + panel) instead of their sum.
 int geqrf_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, double *tau,
               double *T, int64_t ldt) {
     const int64_t k = std::min(m, n);
     const int nb = hd->nb;
     const int64_t pcap = (int64_t)16 * nb * (n + nb);
+    const int64_t pcap2 = (int64_t)64 * nb * nb;
     MQR_CHECK(ws_reserve(hd, hd->ws, sizeof(double) * geqrf_ws_doubles(m, n, nb)));
     double *P = (double *)hd->ws.ptr;
-    double *W2 = P + pcap;
-    void *pws = (void *)(((uintptr_t)(W2 + (int64_t)nb * (n + nb) + 1024) + 255) & ~(uintptr_t)255);
-    for (int64_t j = 0; j < k; j += nb) {
+    double *P2 = P + pcap;
+    double *Ybuf[2] = {P2 + pcap2, P2 + pcap2 + m * nb};
+    void *pws = (void *)(((uintptr_t)(Ybuf[1] + m * nb + 1024) + 255) & ~(uintptr_t)255);
+    hipStream_t s0 = hd->stream, s1 = hd->side_stream;
+    if (k == 0) return MQR_SUCCESS;
+    // the side stream runs after everything queued on the handle's stream
+    MQR_HIP(hipEventRecord(hd->ev_fork, s0));
+    MQR_HIP(hipStreamWaitEvent(s1, hd->ev_fork, 0));
+    bool side_pending = false;
+    const int b0 = (int)std::min<int64_t>(nb, k);
+    MQR_CHECK(panel_factor(hd, m, b0, A, lda, tau, T, ldt, pws));
+    if (b0 < n) MQR_CHECK(form_y(s0, m, b0, A, lda, T, ldt, true, Ybuf[0]));
+    for (int64_t j = 0, it = 0; j < k; j += nb, ++it) {
         const int b = (int)std::min<int64_t>(nb, k - j);
         const int64_t h = m - j;
-        double *V = A + j + j * lda;
-        double *Tj = T + j * ldt;
-        MQR_CHECK(panel_factor(hd, h, b, V, lda, tau + j, Tj, ldt, pws));
-        // C <- (I - V T V^T)^T C for C = A[j:m, j+b:n]
-        const int64_t nt = n - j - b;
-        MQR_CHECK(apply_block(hd, h, b, V, lda, Tj, ldt, A + j + (j + b) * lda, lda, nt, true, P,
-                              pcap, W2));
+        const double *V = A + j + j * lda;
+        const double *Y = Ybuf[it & 1];
+        const int64_t next = j + b;
+        if (next >= n) break;
+        const int64_t bn = std::min<int64_t>(nb, n - next);
+        const int64_t rest0 = next + bn, nrest = n - rest0;
+        // block j-1's update of the remaining columns covers the next block
+        if (side_pending) MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
+        if (nrest > 0) {
+            MQR_HIP(hipEventRecord(hd->ev_fork, s0));
+            MQR_HIP(hipStreamWaitEvent(s1, hd->ev_fork, 0));
+            MQR_CHECK(apply_block(hd, s1, h, b, V, lda, Y, A + j + rest0 * lda, lda, nrest, P,
+                                  pcap));
+            MQR_HIP(hipEventRecord(hd->ev_join, s1));
+            side_pending = true;
+        } else {
+            side_pending = false;
+        }
+        MQR_CHECK(apply_block(hd, s0, h, b, V, lda, Y, A + j + next * lda, lda, bn, P2, pcap2));
+        if (next < k) {
+            const int bnext = (int)std::min<int64_t>(nb, k - next);
+            double *Vn = A + next + next * lda;
+            MQR_CHECK(panel_factor(hd, m - next, bnext, Vn, lda, tau + next, T + next * ldt, ldt,
+                                   pws));
+            if (next + bnext < n)
+                MQR_CHECK(form_y(s0, m - next, bnext, Vn, lda, T + next * ldt, ldt, true,
+                                 Ybuf[(it + 1) & 1]));
+        }
     }
+    if (side_pending) MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
     return MQR_SUCCESS;
 }
 
@@ -140,15 +199,16 @@ int ormqr_f64(mqr_handle_t hd, bool trans, int64_t m, int64_t nrhs, int64_t k, c
               int64_t ldvr, const double *T, int64_t ldt, double *C, int64_t ldc) {
     const int nb = hd->nb;
     const int64_t pcap = (int64_t)16 * nb * (nrhs + nb);
-    MQR_CHECK(ws_reserve(hd, hd->ws, sizeof(double) * (pcap + (int64_t)nb * (nrhs + nb) + 1024)));
+    MQR_CHECK(ws_reserve(hd, hd->ws, sizeof(double) * (pcap + (int64_t)m * nb + 1024)));
     double *P = (double *)hd->ws.ptr;
-    double *W2 = P + pcap;
+    double *Y = P + pcap;
     const int64_t nblk = (k + nb - 1) / nb;
     for (int64_t q = 0; q < nblk; ++q) {
         const int64_t j = trans ? q * nb : (nblk - 1 - q) * nb;
         const int b = (int)std::min<int64_t>(nb, k - j);
-        MQR_CHECK(apply_block(hd, m - j, b, VR + j + j * ldvr, ldvr, T + j * ldt, ldt, C + j, ldc,
-                              nrhs, trans, P, pcap, W2));
+        const double *V = VR + j + j * ldvr;
+        MQR_CHECK(form_y(hd->stream, m - j, b, V, ldvr, T + j * ldt, ldt, trans, Y));
+        MQR_CHECK(apply_block(hd, hd->stream, m - j, b, V, ldvr, Y, C + j, ldc, nrhs, P, pcap));
     }
     return MQR_SUCCESS;
 }
@@ -163,17 +223,19 @@ int orgqr_f64(mqr_handle_t hd, int64_t m, int64_t nq, int64_t k, const double *V
         MQR_LAUNCH_CHECK();
     }
     const int64_t pcap = (int64_t)16 * nb * (nq + nb);
-    MQR_CHECK(ws_reserve(hd, hd->ws, sizeof(double) * (pcap + (int64_t)nb * (nq + nb) + 1024)));
+    MQR_CHECK(ws_reserve(hd, hd->ws, sizeof(double) * (pcap + (int64_t)m * nb + 1024)));
     double *P = (double *)hd->ws.ptr;
-    double *W2 = P + pcap;
+    double *Y = P + pcap;
     const int64_t kk = std::min(k, nq);
     const int64_t nblk = (kk + nb - 1) / nb;
     for (int64_t q = nblk - 1; q >= 0; --q) {
         const int64_t j = q * nb;
         const int b = (int)std::min<int64_t>(nb, kk - j);
+        const double *V = VR + j + j * ldvr;
         // columns < j of Q[j:m, :] are still zero: skip them
-        MQR_CHECK(apply_block(hd, m - j, b, VR + j + j * ldvr, ldvr, T + j * ldt, ldt,
-                              Q + j + j * ldq, ldq, nq - j, false, P, pcap, W2));
+        MQR_CHECK(form_y(hd->stream, m - j, b, V, ldvr, T + j * ldt, ldt, false, Y));
+        MQR_CHECK(apply_block(hd, hd->stream, m - j, b, V, ldvr, Y, Q + j + j * ldq, ldq, nq - j,
+                              P, pcap));
     }
     return MQR_SUCCESS;
 }

@@ -62,30 +62,27 @@ __global__ void k_cast_v(int64_t h, int b, const double *__restrict__ V, int64_t
     }
 }
 
-// P[0:count] += P[s * stride + 0:count] (FP32 partials, FP64 sum)
-__global__ void k_reduce_f32(int64_t count, float *__restrict__ P, int splits, int64_t stride) {
-    const int64_t e = gidx();
-    if (e >= count) return;
-    double acc = 0.0;
-    for (int s = 0; s < splits; ++s) acc += (double)P[(int64_t)s * stride + e];
-    P[e] = (float)acc;
-}
-
-// W2 (f32, b x nc, ld b) = T^T W with W (f32, ld b), products in FP64
+// W2 (f32, b x nc, ld b) = T^T sum_s W_s with the split-K partials W_s (f32,
+// ld b, stride pstride) summed and multiplied in FP64.  T^T is staged so that
+// consecutive lanes (rows r) read consecutive LDS words.
 __global__ __launch_bounds__(256) void k_w2_f32(int b, int64_t nc, const double *__restrict__ T,
                                                 int64_t ldt, const float *__restrict__ W,
+                                                int splits, int64_t pstride,
                                                 float *__restrict__ W2) {
-    __shared__ double Ts[64 * 64];
+    __shared__ double Ts[64 * 64];  // Ts[l * b + r] = T(l, r)
     __shared__ double Ws[64 * 32];
     const int tid = threadIdx.x;
     const int64_t n0 = (int64_t)blockIdx.x * 32;
     for (int e = tid; e < b * b; e += 256) {
         const int r = e % b, c = e / b;
-        Ts[e] = r <= c ? T[r + (int64_t)c * ldt] : 0.0;
+        Ts[r * b + c] = r <= c ? T[r + (int64_t)c * ldt] : 0.0;
     }
     for (int e = tid; e < b * 32; e += 256) {
         const int64_t n = n0 + e / b;
-        Ws[e] = n < nc ? (double)W[e % b + n * b] : 0.0;
+        double acc = 0.0;
+        if (n < nc)
+            for (int s = 0; s < splits; ++s) acc += (double)W[(int64_t)s * pstride + e % b + n * b];
+        Ws[e] = acc;
     }
     __syncthreads();
     for (int e = tid; e < b * 32; e += 256) {
@@ -93,7 +90,7 @@ __global__ __launch_bounds__(256) void k_w2_f32(int b, int64_t nc, const double 
         const int64_t n = n0 + col;
         if (n >= nc) continue;
         double acc = 0.0;
-        for (int l = 0; l <= r; ++l) acc = fma(Ts[l + r * b], Ws[l + col * b], acc);
+        for (int l = 0; l <= r; ++l) acc = fma(Ts[l * b + r], Ws[l + col * b], acc);
         W2[r + n * b] = (float)acc;
     }
 }
@@ -210,7 +207,8 @@ __global__ __launch_bounds__(small::NT) void k_inv_diag_blocks(int64_t n,
                                                                int64_t ldr, double *__restrict__ X,
                                                                int64_t ldx) {
     __shared__ small::Lds S[small::N];
-    __shared__ double rb[2][small::N];
+    __shared__ small::RowBuf rb;
+    __shared__ double rd[small::N];
     const int64_t i0 = (int64_t)blockIdx.x * small::N;
     const int b = (int)std::min<int64_t>(small::N, n - i0);
     for (int e = threadIdx.x; e < small::N * small::N; e += small::NT) {
@@ -220,7 +218,7 @@ __global__ __launch_bounds__(small::NT) void k_inv_diag_blocks(int64_t n,
     }
     __syncthreads();
     double x[small::RK];
-    small::inv_upper(S, rb, x);
+    small::inv_upper(S, rb, rd, x);
     __syncthreads();
     small::to_lds(x, S);
     __syncthreads();
@@ -349,16 +347,11 @@ int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t 
         f.C = w.P32; f.ldc = b; f.splits = sp; f.split_stride = (int64_t)b * nt;
         f.kind = K_VTA;
         MQR_CHECK(gemm_f32(f, s));
-        if (sp > 1) {
-            ProfScope ps(K_OTHER, 0.0, 4.0 * b * nt * (sp + 1), s);
-            hipLaunchKernelGGL(k_reduce_f32, dim3((unsigned)(((int64_t)b * nt + 255) / 256)),
-                               dim3(256), 0, s, (int64_t)b * nt, w.P32, sp, (int64_t)b * nt);
-            MQR_LAUNCH_CHECK();
-        }
         {
-            ProfScope ps(K_W2, (double)b * b * nt, 8.0 * b * nt, s);
+            ProfScope ps(K_W2, (double)b * b * nt, 4.0 * b * nt * (sp + 1), s);
             hipLaunchKernelGGL(k_w2_f32, dim3((unsigned)((nt + 31) / 32)), dim3(256), 0, s, b, nt,
-                               (const double *)Tj, (int64_t)nb, (const float *)w.P32, w.W2);
+                               (const double *)Tj, (int64_t)nb, (const float *)w.P32, sp,
+                               (int64_t)b * nt, w.W2);
             MQR_LAUNCH_CHECK();
         }
         GemmF32 u;

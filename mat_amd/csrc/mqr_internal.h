@@ -40,9 +40,9 @@ struct Workspace {
 
 // Per-kernel-kind timing with HIP event pairs (enabled per handle).
 enum KernelKind {
-    K_PANEL = 0,      // panel reflectors
-    K_VTA = 1,        // split-K V^T [V | C] (Gram + W partials)
-    K_LARFT = 2,      // T recurrence
+    K_PANEL = 0,        // GEMMs and reductions inside the panel factorisation
+    K_VTA = 1,          // split-K W = V^T C partials
+    K_PANEL_SMALL = 2,  // single-workgroup panel kernels (b x b factors, column panel)
     K_W2 = 3,         // W2 = op(T) W
     K_UPDATE = 4,     // C -= V W2 (trailing update / Q formation)
     K_TRSM = 5,       // triangular solves
@@ -83,7 +83,15 @@ struct ProfScope {
 struct mqr_handle_st {
     int device = 0;
     hipStream_t own_stream = nullptr;
-    hipStream_t stream = nullptr;  // current stream (own or caller's)
+    // every kernel runs on `stream` (= own_stream, non-blocking); work is
+    // ordered after and before the caller's user_stream with ev_in / ev_out
+    hipStream_t stream = nullptr;
+    hipStream_t user_stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    // look-ahead: trailing updates run on side_stream, ordered against
+    // `stream` with ev_fork / ev_join; every call joins before returning
+    hipStream_t side_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int nb = 64;                   // factorisation block size
     int cu_count = 256;
     mqr::Workspace ws;             // general scratch
@@ -172,8 +180,5 @@ int panel_factor(mqr_handle_t hd, int64_t h, int b, double *V, int64_t lda, doub
 int reduce_partials(int64_t count, double *P, int splits, int64_t stride, hipStream_t s);
 // Largest block size the LDS-resident small kernels accept.
 constexpr int kMaxNb = 64;
-// W2 = op(T) * sum_s P_s  (P_s: b x nc, ld = b, stride pstride), trans: T^T.
-int block_w2(int b, int64_t nc, const double *T, int64_t ldt, bool trans, const double *P,
-             int splits, int64_t pstride, double *W2, hipStream_t s);
 
 }  // namespace mqr

@@ -1,12 +1,10 @@
-// Panel reflectors, compact-WY T and the T-product of the block reflector.
+// Column-by-column panel reflectors with their compact-WY T, and the split-K
+// partial sums of the block-reflector products.
 //
 //  k_panel_hh  : QR/__Householder_Blocked.h:62-84 -- b Householder reflectors
 //                of an h x b panel, one workgroup, wavefront reductions,
 //                followed by their compact-WY T (:86-103) from the Gram
 //                matrix V^T V.
-//  k_block_w2  : the middle product of the trailing update,
-//                W2 = T^T W (QR/__Householder_Blocked.h:180-190) or T W
-//                (Q formation, :299-308), with W summed from split-K partials.
 #include "mqr_internal.h"
 #include "small_dense.h"
 
@@ -31,7 +29,8 @@ __global__ __launch_bounds__(PNT) void k_panel_hh(int64_t h, int b, double *__re
     __shared__ double red[PNT / 64][CH];
     __shared__ double tau_s[small::N];
     __shared__ small::Lds S[small::N];
-    __shared__ double rbuf[2][small::N];
+    __shared__ small::RowBuf rbuf;
+    __shared__ double rd[small::N];
     if (gate && *gate == 0) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int r = 0; r < b; ++r) {
@@ -126,82 +125,32 @@ __global__ __launch_bounds__(PNT) void k_panel_hh(int64_t h, int b, double *__re
         for (int rr = 0; rr < small::N; ++rr) {
             const double vc = S[rr][c];
 #pragma unroll
-            for (int k = 0; k < small::RK; ++k) g[k] = fma(S[rr][4 * k + q], vc, g[k]);
+            for (int j = 0; j < small::RK; ++j) g[j] = fma(S[rr][small::drow(j, q)], vc, g[j]);
         }
     }
     __syncthreads();
     const double tc = c < b ? tau_s[c] : 1.0;
 #pragma unroll
-    for (int k = 0; k < small::RK; ++k) {
-        const int r = 4 * k + q;
+    for (int j = 0; j < small::RK; ++j) {
+        const int r = small::drow(j, q);
         const double tr = r < b ? tau_s[r] : 1.0;
         double v = 0.0;
-        if (r < c && c < b) v = (tr == 0.0 || tc == 0.0) ? 0.0 : g[k];
+        if (r < c && c < b) v = (tr == 0.0 || tc == 0.0) ? 0.0 : g[j];
         else if (r == c) v = tc != 0.0 ? 1.0 / tc : 1.0;
-        g[k] = v;
+        g[j] = v;
     }
     small::to_lds(g, S);
     __syncthreads();
-    small::inv_upper(S, rbuf, g);
+    small::inv_upper(S, rbuf, rd, g);
 #pragma unroll
-    for (int k = 0; k < small::RK; ++k) {
-        const int r = 4 * k + q;
+    for (int j = 0; j < small::RK; ++j) {
+        const int r = small::drow(j, q);
         if (r < b && c < b) {
-            double v = r <= c ? g[k] : 0.0;
+            double v = r <= c ? g[j] : 0.0;
             if (r == c && tc == 0.0) v = 0.0;
             T[r + (int64_t)c * ldt] = v;
         }
     }
-}
-
-constexpr int W2COLS = 32;
-
-// W2(:, n) = op(T) * sum_s P_s(:, n), for 32 columns per workgroup.
-__global__ __launch_bounds__(256) void k_block_w2(int b, int64_t nc, const double *__restrict__ T,
-                                                  int64_t ldt, int trans,
-                                                  const double *__restrict__ P, int splits,
-                                                  int64_t pstride, double *__restrict__ W2) {
-    extern __shared__ double sm[];
-    double *Ts = sm;          // b * b
-    double *Ws = sm + b * b;  // b * W2COLS
-    const int tid = threadIdx.x;
-    const int64_t n0 = (int64_t)blockIdx.x * W2COLS;
-    for (int e = tid; e < b * b; e += blockDim.x) {
-        const int r = e % b, c = e / b;
-        Ts[e] = r <= c ? T[r + (int64_t)c * ldt] : 0.0;
-    }
-    for (int e = tid; e < b * W2COLS; e += blockDim.x) {
-        const int r = e % b;
-        const int64_t n = n0 + e / b;
-        double acc = 0.0;
-        if (n < nc)
-            for (int s = 0; s < splits; ++s) acc += P[(int64_t)s * pstride + r + n * b];
-        Ws[e] = acc;
-    }
-    __syncthreads();
-    for (int e = tid; e < b * W2COLS; e += blockDim.x) {
-        const int r = e % b, col = e / b;
-        const int64_t n = n0 + col;
-        if (n >= nc) continue;
-        double acc = 0.0;
-        if (trans) {
-            for (int l = 0; l <= r; ++l) acc = fma(Ts[l + r * b], Ws[l + col * b], acc);
-        } else {
-            for (int l = r; l < b; ++l) acc = fma(Ts[r + l * b], Ws[l + col * b], acc);
-        }
-        W2[r + n * b] = acc;
-    }
-}
-
-bool g_lds_set = false;
-
-int ensure_lds_attrs() {
-    if (g_lds_set) return MQR_SUCCESS;
-    MQR_HIP(hipFuncSetAttribute((const void *)k_block_w2,
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (kMaxNb * kMaxNb + kMaxNb * W2COLS) * 8));
-    g_lds_set = true;
-    return MQR_SUCCESS;
 }
 
 }  // namespace
@@ -209,7 +158,7 @@ int ensure_lds_attrs() {
 int panel_householder(int64_t h, int b, double *P, int64_t ld, double *tau, double *T,
                       int64_t ldt, hipStream_t s) {
     const double hh = (double)h, bb = b;
-    ProfScope ps(K_PANEL, 3.0 * hh * bb * bb - 2.0 * bb * bb * bb / 3.0, 24.0 * hh * bb, s);
+    ProfScope ps(K_PANEL_SMALL, 3.0 * hh * bb * bb - 2.0 * bb * bb * bb / 3.0, 24.0 * hh * bb, s);
     hipLaunchKernelGGL(k_panel_hh, dim3(1), dim3(PNT), 0, s, h, b, P, ld, tau, T, ldt,
                        (const int *)nullptr);
     MQR_LAUNCH_CHECK();
@@ -246,19 +195,6 @@ int reduce_partials(int64_t count, double *P, int splits, int64_t stride, hipStr
     ProfScope ps(K_OTHER, (double)count * (splits - 1), 8.0 * count * (splits + 1), s);
     hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
                        count, P, splits, stride);
-    MQR_LAUNCH_CHECK();
-    return MQR_SUCCESS;
-}
-
-int block_w2(int b, int64_t nc, const double *T, int64_t ldt, bool trans, const double *P,
-             int splits, int64_t pstride, double *W2, hipStream_t s) {
-    if (nc <= 0) return MQR_SUCCESS;
-    MQR_CHECK(ensure_lds_attrs());
-    const size_t lds = (size_t)(b * b + b * W2COLS) * sizeof(double);
-    const unsigned blocks = (unsigned)((nc + W2COLS - 1) / W2COLS);
-    ProfScope ps(K_W2, (double)b * b * nc, 8.0 * b * nc * (splits + 1), s);
-    hipLaunchKernelGGL(k_block_w2, dim3(blocks), dim3(256), lds, s, b, nc, T, ldt, trans ? 1 : 0,
-                       P, splits, pstride, W2);
     MQR_LAUNCH_CHECK();
     return MQR_SUCCESS;
 }

@@ -43,7 +43,8 @@ __global__ __launch_bounds__(small::NT) void k_chol_inv(int b, int64_t h,
                                                         double *__restrict__ Rinv,
                                                         int *__restrict__ flag) {
     __shared__ small::Lds S[PB];
-    __shared__ double rb[2][PB];
+    __shared__ small::RowBuf rb;
+    __shared__ double dv[2 * PB];
     __shared__ double shift_s;
     __shared__ int fail;
     if (pass > 1 && *flag) return;
@@ -63,16 +64,16 @@ __global__ __launch_bounds__(small::NT) void k_chol_inv(int b, int64_t h,
     double a[small::RK];
     small::from_lds(a, S);
 #pragma unroll
-    for (int k = 0; k < small::RK; ++k)
-        if (4 * k + q == c && c < b) a[k] += shift;
+    for (int j = 0; j < small::RK; ++j)
+        if (small::drow(j, q) == c && c < b) a[j] += shift;
     // pass 1: a pivot the shift dominates marks a dependent column; later
     // passes see nearly orthonormal columns
-    small::chol(a, rb, b, pass == 1 ? 2.0 * shift : 1e-2, &fail);
+    small::chol(a, rb, dv, b, pass == 1 ? 2.0 * shift : 1e-2, &fail);
     small::to_lds(a, S);
     __syncthreads();
     small::lds_store(S, R, PB, PB, PB);
     double x[small::RK];
-    small::inv_upper(S, rb, x);
+    small::inv_upper(S, rb, dv, x);
     __syncthreads();
     small::to_lds(x, S);
     __syncthreads();
@@ -104,7 +105,8 @@ __global__ __launch_bounds__(small::NT) void k_recon(int b, const double *__rest
                                                      double *__restrict__ Minv,
                                                      int *__restrict__ flag) {
     __shared__ small::Lds S0[PB], S1[PB], S2[PB], S3[PB];
-    __shared__ double sg[PB], rb[2][PB];
+    __shared__ small::RowBuf rb;
+    __shared__ double sg[PB], dv[2 * PB];
     __shared__ int fail;
     if (*flag) return;
     const int tid = threadIdx.x, c = small::lane_id(), q = small::wave_id();
@@ -115,10 +117,10 @@ __global__ __launch_bounds__(small::NT) void k_recon(int b, const double *__rest
     double a[small::RK];
     // R3 = chol(G3) -> S2, R3^{-1} -> S1
     small::from_lds(a, S1);
-    small::chol(a, rb, b, 1e-2, &fail);
+    small::chol(a, rb, dv, b, 1e-2, &fail);
     small::to_lds(a, S2);
     __syncthreads();
-    small::inv_upper(S2, rb, a);
+    small::inv_upper(S2, rb, dv, a);
     __syncthreads();  // S1 (G3) no longer read
     small::to_lds(a, S1);
     __syncthreads();
@@ -127,13 +129,13 @@ __global__ __launch_bounds__(small::NT) void k_recon(int b, const double *__rest
     __syncthreads();
     // LU of Q1 - S -> S3 (L strictly below, U on and above); U -> S0
     small::from_lds(a, S3);
-    small::lu_sign(a, rb, sg);
+    small::lu_sign(a, rb, dv, sg);
     small::to_lds(a, S3);
     small::keep_upper(a);
     small::to_lds(a, S0);
     __syncthreads();
     // U^{-1} -> S0
-    small::inv_upper(S0, rb, a);
+    small::inv_upper(S0, rb, dv, a);
     __syncthreads();
     small::to_lds(a, S0);
     __syncthreads();
@@ -141,9 +143,9 @@ __global__ __launch_bounds__(small::NT) void k_recon(int b, const double *__rest
     small::matmul(S1, S0, true, true, [&](int r, int cc, double v) { Minv[r + cc * PB] = v; });
     // T^T = L^{-1} (-S U^T): B(r, c) = -sg(r) U(c, r) for c <= r
 #pragma unroll
-    for (int k = 0; k < small::RK; ++k) {
-        const int r = 4 * k + q;
-        a[k] = (c <= r) ? -sg[r] * S3[c][r] : 0.0;
+    for (int j = 0; j < small::RK; ++j) {
+        const int r = small::drow(j, q);
+        a[j] = (c <= r) ? -sg[r] * S3[c][r] : 0.0;
     }
     small::solve_unit_lower(S3, rb, a);
     __syncthreads();  // S0 (U^{-1}) and S1 (R3^{-1}) no longer read
@@ -217,21 +219,21 @@ int panel_cholqr(mqr_handle_t hd, int64_t h, int b, double *X, int64_t ld, doubl
     // pass 1 (shifted)
     MQR_CHECK(gram(X, ld));
     {
-        ProfScope ps(K_PANEL, small_flops, small_bytes, s);
+        ProfScope ps(K_PANEL_SMALL, small_flops, small_bytes, s);
         hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(small::NT), 0, s, b, h, Gp, 1, R1, I1, flag);
     }
     MQR_CHECK(apply(h, X, ld, I1, Y, h, nullptr));
     // pass 2
     MQR_CHECK(gram(Y, h));
     {
-        ProfScope ps(K_PANEL, small_flops, small_bytes, s);
+        ProfScope ps(K_PANEL_SMALL, small_flops, small_bytes, s);
         hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(small::NT), 0, s, b, h, Gp, 2, R2, I2, flag);
     }
     MQR_CHECK(apply(h, Y, h, I2, Z, h, nullptr));
     // pass 3 + reconstruction
     MQR_CHECK(gram(Z, h));
     {
-        ProfScope ps(K_PANEL, 4.0 * small_flops, small_bytes, s);
+        ProfScope ps(K_PANEL_SMALL, 4.0 * small_flops, small_bytes, s);
         hipLaunchKernelGGL(k_recon, dim3(1), dim3(small::NT), 0, s, b, Gp, Z, h, R1, R2, X, ld,
                            tau, T, ldt, Mi, flag);
     }

@@ -8,7 +8,7 @@ namespace mqr {
 thread_local Profiler *t_prof = nullptr;
 
 const char *kernel_kind_name(int k) {
-    static const char *names[K_COUNT] = {"panel", "vta_gemm", "larft", "w2",    "update_gemm",
+    static const char *names[K_COUNT] = {"panel_gemm", "vta_gemm", "panel_small", "w2",    "update_gemm",
                                          "trsm",  "gemm",     "resid", "cast",  "other"};
     return (k >= 0 && k < K_COUNT) ? names[k] : "?";
 }

@@ -2,19 +2,23 @@
 // of 256 threads: Cholesky, LU with sign choice, triangular inverse and
 // substitution, and products.
 //
-// Distributed layout: thread t holds column c = t & 63 of the rows
-// r = 4 k + q, k = 0..15, q = t >> 6 (its wave), in a[k].  Sixteen registers
-// per matrix, so several matrices stay resident without spilling.  An
-// elimination step i is done by the wave that owns row i (q = i & 3, a
-// compile-time register index k = i >> 2 once the loop is unrolled): it
-// finishes row i and publishes it through a 64-double LDS buffer; after one
-// barrier every wave updates its own rows.  The buffer alternates between
-// two slots, so one barrier per step suffices: a slot is rewritten two steps
-// later, after every wave has passed the barrier that follows its last read.
-// All barriers are reached by every thread of the block.
+// Distributed layout: thread t holds column c = t & 63 of sixteen rows; its
+// wave q = t >> 6 owns the row blocks {4 m .. 4 m + 3 : m = 4 k + q}, i.e.
+// register j holds row 16 (j >> 2) + 4 q + (j & 3).  Sixteen registers per
+// matrix, so several matrices stay resident without spilling.
 //
-// Products of two LDS-resident matrices run on the FP64 MFMA
-// (v_mfma_f64_16x16x4f64): wave q computes output rows 16 q .. 16 q + 15.
+// The eliminations proceed by blocks of four rows.  The wave owning block m
+// eliminates inside it with v_readlane broadcasts only (no barrier), then
+// publishes the four finished rows through LDS; after one barrier every wave
+// applies the rank-4 update to its own later rows.  The LDS row buffer
+// alternates between two slots, so one barrier per block suffices: a slot is
+// rewritten two blocks later, after every wave has passed the barrier that
+// follows its last read.  All barriers are reached by every thread.
+//
+// Only reciprocals sit on the sequential path: the Cholesky factor is
+// computed in LDL^T form (pivots d_i, multipliers a_ir / d_i) and scaled by
+// d_i^{-1/2} once at the end, and reciprocals are v_rcp_f64 refined by two
+// Newton steps.  Products of two LDS-resident matrices run on the FP64 MFMA.
 #pragma once
 
 #include "mqr_internal.h"
@@ -25,12 +29,16 @@ namespace small {
 constexpr int N = 64;    // matrix order (smaller problems are padded with I)
 constexpr int NT = 256;  // threads of the workgroup
 constexpr int RK = 16;   // rows held per thread
-using Lds = double[N + 1];  // row of an LDS matrix S[r][c]
+constexpr int BR = 4;    // rows per elimination block
+using Lds = double[N + 1];      // row of an LDS matrix S[r][c]
+using RowBuf = double[2][BR][N];  // double-buffered block rows
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 __device__ __forceinline__ int wave_id() { return threadIdx.x >> 6; }
+// row held in register j by wave q
+__device__ __forceinline__ int drow(int j, int q) { return 16 * (j >> 2) + 4 * q + (j & 3); }
 
 // value of v in lane `src` of the calling wave
 __device__ __forceinline__ double bcast(double v, int src) {
@@ -43,6 +51,15 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
+}
+
+// 1 / d: hardware estimate and two Newton steps (relative error ~ u)
+__device__ __forceinline__ double recip(double d) {
+    double y = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-d, y, 1.0);
+    return fma(y, e, y);
 }
 
 // S = M(0:b, 0:b) (column-major, ld), identity outside; coalesced reads.
@@ -65,113 +82,203 @@ __device__ __forceinline__ void lds_store(Lds *S, double *__restrict__ M, int64_
 __device__ __forceinline__ void from_lds(double (&a)[RK], Lds *S) {
     const int c = lane_id(), q = wave_id();
 #pragma unroll
-    for (int k = 0; k < RK; ++k) a[k] = S[4 * k + q][c];
+    for (int j = 0; j < RK; ++j) a[j] = S[drow(j, q)][c];
 }
 
 __device__ __forceinline__ void to_lds(const double (&a)[RK], Lds *S) {
     const int c = lane_id(), q = wave_id();
 #pragma unroll
-    for (int k = 0; k < RK; ++k) S[4 * k + q][c] = a[k];
+    for (int j = 0; j < RK; ++j) S[drow(j, q)][c] = a[j];
 }
 
 __device__ __forceinline__ void set_identity(double (&a)[RK]) {
     const int c = lane_id(), q = wave_id();
 #pragma unroll
-    for (int k = 0; k < RK; ++k) a[k] = (4 * k + q == c) ? 1.0 : 0.0;
+    for (int j = 0; j < RK; ++j) a[j] = (drow(j, q) == c) ? 1.0 : 0.0;
 }
 
 __device__ __forceinline__ void keep_upper(double (&a)[RK]) {
     const int c = lane_id(), q = wave_id();
 #pragma unroll
-    for (int k = 0; k < RK; ++k) a[k] = (4 * k + q <= c) ? a[k] : 0.0;
+    for (int j = 0; j < RK; ++j) a[j] = (drow(j, q) <= c) ? a[j] : 0.0;
 }
 
-// Upper Cholesky factor R (G = R^T R) in place.  Rows below the diagonal
-// are updated as well (their values are never read) and zeroed at the end.
-// *fail (LDS) is set when one of the first b pivots is <= thr or NaN.
-__device__ __forceinline__ void chol(double (&a)[RK], double (*rb)[N], int b, double thr,
+// Upper Cholesky factor R (G = R^T R) in place.  Entries below the diagonal
+// are updated as well (never read) and zeroed at the end.  *fail (LDS) is set
+// when one of the first b pivots is <= thr or NaN.  dv: 2 N doubles of LDS.
+__device__ __forceinline__ void chol(double (&a)[RK], RowBuf &rb, double *dv, int b, double thr,
                                      int *fail) {
     const int c = lane_id(), q = wave_id();
+    double *dinv = dv, *dval = dv + N;
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-        double *buf = rb[i & 1];
-        if (q == (i & 3)) {
-            const double d = bcast(a[i >> 2], i);
-            if (c == 0 && i < b && !(d > thr)) *fail = 1;
-            const double rii = sqrt(fmax(d, 0.0));
-            const double rc = (c == i) ? rii : a[i >> 2] * (1.0 / rii);
-            a[i >> 2] = rc;
-            buf[c] = rc;
+    for (int m = 0; m < N / BR; ++m) {
+        const int j0 = 4 * (m >> 2), i0 = BR * m;
+        double(*raw)[N] = rb[m & 1];
+        if (q == (m & 3)) {
+#pragma unroll
+            for (int s = 0; s < BR; ++s) {
+                const int i = i0 + s;
+                const double d = bcast(a[j0 + s], i);
+                const double di = recip(d);
+                if (c == 0) {
+                    if (i < b && !(d > thr)) *fail = 1;
+                    dinv[i] = di;
+                    dval[i] = d;
+                }
+#pragma unroll
+                for (int t = s + 1; t < BR; ++t) {
+                    const double l = bcast(a[j0 + s], i0 + t) * di;
+                    a[j0 + t] = fma(-l, a[j0 + s], a[j0 + t]);
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < BR; ++s) raw[s][c] = a[j0 + s];
         }
         __syncthreads();
-        const double rcv = buf[c];
+        double u[BR], w[BR];
 #pragma unroll
-        for (int k = 0; k < RK; ++k)
-            if (4 * k + q > i) a[k] = fma(-buf[4 * k + q], rcv, a[k]);
+        for (int s = 0; s < BR; ++s) {
+            u[s] = raw[s][c];
+            w[s] = dinv[i0 + s];
+        }
+#pragma unroll
+        for (int j = 0; j < RK; ++j) {
+            const int r = drow(j, q);
+            if (r >= i0 + BR) {
+#pragma unroll
+                for (int s = 0; s < BR; ++s) a[j] = fma(-(raw[s][r] * w[s]), u[s], a[j]);
+            }
+        }
     }
-    keep_upper(a);
+    __syncthreads();
+    if (threadIdx.x < N) dinv[threadIdx.x] = 1.0 / sqrt(fmax(dval[threadIdx.x], 0.0));
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RK; ++j) {
+        const int r = drow(j, q);
+        a[j] = (r <= c) ? a[j] * dinv[r] : 0.0;
+    }
 }
 
-// x = R^{-1} for the upper-triangular R held in S (read only).
-__device__ __forceinline__ void inv_upper(Lds *S, double (*rb)[N], double (&x)[RK]) {
+// x = R^{-1} for the upper-triangular R held in S (read only).  rd: N
+// doubles of LDS.
+__device__ __forceinline__ void inv_upper(Lds *S, RowBuf &rb, double *rd, double (&x)[RK]) {
     const int c = lane_id(), q = wave_id();
+    if (threadIdx.x < N) rd[threadIdx.x] = 1.0 / S[threadIdx.x][threadIdx.x];
     set_identity(x);
+    __syncthreads();
 #pragma unroll
-    for (int i = N - 1; i >= 0; --i) {
-        double *buf = rb[i & 1];
-        if (q == (i & 3)) {
-            const double xi = x[i >> 2] * (1.0 / S[i][i]);
-            x[i >> 2] = xi;
-            buf[c] = xi;
+    for (int m = N / BR - 1; m >= 0; --m) {
+        const int j0 = 4 * (m >> 2), i0 = BR * m;
+        double(*buf)[N] = rb[m & 1];
+        if (q == (m & 3)) {
+#pragma unroll
+            for (int s = BR - 1; s >= 0; --s) {
+                x[j0 + s] *= rd[i0 + s];
+#pragma unroll
+                for (int t = 0; t < s; ++t) x[j0 + t] = fma(-S[i0 + t][i0 + s], x[j0 + s], x[j0 + t]);
+            }
+#pragma unroll
+            for (int s = 0; s < BR; ++s) buf[s][c] = x[j0 + s];
         }
         __syncthreads();
-        const double v = buf[c];
+        double v[BR];
 #pragma unroll
-        for (int k = 0; k < RK; ++k)
-            if (4 * k + q < i) x[k] = fma(-S[4 * k + q][i], v, x[k]);
+        for (int s = 0; s < BR; ++s) v[s] = buf[s][c];
+#pragma unroll
+        for (int j = 0; j < RK; ++j) {
+            const int r = drow(j, q);
+            if (r < i0) {
+#pragma unroll
+                for (int s = 0; s < BR; ++s) x[j] = fma(-S[r][i0 + s], v[s], x[j]);
+            }
+        }
     }
 }
 
 // y <- L^{-1} y for the unit lower-triangular L stored strictly below the
 // diagonal of S (read only).
-__device__ __forceinline__ void solve_unit_lower(Lds *S, double (*rb)[N], double (&y)[RK]) {
+__device__ __forceinline__ void solve_unit_lower(Lds *S, RowBuf &rb, double (&y)[RK]) {
     const int c = lane_id(), q = wave_id();
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-        double *buf = rb[i & 1];
-        if (q == (i & 3)) buf[c] = y[i >> 2];
-        __syncthreads();
-        const double v = buf[c];
+    for (int m = 0; m < N / BR; ++m) {
+        const int j0 = 4 * (m >> 2), i0 = BR * m;
+        double(*buf)[N] = rb[m & 1];
+        if (q == (m & 3)) {
 #pragma unroll
-        for (int k = 0; k < RK; ++k)
-            if (4 * k + q > i) y[k] = fma(-S[4 * k + q][i], v, y[k]);
+            for (int s = 0; s < BR; ++s)
+#pragma unroll
+                for (int t = s + 1; t < BR; ++t)
+                    y[j0 + t] = fma(-S[i0 + t][i0 + s], y[j0 + s], y[j0 + t]);
+#pragma unroll
+            for (int s = 0; s < BR; ++s) buf[s][c] = y[j0 + s];
+        }
+        __syncthreads();
+        double v[BR];
+#pragma unroll
+        for (int s = 0; s < BR; ++s) v[s] = buf[s][c];
+#pragma unroll
+        for (int j = 0; j < RK; ++j) {
+            const int r = drow(j, q);
+            if (r >= i0 + BR) {
+#pragma unroll
+                for (int s = 0; s < BR; ++s) y[j] = fma(-S[r][i0 + s], v[s], y[j]);
+            }
+        }
     }
 }
 
 // LU without pivoting of M - S with S_j = -sign(current diagonal),
 // sign(0) = +: on exit a holds L (unit, strictly below) and U (on and
-// above); sg[j] (LDS) the signs.  |U_jj| >= 1 by the sign choice.
-__device__ __forceinline__ void lu_sign(double (&a)[RK], double (*rb)[N], double *sg) {
+// above); sg[j] (LDS) the signs.  |U_jj| >= 1 by the sign choice.  ip: N
+// doubles of LDS.
+__device__ __forceinline__ void lu_sign(double (&a)[RK], RowBuf &rb, double *ip, double *sg) {
     const int c = lane_id(), q = wave_id();
 #pragma unroll
-    for (int j = 0; j < N; ++j) {
-        double *buf = rb[j & 1];
-        if (q == (j & 3)) {
-            const double ajj = bcast(a[j >> 2], j);
-            const double sj = ajj >= 0.0 ? -1.0 : 1.0;
-            if (c == j) a[j >> 2] = ajj - sj;
-            buf[c] = a[j >> 2];
-            if (c == 0) sg[j] = sj;
+    for (int m = 0; m < N / BR; ++m) {
+        const int j0 = 4 * (m >> 2), i0 = BR * m;
+        double(*buf)[N] = rb[m & 1];
+        if (q == (m & 3)) {
+#pragma unroll
+            for (int s = 0; s < BR; ++s) {
+                const int jp = i0 + s;
+                const double ajj = bcast(a[j0 + s], jp);
+                const double sj = ajj >= 0.0 ? -1.0 : 1.0;
+                const double piv = ajj - sj;
+                const double ipv = recip(piv);
+                if (c == jp) a[j0 + s] = piv;
+                if (c == 0) {
+                    sg[jp] = sj;
+                    ip[jp] = ipv;
+                }
+#pragma unroll
+                for (int t = s + 1; t < BR; ++t) {
+                    const double l = bcast(a[j0 + t], jp) * ipv;
+                    a[j0 + t] = (c > jp) ? fma(-l, a[j0 + s], a[j0 + t]) : ((c == jp) ? l : a[j0 + t]);
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < BR; ++s) buf[s][c] = a[j0 + s];
         }
         __syncthreads();
-        const double u = buf[c];
-        const double ip = 1.0 / buf[j];
+        double u[BR], w[BR];
 #pragma unroll
-        for (int k = 0; k < RK; ++k)
-            if (4 * k + q > j) {
-                const double l = bcast(a[k], j) * ip;  // L(r, j)
-                a[k] = (c > j) ? fma(-l, u, a[k]) : ((c == j) ? l : a[k]);
+        for (int s = 0; s < BR; ++s) {
+            u[s] = buf[s][c];
+            w[s] = ip[i0 + s];
+        }
+#pragma unroll
+        for (int j = 0; j < RK; ++j) {
+            const int r = drow(j, q);
+            if (r >= i0 + BR) {
+#pragma unroll
+                for (int s = 0; s < BR; ++s) {
+                    const int jp = i0 + s;
+                    const double l = bcast(a[j], jp) * w[s];  // L(r, jp)
+                    a[j] = (c > jp) ? fma(-l, u[s], a[j]) : ((c == jp) ? l : a[j]);
+                }
             }
+        }
     }
 }
 
@@ -186,11 +293,11 @@ __device__ __forceinline__ void matmul(Lds *SA, Lds *SB, bool a_upper, bool b_up
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[j] = (v4d){0.0, 0.0, 0.0, 0.0};
     const int kbeg = a_upper ? r0 : 0;
+    for (int kk = kbeg; kk < N; kk += 4) {
+        const double av = SA[r0 + (l & 15)][kk + (l >> 4)];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int kend = b_upper ? 16 * j + 16 : N;
-        for (int kk = kbeg; kk < kend; kk += 4) {
-            const double av = SA[r0 + (l & 15)][kk + (l >> 4)];
+        for (int j = 0; j < 4; ++j) {
+            if (b_upper && kk >= 16 * j + 16) continue;
             const double bv = SB[kk + (l >> 4)][16 * j + (l & 15)];
             // swapped operands: lane l, register r holds C(r0 + (l & 15),
             // 16 j + (l >> 4) + 4 r)
