@@ -85,16 +85,36 @@ int mqr_memcpy_dtoh(mqr_handle_t handle, void *dst, const void *src, size_t byte
 
 /* ---- kernel timing (no reference counterpart; the reference has no profiler) */
 typedef struct {
-    char name[24];    /* kernel kind: panel, vta_gemm, larft, w2, update_gemm, ... */
+    char name[24];    /* kernel kind: panel_gemm, vta_gemm, panel_small, w2, update_gemm, ... */
     int64_t launches; /* launches timed */
     double total_ms;  /* sum of HIP-event durations */
     double flops;     /* algorithmic flops of those launches */
     double bytes;     /* algorithmic HBM bytes of those launches */
 } mqr_kernel_stat_t;
 
+/* Kernel kinds, in the order of their bit in the mask below. */
+typedef enum {
+    MQR_KIND_PANEL_GEMM = 0,  /* GEMMs and reductions inside the panel */
+    MQR_KIND_VTA_GEMM = 1,    /* split-K W = V^T C */
+    MQR_KIND_PANEL_SMALL = 2, /* single-workgroup panel kernels */
+    MQR_KIND_W2 = 3,          /* Y = V T^T and other T products */
+    MQR_KIND_UPDATE_GEMM = 4, /* trailing update C -= Y W */
+    MQR_KIND_TRSM = 5,        /* triangular solves and inverses */
+    MQR_KIND_GEMM = 6,        /* mqr_dgemm / mqr_sgemm */
+    MQR_KIND_RESID = 7,       /* refinement residual passes */
+    MQR_KIND_CAST = 8,        /* precision conversions */
+    MQR_KIND_OTHER = 9,
+    MQR_KIND_LOOKAHEAD = 10,  /* next-block update on the critical stream */
+    MQR_KIND_APPLY_Q = 11     /* Q / Q^T applications (ormqr, orgqr) */
+} mqr_kernel_kind_t;
+
 /* Turn per-launch HIP event timing on or off for calls on this handle
  * (clears what was recorded). */
 int mqr_profile_enable(mqr_handle_t handle, int on);
+/* Time only the kinds whose bit (1u << kind) is set (default: all).  Two
+ * events per timed launch add to the stream's dispatch gaps; timing one kind
+ * keeps that overhead small. */
+int mqr_profile_set_mask(mqr_handle_t handle, unsigned mask);
 /* Synchronise, then return one entry per kernel kind timed since the last
  * read or enable (at most max_stats are written; *count gets the number of
  * kinds), and clear the records. */

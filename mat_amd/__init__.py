@@ -53,6 +53,7 @@ SIGNATURES = {
     "mqr_memcpy_dtoh": (_i32, [_vp, _vp, _vp, C.c_size_t]),
     "mqr_profile_enable": (_i32, [_vp, _i32]),
     "mqr_profile_read": (_i32, [_vp, _vp, _i32, _vp]),
+    "mqr_profile_set_mask": (_i32, [_vp, C.c_uint]),
     "mqr_dgemm": (_i32, [_vp, C.c_char, C.c_char, _i64, _i64, _i64, _dbl, _vp, _i64, _vp, _i64,
                          _dbl, _vp, _i64]),
     "mqr_sgemm": (_i32, [_vp, C.c_char, C.c_char, _i64, _i64, _i64, C.c_float, _vp, _i64, _vp,
@@ -82,7 +83,15 @@ class KernelStat(C.Structure):
                 ("flops", C.c_double), ("bytes", C.c_double)]
 
 
-def profile_enable(h: "Handle", on: bool = True) -> None:
+# kernel kinds of the profiler, in the order of their mask bit (mqr_kernel_kind_t)
+KINDS = ("panel_gemm", "vta_gemm", "panel_small", "w2", "update_gemm", "trsm", "gemm", "resid",
+         "cast", "other", "lookahead", "apply_q")
+
+
+def profile_enable(h: "Handle", on: bool = True, kinds=None) -> None:
+    """Turn HIP-event timing on or off; `kinds` restricts it to some kinds."""
+    mask = 0xFFFFFFFF if kinds is None else sum(1 << KINDS.index(k) for k in kinds)
+    check(lib().mqr_profile_set_mask(h.h, mask), "mqr_profile_set_mask")
     check(lib().mqr_profile_enable(h.h, 1 if on else 0), "mqr_profile_enable")
 
 

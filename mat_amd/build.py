@@ -17,6 +17,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib", "libmatqr.so")
+# the C++ drop-in test of include/mat/Mat.h, linked against LIB
+CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_mat_api.cpp")
+CPP_TEST_BIN = os.path.join(HERE, "lib", "test_mat_api")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MQR_OFFLOAD_ARCH", "gfx950")
 
@@ -61,7 +64,21 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_cpp_test(verbose)
     return LIB
+
+
+def build_cpp_test(verbose: bool = False) -> str:
+    """Host-only C++ program (g++, no HIP headers) using include/mat/Mat.h."""
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"),
+           CPP_TEST_SRC, "-o", CPP_TEST_BIN, "-L", os.path.dirname(LIB), "-lmatqr",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for {CPP_TEST_SRC}:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {CPP_TEST_BIN}")
+    return CPP_TEST_BIN
 
 
 if __name__ == "__main__":

@@ -90,7 +90,8 @@ int form_y(hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv, cons
 // QR/__Householder_Blocked.h:143-212.  P (pcap doubles) receives the split-K
 // partials of W = V^T C.
 int apply_block(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv,
-                const double *Y, double *C, int64_t ldc, int64_t nc, double *P, int64_t pcap) {
+                const double *Y, double *C, int64_t ldc, int64_t nc, double *P, int64_t pcap,
+                int kind_w = K_VTA, int kind_u = K_UPDATE) {
     if (nc <= 0) return MQR_SUCCESS;
     int splits = gemm_pick_splits(b, (int)nc, (int)h, target_blocks(hd));
     while (splits > 1 && (int64_t)splits * b * nc > pcap) --splits;
@@ -99,13 +100,13 @@ int apply_block(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double 
     g.ta = true; g.amask = MASK_VLOWER;
     g.A = Vp; g.lda = ldv; g.B = C; g.ldb = ldc;
     g.C = P; g.ldc = b; g.alpha = 1.0; g.beta = 0.0;
-    g.splits = splits; g.split_stride = (int64_t)b * nc; g.kind = K_VTA;
+    g.splits = splits; g.split_stride = (int64_t)b * nc; g.kind = kind_w;
     MQR_CHECK(gemm_f64(g, st));
     MQR_CHECK(reduce_partials((int64_t)b * nc, P, splits, (int64_t)b * nc, st));
     GemmF64 u;
     u.M = (int)h; u.N = (int)nc; u.K = b;
     u.A = Y; u.lda = h; u.B = P; u.ldb = b;
-    u.C = C; u.ldc = ldc; u.alpha = -1.0; u.beta = 1.0; u.kind = K_UPDATE;
+    u.C = C; u.ldc = ldc; u.alpha = -1.0; u.beta = 1.0; u.kind = kind_u;
     return gemm_f64(u, st);
 }
 
@@ -180,7 +181,8 @@ int geqrf_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, dou
         } else {
             side_pending = false;
         }
-        MQR_CHECK(apply_block(hd, s0, h, b, V, lda, Y, A + j + next * lda, lda, bn, P2, pcap2));
+        MQR_CHECK(apply_block(hd, s0, h, b, V, lda, Y, A + j + next * lda, lda, bn, P2, pcap2,
+                              K_LOOKAHEAD, K_LOOKAHEAD));
         if (next < k) {
             const int bnext = (int)std::min<int64_t>(nb, k - next);
             double *Vn = A + next + next * lda;
@@ -208,7 +210,8 @@ int ormqr_f64(mqr_handle_t hd, bool trans, int64_t m, int64_t nrhs, int64_t k, c
         const int b = (int)std::min<int64_t>(nb, k - j);
         const double *V = VR + j + j * ldvr;
         MQR_CHECK(form_y(hd->stream, m - j, b, V, ldvr, T + j * ldt, ldt, trans, Y));
-        MQR_CHECK(apply_block(hd, hd->stream, m - j, b, V, ldvr, Y, C + j, ldc, nrhs, P, pcap));
+        MQR_CHECK(apply_block(hd, hd->stream, m - j, b, V, ldvr, Y, C + j, ldc, nrhs, P, pcap,
+                              K_APPLYQ, K_APPLYQ));
     }
     return MQR_SUCCESS;
 }
@@ -235,7 +238,7 @@ int orgqr_f64(mqr_handle_t hd, int64_t m, int64_t nq, int64_t k, const double *V
         // columns < j of Q[j:m, :] are still zero: skip them
         MQR_CHECK(form_y(hd->stream, m - j, b, V, ldvr, T + j * ldt, ldt, false, Y));
         MQR_CHECK(apply_block(hd, hd->stream, m - j, b, V, ldvr, Y, Q + j + j * ldq, ldq, nq - j,
-                              P, pcap));
+                              P, pcap, K_APPLYQ, K_APPLYQ));
     }
     return MQR_SUCCESS;
 }

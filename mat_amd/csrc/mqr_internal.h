@@ -50,12 +50,15 @@ enum KernelKind {
     K_RESID = 7,      // refinement residual passes
     K_CAST = 8,       // precision conversions
     K_OTHER = 9,
-    K_COUNT = 10
+    K_LOOKAHEAD = 10,   // next-block update on the critical stream
+    K_APPLYQ = 11,      // Q / Q^T applications (ormqr, orgqr)
+    K_COUNT = 12
 };
 const char *kernel_kind_name(int k);
 
 struct Profiler {
     bool on = false;
+    unsigned mask = ~0u;  // kernel kinds timed (bit k = KernelKind k)
     struct Rec {
         int kind;
         double flops, bytes;

@@ -8,14 +8,15 @@ namespace mqr {
 thread_local Profiler *t_prof = nullptr;
 
 const char *kernel_kind_name(int k) {
-    static const char *names[K_COUNT] = {"panel_gemm", "vta_gemm", "panel_small", "w2",    "update_gemm",
-                                         "trsm",  "gemm",     "resid", "cast",  "other"};
+    static const char *names[K_COUNT] = {"panel_gemm", "vta_gemm", "panel_small", "w2",
+                                         "update_gemm", "trsm", "gemm", "resid",
+                                         "cast", "other", "lookahead", "apply_q"};
     return (k >= 0 && k < K_COUNT) ? names[k] : "?";
 }
 
 ProfScope::ProfScope(int kind, double flops, double bytes, hipStream_t stream)
     : p(t_prof), idx(0), s(stream) {
-    if (!p || !p->on) {
+    if (!p || !p->on || !((p->mask >> kind) & 1u)) {
         p = nullptr;
         return;
     }
@@ -58,11 +59,18 @@ int mqr_profile_enable(mqr_handle_t h, int on) {
     return MQR_SUCCESS;
 }
 
+int mqr_profile_set_mask(mqr_handle_t h, unsigned mask) {
+    if (!h) return -1;
+    h->prof.mask = mask;
+    return MQR_SUCCESS;
+}
+
 int mqr_profile_read(mqr_handle_t h, mqr_kernel_stat_t *stats, int max_stats, int *count) {
     if (!h) return -1;
     if (max_stats > 0 && !stats) return -2;
     if (!count) return -4;
     MQR_HIP(hipStreamSynchronize(h->stream));
+    MQR_HIP(hipStreamSynchronize(h->side_stream));
     mqr_kernel_stat_t acc[K_COUNT];
     std::memset(acc, 0, sizeof(acc));
     for (const auto &r : h->prof.recs) {

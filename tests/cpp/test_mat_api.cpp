@@ -1,0 +1,162 @@
+// Drop-in check of include/mat/Mat.h: the reference's QR(A, Q, R, QR_Householder)
+// call shapes (QR/QR.h:7-71, Mat.h:132-136) on MAT<double> and MAT<float>,
+// in place and out of place, and LeastSquares.  Prints one line per check and
+// exits non-zero on the first failure.  Inputs: mt19937_64 with a Box-Muller
+// normal (bit-identical across standard libraries).
+#include <cmath>
+#include <cstdio>
+#include <cstdint>
+#include <random>
+#include <vector>
+
+#include "mat/Mat.h"
+
+static int failures = 0;
+
+static void expect(bool ok, const char *what, double value, double bound) {
+    std::printf("%-48s %.3e (bound %.1e) %s\n", what, value, bound, ok ? "ok" : "FAIL");
+    std::fflush(stdout);
+    if (!ok) ++failures;
+}
+
+struct Normal {
+    std::mt19937_64 g;
+    explicit Normal(uint64_t seed) : g(seed) {}
+    double operator()() {
+        const double u1 = ((g() >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        const double u2 = ((g() >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    }
+};
+
+template <typename T>
+MAT<T> randn(int m, int n, uint64_t seed) {
+    Normal rng(seed);
+    MAT<T> A(m, n);
+    for (int64_t i = 0; i < (int64_t)m * n; ++i) A.data[i] = (T)rng();
+    return A;
+}
+
+// ||Q R - A||_F / ||A||_F, ||Q^T Q - I||_F, max |R(r, c)| below the diagonal
+template <typename T>
+void qr_errors(const MAT<T> &A, const MAT<T> &Q, const MAT<T> &R, double *rec, double *orth,
+               double *low) {
+    const int m = A.rows, n = A.cols;
+    double num = 0, den = 0, o = 0, l = 0;
+    for (int c = 0; c < n; ++c)
+        for (int r = 0; r < m; ++r) {
+            double s = 0;
+            for (int k = 0; k < m; ++k) s += (double)Q(r, k) * (double)R(k, c);
+            const double a = (double)A(r, c);
+            num += (s - a) * (s - a);
+            den += a * a;
+            if (r > c) l = std::max(l, std::fabs((double)R(r, c)));
+        }
+    for (int c = 0; c < m; ++c)
+        for (int r = 0; r < m; ++r) {
+            double s = 0;
+            for (int k = 0; k < m; ++k) s += (double)Q(k, r) * (double)Q(k, c);
+            s -= (r == c) ? 1.0 : 0.0;
+            o += s * s;
+        }
+    *rec = std::sqrt(num / den);
+    *orth = std::sqrt(o);
+    *low = l;
+}
+
+int main() {
+    const double u = 1.1102230246251565e-16;
+    {
+        // blocked-size FP64 call, the reference's dispatch condition
+        // min(m, n) > 2 * BS on both ISAs
+        MAT<double> A = randn<double>(400, 260, 1), Q, R;
+        QR_init(A, Q, R);
+        QR(A, Q, R, QR_Householder);
+        double rec, orth, low;
+        qr_errors(A, Q, R, &rec, &orth, &low);
+        expect(Q.rows == 400 && Q.cols == 400 && R.rows == 400 && R.cols == 260,
+               "QR(A,Q,R,QR_Householder) shapes", 0.0, 0.0);
+        expect(rec < 64 * 400 * u, "FP64 400x260 ||QR-A||/||A||", rec, 64 * 400 * u);
+        expect(orth < 64 * 400 * u, "FP64 400x260 ||Q^TQ-I||", orth, 64 * 400 * u);
+        expect(low == 0.0, "FP64 400x260 R zero below the diagonal", low, 0.0);
+    }
+    {
+        // the template form, a small (unblocked-size) matrix, wide shape
+        MAT<double> A = randn<double>(40, 70, 2), Q, R;
+        QR<QR_Householder>(A, Q, R);
+        double rec, orth, low;
+        qr_errors(A, Q, R, &rec, &orth, &low);
+        expect(rec < 64 * 70 * u, "QR<QR_Householder> 40x70 ||QR-A||/||A||", rec, 64 * 70 * u);
+        expect(orth < 64 * 70 * u, "QR<QR_Householder> 40x70 ||Q^TQ-I||", orth, 64 * 70 * u);
+    }
+    {
+        // in place: A is R (QR/QR.h:12-13)
+        MAT<double> A = randn<double>(300, 300, 3), A0(A), Q;
+        QR(A, Q, A, QR_Householder);
+        double rec, orth, low;
+        qr_errors(A0, Q, A, &rec, &orth, &low);
+        expect(rec < 64 * 300 * u, "in place 300x300 ||QR-A||/||A||", rec, 64 * 300 * u);
+    }
+    {
+        // FP32 matrices (computed in FP64, converted like the A -> R copy)
+        MAT<float> A = randn<float>(200, 150, 4), Q, R;
+        QR_init(A, Q, R);
+        QR(A, Q, R, QR_Householder);
+        double rec, orth, low;
+        qr_errors(A, Q, R, &rec, &orth, &low);
+        const double uf = 5.960464477539063e-08;
+        expect(rec < 64 * uf, "FP32 200x150 ||QR-A||/||A||", rec, 64 * uf);
+        expect(orth < 64 * uf * 14, "FP32 200x150 ||Q^TQ-I||", orth, 64 * uf * 14);
+    }
+    {
+        // least squares, plain and mixed: the normal equations hold
+        MAT<double> A = randn<double>(900, 300, 5), b = randn<double>(900, 1, 6), x, x32;
+        LeastSquares(A, b, x);
+        mqr_ir_info_t info = LeastSquares(A, b, x32, MQR_PREC_FP32);
+        double na = 0, g = 0, g32 = 0, nx = 0, dx = 0;
+        for (int64_t i = 0; i < (int64_t)A.rows * A.cols; ++i) na += A.data[i] * A.data[i];
+        // r = b - A x and A^T r
+        std::vector<double> r(A.rows), r32(A.rows);
+        for (int i = 0; i < A.rows; ++i) {
+            double s = b(i, 0), s32 = b(i, 0);
+            for (int c = 0; c < A.cols; ++c) {
+                s -= A(i, c) * x(c, 0);
+                s32 -= A(i, c) * x32(c, 0);
+            }
+            r[i] = s;
+            r32[i] = s32;
+        }
+        for (int c = 0; c < A.cols; ++c) {
+            double s = 0, s32 = 0;
+            for (int i = 0; i < A.rows; ++i) {
+                s += A(i, c) * r[i];
+                s32 += A(i, c) * r32[i];
+            }
+            g += s * s;
+            g32 += s32 * s32;
+            nx += x(c, 0) * x(c, 0);
+            dx += (x(c, 0) - x32(c, 0)) * (x(c, 0) - x32(c, 0));
+        }
+        double nb = 0;
+        for (int i = 0; i < A.rows; ++i) nb += b(i, 0) * b(i, 0);
+        const double eta = std::sqrt(g) / (na * std::sqrt(nx) + std::sqrt(na * nb));
+        const double eta32 = std::sqrt(g32) / (na * std::sqrt(nx) + std::sqrt(na * nb));
+        expect(eta < 1e-15, "LeastSquares FP64 ||A^T r|| / (||A||(||A|| ||x|| + ||b||))", eta, 1e-15);
+        expect(eta32 < 1e-15, "LeastSquares FP32+IR same", eta32, 1e-15);
+        expect(info.converged == 1, "LeastSquares FP32+IR converged", info.iterations, 30);
+        expect(std::sqrt(dx / nx) < 1e-12, "LeastSquares FP32+IR vs FP64 x", std::sqrt(dx / nx), 1e-12);
+    }
+    {
+        // QR types this library does not provide raise mat::Error
+        MAT<double> A = randn<double>(10, 5, 7), Q, R;
+        bool threw = false;
+        try {
+            QR(A, Q, R, QR_Givens);
+        } catch (const mat::Error &e) {
+            threw = e.status == MQR_ERR_UNSUPPORTED;
+        }
+        expect(threw, "QR_Givens raises mat::Error (unsupported)", 0.0, 0.0);
+    }
+    std::printf("%s\n", failures ? "FAILED" : "ALL PASSED");
+    return failures ? 1 : 0;
+}
