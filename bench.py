@@ -235,6 +235,47 @@ def gemm_rates(M, h) -> list:
     return out
 
 
+def batched_rates(M, h, count: int = 10000, seed: int = 4321) -> list:
+    """Batched uneven QR (m ~ U{16..512}, n ~ U{8..m}, iid N(0,1)) in FP64 and
+    FP32: one persistent launch replayed from its hipGraph, timed with events
+    on the handle's stream around the replay (inputs restored before each)."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(seed)
+    ms = rng.integers(16, 513, count)
+    ns = np.array([rng.integers(8, m + 1) for m in ms])
+    flops = float(np.sum(2.0 * ms * ns * ns - 2.0 * ns ** 3 / 3.0))
+    out = []
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    for dt, name in ((torch.float64, "fp64"), (torch.float32, "fp32")):
+        src = [torch.randn(int(n), int(m), generator=g, device="cuda", dtype=dt).T
+               for m, n in zip(ms, ns)]
+        work = [torch.empty(int(n), int(m), device="cuda", dtype=dt).T for m, n in zip(ms, ns)]
+        plan = M.BatchedPlan(h, work)
+        times = []
+        for rep in range(4):
+            for w, s in zip(work, src):
+                w.copy_(s)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            plan.run()
+            e1.record()
+            torch.cuda.synchronize()
+            times.append(e0.elapsed_time(e1))
+        plan.close()
+        t = float(np.median(times[1:]))
+        peak = PEAK_TFLOPS[name]
+        rec = {"dtype": name, "count": count, "ms": round(t, 3),
+               "tflops": round(flops / (t * 1e-3) / 1e12, 3),
+               "frac_mfma_peak": round(flops / (t * 1e-3) / 1e12 / peak, 4), "flops": flops}
+        print(f"[bench] batched {name}: {t:.3f} ms {rec['tflops']} TFLOP/s", file=sys.stderr,
+              flush=True)
+        out.append(rec)
+        del src, work
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,9 +287,14 @@ def main() -> int:
     ap.add_argument("--prec", default="fp32", choices=["fp32", "bf16x3", "bf16", "none"],
                     help="mixed-precision trailing updates ('none': plain FP64 only)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernel", choices=["gemm"], default=None,
+    ap.add_argument("--kernel", choices=["gemm", "batched"], default=None,
                     help="time one kernel family instead of the solve")
     args = ap.parse_args()
+    if args.kernel == "batched":
+        import mat_amd as M
+        h = M.Handle(0)
+        print(json.dumps({"batched_qr": batched_rates(M, h)}))
+        return 0
     if args.kernel == "gemm":
         import mat_amd as M
         h = M.Handle(0)

@@ -199,6 +199,32 @@ int mqr_gels_mixed(mqr_handle_t handle, mqr_precision_t prec, int64_t m, int64_t
                    const double *A, int64_t lda, const double *b, double *x,
                    const mqr_ir_opts_t *opts, mqr_ir_info_t *info);
 
+/* ---- batched uneven QR ---------------------------------------------------- */
+/* No reference counterpart: the reference factors one MAT per QR() call
+ * (QR/QR.h:7-46); a batch there is a loop of such calls.
+ * Factor `batch` independent column-major matrices in place with the
+ * semantics of mqr_dgeqrf per matrix (R on and above the diagonal, reflectors
+ * below, tau[min(m,n)]), in the precision of `dtype`, in one persistent
+ * launch.  m, n, lda: host arrays (0 <= m[i] <= 512, n[i] >= 0,
+ * lda[i] >= max(1, m[i])); A, tau: host arrays of device pointers (double *
+ * for MQR_F64, float * for MQR_F32).  tile: -1 = the tuned per-size table,
+ * 0 = panel width 8, 1 = panel width 16.  Blocks until done. */
+int mqr_geqrf_batched_uneven(mqr_handle_t handle, mqr_dtype_t dtype, int64_t batch,
+                             const int32_t *m, const int32_t *n, const int64_t *lda,
+                             void *const *A, void *const *tau, int tile);
+
+/* The same work captured once in a hipGraph: create uploads the descriptors
+ * (the device pointers are fixed from then on) and instantiates the graph;
+ * run replays it on the handle's stream (stream-ordered, returns at once);
+ * destroy frees it. */
+typedef struct mqr_batched_plan_st *mqr_batched_plan_t;
+int mqr_batched_plan_create(mqr_handle_t handle, mqr_dtype_t dtype, int64_t batch,
+                            const int32_t *m, const int32_t *n, const int64_t *lda,
+                            void *const *A, void *const *tau, int tile,
+                            mqr_batched_plan_t *plan);
+int mqr_batched_plan_run(mqr_handle_t handle, mqr_batched_plan_t plan);
+int mqr_batched_plan_destroy(mqr_batched_plan_t plan);
+
 /* ---- host-memory convenience (used by include/mat/Mat.h) ------------------ */
 /* QR(A, Q, R, QR_Householder) semantics (QR/QR.h:7-24 with
  * QR/__Householder_Blocked.h:335-353): A m x n (lda = m) is read, Q m x m and

@@ -64,6 +64,10 @@ SIGNATURES = {
     "mqr_dtrsm_upper": (_i32, [_vp, C.c_char, _i64, _i64, _vp, _i64, _vp, _i64]),
     "mqr_dgels": (_i32, [_vp, _i64, _i64, _i64, _vp, _i64, _vp, _i64]),
     "mqr_gels_mixed": (_i32, [_vp, _i32, _i64, _i64, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "mqr_geqrf_batched_uneven": (_i32, [_vp, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _i32]),
+    "mqr_batched_plan_create": (_i32, [_vp, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _i32, _vp]),
+    "mqr_batched_plan_run": (_i32, [_vp, _vp]),
+    "mqr_batched_plan_destroy": (_i32, [_vp]),
     "mqr_qr_householder_host": (_i32, [_vp, _i64, _i64, _vp, _vp, _vp]),
     "mqr_gels_host": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp]),
 }
@@ -220,9 +224,15 @@ def empty_colmajor(m: int, n: int, device="cuda", dtype=None):
 
 
 def ld(t) -> int:
-    """Leading dimension of a column-major (m, n) tensor."""
-    assert t.dim() == 2 and t.stride(0) == 1, "expected a column-major tensor"
-    return max(t.stride(1), t.shape[0], 1)
+    """Leading dimension of a column-major (m, n) tensor (strides of size-1
+    dimensions are irrelevant and not checked)."""
+    assert t.dim() == 2, "expected a matrix"
+    m, n = t.shape
+    if n <= 1 or t.numel() == 0:
+        assert t.numel() <= 1 or n == 0 or t.stride(0) == 1, "expected a column-major tensor"
+        return max(m, 1)
+    assert t.stride(0) == 1 or m <= 1, "expected a column-major tensor"
+    return max(t.stride(1), m, 1)
 
 
 def _ptr(t) -> C.c_void_p:
@@ -303,6 +313,68 @@ def gels_mixed(h: Handle, prec: str, A, b, max_iter: int = 30, tol: float = 0.0,
     check(lib().mqr_gels_mixed(h.h, PREC[prec], m, n, _ptr(A), ld(A), _ptr(b), _ptr(x),
                                C.byref(opts), C.byref(info)), "mqr_gels_mixed")
     return x, info
+
+
+def _batch_args(mats, taus):
+    import numpy as np
+    import torch
+    b = len(mats)
+    f64 = mats[0].dtype == torch.float64 if b else True
+    m = np.array([t.shape[0] for t in mats], dtype=np.int32)
+    n = np.array([t.shape[1] for t in mats], dtype=np.int32)
+    lda = np.array([ld(t) for t in mats], dtype=np.int64)
+    A = (C.c_void_p * max(b, 1))(*[t.data_ptr() for t in mats])
+    T = (C.c_void_p * max(b, 1))(*[t.data_ptr() for t in taus])
+    keep = (m, n, lda, A, T)
+    return (0 if f64 else 1), b, keep
+
+
+def geqrf_batched(h: Handle, mats, tile: int = -1):
+    """Factor a list of column-major device matrices (all float64 or all
+    float32, m <= 512) in place in one launch; returns their tau vectors."""
+    import torch
+    taus = [torch.zeros(max(min(t.shape), 1), device=t.device, dtype=t.dtype) for t in mats]
+    if not mats:
+        return taus
+    dt, b, (m, n, lda, A, T) = _batch_args(mats, taus)
+    check(lib().mqr_geqrf_batched_uneven(h.h, dt, b, m.ctypes.data_as(C.c_void_p),
+                                         n.ctypes.data_as(C.c_void_p),
+                                         lda.ctypes.data_as(C.c_void_p), C.cast(A, C.c_void_p),
+                                         C.cast(T, C.c_void_p), tile), "mqr_geqrf_batched_uneven")
+    return taus
+
+
+class BatchedPlan:
+    """The batched factorisation of fixed device matrices captured in a
+    hipGraph; run() replays it on the handle's stream."""
+
+    def __init__(self, h: Handle, mats, tile: int = -1):
+        import torch
+        self.h = h
+        self.mats = mats
+        self.taus = [torch.zeros(max(min(t.shape), 1), device=t.device, dtype=t.dtype)
+                     for t in mats]
+        self.plan = C.c_void_p()
+        dt, b, (m, n, lda, A, T) = _batch_args(mats, self.taus)
+        check(lib().mqr_batched_plan_create(h.h, dt, b, m.ctypes.data_as(C.c_void_p),
+                                            n.ctypes.data_as(C.c_void_p),
+                                            lda.ctypes.data_as(C.c_void_p),
+                                            C.cast(A, C.c_void_p), C.cast(T, C.c_void_p), tile,
+                                            C.byref(self.plan)), "mqr_batched_plan_create")
+
+    def run(self):
+        check(lib().mqr_batched_plan_run(self.h.h, self.plan), "mqr_batched_plan_run")
+
+    def close(self):
+        if self.plan:
+            lib().mqr_batched_plan_destroy(self.plan)
+            self.plan = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------

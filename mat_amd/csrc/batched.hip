@@ -1,0 +1,499 @@
+// Batched uneven QR: thousands of independent small matrices (m <= 512)
+// factored in one persistent launch.  Per matrix the result is what
+// mqr_dgeqrf gives (R on and above the diagonal, reflectors below, tau): the
+// reference's blocked Householder conventions (QR/__Householder_Blocked.h:62-245),
+// computed here in the precision of the data (FP32 or FP64).
+//
+// Scheduling: the host orders the matrices by decreasing flop count; every
+// workgroup (256 threads) takes the next matrix from an atomic counter until
+// the list is exhausted (longest-processing-time-first, so the big ones do
+// not straggle).  The last workgroup to leave resets the counters, so the
+// launch can be replayed from a hipGraph without a memset node.
+//
+// Per matrix, blocks of NB columns (NB = 8 or 16, per size bin from a tuned
+// table):
+//   panel   : the h x b panel is staged in LDS (column-major, ld = h + 1) and
+//             factored column by column with workgroup reductions
+//             (QR/__Householder_Blocked.h:62-84), T from
+//             T^{-1} = strict_upper(V^T V) + diag(1 / tau) (:86-103);
+//   update  : the trailing columns, in chunks of 16, one chunk per wave:
+//             W = V^T C (MFMA, K = h, C straight from global memory),
+//             W2 = T^T W (LDS), C -= V W2 (MFMA with swapped operands, so
+//             that 16 lanes cover 16 consecutive rows of C: coalesced
+//             read-modify-write) (:143-212).
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "mqr_internal.h"
+
+namespace mqr {
+
+// Per-matrix descriptor, in execution order.
+struct BatchDesc {
+    void *A;
+    void *tau;
+    int64_t lda;
+    int m, n;
+    int nb;   // panel width: 8 or 16
+    int pad;
+};
+
+namespace {
+
+constexpr int BNT = 256;       // threads per workgroup
+constexpr int BMAXM = 512;     // largest m accepted
+constexpr int BLD = BMAXM + 1; // LDS leading dimension bound
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mf;
+template <>
+struct Mf<double> {
+    typedef v4d acc;
+    static __device__ __forceinline__ acc run(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // output row of lane l, register r (the column is l & 15)
+    static __device__ __forceinline__ int row(int l, int r) { return (l >> 4) + 4 * r; }
+};
+template <>
+struct Mf<float> {
+    typedef v4f acc;
+    static __device__ __forceinline__ acc run(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+
+template <typename T>
+__device__ __forceinline__ T wsum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <typename T>
+struct Shared {
+    T P[16 * BLD];        // panel, column-major, ld = h + 1
+    T W[4][16][17];       // per-wave W, then W2; partial V^T V
+    T Tm[16][17];         // T of the panel
+    T G[16][17];          // V^T V of the panel
+    T red[4][16];
+    T tau[16];
+    int idx;
+};
+
+// unit-lower-trapezoid V(r, p) of the b-column panel factored in LDS; zero
+// outside rows [0, h) and columns [0, b)
+template <typename T>
+__device__ __forceinline__ T vget(const T *P, int ld, int h, int b, int r, int p) {
+    const T x = (p < b && r < h) ? P[p * ld + r] : T(0);
+    return r > p ? x : (r == p && p < b ? T(1) : T(0));
+}
+
+template <typename T>
+__device__ void factor_one(const BatchDesc &d, Shared<T> &sh) {
+    T *A = (T *)d.A;
+    T *tau = (T *)d.tau;
+    const int64_t lda = d.lda;
+    const int m = d.m, n = d.n, k = min(m, n), NB = d.nb;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int j = 0; j < k; j += NB) {
+        const int b = min(NB, k - j);
+        const int h = m - j;
+        const int ld = h + 1;
+        T *Aj = A + j + (int64_t)j * lda;
+        // ---- panel to LDS (coalesced along rows) ----
+        for (int e = tid; e < b * h; e += BNT) {
+            const int r = e % h, c = e / h;
+            sh.P[c * ld + r] = Aj[r + (int64_t)c * lda];
+        }
+        __syncthreads();
+        // ---- column-by-column reflectors (QR/__Householder_Blocked.h:62-84) ----
+        for (int r = 0; r < b; ++r) {
+            T *w = sh.P + r * ld;
+            T s = 0;
+            for (int i = r + tid; i < h; i += BNT) s += w[i] * w[i];
+            s = wsum(s);
+            if (lane == 0) sh.red[wave][0] = s;
+            __syncthreads();
+            const T nrm2 = (sh.red[0][0] + sh.red[1][0]) + (sh.red[2][0] + sh.red[3][0]);
+            T cr = sqrt(nrm2);
+            const T alpha = w[r];
+            __syncthreads();
+            if (cr == T(0)) {
+                if (tid == 0) sh.tau[r] = T(0);
+                continue;  // uniform: every thread read the same sum
+            }
+            if (alpha < T(0)) cr = -cr;
+            const T w0 = alpha + cr;
+            const T hr = T(1) / w0;
+            const T t = w0 / cr;
+            for (int i = r + 1 + tid; i < h; i += BNT) w[i] *= hr;
+            if (tid == 0) {
+                w[r] = -cr;
+                sh.tau[r] = t;
+            }
+            __syncthreads();
+            // apply I - t v v^T to the panel columns c > r
+            T part[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) part[c] = 0;
+            for (int i = r + 1 + tid; i < h; i += BNT) {
+                const T vi = w[i];
+#pragma unroll
+                for (int c = 0; c < 16; ++c)
+                    if (c > r && c < b) part[c] += vi * sh.P[c * ld + i];
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                if (c > r && c < b) {
+                    const T v = wsum(part[c]);
+                    if (lane == 0) sh.red[wave][c] = v;
+                }
+            __syncthreads();
+            T coef[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                coef[c] = (c > r && c < b)
+                              ? -t * (sh.P[c * ld + r] + ((sh.red[0][c] + sh.red[1][c]) +
+                                                          (sh.red[2][c] + sh.red[3][c])))
+                              : T(0);
+            __syncthreads();
+            for (int i = r + tid; i < h; i += BNT) {
+                const T vi = i == r ? T(1) : w[i];
+#pragma unroll
+                for (int c = 0; c < 16; ++c)
+                    if (c > r && c < b) sh.P[c * ld + i] += coef[c] * vi;
+            }
+            __syncthreads();
+        }
+        // ---- T from G = V^T V: T^{-1} = strict_upper(G) + diag(1 / tau) ----
+        // G on the MFMA: wave w sums rows [w hq, (w + 1) hq), then the four
+        // partial Gram matrices are added
+        {
+            typedef typename Mf<T>::acc acc_t;
+            const int hq = (((h + 3) / 4) + 3) & ~3;
+            acc_t g = {0, 0, 0, 0};
+            for (int r0 = wave * hq; r0 < min(h, (wave + 1) * hq); r0 += 4) {
+                const T v = vget(sh.P, ld, h, b, r0 + (lane >> 4), lane & 15);
+                g = Mf<T>::run(v, v, g);
+            }
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) sh.W[wave][Mf<T>::row(lane, rr)][lane & 15] = g[rr];
+        }
+        __syncthreads();
+        {
+            const int p = tid & 15, q = tid >> 4;
+            sh.G[p][q] = (sh.W[0][p][q] + sh.W[1][p][q]) + (sh.W[2][p][q] + sh.W[3][p][q]);
+        }
+        __syncthreads();
+        T(*G)[17] = sh.G;
+        if (tid < 16) {
+            // column c = tid of T = M^{-1} by back substitution, M(i, i) =
+            // 1 / tau_i; a tau = 0 reflector (the identity) is removed from M
+            // (zero row and column, unit diagonal) and left zero in T, which is
+            // what the forward recurrence produces
+            const int c = tid;
+            T x[16];
+#pragma unroll
+            for (int i = 15; i >= 0; --i) {
+                T s = (i == c) ? T(1) : T(0);
+                const bool zi = i < b && sh.tau[i] == T(0);
+#pragma unroll
+                for (int l = 0; l < 16; ++l)
+                    if (l > i && l <= c) {
+                        const bool zl = l < b && sh.tau[l] == T(0);
+                        s -= ((zi || zl) ? T(0) : G[i][l]) * x[l];
+                    }
+                const T ti = (i < b && !zi) ? sh.tau[i] : T(1);
+                x[i] = (i <= c) ? s * ti : T(0);
+            }
+            const bool zc = c < b && sh.tau[c] == T(0);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) sh.Tm[i][c] = (c < b && i < b && !zc) ? x[i] : T(0);
+        }
+        // ---- panel back to global, tau ----
+        for (int e = tid; e < b * h; e += BNT) {
+            const int r = e % h, c = e / h;
+            Aj[r + (int64_t)c * lda] = sh.P[c * ld + r];
+        }
+        if (tid < b) tau[j + tid] = sh.tau[tid];
+        __syncthreads();
+        // ---- trailing update, one 16-column chunk per wave ----
+        typedef typename Mf<T>::acc acc_t;
+        const int c0base = j + b;
+        const int nchunks = (n - c0base + 15) / 16;
+        const int hp4 = (h + 3) & ~3;
+        for (int q = wave; q < nchunks; q += 4) {
+            const int c0 = c0base + 16 * q;
+            T *C = A + j;  // row j of the matrix, columns absolute
+            // W = V^T C  (16 x 16, K = h)
+            acc_t acc = {0, 0, 0, 0};
+            {
+                // eight row groups per round: all eight loads are issued
+                // before the MFMAs that consume them
+                const int col = c0 + (lane & 15);
+                const bool cin = col < n;
+                const T *Ccol = C + (int64_t)(cin ? col : c0) * lda;
+                for (int r0 = 0; r0 < hp4; r0 += 32) {
+                    T bv[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int r = r0 + 4 * u + (lane >> 4);
+                        bv[u] = (cin && r < h) ? Ccol[r] : T(0);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int r = r0 + 4 * u + (lane >> 4);
+                        acc = Mf<T>::run(vget(sh.P, ld, h, b, r, lane & 15), bv[u], acc);
+                    }
+                }
+            }
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) sh.W[wave][Mf<T>::row(lane, rr)][lane & 15] = acc[rr];
+            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_wave_barrier();
+            // W2 = T^T W (rows and columns >= b of T are zero), written over W
+            T w2[4];
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                const int i = Mf<T>::row(lane, rr), jj = lane & 15;
+                T s = 0;
+                for (int p = 0; p <= i; ++p) s += sh.Tm[p][i] * sh.W[wave][p][jj];
+                w2[rr] = s;
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) sh.W[wave][Mf<T>::row(lane, rr)][lane & 15] = w2[rr];
+            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_wave_barrier();
+            // C -= V W2, 16 rows at a time; operands swapped so that lane l
+            // holds row (l & 15) of the tile
+            T w2v[4];
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) w2v[kk] = sh.W[wave][4 * kk + (lane >> 4)][lane & 15];
+            // two 16-row tiles per round: their C loads are issued together
+            for (int row0 = 0; row0 < h; row0 += 32) {
+                acc_t d[2];
+                T cv[2][4];
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    const int row = row0 + 16 * t + (lane & 15);
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr) {
+                        const int col = c0 + Mf<T>::row(lane, rr);
+                        cv[t][rr] = (row < h && col < n) ? C[row + (int64_t)col * lda] : T(0);
+                    }
+                    d[t] = (acc_t){0, 0, 0, 0};
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk) {
+                        const T bv = vget(sh.P, ld, h, b, row, 4 * kk + (lane >> 4));
+                        d[t] = Mf<T>::run(w2v[kk], bv, d[t]);
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    const int row = row0 + 16 * t + (lane & 15);
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr) {
+                        const int col = c0 + Mf<T>::row(lane, rr);
+                        if (row < h && col < n) C[row + (int64_t)col * lda] = cv[t][rr] - d[t][rr];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(BNT) void k_qr_batched(const BatchDesc *__restrict__ desc, int batch,
+                                                    int *__restrict__ counters) {
+    extern __shared__ char smem[];
+    Shared<T> &sh = *reinterpret_cast<Shared<T> *>(smem);
+    for (;;) {
+        if (threadIdx.x == 0) sh.idx = atomicAdd(&counters[0], 1);
+        __syncthreads();
+        const int i = sh.idx;
+        __syncthreads();
+        if (i >= batch) break;
+        factor_one<T>(desc[i], sh);
+    }
+    // the last workgroup out resets the counters for the next replay
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const int done = atomicAdd(&counters[1], 1);
+        if (done == (int)gridDim.x - 1) {
+            atomicExch(&counters[0], 0);
+            atomicExch(&counters[1], 0);
+        }
+    }
+}
+
+template <typename T>
+size_t smem_bytes() {
+    return sizeof(Shared<T>);
+}
+
+}  // namespace
+
+// Panel widths per m bin (<= 64, <= 128, <= 256, <= 512) from the
+// autotuner (mat_amd/autotune_batched.py), kTunedNb[double][bin].
+#include "tuned_batched.h"
+
+int batched_nb(bool f64, int m, int tile) {
+    if (tile == 0) return 8;
+    if (tile == 1) return 16;
+    const int bin = m <= 64 ? 0 : m <= 128 ? 1 : m <= 256 ? 2 : 3;
+    return kTunedNb[f64 ? 1 : 0][bin];
+}
+
+struct BatchedPlan {
+    bool f64 = true;
+    int batch = 0;
+    BatchDesc *d_desc = nullptr;
+    int *d_counters = nullptr;
+    unsigned grid = 0;
+    size_t smem = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static int batched_setup(mqr_handle_t hd, bool f64, int64_t batch, const int32_t *m,
+                         const int32_t *n, const int64_t *lda, void *const *A, void *const *tau,
+                         int tile, BatchedPlan &p) {
+    std::vector<int> order(batch);
+    std::iota(order.begin(), order.end(), 0);
+    auto cost = [&](int i) {
+        const double mm = m[i], nn = std::min(m[i], n[i]);
+        return 2.0 * mm * nn * n[i] - 2.0 * nn * nn * nn / 3.0 + mm;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+    std::vector<BatchDesc> h(batch);
+    for (int64_t e = 0; e < batch; ++e) {
+        const int i = order[e];
+        h[e].A = A[i];
+        h[e].tau = tau[i];
+        h[e].lda = lda[i];
+        h[e].m = m[i];
+        h[e].n = n[i];
+        h[e].nb = batched_nb(f64, m[i], tile);
+        h[e].pad = 0;
+    }
+    p.f64 = f64;
+    p.batch = (int)batch;
+    MQR_HIP(hipMalloc(&p.d_desc, sizeof(BatchDesc) * std::max<int64_t>(batch, 1)));
+    MQR_HIP(hipMalloc(&p.d_counters, 2 * sizeof(int)));
+    MQR_HIP(hipMemcpyAsync(p.d_desc, h.data(), sizeof(BatchDesc) * batch, hipMemcpyHostToDevice,
+                           hd->stream));
+    MQR_HIP(hipMemsetAsync(p.d_counters, 0, 2 * sizeof(int), hd->stream));
+    MQR_HIP(hipStreamSynchronize(hd->stream));
+    p.smem = f64 ? smem_bytes<double>() : smem_bytes<float>();
+    const void *fn = f64 ? (const void *)k_qr_batched<double> : (const void *)k_qr_batched<float>;
+    MQR_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.smem));
+    int per_cu = 0;
+    MQR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BNT, p.smem));
+    per_cu = std::max(per_cu, 1);
+    p.grid = (unsigned)std::min<int64_t>(batch, (int64_t)per_cu * hd->cu_count);
+    return MQR_SUCCESS;
+}
+
+static int batched_launch(const BatchedPlan &p, hipStream_t s) {
+    if (p.batch == 0) return MQR_SUCCESS;
+    if (p.f64)
+        hipLaunchKernelGGL(k_qr_batched<double>, dim3(p.grid), dim3(BNT), p.smem, s, p.d_desc,
+                           p.batch, p.d_counters);
+    else
+        hipLaunchKernelGGL(k_qr_batched<float>, dim3(p.grid), dim3(BNT), p.smem, s, p.d_desc,
+                           p.batch, p.d_counters);
+    MQR_LAUNCH_CHECK();
+    return MQR_SUCCESS;
+}
+
+static void batched_free(BatchedPlan &p) {
+    if (p.exec) (void)hipGraphExecDestroy(p.exec);
+    if (p.graph) (void)hipGraphDestroy(p.graph);
+    if (p.d_desc) (void)hipFree(p.d_desc);
+    if (p.d_counters) (void)hipFree(p.d_counters);
+    p = BatchedPlan();
+}
+
+int geqrf_batched(mqr_handle_t hd, bool f64, int64_t batch, const int32_t *m, const int32_t *n,
+                  const int64_t *lda, void *const *A, void *const *tau, int tile) {
+    BatchedPlan p;
+    int st = batched_setup(hd, f64, batch, m, n, lda, A, tau, tile, p);
+    if (st == MQR_SUCCESS) {
+        ProfScope ps(K_PANEL_SMALL, 0.0, 0.0, hd->stream);
+        st = batched_launch(p, hd->stream);
+    }
+    if (st == MQR_SUCCESS && hipStreamSynchronize(hd->stream) != hipSuccess) st = MQR_ERR_HIP;
+    batched_free(p);
+    return st;
+}
+
+}  // namespace mqr
+
+struct mqr_batched_plan_st {
+    mqr::BatchedPlan p;
+    int device = 0;
+};
+
+namespace mqr {
+
+int batched_plan_create(mqr_handle_t hd, bool f64, int64_t batch, const int32_t *m,
+                        const int32_t *n, const int64_t *lda, void *const *A, void *const *tau,
+                        int tile, mqr_batched_plan_st **out) {
+    mqr_batched_plan_st *pl = new (std::nothrow) mqr_batched_plan_st();
+    if (!pl) return MQR_ERR_ALLOC;
+    pl->device = hd->device;
+    int st = batched_setup(hd, f64, batch, m, n, lda, A, tau, tile, pl->p);
+    if (st == MQR_SUCCESS && batch > 0) {
+        // capture the single persistent launch; replays need no memset node
+        // because the kernel resets its own counters
+        hipStream_t cs = nullptr;
+        if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) st = MQR_ERR_HIP;
+        if (st == MQR_SUCCESS &&
+            hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            st = MQR_ERR_HIP;
+        if (st == MQR_SUCCESS) {
+            const int ls = batched_launch(pl->p, cs);
+            const hipError_t ec = hipStreamEndCapture(cs, &pl->p.graph);
+            if (ls != MQR_SUCCESS) st = ls;
+            else if (ec != hipSuccess) st = MQR_ERR_HIP;
+        }
+        if (st == MQR_SUCCESS &&
+            hipGraphInstantiate(&pl->p.exec, pl->p.graph, nullptr, nullptr, 0) != hipSuccess)
+            st = MQR_ERR_HIP;
+        if (cs) (void)hipStreamDestroy(cs);
+        if (st != MQR_SUCCESS) (void)hipGetLastError();
+    }
+    if (st != MQR_SUCCESS) {
+        batched_free(pl->p);
+        delete pl;
+        return st;
+    }
+    *out = pl;
+    return MQR_SUCCESS;
+}
+
+int batched_plan_run(mqr_handle_t hd, mqr_batched_plan_st *pl) {
+    if (pl->p.batch == 0) return MQR_SUCCESS;
+    ProfScope ps(K_PANEL_SMALL, 0.0, 0.0, hd->stream);
+    MQR_HIP(hipGraphLaunch(pl->p.exec, hd->stream));
+    return MQR_SUCCESS;
+}
+
+void batched_plan_destroy(mqr_batched_plan_st *pl) {
+    batched_free(pl->p);
+    delete pl;
+}
+
+int batched_plan_device(const mqr_batched_plan_st *pl) { return pl->device; }
+
+}  // namespace mqr

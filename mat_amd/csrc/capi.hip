@@ -17,6 +17,14 @@ int orgqr_f64(mqr_handle_t hd, int64_t m, int64_t nq, int64_t k, const double *V
               const double *T, int64_t ldt, double *Q, int64_t ldq);
 int trsm_upper_f64(mqr_handle_t hd, bool trans, int64_t n, int64_t nrhs, const double *R,
                    int64_t ldr, double *B, int64_t ldb);
+int geqrf_batched(mqr_handle_t hd, bool f64, int64_t batch, const int32_t *m, const int32_t *n,
+                  const int64_t *lda, void *const *A, void *const *tau, int tile);
+int batched_plan_create(mqr_handle_t hd, bool f64, int64_t batch, const int32_t *m,
+                        const int32_t *n, const int64_t *lda, void *const *A, void *const *tau,
+                        int tile, mqr_batched_plan_st **out);
+int batched_plan_run(mqr_handle_t hd, mqr_batched_plan_st *pl);
+void batched_plan_destroy(mqr_batched_plan_st *pl);
+int batched_plan_device(const mqr_batched_plan_st *pl);
 int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, const double *A,
                int64_t lda, const double *b, double *x, const mqr_ir_opts_t *opts,
                mqr_ir_info_t *info);
@@ -439,6 +447,62 @@ int mqr_gels_mixed(mqr_handle_t h, mqr_precision_t prec, int64_t m, int64_t n, c
     MQR_GUARD(h);
     MQR_BRIDGE(h);
     return gels_mixed(h, prec, m, n, A, lda, b, x, opts, info);
+}
+
+// argument checks shared by the batched entry points (LAPACK-style -i)
+static int check_batched(mqr_handle_t h, mqr_dtype_t dtype, int64_t batch, const int32_t *m,
+                         const int32_t *n, const int64_t *lda, void *const *A, void *const *tau,
+                         int tile) {
+    if (!h) return -1;
+    if (dtype != MQR_F64 && dtype != MQR_F32) return -2;
+    if (batch < 0 || batch > INT32_MAX) return -3;
+    if (batch > 0 && (!m || !n || !lda || !A || !tau)) return !m ? -4 : !n ? -5 : !lda ? -6 : !A ? -7 : -8;
+    for (int64_t i = 0; i < batch; ++i) {
+        if (m[i] < 0 || m[i] > 512) return -4;
+        if (n[i] < 0) return -5;
+        if (lda[i] < std::max<int64_t>(1, m[i])) return -6;
+        if ((int64_t)m[i] * n[i] > 0 && !A[i]) return -7;
+        if (std::min(m[i], n[i]) > 0 && !tau[i]) return -8;
+    }
+    if (tile < -1 || tile > 1) return -9;
+    return MQR_SUCCESS;
+}
+
+int mqr_geqrf_batched_uneven(mqr_handle_t h, mqr_dtype_t dtype, int64_t batch, const int32_t *m,
+                             const int32_t *n, const int64_t *lda, void *const *A,
+                             void *const *tau, int tile) {
+    const int a = check_batched(h, dtype, batch, m, n, lda, A, tau, tile);
+    if (a) return a;
+    if (batch == 0) return MQR_SUCCESS;
+    MQR_GUARD(h);
+    MQR_BRIDGE(h);
+    return geqrf_batched(h, dtype == MQR_F64, batch, m, n, lda, A, tau, tile);
+}
+
+int mqr_batched_plan_create(mqr_handle_t h, mqr_dtype_t dtype, int64_t batch, const int32_t *m,
+                            const int32_t *n, const int64_t *lda, void *const *A,
+                            void *const *tau, int tile, mqr_batched_plan_t *plan) {
+    const int a = check_batched(h, dtype, batch, m, n, lda, A, tau, tile);
+    if (a) return a;
+    if (!plan) return -10;
+    *plan = nullptr;
+    MQR_GUARD(h);
+    return batched_plan_create(h, dtype == MQR_F64, batch, m, n, lda, A, tau, tile, plan);
+}
+
+int mqr_batched_plan_run(mqr_handle_t h, mqr_batched_plan_t plan) {
+    if (!plan) return -2;
+    MQR_GUARD(h);
+    if (batched_plan_device(plan) != h->device) return -2;
+    MQR_BRIDGE(h);
+    return batched_plan_run(h, plan);
+}
+
+int mqr_batched_plan_destroy(mqr_batched_plan_t plan) {
+    if (!plan) return MQR_SUCCESS;
+    DeviceGuard guard(batched_plan_device(plan));
+    batched_plan_destroy(plan);
+    return MQR_SUCCESS;
 }
 
 int mqr_qr_householder_host(mqr_handle_t h, int64_t m, int64_t n, const double *A, double *Q,
