@@ -276,6 +276,41 @@ def batched_rates(M, h, count: int = 10000, seed: int = 4321) -> list:
     return out
 
 
+def adjoint_rates(M, h, batch: int = 256, m: int = 1024, n: int = 256, seed: int = 99) -> list:
+    """QR adjoint (mqr_qr_backward) on a batch of thin QRs of iid N(0,1)
+    matrices, FP32 and FP64, loss sum(Wq o Q) + sum(Wr o R) so that dQ = Wq,
+    dR = Wr; events on the handle's stream around the backward call only."""
+    import torch
+    out = []
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    A = torch.randn(batch, m, n, generator=g, device="cuda", dtype=torch.float64)
+    Q, R = torch.linalg.qr(A.cpu(), mode="reduced")
+    flops = batch * (5.0 * m * n * n + 2.0 * n ** 3 / 3.0)
+    for dt, name in ((torch.float32, "fp32"), (torch.float64, "fp64")):
+        Qd = M.colmajor_batched(Q.to(device="cuda", dtype=dt))
+        Rd = M.colmajor_batched(R.to(device="cuda", dtype=dt))
+        Wq = M.colmajor_batched(torch.randn(batch, m, n, generator=g, device="cuda", dtype=dt))
+        Wr = M.colmajor_batched(torch.randn(batch, n, n, generator=g, device="cuda", dtype=dt))
+        for _ in range(2):
+            M.qr_backward(h, Qd, Rd, Wq, Wr)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 5
+        e0.record()
+        for _ in range(reps):
+            M.qr_backward(h, Qd, Rd, Wq, Wr)
+        e1.record()
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) / reps
+        rec = {"dtype": name, "batch": batch, "m": m, "n": n, "ms": round(t, 3),
+               "tflops": round(flops / (t * 1e-3) / 1e12, 3),
+               "frac_mfma_peak": round(flops / (t * 1e-3) / 1e12 / PEAK_TFLOPS[name], 4)}
+        print(f"[bench] adjoint {name}: {t:.3f} ms {rec['tflops']} TFLOP/s", file=sys.stderr,
+              flush=True)
+        out.append(rec)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -287,9 +322,14 @@ def main() -> int:
     ap.add_argument("--prec", default="fp32", choices=["fp32", "bf16x3", "bf16", "none"],
                     help="mixed-precision trailing updates ('none': plain FP64 only)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernel", choices=["gemm", "batched"], default=None,
+    ap.add_argument("--kernel", choices=["gemm", "batched", "adjoint"], default=None,
                     help="time one kernel family instead of the solve")
     args = ap.parse_args()
+    if args.kernel == "adjoint":
+        import mat_amd as M
+        h = M.Handle(0)
+        print(json.dumps({"qr_adjoint": adjoint_rates(M, h)}))
+        return 0
     if args.kernel == "batched":
         import mat_amd as M
         h = M.Handle(0)

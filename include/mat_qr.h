@@ -225,6 +225,22 @@ int mqr_batched_plan_create(mqr_handle_t handle, mqr_dtype_t dtype, int64_t batc
 int mqr_batched_plan_run(mqr_handle_t handle, mqr_batched_plan_t plan);
 int mqr_batched_plan_destroy(mqr_batched_plan_t plan);
 
+/* ---- QR adjoint ------------------------------------------------------------ */
+/* No reference counterpart.  Reverse-mode derivative of the thin QR A = Q R
+ * (m >= n) for `batch` equally sized matrices at fixed strides:
+ *   dA = [dQ + Q copyltu(M)] R^{-T},  M = R dR^T - dQ^T Q,
+ *   copyltu(M) = tril(M) + tril(M, -1)^T.
+ * Q, dQ, dA: m x n; R, dR: n x n (R upper triangular with a nonzero
+ * diagonal; only the upper triangle of dR is read, since R's strictly lower
+ * part is identically zero).  Matrix i of X is at X + i * strideX (elements).  dQ or dR may
+ * be NULL (taken as zero).  All device pointers of `dtype` (double or float);
+ * dA must not overlap the inputs. */
+int mqr_qr_backward(mqr_handle_t handle, mqr_dtype_t dtype, int64_t batch, int64_t m, int64_t n,
+                    const void *Q, int64_t ldq, int64_t strideQ, const void *R, int64_t ldr,
+                    int64_t strideR, const void *dQ, int64_t lddq, int64_t stridedQ,
+                    const void *dR, int64_t lddr, int64_t stridedR, void *dA, int64_t ldda,
+                    int64_t stridedA);
+
 /* ---- host-memory convenience (used by include/mat/Mat.h) ------------------ */
 /* QR(A, Q, R, QR_Householder) semantics (QR/QR.h:7-24 with
  * QR/__Householder_Blocked.h:335-353): A m x n (lda = m) is read, Q m x m and

@@ -68,6 +68,8 @@ SIGNATURES = {
     "mqr_batched_plan_create": (_i32, [_vp, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _i32, _vp]),
     "mqr_batched_plan_run": (_i32, [_vp, _vp]),
     "mqr_batched_plan_destroy": (_i32, [_vp]),
+    "mqr_qr_backward": (_i32, [_vp, _i32, _i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64,
+                               _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64]),
     "mqr_qr_householder_host": (_i32, [_vp, _i64, _i64, _vp, _vp, _vp]),
     "mqr_gels_host": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp]),
 }
@@ -206,13 +208,14 @@ class Handle:
 # ---------------------------------------------------------------------------
 
 def colmajor(x, device="cuda", dtype=None):
-    """A (m, n) torch tensor with strides (1, m) holding x (numpy or torch)."""
+    """A new (m, n) torch tensor with strides (1, m) holding a copy of x
+    (numpy or torch); never a view of x."""
     import numpy as np
     import torch
     if isinstance(x, np.ndarray):
         t = torch.from_numpy(np.ascontiguousarray(x.T))
     else:
-        t = x.T.contiguous()
+        t = x.T.clone(memory_format=torch.contiguous_format)
     if dtype is not None:
         t = t.to(dtype)
     return t.to(device).T
@@ -375,6 +378,102 @@ class BatchedPlan:
             self.close()
         except Exception:
             pass
+
+
+def colmajor_batched(x):
+    """(batch, m, n) tensor with each matrix column-major (strides (m n, 1, m))."""
+    return x.transpose(1, 2).contiguous().transpose(1, 2)
+
+
+def _bstride(t) -> int:
+    return t.stride(0) if t.shape[0] > 1 else t.shape[1] * t.shape[2]
+
+
+def _bld(t) -> int:
+    assert t.dim() == 3 and (t.stride(1) == 1 or t.shape[1] <= 1), "expected column-major matrices"
+    return max(t.stride(2) if t.shape[2] > 1 else t.shape[1], t.shape[1], 1)
+
+
+def qr_backward(h: Handle, Q, R, dQ=None, dR=None):
+    """dA for a batch of thin QRs A = Q R (tensors (batch, m, n) / (batch, n,
+    n), column-major per matrix, float32 or float64) via mqr_qr_backward."""
+    import torch
+    bsz, m, n = Q.shape
+    dA = colmajor_batched(torch.empty((bsz, m, n), device=Q.device, dtype=Q.dtype))
+    dt = 0 if Q.dtype == torch.float64 else 1
+
+    def arg(t):
+        return (_ptr(t), _bld(t), _bstride(t)) if t is not None else (_ptr(None), 1, 0)
+
+    check(lib().mqr_qr_backward(h.h, dt, bsz, m, n, *arg(Q), *arg(R), *arg(dQ), *arg(dR),
+                                *arg(dA)), "mqr_qr_backward")
+    return dA
+
+
+def qr_thin_batched(h: Handle, A):
+    """Thin FP64 QR of each matrix of a (batch, m, n) tensor on the GPU
+    (mqr_dgeqrf + mqr_dorgqr): returns Q (batch, m, n), R (batch, n, n)."""
+    import torch
+    bsz, m, n = A.shape
+    Qs, Rs = [], []
+    for i in range(bsz):
+        VR = colmajor(A[i].detach(), device=A.device)
+        _, T = geqrf(h, VR)
+        Q = orgqr(h, VR, T, n, n)
+        Qs.append(Q)
+        Rs.append(torch.triu(VR[:n, :n]))
+    return colmajor_batched(torch.stack(Qs)), colmajor_batched(torch.stack(Rs))
+
+
+_torch_handle: Optional[Handle] = None
+
+
+def torch_handle() -> Handle:
+    """A handle ordered on torch's current CUDA stream (re-read at every call)."""
+    global _torch_handle
+    import torch
+    if _torch_handle is None:
+        _torch_handle = Handle(0)
+    s = torch.cuda.current_stream(0)
+    check(lib().mqr_set_stream(_torch_handle.h, C.c_void_p(s.cuda_stream)), "mqr_set_stream")
+    return _torch_handle
+
+
+def _qr_function():
+    import torch
+
+    class QRFunction(torch.autograd.Function):
+        """Thin QR with the library's forward (FP64 blocked Householder) and
+        backward (mqr_qr_backward).  Test harness for gradient checks."""
+
+        @staticmethod
+        def forward(ctx, A):
+            h = torch_handle()
+            Q, R = qr_thin_batched(h, A)
+            ctx.save_for_backward(Q, R)
+            return Q, R
+
+        @staticmethod
+        def backward(ctx, gQ, gR):
+            Q, R = ctx.saved_tensors
+            h = torch_handle()
+            gQ = colmajor_batched(gQ) if gQ is not None else None
+            gR = colmajor_batched(gR) if gR is not None else None
+            return qr_backward(h, Q, R, gQ, gR)
+
+    return QRFunction
+
+
+_QRF = None
+
+
+def qr_autograd(A):
+    """Q, R = thin QR of a (batch, m, n) float64 CUDA tensor, differentiable
+    through mqr_qr_backward."""
+    global _QRF
+    if _QRF is None:
+        _QRF = _qr_function()
+    return _QRF.apply(A)
 
 
 # ---------------------------------------------------------------------------

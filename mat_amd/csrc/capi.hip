@@ -25,6 +25,10 @@ int batched_plan_create(mqr_handle_t hd, bool f64, int64_t batch, const int32_t 
 int batched_plan_run(mqr_handle_t hd, mqr_batched_plan_st *pl);
 void batched_plan_destroy(mqr_batched_plan_st *pl);
 int batched_plan_device(const mqr_batched_plan_st *pl);
+int qr_backward(mqr_handle_t hd, bool f64, int64_t batch, int64_t m, int64_t n, const void *Q,
+                int64_t ldq, int64_t sq, const void *R, int64_t ldr, int64_t sr, const void *dQ,
+                int64_t lddq, int64_t sdq, const void *dR, int64_t lddr, int64_t sdr, void *dA,
+                int64_t ldda, int64_t sda);
 int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, const double *A,
                int64_t lda, const double *b, double *x, const mqr_ir_opts_t *opts,
                mqr_ir_info_t *info);
@@ -503,6 +507,37 @@ int mqr_batched_plan_destroy(mqr_batched_plan_t plan) {
     DeviceGuard guard(batched_plan_device(plan));
     batched_plan_destroy(plan);
     return MQR_SUCCESS;
+}
+
+int mqr_qr_backward(mqr_handle_t h, mqr_dtype_t dtype, int64_t batch, int64_t m, int64_t n,
+                    const void *Q, int64_t ldq, int64_t strideQ, const void *R, int64_t ldr,
+                    int64_t strideR, const void *dQ, int64_t lddq, int64_t stridedQ,
+                    const void *dR, int64_t lddr, int64_t stridedR, void *dA, int64_t ldda,
+                    int64_t stridedA) {
+    if (!h) return -1;
+    if (dtype != MQR_F64 && dtype != MQR_F32) return -2;
+    if (batch < 0 || batch > 65535) return -3;
+    if (m < 0 || m > INT32_MAX) return -4;
+    if (n < 0 || n > m) return -5;
+    const bool any = batch > 0 && n > 0;
+    if (any && !Q) return -6;
+    if (ldq < std::max<int64_t>(1, m)) return -7;
+    if (batch > 1 && strideQ < ldq * n) return -8;
+    if (any && !R) return -9;
+    if (ldr < std::max<int64_t>(1, n)) return -10;
+    if (batch > 1 && strideR < ldr * n) return -11;
+    if (dQ && lddq < std::max<int64_t>(1, m)) return -13;
+    if (dQ && batch > 1 && stridedQ < lddq * n) return -14;
+    if (dR && lddr < std::max<int64_t>(1, n)) return -16;
+    if (dR && batch > 1 && stridedR < lddr * n) return -17;
+    if (any && !dA) return -18;
+    if (ldda < std::max<int64_t>(1, m)) return -19;
+    if (batch > 1 && stridedA < ldda * n) return -20;
+    if (!any) return MQR_SUCCESS;
+    MQR_GUARD(h);
+    MQR_BRIDGE(h);
+    return qr_backward(h, dtype == MQR_F64, batch, m, n, Q, ldq, strideQ, R, ldr, strideR, dQ,
+                       lddq, stridedQ, dR, lddr, stridedR, dA, ldda, stridedA);
 }
 
 int mqr_qr_householder_host(mqr_handle_t h, int64_t m, int64_t n, const double *A, double *Q,

@@ -6,6 +6,7 @@
 // Same structure as the FP64 tiles: a TILE x TILE output tile per workgroup
 // of four waves, 16-deep K slices staged in LDS with register prefetch and
 // double buffering, branch-free clamped loads, split-K partial outputs.
+#include <algorithm>
 #include <climits>
 
 #include "mqr_internal.h"
@@ -59,14 +60,24 @@ __device__ __forceinline__ void stash(float (*S)[TILE + 4], int tid, const float
 }
 
 template <int TILE, bool TA, bool TB>
-__global__ __launch_bounds__(NT) void k_gemm_f32(GemmF32 g, int kchunk) {
+__global__ __launch_bounds__(NT) void k_gemm_f32(GemmF32 gin, int kchunk) {
     constexpr int NV = TILE * BK / NT, BW = TILE / 32, HALF = TILE / 2;
     __shared__ float As[2][BK][TILE + 4];
     __shared__ float Bs[2][BK][TILE + 4];
+    // blockIdx.z = batch index * splits + split index
+    GemmF32 g = gin;
+    int split = blockIdx.z;
+    if (g.batch > 1) {
+        const int bz = blockIdx.z / g.splits;
+        split = blockIdx.z - bz * g.splits;
+        g.A += bz * g.strideA;
+        g.B += bz * g.strideB;
+        g.C += bz * g.strideC;
+    }
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int m0 = blockIdx.x * TILE, n0 = blockIdx.y * TILE;
-    const int kb = blockIdx.z * kchunk;
+    const int kb = split * kchunk;
     const int ke = min(g.K, kb + kchunk);
 
     v4f acc[BW][BW];
@@ -115,7 +126,7 @@ __global__ __launch_bounds__(NT) void k_gemm_f32(GemmF32 g, int kchunk) {
     // v_mfma_f32_16x16x4_f32 with swapped operands: lane l, register r holds
     // output row (l >> 4) * 4 + r of the transposed product, column l & 15,
     // i.e. C row = base + (l & 15), column = base + 4 (l >> 4) + r.
-    float *Cout = g.C + (g.splits > 1 ? (int64_t)blockIdx.z * g.split_stride : 0);
+    float *Cout = g.C + (g.splits > 1 ? (int64_t)split * g.split_stride : 0);
     const bool direct = g.splits > 1 || g.beta == 0.f;
     const bool interior = (m0 + TILE <= g.M) && (n0 + TILE <= g.N);
 #pragma unroll
@@ -142,7 +153,7 @@ void launch(const GemmF32 &g, hipStream_t s) {
     const int tile = (g.M >= 128 && g.N >= 128) ? 128 : 64;
     int kchunk = (g.K + g.splits - 1) / g.splits;
     kchunk = ((kchunk + BK - 1) / BK) * BK;
-    dim3 grid((g.M + tile - 1) / tile, (g.N + tile - 1) / tile, g.splits);
+    dim3 grid((g.M + tile - 1) / tile, (g.N + tile - 1) / tile, g.splits * std::max(g.batch, 1));
     if (tile == 128)
         hipLaunchKernelGGL((k_gemm_f32<128, TA, TB>), grid, dim3(NT), 0, s, g, kchunk);
     else
@@ -155,7 +166,9 @@ int gemm_f32(const GemmF32 &g, hipStream_t s) {
     if (g.M <= 0 || g.N <= 0) return MQR_SUCCESS;
     if (g.K <= 0) return MQR_ERR_INTERNAL;
     const double M = g.M, N = g.N, K = g.K;
-    ProfScope ps(g.kind, 2.0 * M * N * K, 4.0 * (M * K + K * N + 2.0 * M * N * g.splits), s);
+    const double nb = std::max(g.batch, 1);
+    ProfScope ps(g.kind, 2.0 * M * N * K * nb, 4.0 * nb * (M * K + K * N + 2.0 * M * N * g.splits),
+                 s);
     if (!g.ta && !g.tb) launch<false, false>(g, s);
     else if (g.ta && !g.tb) launch<true, false>(g, s);
     else if (!g.ta && g.tb) launch<false, true>(g, s);

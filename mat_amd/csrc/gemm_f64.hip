@@ -12,6 +12,7 @@
 // Two tiles: 64 x 64 (four waves, 2 x 2 MFMA blocks each) for small or thin
 // products, and 128 x 128 with a register-prefetched double-buffered LDS
 // pipeline (four waves, 4 x 4 MFMA blocks each) for the large ones.
+#include <algorithm>
 #include <climits>
 
 #include "mqr_internal.h"
@@ -31,9 +32,9 @@ constexpr int NT = 256;
 // row group, i.e. coalesced column-major stores.
 template <int BI, int BJ>
 __device__ __forceinline__ void epilogue(const GemmF64 &g, v4d (&acc)[BI][BJ], int rowbase,
-                                         int colbase, bool interior) {
+                                         int colbase, bool interior, int split) {
     const int lane = threadIdx.x & 63;
-    double *Cout = g.C + (g.splits > 1 ? (int64_t)blockIdx.z * g.split_stride : 0);
+    double *Cout = g.C + (g.splits > 1 ? (int64_t)split * g.split_stride : 0);
     const bool direct = g.splits > 1 || g.beta == 0.0;
 #pragma unroll
     for (int i = 0; i < BI; ++i) {
@@ -131,15 +132,25 @@ __device__ __forceinline__ void stash(double (*S)[TILE + 2], int tid, const doub
 }
 
 template <int TILE, bool TA, bool TB, int AM, int BMK>
-__global__ __launch_bounds__(NT) void k_gemm_f64(GemmF64 g, int kchunk) {
+__global__ __launch_bounds__(NT) void k_gemm_f64(GemmF64 gin, int kchunk) {
     constexpr int NV = TILE * BK / NT, BW = TILE / 32, HALF = TILE / 2;
     __shared__ double As[2][BK][TILE + 2];
     __shared__ double Bs[2][BK][TILE + 2];
-    if (g.skip_if && *g.skip_if) return;
+    if (gin.skip_if && *gin.skip_if) return;
+    // blockIdx.z = batch index * splits + split index
+    GemmF64 g = gin;
+    int split = blockIdx.z;
+    if (g.batch > 1) {
+        const int bz = blockIdx.z / g.splits;
+        split = blockIdx.z - bz * g.splits;
+        g.A += bz * g.strideA;
+        g.B += bz * g.strideB;
+        g.C += bz * g.strideC;
+    }
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int m0 = blockIdx.x * TILE, n0 = blockIdx.y * TILE;
-    const int kb = blockIdx.z * kchunk;
+    const int kb = split * kchunk;
     const int ke = min(g.K, kb + kchunk);
 
     v4d acc[BW][BW];
@@ -188,7 +199,7 @@ __global__ __launch_bounds__(NT) void k_gemm_f64(GemmF64 g, int kchunk) {
         }
     }
     const bool interior = (m0 + TILE <= g.M) && (n0 + TILE <= g.N);
-    epilogue<BW, BW>(g, acc, m0 + wm * HALF, n0 + wn * HALF, interior);
+    epilogue<BW, BW>(g, acc, m0 + wm * HALF, n0 + wn * HALF, interior, split);
 }
 
 constexpr int BIG = 128, SMALL = 64;
@@ -198,7 +209,7 @@ void launch(const GemmF64 &g, hipStream_t s) {
     const int tile = (g.M >= BIG && g.N >= BIG) ? BIG : SMALL;
     int kchunk = (g.K + g.splits - 1) / g.splits;
     kchunk = ((kchunk + BK - 1) / BK) * BK;
-    dim3 grid((g.M + tile - 1) / tile, (g.N + tile - 1) / tile, g.splits);
+    dim3 grid((g.M + tile - 1) / tile, (g.N + tile - 1) / tile, g.splits * std::max(g.batch, 1));
     if (tile == BIG)
         hipLaunchKernelGGL((k_gemm_f64<BIG, TA, TB, AM, BMK>), grid, dim3(NT), 0, s, g, kchunk);
     else
@@ -239,7 +250,9 @@ int gemm_f64(const GemmF64 &g, hipStream_t s) {
     const int key = (g.ta ? 8 : 0) | (g.tb ? 4 : 0) | (g.amask ? 2 : 0) | (g.bmask ? 1 : 0);
     const double M = g.M, N = g.N, K = g.K;
     const double out_rw = (g.splits > 1 || g.beta == 0.0) ? 1.0 : 2.0;
-    ProfScope ps(g.kind, 2.0 * M * N * K, 8.0 * (M * K + K * N + out_rw * M * N * g.splits), s);
+    const double nb = std::max(g.batch, 1);
+    ProfScope ps(g.kind, 2.0 * M * N * K * nb,
+                 8.0 * nb * (M * K + K * N + out_rw * M * N * g.splits), s);
     switch (key) {
         case 0: launch<false, false, MASK_NONE, MASK_NONE>(g, s); break;
         case 2: launch<false, false, MASK_VLOWER, MASK_NONE>(g, s); break;

@@ -134,6 +134,9 @@ struct GemmF64 {
     int kind = K_GEMM;  // profiler bucket
     // when non-null and *skip_if != 0 at run time, the launch does nothing
     const int *skip_if = nullptr;
+    // strided batch: `batch` products, operand i at A + i * strideA, ...
+    int batch = 1;
+    int64_t strideA = 0, strideB = 0, strideC = 0;
 };
 int gemm_f64(const GemmF64 &g, hipStream_t s);
 
@@ -152,6 +155,8 @@ struct GemmF32 {
     int64_t split_stride = 0;
     int kind = K_GEMM;
     int layout = 0;  // accumulator layout selector (0 is the gfx950 one)
+    int batch = 1;   // strided batch as in GemmF64
+    int64_t strideA = 0, strideB = 0, strideC = 0;
 };
 int gemm_f32(const GemmF32 &g, hipStream_t s);
 // C = beta * C (beta == 0 writes zeros).
