@@ -112,6 +112,11 @@ int apply_block(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double 
 
 }  // namespace
 
+int form_y_f64(hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv, const double *Tp,
+               int64_t ldt, bool trans, double *Y) {
+    return form_y(st, h, b, Vp, ldv, Tp, ldt, trans, Y);
+}
+
 // Workspace (doubles) the factorisation of an m x n matrix uses: split-K
 // partials of the trailing update and of the next-block update, two Y
 // factors, and the tall-panel scratch.

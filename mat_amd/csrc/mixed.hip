@@ -62,37 +62,36 @@ __global__ void k_cast_v(int64_t h, int b, const double *__restrict__ V, int64_t
     }
 }
 
-// W2 (f32, b x nc, ld b) = T^T sum_s W_s with the split-K partials W_s (f32,
-// ld b, stride pstride) summed and multiplied in FP64.  T^T is staged so that
-// consecutive lanes (rows r) read consecutive LDS words.
-__global__ __launch_bounds__(256) void k_w2_f32(int b, int64_t nc, const double *__restrict__ T,
-                                                int64_t ldt, const float *__restrict__ W,
-                                                int splits, int64_t pstride,
-                                                float *__restrict__ W2) {
-    __shared__ double Ts[64 * 64];  // Ts[l * b + r] = T(l, r)
-    __shared__ double Ws[64 * 32];
-    const int tid = threadIdx.x;
-    const int64_t n0 = (int64_t)blockIdx.x * 32;
-    for (int e = tid; e < b * b; e += 256) {
-        const int r = e % b, c = e / b;
-        Ts[r * b + c] = r <= c ? T[r + (int64_t)c * ldt] : 0.0;
+// P[0:count] = sum over s of P[s * stride + 0:count] (FP32 partials summed in
+// FP64)
+__global__ void k_reduce_f32(int64_t count, float *__restrict__ P, int splits, int64_t stride) {
+    const int64_t e = gidx();
+    if (e >= count) return;
+    double a0 = 0.0, a1 = 0.0;
+    int s = 0;
+    for (; s + 1 < splits; s += 2) {
+        a0 += (double)P[(int64_t)s * stride + e];
+        a1 += (double)P[(int64_t)(s + 1) * stride + e];
     }
-    for (int e = tid; e < b * 32; e += 256) {
-        const int64_t n = n0 + e / b;
-        double acc = 0.0;
-        if (n < nc)
-            for (int s = 0; s < splits; ++s) acc += (double)W[(int64_t)s * pstride + e % b + n * b];
-        Ws[e] = acc;
+    if (s < splits) a0 += (double)P[(int64_t)s * stride + e];
+    P[e] = (float)(a0 + a1);
+}
+
+// R (ldr) entries of the block rows right of the diagonal blocks:
+// R[r][c] = Aw[r][c] for c >= the end of r's nb-block (FP32 -> FP64)
+__global__ void k_copy_rupper(int64_t n, int nb, const float *__restrict__ Aw, int64_t ldw,
+                              double *__restrict__ R, int64_t ldr) {
+    for (int64_t e = gidx(); e < n * n; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = e % n, c = e / n;
+        if (c >= (r / nb + 1) * nb) R[r + c * ldr] = (double)Aw[r + c * ldw];
+        else if (r > c) R[r + c * ldr] = 0.0;
     }
-    __syncthreads();
-    for (int e = tid; e < b * 32; e += 256) {
-        const int r = e % b, col = e / b;
-        const int64_t n = n0 + col;
-        if (n >= nc) continue;
-        double acc = 0.0;
-        for (int l = 0; l <= r; ++l) acc = fma(Ts[l * b + r], Ws[l + col * b], acc);
-        W2[r + n * b] = (float)acc;
-    }
+}
+
+// x[0:n] = (double) src[0:n]
+__global__ void k_vec_s2d(int64_t n, const float *__restrict__ src, double *__restrict__ x) {
+    const int64_t i = gidx();
+    if (i < n) x[i] = (double)src[i];
 }
 
 // Sum of squares of an m x n FP64 matrix into out[blockIdx.x] (partials).
@@ -225,13 +224,6 @@ __global__ __launch_bounds__(small::NT) void k_inv_diag_blocks(int64_t n,
     small::lds_store(S, X + i0 + i0 * ldx, ldx, b, b);
 }
 
-// R (ldr) rows [0, b), columns [0, nt) = Aw rows [0, b) (FP32 -> FP64)
-__global__ void k_copy_rrows(int b, int64_t nt, const float *__restrict__ Aw, int64_t ldw,
-                             double *__restrict__ R, int64_t ldr) {
-    for (int64_t e = gidx(); e < (int64_t)b * nt; e += (int64_t)gridDim.x * blockDim.x)
-        R[e % b + (e / b) * ldr] = (double)Aw[e % b + (e / b) * ldw];
-}
-
 // R (ldr) b x b = upper triangle of the factored panel P (ld), zero below
 __global__ void k_copy_rdiag(int b, const double *__restrict__ P, int64_t ld,
                              double *__restrict__ R, int64_t ldr) {
@@ -252,7 +244,9 @@ struct MixedWs {
     double *Pb = nullptr;  // m x nb FP64 panel (FP32 mode)
     double *R = nullptr;   // R factor, ld ldr
     int64_t ldr = 0;
-    float *V32 = nullptr, *P32 = nullptr, *W2 = nullptr;
+    float *V32[2] = {nullptr, nullptr}, *Y32[2] = {nullptr, nullptr};
+    float *P32 = nullptr, *P32b = nullptr;
+    double *Pb2 = nullptr, *Yd = nullptr;
     double *X = nullptr, *T = nullptr, *tau = nullptr;
     double *r = nullptr, *z = nullptr, *y = nullptr, *part = nullptr, *sc = nullptr;
     double *xx = nullptr, *zz = nullptr, *tmp = nullptr;
@@ -276,13 +270,18 @@ MixedWs carve(char *base, bool fp64, int64_t m, int64_t n, int nb) {
         w.R = w.F;
         w.ldr = m;
     } else {
-        w.Aw = (float *)take(sizeof(float) * m * n);
+        w.Aw = (float *)take(sizeof(float) * m * (n + 1));
         w.Pb = (double *)take(sizeof(double) * m * nb);
+        w.Pb2 = (double *)take(sizeof(double) * m * nb);
+        w.Yd = (double *)take(sizeof(double) * m * nb);
         w.R = (double *)take(sizeof(double) * n * n);
         w.ldr = n;
-        w.V32 = (float *)take(sizeof(float) * m * nb);
-        w.P32 = (float *)take(sizeof(float) * kMaxSplits * nb * n);
-        w.W2 = (float *)take(sizeof(float) * nb * n);
+        for (int i = 0; i < 2; ++i) {
+            w.V32[i] = (float *)take(sizeof(float) * m * nb);
+            w.Y32[i] = (float *)take(sizeof(float) * m * nb);
+        }
+        w.P32 = (float *)take(sizeof(float) * kMaxSplits * nb * (n + 1));
+        w.P32b = (float *)take(sizeof(float) * kMaxSplits * nb * nb);
     }
     w.X = (double *)take(sizeof(double) * n * n);
     w.tmp = (double *)take(sizeof(double) * ((n + 1) / 2) * n);
@@ -300,71 +299,117 @@ MixedWs carve(char *base, bool fp64, int64_t m, int64_t n, int nb) {
     return w;
 }
 
-// Factorisation with FP32 trailing updates: R (upper, ld ldr) is written.
-int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t lda,
-                MixedWs &w) {
-    hipStream_t s = hd->stream;
-    const int nb = hd->nb;
-    const int64_t k = std::min(m, n);
-    {
-        ProfScope ps(K_CAST, 0.0, 12.0 * m * n, s);
-        hipLaunchKernelGGL(k_cast_d2s, dim3(grid_for(m * n)), dim3(256), 0, s, m, n, A, lda, w.Aw,
-                           m);
+// C <- C - Y32 (V32^T C) in FP32 on stream st (C: h x nc at ldc), the block
+// reflector application of the FP32 path; P: split-K partials (pcap floats).
+int apply_block_f32(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const float *V32,
+                    const float *Y32, float *C, int64_t ldc, int64_t nc, float *P, int64_t pcap,
+                    int kind_w, int kind_u) {
+    if (nc <= 0) return MQR_SUCCESS;
+    int sp = std::min(kMaxSplits, gemm_pick_splits(b, (int)nc, (int)h, 4 * hd->cu_count));
+    while (sp > 1 && (int64_t)sp * b * nc > pcap) --sp;
+    GemmF32 f;
+    f.M = b; f.N = (int)nc; f.K = (int)h; f.ta = true;
+    f.A = V32; f.lda = h; f.B = C; f.ldb = ldc;
+    f.C = P; f.ldc = b; f.splits = sp; f.split_stride = (int64_t)b * nc;
+    f.kind = kind_w;
+    MQR_CHECK(gemm_f32(f, st));
+    if (sp > 1) {
+        ProfScope ps(kind_w, 0.0, 4.0 * b * nc * (sp + 1), st);
+        hipLaunchKernelGGL(k_reduce_f32, dim3((unsigned)(((int64_t)b * nc + 255) / 256)),
+                           dim3(256), 0, st, (int64_t)b * nc, P, sp, (int64_t)b * nc);
         MQR_LAUNCH_CHECK();
     }
-    for (int64_t j = 0; j < k; j += nb) {
-        const int b = (int)std::min<int64_t>(nb, k - j);
+    GemmF32 u;
+    u.M = (int)h; u.N = (int)nc; u.K = b;
+    u.A = Y32; u.lda = h; u.B = P; u.ldb = b;
+    u.C = C; u.ldc = ldc; u.alpha = -1.f; u.beta = 1.f; u.kind = kind_u;
+    return gemm_f32(u, st);
+}
+
+// FP64 panel j of the FP32 working matrix: cast to FP64 (V, ld m), factor,
+// copy its R block, form Y = V T^T and the FP32 copies V32, Y32 (ld h).
+int panel_fp32(mqr_handle_t hd, int64_t m, int64_t j, int b, MixedWs &w, double *V, float *V32,
+               float *Y32) {
+    hipStream_t s = hd->stream;
+    const int nb = hd->nb;
+    const int64_t h = m - j;
+    {
+        ProfScope ps(K_CAST, 0.0, 12.0 * h * b, s);
+        hipLaunchKernelGGL(k_cast_s2d, dim3(grid_for(h * b)), dim3(256), 0, s, h, (int64_t)b,
+                           (const float *)(w.Aw + j + j * m), m, V, m);
+        MQR_LAUNCH_CHECK();
+    }
+    double *Tj = w.T + j * nb;
+    MQR_CHECK(panel_factor(hd, h, b, V, m, w.tau + j, Tj, nb, w.pws));
+    MQR_CHECK(form_y_f64(s, h, b, V, m, Tj, nb, true, w.Yd));
+    {
+        ProfScope ps(K_CAST, 0.0, 16.0 * b * b + 28.0 * h * b, s);
+        hipLaunchKernelGGL(k_copy_rdiag, dim3((unsigned)((b * b + 255) / 256)), dim3(256), 0, s, b,
+                           (const double *)V, m, w.R + j + j * w.ldr, w.ldr);
+        hipLaunchKernelGGL(k_cast_v, dim3(grid_for(h * b)), dim3(256), 0, s, h, b,
+                           (const double *)V, m, V32);
+        hipLaunchKernelGGL(k_cast_d2s, dim3(grid_for(h * b)), dim3(256), 0, s, h, (int64_t)b,
+                           (const double *)w.Yd, h, Y32, h);
+        MQR_LAUNCH_CHECK();
+    }
+    return MQR_SUCCESS;
+}
+
+// Factorisation with FP32 trailing updates of [A | b]: R (upper, ld ldr) is
+// written, and Aw[0:n, n] ends as Q^T b (FP32).  Look-ahead of one block as
+// in the FP64 factorisation: the handle's stream updates the next block and
+// factors the next panel while the side stream updates the rest.
+int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t lda,
+                const double *b, MixedWs &w) {
+    hipStream_t s0 = hd->stream, s1 = hd->side_stream;
+    const int nb = hd->nb;
+    const int64_t k = std::min(m, n), nc = n + 1;
+    const int64_t pcap = (int64_t)kMaxSplits * nb * nc, pcap2 = (int64_t)kMaxSplits * nb * nb;
+    {
+        ProfScope ps(K_CAST, 0.0, 12.0 * m * nc, s0);
+        hipLaunchKernelGGL(k_cast_d2s, dim3(grid_for(m * n)), dim3(256), 0, s0, m, n, A, lda,
+                           w.Aw, m);
+        hipLaunchKernelGGL(k_cast_d2s, dim3(grid_for(m)), dim3(256), 0, s0, m, (int64_t)1, b, m,
+                           w.Aw + n * m, m);
+        MQR_LAUNCH_CHECK();
+    }
+    MQR_HIP(hipEventRecord(hd->ev_fork, s0));
+    MQR_HIP(hipStreamWaitEvent(s1, hd->ev_fork, 0));
+    double *Vb[2] = {w.Pb, w.Pb2};
+    bool side_pending = false;
+    MQR_CHECK(panel_fp32(hd, m, 0, (int)std::min<int64_t>(nb, k), w, Vb[0], w.V32[0], w.Y32[0]));
+    for (int64_t j = 0, it = 0; j < k; j += nb, ++it) {
+        const int bj = (int)std::min<int64_t>(nb, k - j);
         const int64_t h = m - j;
-        double *V = w.Pb;  // ld m, h rows
-        {
-            ProfScope ps(K_CAST, 0.0, 12.0 * h * b, s);
-            hipLaunchKernelGGL(k_cast_s2d, dim3(grid_for(h * b)), dim3(256), 0, s, h, (int64_t)b,
-                               (const float *)(w.Aw + j + j * m), m, V, m);
-            MQR_LAUNCH_CHECK();
+        const float *V32 = w.V32[it & 1], *Y32 = w.Y32[it & 1];
+        const int64_t next = j + bj;
+        const int64_t bn = std::min<int64_t>(nb, nc - next);
+        const int64_t rest0 = next + bn, nrest = nc - rest0;
+        if (side_pending) MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
+        if (nrest > 0) {
+            MQR_HIP(hipEventRecord(hd->ev_fork, s0));
+            MQR_HIP(hipStreamWaitEvent(s1, hd->ev_fork, 0));
+            MQR_CHECK(apply_block_f32(hd, s1, h, bj, V32, Y32, w.Aw + j + rest0 * m, m, nrest,
+                                      w.P32, pcap, K_VTA, K_UPDATE));
+            MQR_HIP(hipEventRecord(hd->ev_join, s1));
+            side_pending = true;
+        } else {
+            side_pending = false;
         }
-        double *Tj = w.T + j * nb;
-        MQR_CHECK(panel_factor(hd, h, b, V, m, w.tau + j, Tj, nb, w.pws));
-        {
-            ProfScope ps(K_CAST, 0.0, 16.0 * b * b, s);
-            hipLaunchKernelGGL(k_copy_rdiag, dim3((unsigned)((b * b + 255) / 256)), dim3(256), 0, s,
-                               b, (const double *)V, m, w.R + j + j * w.ldr, w.ldr);
-            MQR_LAUNCH_CHECK();
+        MQR_CHECK(apply_block_f32(hd, s0, h, bj, V32, Y32, w.Aw + j + next * m, m, bn, w.P32b,
+                                  pcap2, K_LOOKAHEAD, K_LOOKAHEAD));
+        if (next < k) {
+            const int bnext = (int)std::min<int64_t>(nb, k - next);
+            MQR_CHECK(panel_fp32(hd, m, next, bnext, w, Vb[(it + 1) & 1], w.V32[(it + 1) & 1],
+                                 w.Y32[(it + 1) & 1]));
         }
-        const int64_t nt = n - j - b;
-        if (nt <= 0) continue;
-        {
-            ProfScope ps(K_CAST, 0.0, 12.0 * h * b, s);
-            hipLaunchKernelGGL(k_cast_v, dim3(grid_for(h * b)), dim3(256), 0, s, h, b,
-                               (const double *)V, m, w.V32);
-            MQR_LAUNCH_CHECK();
-        }
-        // C <- (I - V T V^T)^T C in FP32: W = V^T C, W2 = T^T W, C -= V W2
-        float *C = w.Aw + j + (j + b) * m;
-        const int sp = std::min(kMaxSplits, gemm_pick_splits(b, (int)nt, (int)h, 4 * hd->cu_count));
-        GemmF32 f;
-        f.M = b; f.N = (int)nt; f.K = (int)h; f.ta = true;
-        f.A = w.V32; f.lda = h; f.B = C; f.ldb = m;
-        f.C = w.P32; f.ldc = b; f.splits = sp; f.split_stride = (int64_t)b * nt;
-        f.kind = K_VTA;
-        MQR_CHECK(gemm_f32(f, s));
-        {
-            ProfScope ps(K_W2, (double)b * b * nt, 4.0 * b * nt * (sp + 1), s);
-            hipLaunchKernelGGL(k_w2_f32, dim3((unsigned)((nt + 31) / 32)), dim3(256), 0, s, b, nt,
-                               (const double *)Tj, (int64_t)nb, (const float *)w.P32, sp,
-                               (int64_t)b * nt, w.W2);
-            MQR_LAUNCH_CHECK();
-        }
-        GemmF32 u;
-        u.M = (int)h; u.N = (int)nt; u.K = b;
-        u.A = w.V32; u.lda = h; u.B = w.W2; u.ldb = b;
-        u.C = C; u.ldc = m; u.alpha = -1.f; u.beta = 1.f; u.kind = K_UPDATE;
-        MQR_CHECK(gemm_f32(u, s));
-        {
-            ProfScope ps(K_CAST, 0.0, 12.0 * b * nt, s);
-            hipLaunchKernelGGL(k_copy_rrows, dim3(grid_for((int64_t)b * nt)), dim3(256), 0, s, b,
-                               nt, (const float *)C, m, w.R + j + (j + b) * w.ldr, w.ldr);
-            MQR_LAUNCH_CHECK();
-        }
+    }
+    if (side_pending) MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
+    {
+        ProfScope ps(K_CAST, 0.0, 12.0 * n * n, s0);
+        hipLaunchKernelGGL(k_copy_rupper, dim3(grid_for(n * n, 256, 1 << 20)), dim3(256), 0, s0, n,
+                           nb, (const float *)w.Aw, m, w.R, w.ldr);
+        MQR_LAUNCH_CHECK();
     }
     return MQR_SUCCESS;
 }
@@ -384,16 +429,36 @@ int tri_inverse(mqr_handle_t hd, int64_t n, const double *R, int64_t ldr, double
         MQR_LAUNCH_CHECK();
     }
     for (int64_t sz = 64; sz < n; sz *= 2) {
-        for (int64_t i = 0; i + sz < n; i += 2 * sz) {
-            const int64_t sc = std::min<int64_t>(sz, n - i - sz);
-            // tmp (sz x sc) = R[i:i+sz, i+sz:i+sz+sc] * X_C
+        // all full pairs of one level as one batched product each:
+        // tmp = R[i:i+sz, i+sz:i+2sz] X_C,  X[i:i+sz, i+sz:i+2sz] = -X_A tmp
+        const int64_t full = n / (2 * sz);
+        const int64_t dstep = 2 * sz * (1 + ldr), xstep = 2 * sz * (1 + n);
+        if (full > 0) {
+            GemmF64 g;
+            g.M = (int)sz; g.N = (int)sz; g.K = (int)sz;
+            g.A = R + sz * ldr; g.lda = ldr; g.strideA = dstep;
+            g.B = X + sz + sz * n; g.ldb = n; g.strideB = xstep;
+            g.C = tmp; g.ldc = sz; g.strideC = sz * sz;
+            g.batch = (int)full; g.kind = K_TRSM;
+            MQR_CHECK(gemm_f64(g, s));
+            GemmF64 h2;
+            h2.M = (int)sz; h2.N = (int)sz; h2.K = (int)sz;
+            h2.A = X; h2.lda = n; h2.strideA = xstep;
+            h2.B = tmp; h2.ldb = sz; h2.strideB = sz * sz;
+            h2.C = X + sz * n; h2.ldc = n; h2.strideC = xstep;
+            h2.alpha = -1.0; h2.batch = (int)full; h2.kind = K_TRSM;
+            MQR_CHECK(gemm_f64(h2, s));
+        }
+        // a trailing partial pair
+        const int64_t i = full * 2 * sz;
+        if (i + sz < n) {
+            const int64_t sc = n - i - sz;
             GemmF64 g;
             g.M = (int)sz; g.N = (int)sc; g.K = (int)sc;
             g.A = R + i + (i + sz) * ldr; g.lda = ldr;
             g.B = X + (i + sz) + (i + sz) * n; g.ldb = n;
             g.C = tmp; g.ldc = sz; g.kind = K_TRSM;
             MQR_CHECK(gemm_f64(g, s));
-            // X[i:i+sz, i+sz:i+sz+sc] = -X_A * tmp
             GemmF64 h2;
             h2.M = (int)sz; h2.N = (int)sc; h2.K = (int)sz;
             h2.A = X + i + i * n; h2.lda = n; h2.B = tmp; h2.ldb = sz;
@@ -520,12 +585,24 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
     if (fp64)
         MQR_CHECK(geqrf_f64(hd, m, n, w.F, m, w.tau, w.T, nb));
     else
-        MQR_CHECK(factor_fp32(hd, m, n, A, lda, w));
+        MQR_CHECK(factor_fp32(hd, m, n, A, lda, b, w));
     MQR_CHECK(tri_inverse(hd, n, w.R, w.ldr, w.X, w.tmp));
 
+    // starting point: the QR solution x0 = R^{-1} (Q^T b)[0:n] of the
+    // factorisation (the FP32 path carried b through its updates as column n)
     hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, s, n, x, 0.0);
     hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, s, n, w.xx, 0.0);
     MQR_LAUNCH_CHECK();
+    if (fp64) {
+        MQR_HIP(hipMemcpyAsync(w.r, b, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
+        MQR_CHECK(ormqr_f64(hd, true, m, 1, n, w.F, m, w.T, nb, w.r, m));
+        MQR_HIP(hipMemcpyAsync(w.y, w.r, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    } else {
+        hipLaunchKernelGGL(k_vec_s2d, dim3(grid_for(n)), dim3(256), 0, s, n,
+                           (const float *)(w.Aw + n * m), w.y);
+        MQR_LAUNCH_CHECK();
+    }
+    MQR_CHECK(correction(hd, n, x, w));
     double hA;
     MQR_CHECK(read_scalars(hd, w.sc, 1, &hA));
     const double nA = std::sqrt(hA);

@@ -184,6 +184,10 @@ constexpr int64_t kCholqrMinRows = 384;
 // Either of the above by height.
 int panel_factor(mqr_handle_t hd, int64_t h, int b, double *V, int64_t lda, double *tau,
                  double *T, int64_t ldt, void *ws);
+// Y (h x b, ld h) = V op(T) (op = transpose when trans): the left factor of
+// the block reflector I - V T V^T = I - (V T) V^T.
+int form_y_f64(hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv, const double *Tp,
+               int64_t ldt, bool trans, double *Y);
 // P[0:count] += P[s*stride + 0:count] for s = 1..splits-1.
 int reduce_partials(int64_t count, double *P, int splits, int64_t stride, hipStream_t s);
 // Largest block size the LDS-resident small kernels accept.
