@@ -105,9 +105,10 @@ __device__ __forceinline__ void keep_upper(double (&a)[RK]) {
 
 // Upper Cholesky factor R (G = R^T R) in place.  Entries below the diagonal
 // are updated as well (never read) and zeroed at the end.  *fail (LDS) is set
-// when one of the first b pivots is <= thr or NaN.  dv: 2 N doubles of LDS.
+// when one of the first b pivots is <= its threshold (thr_row[i] when given,
+// else thr) or NaN.  dv: 2 N doubles of LDS.
 __device__ __forceinline__ void chol(double (&a)[RK], RowBuf &rb, double *dv, int b, double thr,
-                                     int *fail) {
+                                     int *fail, const double *thr_row = nullptr) {
     const int c = lane_id(), q = wave_id();
     double *dinv = dv, *dval = dv + N;
 #pragma unroll
@@ -121,7 +122,7 @@ __device__ __forceinline__ void chol(double (&a)[RK], RowBuf &rb, double *dv, in
                 const double d = bcast(a[j0 + s], i);
                 const double di = recip(d);
                 if (c == 0) {
-                    if (i < b && !(d > thr)) *fail = 1;
+                    if (i < b && !(d > (thr_row ? thr_row[i] : thr))) *fail = 1;
                     dinv[i] = di;
                     dval[i] = d;
                 }
