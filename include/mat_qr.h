@@ -58,8 +58,11 @@ typedef enum { MQR_F64 = 0, MQR_F32 = 1 } mqr_dtype_t;
 typedef enum {
     MQR_PREC_FP64 = 0,   /* plain FP64 MFMA */
     MQR_PREC_FP32 = 1,   /* FP32 MFMA, FP32 accumulation */
-    MQR_PREC_BF16X3 = 2, /* split-BF16 (hi*hi + hi*lo + lo*hi) MFMA, FP32 accumulation */
-    MQR_PREC_BF16 = 3    /* single BF16 MFMA product, FP32 accumulation */
+    MQR_PREC_BF16X3 = 2, /* split-BF16 (hi*hi + hi*lo + lo*hi) MFMA, FP32 accumulation:
+                            ~2^-16 per product, converges for kappa up to ~1e4 */
+    MQR_PREC_BF16 = 3    /* single BF16 MFMA product, FP32 accumulation: ~2^-8,
+                            converges for kappa up to ~1e2; beyond, refinement
+                            stagnates and the FP64 fallback runs */
 } mqr_precision_t;
 
 /* ---- library / handle ---------------------------------------------------- */
@@ -134,6 +137,15 @@ int mqr_dgemm(mqr_handle_t handle, char transa, char transb, int64_t m, int64_t 
 int mqr_sgemm(mqr_handle_t handle, char transa, char transb, int64_t m, int64_t n, int64_t k,
               float alpha, const float *A, int64_t lda, const float *B, int64_t ldb, float beta,
               float *C, int64_t ldc);
+
+/* The same product on the BF16 MFMA with FP32 accumulation: every FP32
+ * operand x is split as hi = bf16(x), lo = bf16(x - hi); terms = 1 multiplies
+ * hi * hi, terms = 3 adds hi * lo + lo * hi (relative error ~2^-16).  The
+ * precision the mixed solve's MQR_PREC_BF16 / MQR_PREC_BF16X3 updates use.
+ * No reference counterpart (the reference has no BF16 path). */
+int mqr_sgemm_bf16(mqr_handle_t handle, int terms, char transa, char transb, int64_t m, int64_t n,
+                   int64_t k, float alpha, const float *A, int64_t lda, const float *B,
+                   int64_t ldb, float beta, float *C, int64_t ldc);
 
 /* ---- FP64 blocked Householder QR (device pointers) ------------------------ */
 /* Factor the m x n matrix A in place: R on and above the diagonal, the

@@ -58,6 +58,8 @@ SIGNATURES = {
                          _dbl, _vp, _i64]),
     "mqr_sgemm": (_i32, [_vp, C.c_char, C.c_char, _i64, _i64, _i64, C.c_float, _vp, _i64, _vp,
                          _i64, C.c_float, _vp, _i64]),
+    "mqr_sgemm_bf16": (_i32, [_vp, _i32, C.c_char, C.c_char, _i64, _i64, _i64, C.c_float, _vp,
+                              _i64, _vp, _i64, C.c_float, _vp, _i64]),
     "mqr_dgeqrf": (_i32, [_vp, _i64, _i64, _vp, _i64, _vp, _vp, _i64]),
     "mqr_dorgqr": (_i32, [_vp, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64]),
     "mqr_dormqr": (_i32, [_vp, C.c_char, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64]),
@@ -245,6 +247,18 @@ def _ptr(t) -> C.c_void_p:
 # ---------------------------------------------------------------------------
 # device-level entry points
 # ---------------------------------------------------------------------------
+
+def gemm_bf16(h: Handle, terms: int, transa: str, transb: str, alpha: float, A, B, beta: float,
+              Cm):
+    """C = alpha op(A) op(B) + beta C for float32 column-major tensors on the
+    BF16 MFMA (terms 1: hi*hi; 3: hi*hi + hi*lo + lo*hi), FP32 accumulation."""
+    m, n = Cm.shape
+    k = A.shape[0] if transa == "T" else A.shape[1]
+    check(lib().mqr_sgemm_bf16(h.h, terms, transa.encode(), transb.encode(), m, n, k, alpha,
+                               _ptr(A), ld(A), _ptr(B), ld(B), beta, _ptr(Cm), ld(Cm)),
+          "mqr_sgemm_bf16")
+    return Cm
+
 
 def gemm(h: Handle, transa: str, transb: str, alpha: float, A, B, beta: float, Cm):
     """C = alpha op(A) op(B) + beta C on column-major device tensors (float64

@@ -310,9 +310,9 @@ int mqr_dgemm(mqr_handle_t h, char transa, char transb, int64_t m, int64_t n, in
     return gemm_f64(g, h->stream);
 }
 
-int mqr_sgemm(mqr_handle_t h, char transa, char transb, int64_t m, int64_t n, int64_t k,
-              float alpha, const float *A, int64_t lda, const float *B, int64_t ldb, float beta,
-              float *C, int64_t ldc) {
+static int sgemm_common(mqr_handle_t h, int terms, char transa, char transb, int64_t m,
+                        int64_t n, int64_t k, float alpha, const float *A, int64_t lda,
+                        const float *B, int64_t ldb, float beta, float *C, int64_t ldc) {
     if (!h) return -1;
     const bool ta = transa == 'T' || transa == 't', tb = transb == 'T' || transb == 't';
     if (!ta && transa != 'N' && transa != 'n') return -2;
@@ -335,7 +335,24 @@ int mqr_sgemm(mqr_handle_t h, char transa, char transb, int64_t m, int64_t n, in
     g.ta = ta; g.tb = tb;
     g.A = A; g.lda = lda; g.B = B; g.ldb = ldb;
     g.C = C; g.ldc = ldc; g.alpha = alpha; g.beta = beta;
+    g.bf16_terms = terms;
     return gemm_f32(g, h->stream);
+}
+
+int mqr_sgemm(mqr_handle_t h, char transa, char transb, int64_t m, int64_t n, int64_t k,
+              float alpha, const float *A, int64_t lda, const float *B, int64_t ldb, float beta,
+              float *C, int64_t ldc) {
+    return sgemm_common(h, 0, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int mqr_sgemm_bf16(mqr_handle_t h, int terms, char transa, char transb, int64_t m, int64_t n,
+                   int64_t k, float alpha, const float *A, int64_t lda, const float *B,
+                   int64_t ldb, float beta, float *C, int64_t ldc) {
+    if (!h) return -1;
+    if (terms != 1 && terms != 3) return -2;
+    const int st = sgemm_common(h, terms, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                                ldc);
+    return st < 0 ? st - 1 : st;  // argument positions shift by one
 }
 
 int mqr_dgeqrf(mqr_handle_t h, int64_t m, int64_t n, double *A, int64_t lda, double *tau,

@@ -6,6 +6,12 @@
 // Same structure as the FP64 tiles: a TILE x TILE output tile per workgroup
 // of four waves, 16-deep K slices staged in LDS with register prefetch and
 // double buffering, branch-free clamped loads, split-K partial outputs.
+//
+// Split-BF16 variants (MODE 1 and 3) keep the FP32 operands in LDS and round
+// them to BF16 when the MFMA operands are formed: x = hi + lo with
+// hi = bf16(x), lo = bf16(x - hi); MODE 1 multiplies hi * hi, MODE 3 adds
+// hi * lo + lo * hi (the lo * lo term, ~2^-16 relative, is dropped), all on
+// v_mfma_f32_16x16x16_bf16 with FP32 accumulation.
 #include <algorithm>
 #include <climits>
 
@@ -15,7 +21,25 @@ namespace mqr {
 namespace {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef short v4s __attribute__((ext_vector_type(4)));
 constexpr int NT = 256, BK = 16;
+
+// float -> bf16 bits, round to nearest even (finite inputs)
+__device__ __forceinline__ unsigned bf16_bits(float x) {
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+
+// hi/lo BF16 operands of four FP32 values
+__device__ __forceinline__ void split4(const float (&x)[4], v4s &hi, v4s &lo) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const unsigned h = bf16_bits(x[q]);
+        hi[q] = (short)h;
+        lo[q] = (short)bf16_bits(x[q] - __uint_as_float(h << 16));
+    }
+}
 
 template <int TILE, bool KMAJOR>
 __device__ __forceinline__ void slot(int tid, int q, int &r, int &k) {
@@ -59,7 +83,7 @@ __device__ __forceinline__ void stash(float (*S)[TILE + 4], int tid, const float
     }
 }
 
-template <int TILE, bool TA, bool TB>
+template <int TILE, bool TA, bool TB, int MODE>
 __global__ __launch_bounds__(NT) void k_gemm_f32(GemmF32 gin, int kchunk) {
     constexpr int NV = TILE * BK / NT, BW = TILE / 32, HALF = TILE / 2;
     __shared__ float As[2][BK][TILE + 4];
@@ -100,20 +124,56 @@ __global__ __launch_bounds__(NT) void k_gemm_f32(GemmF32 gin, int kchunk) {
                 fetch<TILE, TA>(g.A, g.lda, m0, k0 + BK, g.M, ke, tid, va);
                 fetch<TILE, !TB>(g.B, g.ldb, n0, k0 + BK, g.N, ke, tid, vb);
             }
+            if constexpr (MODE == 0) {
 #pragma unroll
-            for (int kk = 0; kk < BK; kk += 4) {
-                float a[BW], b[BW];
+                for (int kk = 0; kk < BK; kk += 4) {
+                    float a[BW], b[BW];
 #pragma unroll
-                for (int i = 0; i < BW; ++i)
-                    a[i] = As[buf][kk + (lane >> 4)][wm * HALF + i * 16 + (lane & 15)];
-#pragma unroll
-                for (int j = 0; j < BW; ++j)
-                    b[j] = Bs[buf][kk + (lane >> 4)][wn * HALF + j * 16 + (lane & 15)];
-#pragma unroll
-                for (int i = 0; i < BW; ++i)
+                    for (int i = 0; i < BW; ++i)
+                        a[i] = As[buf][kk + (lane >> 4)][wm * HALF + i * 16 + (lane & 15)];
 #pragma unroll
                     for (int j = 0; j < BW; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(b[j], a[i], acc[i][j], 0, 0, 0);
+                        b[j] = Bs[buf][kk + (lane >> 4)][wn * HALF + j * 16 + (lane & 15)];
+#pragma unroll
+                    for (int i = 0; i < BW; ++i)
+#pragma unroll
+                        for (int j = 0; j < BW; ++j)
+                            acc[i][j] =
+                                __builtin_amdgcn_mfma_f32_16x16x4f32(b[j], a[i], acc[i][j], 0, 0, 0);
+                }
+            } else {
+                // one 16-deep MFMA per tile and term: lane l supplies
+                // k = 4 (l >> 4) + q, q = 0..3, of row / column l & 15
+                v4s ah[BW], al[BW], bh[BW], bl[BW];
+#pragma unroll
+                for (int i = 0; i < BW; ++i) {
+                    float x[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        x[q] = As[buf][4 * (lane >> 4) + q][wm * HALF + i * 16 + (lane & 15)];
+                    split4(x, ah[i], al[i]);
+                }
+#pragma unroll
+                for (int j = 0; j < BW; ++j) {
+                    float x[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        x[q] = Bs[buf][4 * (lane >> 4) + q][wn * HALF + j * 16 + (lane & 15)];
+                    split4(x, bh[j], bl[j]);
+                }
+#pragma unroll
+                for (int i = 0; i < BW; ++i)
+#pragma unroll
+                    for (int j = 0; j < BW; ++j) {
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bh[j], ah[i],
+                                                                              acc[i][j], 0, 0, 0);
+                        if constexpr (MODE == 3) {
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                                bh[j], al[i], acc[i][j], 0, 0, 0);
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                                bl[j], ah[i], acc[i][j], 0, 0, 0);
+                        }
+                    }
             }
             if (more) {
                 stash<TILE, TA>(As[buf ^ 1], tid, va, m0, k0 + BK, g.M, ke);
@@ -148,16 +208,23 @@ __global__ __launch_bounds__(NT) void k_gemm_f32(GemmF32 gin, int kchunk) {
     }
 }
 
-template <bool TA, bool TB>
-void launch(const GemmF32 &g, hipStream_t s) {
+template <bool TA, bool TB, int MODE>
+void launch_mode(const GemmF32 &g, hipStream_t s) {
     const int tile = (g.M >= 128 && g.N >= 128) ? 128 : 64;
     int kchunk = (g.K + g.splits - 1) / g.splits;
     kchunk = ((kchunk + BK - 1) / BK) * BK;
     dim3 grid((g.M + tile - 1) / tile, (g.N + tile - 1) / tile, g.splits * std::max(g.batch, 1));
     if (tile == 128)
-        hipLaunchKernelGGL((k_gemm_f32<128, TA, TB>), grid, dim3(NT), 0, s, g, kchunk);
+        hipLaunchKernelGGL((k_gemm_f32<128, TA, TB, MODE>), grid, dim3(NT), 0, s, g, kchunk);
     else
-        hipLaunchKernelGGL((k_gemm_f32<64, TA, TB>), grid, dim3(NT), 0, s, g, kchunk);
+        hipLaunchKernelGGL((k_gemm_f32<64, TA, TB, MODE>), grid, dim3(NT), 0, s, g, kchunk);
+}
+
+template <bool TA, bool TB>
+void launch(const GemmF32 &g, hipStream_t s) {
+    if (g.bf16_terms == 3) launch_mode<TA, TB, 3>(g, s);
+    else if (g.bf16_terms == 1) launch_mode<TA, TB, 1>(g, s);
+    else launch_mode<TA, TB, 0>(g, s);
 }
 
 }  // namespace

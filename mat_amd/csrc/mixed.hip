@@ -4,8 +4,9 @@
 //
 // Factorisation: A is copied to an FP32 working matrix; each panel is cast
 // back to FP64 and factored in FP64 (reflectors, tau, T and the diagonal
-// block of R stay FP64); the trailing update C -= V (T^T (V^T C)) runs as two
-// FP32 MFMA GEMMs with FP32 accumulation (the T product is formed in FP64).
+// block of R stay FP64); the trailing update C -= (V T^T) (V^T C) runs as two
+// MFMA GEMMs with FP32 accumulation on FP32, BF16 (hi * hi) or split-BF16
+// (hi * hi + hi * lo + lo * hi) products (Y = V T^T is formed in FP64).
 // The computed R is the exact R factor of a nearby matrix A + dA with
 // ||dA|| = O(u_32 ||A||).
 //
@@ -303,7 +304,7 @@ MixedWs carve(char *base, bool fp64, int64_t m, int64_t n, int nb) {
 // reflector application of the FP32 path; P: split-K partials (pcap floats).
 int apply_block_f32(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const float *V32,
                     const float *Y32, float *C, int64_t ldc, int64_t nc, float *P, int64_t pcap,
-                    int kind_w, int kind_u) {
+                    int kind_w, int kind_u, int bf16_terms) {
     if (nc <= 0) return MQR_SUCCESS;
     int sp = std::min(kMaxSplits, gemm_pick_splits(b, (int)nc, (int)h, 4 * hd->cu_count));
     while (sp > 1 && (int64_t)sp * b * nc > pcap) --sp;
@@ -311,7 +312,7 @@ int apply_block_f32(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const flo
     f.M = b; f.N = (int)nc; f.K = (int)h; f.ta = true;
     f.A = V32; f.lda = h; f.B = C; f.ldb = ldc;
     f.C = P; f.ldc = b; f.splits = sp; f.split_stride = (int64_t)b * nc;
-    f.kind = kind_w;
+    f.kind = kind_w; f.bf16_terms = bf16_terms;
     MQR_CHECK(gemm_f32(f, st));
     if (sp > 1) {
         ProfScope ps(kind_w, 0.0, 4.0 * b * nc * (sp + 1), st);
@@ -323,6 +324,7 @@ int apply_block_f32(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const flo
     u.M = (int)h; u.N = (int)nc; u.K = b;
     u.A = Y32; u.lda = h; u.B = P; u.ldb = b;
     u.C = C; u.ldc = ldc; u.alpha = -1.f; u.beta = 1.f; u.kind = kind_u;
+    u.bf16_terms = bf16_terms;
     return gemm_f32(u, st);
 }
 
@@ -360,7 +362,7 @@ int panel_fp32(mqr_handle_t hd, int64_t m, int64_t j, int b, MixedWs &w, double 
 // in the FP64 factorisation: the handle's stream updates the next block and
 // factors the next panel while the side stream updates the rest.
 int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t lda,
-                const double *b, MixedWs &w) {
+                const double *b, MixedWs &w, int bf16_terms) {
     hipStream_t s0 = hd->stream, s1 = hd->side_stream;
     const int nb = hd->nb;
     const int64_t k = std::min(m, n), nc = n + 1;
@@ -390,14 +392,14 @@ int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t 
             MQR_HIP(hipEventRecord(hd->ev_fork, s0));
             MQR_HIP(hipStreamWaitEvent(s1, hd->ev_fork, 0));
             MQR_CHECK(apply_block_f32(hd, s1, h, bj, V32, Y32, w.Aw + j + rest0 * m, m, nrest,
-                                      w.P32, pcap, K_VTA, K_UPDATE));
+                                      w.P32, pcap, K_VTA, K_UPDATE, bf16_terms));
             MQR_HIP(hipEventRecord(hd->ev_join, s1));
             side_pending = true;
         } else {
             side_pending = false;
         }
         MQR_CHECK(apply_block_f32(hd, s0, h, bj, V32, Y32, w.Aw + j + next * m, m, bn, w.P32b,
-                                  pcap2, K_LOOKAHEAD, K_LOOKAHEAD));
+                                  pcap2, K_LOOKAHEAD, K_LOOKAHEAD, bf16_terms));
         if (next < k) {
             const int bnext = (int)std::min<int64_t>(nb, k - next);
             MQR_CHECK(panel_fp32(hd, m, next, bnext, w, Vb[(it + 1) & 1], w.V32[(it + 1) & 1],
@@ -557,7 +559,7 @@ int estimate(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t lda
 int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, const double *A,
                int64_t lda, const double *b, double *x, const mqr_ir_opts_t *opts,
                mqr_ir_info_t *info) {
-    if (prec != MQR_PREC_FP64 && prec != MQR_PREC_FP32) return MQR_ERR_UNSUPPORTED;
+
     if (m > INT32_MAX / 2 || n > INT32_MAX / 2) return MQR_ERR_UNSUPPORTED;
     const int max_iter = (opts && opts->max_iter > 0) ? opts->max_iter : 30;
     const double tol = (opts && opts->tol > 0.0) ? opts->tol : 1e-14;
@@ -585,7 +587,8 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
     if (fp64)
         MQR_CHECK(geqrf_f64(hd, m, n, w.F, m, w.tau, w.T, nb));
     else
-        MQR_CHECK(factor_fp32(hd, m, n, A, lda, b, w));
+        MQR_CHECK(factor_fp32(hd, m, n, A, lda, b, w,
+                              prec == MQR_PREC_BF16X3 ? 3 : prec == MQR_PREC_BF16 ? 1 : 0));
     MQR_CHECK(tri_inverse(hd, n, w.R, w.ldr, w.X, w.tmp));
 
     // starting point: the QR solution x0 = R^{-1} (Q^T b)[0:n] of the

@@ -157,6 +157,9 @@ struct GemmF32 {
     int layout = 0;  // accumulator layout selector (0 is the gfx950 one)
     int batch = 1;   // strided batch as in GemmF64
     int64_t strideA = 0, strideB = 0, strideC = 0;
+    // 0: FP32 MFMA; 1: BF16 products (hi * hi); 3: split BF16 (hi * hi +
+    // hi * lo + lo * hi); always FP32 accumulation
+    int bf16_terms = 0;
 };
 int gemm_f32(const GemmF32 &g, hipStream_t s);
 // C = beta * C (beta == 0 writes zeros).

@@ -193,3 +193,65 @@ def test_host_least_squares_api(h):
     x, info = M.LeastSquares(A, b, prec="fp32", handle=h)
     assert info.converged == 1
     assert ls_backward_error(A, b, x)["kw"] <= 1e-14
+
+
+# ---------------------------------------------------------------------------
+# BF16 MFMA products (split-BF16 trailing updates)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("terms", [1, 3])
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+def test_bf16_gemm_exact_on_small_integers(h, torch_mod, terms, ta, tb):
+    """|x| <= 8 is exact in BF16 (lo = 0) and every sum stays below 2^24, so
+    both modes must return the integer product exactly: this pins the BF16
+    MFMA operand and accumulator layouts and the edge masks."""
+    rng = np.random.default_rng(17)
+    f32 = torch_mod.float32
+    for (m, n, k) in [(16, 16, 16), (37, 29, 41), (200, 150, 90), (513, 384, 200), (128, 260, 33)]:
+        A = rng.integers(-8, 9, (k, m) if ta == "T" else (m, k)).astype(np.float32)
+        B = rng.integers(-8, 9, (n, k) if tb == "T" else (k, n)).astype(np.float32)
+        Cm = rng.integers(-8, 9, (m, n)).astype(np.float32)
+        opA = A.T if ta == "T" else A
+        opB = B.T if tb == "T" else B
+        exact = Cm.astype(np.int64) - opA.astype(np.int64) @ opB.astype(np.int64)
+        dC = M.colmajor(Cm, dtype=f32)
+        M.gemm_bf16(h, terms, ta, tb, -1.0, M.colmajor(A, dtype=f32), M.colmajor(B, dtype=f32),
+                    1.0, dC)
+        assert np.array_equal(host(dC).astype(np.int64), exact), (terms, ta, tb, m, n, k)
+
+
+@pytest.mark.parametrize("terms,bits", [(1, 8), (3, 16)])
+def test_bf16_gemm_error_bound(h, torch_mod, terms, bits):
+    """Random operands: |C - AB| <= k 2^-bits |A||B| (relative error of one
+    product 2^-(bits-1), FP32 accumulation on top)."""
+    rng = np.random.default_rng(19)
+    m, n, k = 300, 280, 700
+    A = rng.standard_normal((m, k)).astype(np.float32)
+    B = rng.standard_normal((k, n)).astype(np.float32)
+    f32 = torch_mod.float32
+    dC = M.empty_colmajor(m, n, dtype=f32)
+    M.gemm_bf16(h, terms, "N", "N", 1.0, M.colmajor(A, dtype=f32), M.colmajor(B, dtype=f32), 0.0,
+                dC)
+    ref = A.astype(np.float64) @ B.astype(np.float64)
+    bound = (2.0 ** (1 - bits) + k * np.finfo(np.float32).eps) * (np.abs(A).astype(np.float64)
+                                                                   @ np.abs(B))
+    err = np.abs(host(dC).astype(np.float64) - ref)
+    assert np.all(err <= bound)
+    # and the split form is markedly better than one BF16 product
+    if terms == 3:
+        assert np.max(err / np.maximum(bound, 1e-300)) < 0.5
+
+
+@pytest.mark.parametrize("prec,cond,converges", [("bf16x3", 1e3, True), ("bf16", 10.0, True),
+                                                 ("bf16", 1e6, False)])
+def test_mixed_bf16_refinement(h, orc, prec, cond, converges):
+    """BF16 trailing updates: refinement reaches the FP64 solution while
+    u_bf16 * kappa is small; otherwise it stagnates and the FP64 solve runs
+    (fell_back), still returning the FP64 least-squares solution."""
+    A, b = ls_problem(1500, 400, cond, seed=int(cond) % 97 + 1, resid=1e-8)
+    x, info = run_mixed(h, A, b, prec=prec)
+    note(f"  {prec} cond={cond:g}: iterations={info.iterations} converged={info.converged} "
+         f"fell_back={info.fell_back}")
+    assert (info.converged == 1) == converges
+    assert info.fell_back == (0 if converges else 1)
+    assert ls_backward_error(A, b, x)["kw"] <= 1e-14
