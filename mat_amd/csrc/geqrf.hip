@@ -89,10 +89,10 @@ int form_y(hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv, cons
 // applies (I - V T V^T)^T (Y = V T^T) or I - V T V^T (Y = V T) as in
 // QR/__Householder_Blocked.h:143-212.  P (pcap doubles) receives the split-K
 // partials of W = V^T C.
-int apply_block(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv,
-                const double *Y, double *C, int64_t ldc, int64_t nc, double *P, int64_t pcap,
-                int kind_w = K_VTA, int kind_u = K_UPDATE) {
-    if (nc <= 0) return MQR_SUCCESS;
+// W = V^T C (b x nc, ld b) into P[0 : b nc] on stream st; P (pcap doubles)
+// holds the split-K partials.
+int block_w(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv,
+            const double *C, int64_t ldc, int64_t nc, double *P, int64_t pcap, int kind_w) {
     int splits = gemm_pick_splits(b, (int)nc, (int)h, target_blocks(hd));
     while (splits > 1 && (int64_t)splits * b * nc > pcap) --splits;
     GemmF64 g;
@@ -102,7 +102,14 @@ int apply_block(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double 
     g.C = P; g.ldc = b; g.alpha = 1.0; g.beta = 0.0;
     g.splits = splits; g.split_stride = (int64_t)b * nc; g.kind = kind_w;
     MQR_CHECK(gemm_f64(g, st));
-    MQR_CHECK(reduce_partials((int64_t)b * nc, P, splits, (int64_t)b * nc, st));
+    return reduce_partials((int64_t)b * nc, P, splits, (int64_t)b * nc, st);
+}
+
+int apply_block(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv,
+                const double *Y, double *C, int64_t ldc, int64_t nc, double *P, int64_t pcap,
+                int kind_w = K_VTA, int kind_u = K_UPDATE) {
+    if (nc <= 0) return MQR_SUCCESS;
+    MQR_CHECK(block_w(hd, st, h, b, Vp, ldv, C, ldc, nc, P, pcap, kind_w));
     GemmF64 u;
     u.M = (int)h; u.N = (int)nc; u.K = b;
     u.A = Y; u.lda = h; u.B = P; u.ldb = b;
@@ -162,9 +169,19 @@ int geqrf_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, dou
     MQR_HIP(hipEventRecord(hd->ev_fork, s0));
     MQR_HIP(hipStreamWaitEvent(s1, hd->ev_fork, 0));
     bool side_pending = false;
+    // a tall full-width panel runs the fused CholeskyQR sequence, which also
+    // forms Y = V T^T and takes the next-block update into its first launch
+    auto fused = [&](int64_t hp, int bp, int64_t bcols) {
+        return hp >= kCholqrMinRows && bp == bcols;
+    };
     const int b0 = (int)std::min<int64_t>(nb, k);
-    MQR_CHECK(panel_factor(hd, m, b0, A, lda, tau, T, ldt, pws));
-    if (b0 < n) MQR_CHECK(form_y(s0, m, b0, A, lda, T, ldt, true, Ybuf[0]));
+    if (fused(m, b0, b0)) {
+        MQR_CHECK(cholqr_gram1(s0, m, b0, 0, nullptr, 0, nullptr, A, lda, 0, pws));
+        MQR_CHECK(cholqr_rest(s0, m, b0, A, lda, tau, T, ldt, b0 < n ? Ybuf[0] : nullptr, pws));
+    } else {
+        MQR_CHECK(panel_factor(hd, m, b0, A, lda, tau, T, ldt, pws));
+        if (b0 < n) MQR_CHECK(form_y(s0, m, b0, A, lda, T, ldt, true, Ybuf[0]));
+    }
     for (int64_t j = 0, it = 0; j < k; j += nb, ++it) {
         const int b = (int)std::min<int64_t>(nb, k - j);
         const int64_t h = m - j;
@@ -186,10 +203,18 @@ int geqrf_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, dou
         } else {
             side_pending = false;
         }
+        const int bnext = next < k ? (int)std::min<int64_t>(nb, k - next) : 0;
+        if (next < k && fused(m - next, bnext, bn)) {
+            double *Cn = A + j + next * lda;
+            MQR_CHECK(block_w(hd, s0, h, b, V, lda, Cn, lda, bn, P2, pcap2, K_LOOKAHEAD));
+            MQR_CHECK(cholqr_gram1(s0, h, (int)bn, b, Y, h, P2, Cn, lda, b, pws));
+            MQR_CHECK(cholqr_rest(s0, m - next, bnext, Cn + b, lda, tau + next, T + next * ldt,
+                                  ldt, next + bnext < n ? Ybuf[(it + 1) & 1] : nullptr, pws));
+            continue;
+        }
         MQR_CHECK(apply_block(hd, s0, h, b, V, lda, Y, A + j + next * lda, lda, bn, P2, pcap2,
                               K_LOOKAHEAD, K_LOOKAHEAD));
         if (next < k) {
-            const int bnext = (int)std::min<int64_t>(nb, k - next);
             double *Vn = A + next + next * lda;
             MQR_CHECK(panel_factor(hd, m - next, bnext, Vn, lda, tau + next, T + next * ldt, ldt,
                                    pws));

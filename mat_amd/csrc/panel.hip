@@ -173,28 +173,44 @@ int panel_householder_gated(int64_t h, int b, double *P, int64_t ld, double *tau
 }
 
 // P[0:count] = sum over s of P[s * stride + 0:count]
-__global__ void k_reduce_partials(int64_t count, double *__restrict__ P, int splits,
-                                  int64_t stride) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= count) return;
-    // four independent chains keep several loads in flight
-    double a0 = P[e], a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int s = 1;
-    for (; s + 3 < splits; s += 4) {
-        a1 += P[(int64_t)s * stride + e];
-        a2 += P[(int64_t)(s + 1) * stride + e];
-        a3 += P[(int64_t)(s + 2) * stride + e];
-        a0 += P[(int64_t)(s + 3) * stride + e];
+// P[0:count] = sum of the splits partials P[s * stride : s * stride + count].
+// A workgroup covers 256 / G entries with G threads per entry (thread group
+// g sums partials g, g + G, ...); the G sums meet in LDS.  Spreading one
+// entry over G threads keeps many loads in flight when there are few entries
+// and many partials (a 64 x 64 Gram matrix from 128 row chunks).
+__global__ __launch_bounds__(256) void k_reduce_partials(int64_t count, double *__restrict__ P,
+                                                         int splits, int64_t stride, int G) {
+    __shared__ double part[256];
+    const int E = 256 / G;
+    const int t = threadIdx.x, g = t / E;
+    const int64_t e = (int64_t)blockIdx.x * E + (t % E);
+    double a0 = 0.0, a1 = 0.0;
+    if (e < count) {
+        int s = g;
+        for (; s + G < splits; s += 2 * G) {
+            a0 += P[(int64_t)s * stride + e];
+            a1 += P[(int64_t)(s + G) * stride + e];
+        }
+        if (s < splits) a0 += P[(int64_t)s * stride + e];
     }
-    for (; s < splits; ++s) a1 += P[(int64_t)s * stride + e];
-    P[e] = (a0 + a1) + (a2 + a3);
+    part[t] = a0 + a1;
+    __syncthreads();
+    if (g == 0 && e < count) {
+        double sum = part[t];
+        for (int k = 1; k < G; ++k) sum += part[t + k * E];
+        P[e] = sum;
+    }
 }
 
 int reduce_partials(int64_t count, double *P, int splits, int64_t stride, hipStream_t s) {
     if (splits <= 1 || count <= 0) return MQR_SUCCESS;
     ProfScope ps(K_OTHER, (double)count * (splits - 1), 8.0 * count * (splits + 1), s);
-    hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
-                       count, P, splits, stride);
+    // threads per entry: enough workgroups to spread the loads over the chip
+    int G = 1;
+    while (G < 16 && G * 2 <= splits && (count * G) / 256 < 512) G *= 2;
+    const int E = 256 / G;
+    hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)((count + E - 1) / E)), dim3(256), 0, s,
+                       count, P, splits, stride, G);
     MQR_LAUNCH_CHECK();
     return MQR_SUCCESS;
 }

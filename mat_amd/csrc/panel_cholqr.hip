@@ -24,16 +24,17 @@
 // The b x b factorisations run register-resident in one workgroup
 // (small_dense.h); every barrier is executed by all of its waves.  The
 // compact-WY factor T comes out of the reconstruction as well.
+#include <algorithm>
 #include <climits>
 
 #include "mqr_internal.h"
+#include "small_blocked.h"
 #include "small_dense.h"
 
 namespace mqr {
 namespace {
 
 constexpr int PB = kMaxNb;  // padded panel width (small::N)
-using SM = double[small::N + 1];
 
 // Cholesky + inverse of the first pass.  G: reduced Gram (b x b, ld b).
 // A pivot d_i <= 1e-12 G_ii (column i nearly in the span of the previous
@@ -41,31 +42,21 @@ using SM = double[small::N + 1];
 // orthonormal factor to working accuracy), a zero column or a non-finite
 // value sets the flag; the rest of the sequence then skips the panel.
 __global__ __launch_bounds__(small::NT) void k_chol_inv(int b, const double *__restrict__ G,
-                                                        double *__restrict__ R,
+                                                        int64_t ldg, double *__restrict__ R,
                                                         double *__restrict__ Rinv,
                                                         int *__restrict__ flag) {
-    __shared__ small::Lds S[PB];
-    __shared__ small::RowBuf rb;
-    __shared__ double dv[2 * PB], thr[PB];
+    __shared__ small::Lds S[PB], X[PB];
+    __shared__ double thr[PB];
     __shared__ int fail;
     const int tid = threadIdx.x;
     if (tid == 0) fail = 0;
-    small::lds_load(S, G, b, b);
+    small::lds_load(S, G, ldg, b);
     __syncthreads();
     if (tid < PB) thr[tid] = tid < b ? 1e-12 * S[tid][tid] : 0.0;
     __syncthreads();
-    double a[small::RK];
-    small::from_lds(a, S);
-    small::chol(a, rb, dv, b, 0.0, &fail, thr);
-    small::to_lds(a, S);
-    __syncthreads();
+    sb::chol_inv(S, X, b, [&](int i) { return thr[i]; }, &fail);
     small::lds_store(S, R, PB, PB, PB);
-    double x[small::RK];
-    small::inv_upper(S, rb, dv, x);
-    __syncthreads();
-    small::to_lds(x, S);
-    __syncthreads();
-    small::lds_store(S, Rinv, PB, PB, PB);
+    small::lds_store(X, Rinv, PB, PB, PB);
     if (tid == 0) *flag = fail;
 }
 
@@ -78,10 +69,10 @@ __global__ __launch_bounds__(small::NT) void k_chol_inv(int b, const double *__r
 // With Q = Z R2^{-1}, Q1 = Q(0:b, :), and the LU factorisation
 // Q1 - S = L U (S the signs), the reflectors I - V T V^T whose first b
 // columns are Q S satisfy V1 = L, V2 = Q2 U^{-1} and T = -U S L^{-T}
-// (top block: Q1 S - I = -V1 T V1^T).  T^T = L^{-1} (-S U^T) is one
-// forward substitution.
+// (top block: Q1 S - I = -V1 T V1^T).
 __global__ __launch_bounds__(small::NT) void k_recon(int b, const double *__restrict__ G,
-                                                     const double *__restrict__ Z, int64_t ldz,
+                                                     int64_t ldg, const double *__restrict__ Z,
+                                                     int64_t ldz,
                                                      const double *__restrict__ R1,
                                                      double *__restrict__ A, int64_t lda,
                                                      double *__restrict__ tau,
@@ -89,125 +80,414 @@ __global__ __launch_bounds__(small::NT) void k_recon(int b, const double *__rest
                                                      double *__restrict__ Minv,
                                                      int *__restrict__ flag) {
     __shared__ small::Lds S0[PB], S1[PB], S2[PB], S3[PB];
-    __shared__ small::RowBuf rb;
-    __shared__ double sg[PB], dv[2 * PB];
+    __shared__ double sg[PB];
     __shared__ int fail;
     if (*flag) return;
-    const int tid = threadIdx.x, c = small::lane_id(), q = small::wave_id();
+    const int tid = threadIdx.x, q = small::wave_id();
     if (tid == 0) fail = 0;
     small::lds_load(S0, Z, ldz, b);
-    small::lds_load(S1, G, b, b);
+    small::lds_load(S1, G, ldg, b);
     __syncthreads();
-    double a[small::RK];
-    // R2 = chol(G2) -> S2, R2^{-1} -> S1
-    small::from_lds(a, S1);
-    small::chol(a, rb, dv, b, 1e-2, &fail);
-    small::to_lds(a, S2);
-    __syncthreads();
-    small::inv_upper(S2, rb, dv, a);
-    __syncthreads();  // S1 (G2) no longer read
-    small::to_lds(a, S1);
-    __syncthreads();
-    // Q1 = Z_top R2^{-1} -> S3
-    small::matmul(S0, S1, false, true, [&](int r, int cc, double v) { S3[r][cc] = v; });
-    __syncthreads();
-    // LU of Q1 - S -> S3 (L strictly below, U on and above); U -> S0
-    small::from_lds(a, S3);
-    small::lu_sign(a, rb, dv, sg);
-    small::to_lds(a, S3);
-    small::keep_upper(a);
-    small::to_lds(a, S0);
-    __syncthreads();
-    // U^{-1} -> S0
-    small::inv_upper(S0, rb, dv, a);
-    __syncthreads();
-    small::to_lds(a, S0);
-    __syncthreads();
-    // Minv = R2^{-1} U^{-1} straight to global
-    small::matmul(S1, S0, true, true, [&](int r, int cc, double v) { Minv[r + cc * PB] = v; });
-    // T^T = L^{-1} (-S U^T): B(r, c) = -sg(r) U(c, r) for c <= r
-#pragma unroll
-    for (int j = 0; j < small::RK; ++j) {
-        const int r = small::drow(j, q);
-        a[j] = (c <= r) ? -sg[r] * S3[c][r] : 0.0;
+    // R2 = chol(G2) -> S1, R2^{-1} -> S2
+    sb::chol_inv(S1, S2, b, [](int) { return 1e-2; }, &fail);
+    if (fail) {  // uniform: read after the barrier that ends chol_inv
+        if (tid == 0) *flag = 1;
+        return;
     }
-    small::solve_unit_lower(S3, rb, a);
-    __syncthreads();  // S0 (U^{-1}) and S1 (R2^{-1}) no longer read
-    small::to_lds(a, S0);
-    small::lds_load(S1, R1, PB, PB);
+    // Q1 = Z_top R2^{-1} -> S3 (wave q: tile row q)
+    for (int jb = 0; jb < 4; ++jb) {
+        sb::v4d acc = sb::zero4();
+        for (int k = 0; k <= jb; ++k) acc = sb::tmma<false, false>(acc, S0, q, k, S2, k, jb);
+        sb::tstore(S3, q, jb, acc);
+    }
     __syncthreads();
+    // RR = R2 R1, kept in registers (tile row q)
+    small::lds_load(S0, R1, PB, PB);
+    __syncthreads();
+    sb::v4d rr[4];
+    for (int jb = 0; jb < 4; ++jb) {
+        rr[jb] = sb::zero4();
+        for (int k = q; k <= jb; ++k) rr[jb] = sb::tmma<false, false>(rr[jb], S1, q, k, S0, k, jb);
+    }
+    __syncthreads();
+    // LU of Q1 - S -> S3; U^{-1} -> S0, L^{-1} -> S1
+    sb::lu_sign_inv(S3, S0, S1, sg);
+    // Minv = R2^{-1} U^{-1} straight to global
+    for (int jb = 0; jb < 4; ++jb) {
+        sb::v4d acc = sb::zero4();
+        for (int k = q; k <= jb; ++k) acc = sb::tmma<false, false>(acc, S2, q, k, S0, k, jb);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            Minv[16 * q + sb::trow(r) + (int64_t)(16 * jb + sb::tcol()) * PB] = acc[r];
+    }
+    __syncthreads();  // S2 no longer read
+    // U S -> S2
     for (int e = tid; e < PB * PB; e += small::NT) {
         const int r = e & (PB - 1), cc = e >> 6;
-        if (r < b && cc < b) T[r + (int64_t)cc * ldt] = r <= cc ? S0[cc][r] : 0.0;
+        S2[r][cc] = r <= cc ? S3[r][cc] * sg[cc] : 0.0;
     }
-    // R = S (R2 R1) with V1 (strictly below, from the LU) into the panel
-    small::matmul(S2, S1, true, true, [&](int r, int cc, double v) {
-        if (r < b && cc < b) A[r + (int64_t)cc * lda] = r <= cc ? sg[r] * v : S3[r][cc];
-    });
-    for (int i = tid; i < b; i += small::NT) tau[i] = -sg[i] * S3[i][i];
     __syncthreads();
-    if (tid == 0 && fail) *flag = 1;
+    // T = -(U S) L^{-T} (upper), R = S (R2 R1) with V1 strictly below
+    for (int jb = 0; jb < 4; ++jb) {
+        sb::v4d acc = sb::zero4();
+        for (int k = q; k <= jb; ++k) acc = sb::tmma<false, true>(acc, S2, q, k, S1, k, jb, -1.0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int rr_ = 16 * q + sb::trow(r), cc = 16 * jb + sb::tcol();
+            if (rr_ < b && cc < b) {
+                T[rr_ + (int64_t)cc * ldt] = rr_ <= cc ? acc[r] : 0.0;
+                A[rr_ + (int64_t)cc * lda] = rr_ <= cc ? sg[rr_] * rr[jb][r] : S3[rr_][cc];
+            }
+        }
+    }
+    for (int i = tid; i < b; i += small::NT) tau[i] = -sg[i] * S3[i][i];
+}
+
+// ---- row-chunk kernels of the tall-panel sequence ----------------------
+//
+// Each workgroup streams 64-row chunks of the h x b panel through LDS, forms
+// its products on the FP64 MFMA and, where a Gram matrix is needed, adds the
+// chunk's contribution to the upper 64 x 64 tiles it owns; one partial per
+// workgroup is summed by reduce_partials.  Results are held in the
+// row-per-lane tile layout (lane l, register r: row 16 i + (l & 15), column
+// 16 j + (l >> 4) + 4 r), so global loads and stores of a 64-row chunk are
+// contiguous 128-byte column pieces.
+
+constexpr int RC = 64;              // rows per chunk
+constexpr int kMaxGramParts = 128;  // Gram partials per pass (workgroups)
+
+// acc += s op(A)[ia, ka] op(B)[kb, jb] in the row-per-lane layout
+template <bool TA, bool TB>
+__device__ __forceinline__ sb::v4d tmmaT(sb::v4d acc, const small::Lds *A, int ia, int ka,
+                                         const small::Lds *B, int kb, int jb, double s = 1.0) {
+    const int l = small::lane_id(), i = l & 15, kq = l >> 4;
+#pragma unroll
+    for (int k4 = 0; k4 < 4; ++k4) {
+        const int ak = 16 * ka + 4 * k4 + kq, bk = 16 * kb + 4 * k4 + kq;
+        const double av = TA ? A[ak][16 * ia + i] : A[16 * ia + i][ak];
+        const double bv = TB ? B[16 * jb + i][bk] : B[bk][16 * jb + i];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, s * av, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ int lrow() { return small::lane_id() & 15; }
+__device__ __forceinline__ int lcol(int r) { return (small::lane_id() >> 4) + 4 * r; }
+
+__device__ __forceinline__ void tstoreT(small::Lds *S, int ia, int jb, sb::v4d v) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) S[16 * ia + lrow()][16 * jb + lcol(r)] = v[r];
+}
+
+// S[r][c] = M[r0 + r, c] for r0 + r < rows and c < cols, else 0
+__device__ __forceinline__ void chunk_load(small::Lds *S, const double *__restrict__ M, int64_t ld,
+                                           int64_t r0, int64_t rows, int cols) {
+    for (int e = threadIdx.x; e < RC * PB; e += small::NT) {
+        const int r = e & (RC - 1), c = e >> 6;
+        S[r][c] = (r0 + r < rows && c < cols) ? M[r0 + r + (int64_t)c * ld] : 0.0;
+    }
+}
+
+// upper tile u (0..9) in row order: (0,0) (0,1) (0,2) (0,3) (1,1) ... (3,3)
+__device__ __forceinline__ void upper_tile(int u, int &i, int &j) {
+    if (u < 4) { i = 0; j = u; }
+    else if (u < 7) { i = 1; j = u - 3; }
+    else if (u < 9) { i = 2; j = u - 5; }
+    else { i = 3; j = 3; }
+}
+// strictly lower tile u (0..5): (1,0) (2,0) (2,1) (3,0) (3,1) (3,2)
+__device__ __forceinline__ void lower_tile(int u, int &i, int &j) {
+    if (u < 1) { i = 1; j = 0; }
+    else if (u < 3) { i = 2; j = u - 1; }
+    else { i = 3; j = u - 3; }
+}
+
+// g += S^T S on the upper tiles wave q owns (tiles q, q + 4, q + 8)
+__device__ __forceinline__ void gram_acc(const small::Lds *S, sb::v4d (&g)[3]) {
+    const int q = small::wave_id();
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+        const int t = q + 4 * u;
+        if (t < 10) {
+            int i, j;
+            upper_tile(t, i, j);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) g[u] = tmmaT<true, false>(g[u], S, i, k, S, k, j);
+        }
+    }
+}
+
+// Gp (64 x 64, ld 64) = the upper tiles in g, zero below the diagonal tiles
+__device__ __forceinline__ void gram_store(const sb::v4d (&g)[3], double *__restrict__ Gp) {
+    const int q = small::wave_id();
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+        const int t = q + 4 * u;
+        if (t < 10) {
+            int i, j;
+            upper_tile(t, i, j);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Gp[16 * i + lrow() + (16 * j + lcol(r)) * PB] = g[u][r];
+        }
+        if (t < 6) {
+            int i, j;
+            lower_tile(t, i, j);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Gp[16 * i + lrow() + (16 * j + lcol(r)) * PB] = 0.0;
+        }
+    }
+}
+
+// C (hc x bn, ldc) -= Y (hc x kin, ldy) W (kin x bn, ld kin) when kin > 0,
+// then the Gram partial of rows [g0, hc) of C into Gp[blockIdx.x].
+__global__ __launch_bounds__(small::NT) void k_upd_gram(int64_t hc, int kin, int bn,
+                                                        const double *__restrict__ Y, int64_t ldy,
+                                                        const double *__restrict__ W,
+                                                        double *__restrict__ C, int64_t ldc,
+                                                        int64_t g0, double *__restrict__ Gp) {
+    __shared__ small::Lds Ys[RC], Ws[PB];
+    const int q = small::wave_id();
+    sb::v4d g[3] = {sb::zero4(), sb::zero4(), sb::zero4()};
+    if (kin > 0) {
+        for (int e = threadIdx.x; e < PB * PB; e += small::NT) {
+            const int r = e & (PB - 1), c = e >> 6;
+            Ws[r][c] = (r < kin && c < bn) ? W[r + (int64_t)c * kin] : 0.0;
+        }
+    }
+    for (int64_t r0 = (int64_t)blockIdx.x * RC; r0 < hc; r0 += (int64_t)gridDim.x * RC) {
+        __syncthreads();  // the previous chunk's Gram reads are done
+        if (kin > 0) chunk_load(Ys, Y, ldy, r0, hc, kin);
+        const int64_t row = r0 + 16 * q + lrow();
+        sb::v4d acc[4];
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = 16 * jb + lcol(r);
+                acc[jb][r] = (row < hc && col < bn) ? C[row + (int64_t)col * ldc] : 0.0;
+            }
+        __syncthreads();
+        if (kin > 0) {
+#pragma unroll
+            for (int jb = 0; jb < 4; ++jb)
+                for (int k = 0; 16 * k < kin; ++k)
+                    acc[jb] = tmmaT<false, false>(acc[jb], Ys, q, k, Ws, k, jb, -1.0);
+#pragma unroll
+            for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = 16 * jb + lcol(r);
+                    if (row < hc && col < bn) C[row + (int64_t)col * ldc] = acc[jb][r];
+                }
+        }
+        __syncthreads();  // Ys no longer read by the update
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb) {
+            sb::v4d v = acc[jb];
+            if (row < g0) v = sb::zero4();
+            tstoreT(Ys, q, jb, v);
+        }
+        __syncthreads();
+        gram_acc(Ys, g);
+    }
+    gram_store(g, Gp + (size_t)blockIdx.x * PB * PB);
+}
+
+// Z (h x b, ldz) = X (h x b, ldx) Rinv (upper, ld PB) and the Gram partial
+// of Z into Gp[blockIdx.x]; nothing when *flag is set.
+__global__ __launch_bounds__(small::NT) void k_apply_gram(int64_t h, int b,
+                                                          const double *__restrict__ X,
+                                                          int64_t ldx,
+                                                          const double *__restrict__ Rinv,
+                                                          double *__restrict__ Z, int64_t ldz,
+                                                          double *__restrict__ Gp,
+                                                          const int *__restrict__ flag) {
+    if (*flag) return;
+    __shared__ small::Lds Xs[RC], Ms[PB];
+    const int q = small::wave_id();
+    sb::v4d g[3] = {sb::zero4(), sb::zero4(), sb::zero4()};
+    small::lds_load(Ms, Rinv, PB, PB);
+    for (int64_t r0 = (int64_t)blockIdx.x * RC; r0 < h; r0 += (int64_t)gridDim.x * RC) {
+        __syncthreads();
+        chunk_load(Xs, X, ldx, r0, h, b);
+        __syncthreads();
+        sb::v4d acc[4];
+        const int64_t row = r0 + 16 * q + lrow();
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb) {
+            acc[jb] = sb::zero4();
+            for (int k = 0; k <= jb; ++k) acc[jb] = tmmaT<false, false>(acc[jb], Xs, q, k, Ms, k, jb);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = 16 * jb + lcol(r);
+                if (row < h && col < b) Z[row + (int64_t)col * ldz] = acc[jb][r];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb) tstoreT(Xs, q, jb, acc[jb]);
+        __syncthreads();
+        gram_acc(Xs, g);
+    }
+    gram_store(g, Gp + (size_t)blockIdx.x * PB * PB);
+}
+
+// One 64-row chunk per workgroup: V(b:h) = Z(b:h) Minv into the panel A
+// (unless *flag: then the fallback has written V), and, when Yf, the block
+// reflector's left factor Yf (ld h) = V T^T with V = [unit-lower V1; V2].
+__global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
+                                                        const double *__restrict__ Z,
+                                                        const double *__restrict__ Minv,
+                                                        double *__restrict__ A, int64_t lda,
+                                                        const double *__restrict__ T, int64_t ldt,
+                                                        double *__restrict__ Yf,
+                                                        const int *__restrict__ flag) {
+    __shared__ small::Lds Vs[RC], Ms[PB], Ts[PB];
+    const int q = small::wave_id();
+    const bool fb = *flag != 0;
+    const int64_t r0 = (int64_t)blockIdx.x * RC;
+    const int64_t row = r0 + 16 * q + lrow();
+    if (!fb) {
+        small::lds_load(Ms, Minv, PB, PB);
+        chunk_load(Vs, Z, h, r0, h, b);
+        __syncthreads();
+        sb::v4d acc[4];
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb) {
+            acc[jb] = sb::zero4();
+            for (int k = 0; k <= jb; ++k) acc[jb] = tmmaT<false, false>(acc[jb], Vs, q, k, Ms, k, jb);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = 16 * jb + lcol(r);
+                if (row >= b && row < h && col < b) A[row + (int64_t)col * lda] = acc[jb][r];
+            }
+        }
+        if (!Yf) return;
+        __syncthreads();
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb) tstoreT(Vs, q, jb, acc[jb]);
+    } else {
+        if (!Yf) return;
+        chunk_load(Vs, A, lda, r0, h, b);
+    }
+    __syncthreads();
+    // rows above b: the unit-lower V1 (R sits above its diagonal in A)
+    if (r0 < b) {
+        for (int e = threadIdx.x; e < RC * PB; e += small::NT) {
+            const int r = e & (RC - 1), c = e >> 6;
+            const int64_t gr = r0 + r;
+            if (gr < b)
+                Vs[r][c] = c < b ? (gr > c ? A[gr + (int64_t)c * lda] : (gr == c ? 1.0 : 0.0)) : 0.0;
+        }
+    }
+    for (int e = threadIdx.x; e < PB * PB; e += small::NT) {
+        const int r = e & (PB - 1), c = e >> 6;
+        Ts[r][c] = (r < b && c < b && r <= c) ? T[r + (int64_t)c * ldt] : 0.0;
+    }
+    __syncthreads();
+    // Yf = V T^T: op(T)(k, j) = T(j, k), nonzero for k >= j
+#pragma unroll
+    for (int jb = 0; jb < 4; ++jb) {
+        sb::v4d acc = sb::zero4();
+        for (int k = jb; k < 4; ++k) acc = tmmaT<false, true>(acc, Vs, q, k, Ts, k, jb);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int col = 16 * jb + lcol(r);
+            if (row < h && col < b) Yf[row + (int64_t)col * h] = acc[r];
+        }
+    }
+}
+
+struct CqrWs {
+    double *Z, *R1, *I1, *Mi, *Gp;
+    int *flag;
+};
+
+CqrWs carve(void *ws, int64_t h) {
+    CqrWs w;
+    w.Z = (double *)ws;
+    w.R1 = w.Z + (size_t)h * PB;
+    w.I1 = w.R1 + PB * PB;
+    w.Mi = w.I1 + PB * PB;
+    w.Gp = w.Mi + PB * PB;
+    w.flag = (int *)(w.Gp + (size_t)kMaxGramParts * PB * PB);
+    return w;
+}
+
+int gram_parts(int64_t rows) {
+    return (int)std::min<int64_t>((rows + RC - 1) / RC, kMaxGramParts);
 }
 
 }  // namespace
 
-// Workspace: Y (h x PB), R1, R1^{-1}, Minv (PB x PB), 64 Gram partials, flag.
+// Workspace: Z (h x PB), R1, R1^{-1}, Minv (PB x PB), the Gram partials, flag.
 size_t panel_cholqr_ws_bytes(int64_t h) {
-    return sizeof(double) * ((size_t)h * PB + 3 * PB * PB + (size_t)64 * PB * PB) + 256;
+    return sizeof(double) * ((size_t)h * PB + 3 * PB * PB + (size_t)kMaxGramParts * PB * PB) + 256;
+}
+
+// First pass: C (hc x bn) -= Y W when kin > 0, the Gram matrix of rows
+// [g0, hc) of C (the panel), and its Cholesky factor R1 with R1^{-1}.
+int cholqr_gram1(hipStream_t s, int64_t hc, int bn, int kin, const double *Y, int64_t ldy,
+                 const double *W, double *C, int64_t ldc, int64_t g0, void *ws) {
+    const CqrWs w = carve(ws, hc - g0);
+    const int parts = gram_parts(hc);
+    {
+        const double hh = (double)hc;
+        ProfScope ps(kin > 0 ? K_LOOKAHEAD : K_PANEL,
+                     2.0 * hh * bn * kin + hh * bn * bn,
+                     8.0 * hh * (kin + (kin > 0 ? 2.0 : 1.0) * bn), s);
+        hipLaunchKernelGGL(k_upd_gram, dim3(parts), dim3(small::NT), 0, s, hc, kin, bn, Y, ldy, W,
+                           C, ldc, g0, w.Gp);
+        MQR_LAUNCH_CHECK();
+    }
+    MQR_CHECK(reduce_partials((int64_t)PB * PB, w.Gp, parts, (int64_t)PB * PB, s));
+    const double bb = bn;
+    ProfScope ps(K_PANEL_SMALL, 2.0 * bb * bb * bb / 3.0, 8.0 * bb * bb * 3.0, s);
+    hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(small::NT), 0, s, bn, w.Gp, (int64_t)PB, w.R1,
+                       w.I1, w.flag);
+    MQR_LAUNCH_CHECK();
+    return MQR_SUCCESS;
+}
+
+// Second pass, reconstruction, gated fallback and V2 of the h x b panel X
+// after cholqr_gram1; with Yf, also Yf (h x b, ld h) = V T^T.
+int cholqr_rest(hipStream_t s, int64_t h, int b, double *X, int64_t ld, double *tau, double *T,
+                int64_t ldt, double *Yf, void *ws) {
+    const CqrWs w = carve(ws, h);
+    const int parts = gram_parts(h);
+    const double hh = (double)h, bb = b;
+    {
+        ProfScope ps(K_PANEL, hh * bb * bb + hh * bb * bb, 16.0 * hh * bb, s);
+        hipLaunchKernelGGL(k_apply_gram, dim3(parts), dim3(small::NT), 0, s, h, b, (const double *)X,
+                           ld, (const double *)w.I1, w.Z, h, w.Gp, (const int *)w.flag);
+        MQR_LAUNCH_CHECK();
+    }
+    MQR_CHECK(reduce_partials((int64_t)PB * PB, w.Gp, parts, (int64_t)PB * PB, s));
+    {
+        ProfScope ps(K_PANEL_SMALL, 8.0 * bb * bb * bb / 3.0, 8.0 * bb * bb * 6.0, s);
+        hipLaunchKernelGGL(k_recon, dim3(1), dim3(small::NT), 0, s, b, (const double *)w.Gp,
+                           (int64_t)PB, (const double *)w.Z, h, (const double *)w.R1, X, ld, tau,
+                           T, ldt, w.Mi, w.flag);
+        MQR_LAUNCH_CHECK();
+    }
+    // fallback: the column-by-column kernel runs only when the flag is up
+    MQR_CHECK(panel_householder_gated(h, b, X, ld, tau, T, ldt, w.flag, s));
+    {
+        ProfScope ps(K_PANEL, hh * bb * bb + (Yf ? hh * bb * bb : 0.0),
+                     8.0 * hh * bb * (Yf ? 3.0 : 2.0), s);
+        hipLaunchKernelGGL(k_v2_formy, dim3((unsigned)((h + RC - 1) / RC)), dim3(small::NT), 0, s,
+                           h, b, (const double *)w.Z, (const double *)w.Mi, X, ld,
+                           (const double *)T, ldt, Yf, (const int *)w.flag);
+        MQR_LAUNCH_CHECK();
+    }
+    return MQR_SUCCESS;
 }
 
 // Factor the h x b panel X (ld) in place.
 int panel_cholqr(mqr_handle_t hd, int64_t h, int b, double *X, int64_t ld, double *tau,
                  double *T, int64_t ldt, void *ws) {
-    hipStream_t s = hd->stream;
-    double *Y = (double *)ws;
-    double *R1 = Y + (size_t)h * PB;
-    double *I1 = R1 + PB * PB;
-    double *Mi = I1 + PB * PB;
-    double *Gp = Mi + PB * PB;
-    int *flag = (int *)(Gp + (size_t)64 * PB * PB);
-
-    int splits = gemm_pick_splits(b, b, (int)h, 4 * hd->cu_count);
-    if (splits > 64) splits = 64;
-    // Gram of W reduced into Gp[0 : b*b]
-    auto gram = [&](const double *W, int64_t ldw) -> int {
-        GemmF64 g;
-        g.M = b; g.N = b; g.K = (int)h; g.ta = true;
-        g.A = W; g.lda = ldw; g.B = W; g.ldb = ldw;
-        g.C = Gp; g.ldc = b; g.splits = splits; g.split_stride = (int64_t)b * b;
-        g.kind = K_PANEL;
-        MQR_CHECK(gemm_f64(g, s));
-        return reduce_partials((int64_t)b * b, Gp, splits, (int64_t)b * b, s);
-    };
-    // out(rows x b) = in(rows x b) * Tinv (padded, ld PB)
-    auto apply = [&](int64_t rows, const double *in, int64_t ldi, const double *Tinv, double *out,
-                     int64_t ldo, const int *skip) -> int {
-        GemmF64 g;
-        g.M = (int)rows; g.N = b; g.K = b;
-        g.A = in; g.lda = ldi; g.B = Tinv; g.ldb = PB;
-        g.C = out; g.ldc = ldo; g.kind = K_PANEL; g.skip_if = skip;
-        return gemm_f64(g, s);
-    };
-    const double bb = b;
-    const double small_flops = bb * bb * bb / 3.0, small_bytes = 8.0 * bb * bb * 3.0;
-    // pass 1: Y = X R1^{-1}
-    MQR_CHECK(gram(X, ld));
-    {
-        ProfScope ps(K_PANEL_SMALL, small_flops, small_bytes, s);
-        hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(small::NT), 0, s, b, Gp, R1, I1, flag);
-    }
-    MQR_CHECK(apply(h, X, ld, I1, Y, h, flag));
-    // pass 2 + reconstruction
-    MQR_CHECK(gram(Y, h));
-    {
-        ProfScope ps(K_PANEL_SMALL, 4.0 * small_flops, small_bytes, s);
-        hipLaunchKernelGGL(k_recon, dim3(1), dim3(small::NT), 0, s, b, Gp, Y, h, R1, X, ld, tau,
-                           T, ldt, Mi, flag);
-    }
-    MQR_LAUNCH_CHECK();
-    if (h > b) MQR_CHECK(apply(h - b, Y + b, h, Mi, X + b, ld, flag));
-    // fallback: the column-by-column kernel runs only when the flag is up
-    return panel_householder_gated(h, b, X, ld, tau, T, ldt, flag, s);
+    MQR_CHECK(cholqr_gram1(hd->stream, h, b, 0, nullptr, 0, nullptr, X, ld, 0, ws));
+    return cholqr_rest(hd->stream, h, b, X, ld, tau, T, ldt, nullptr, ws);
 }
 
 }  // namespace mqr
