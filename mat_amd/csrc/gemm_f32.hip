@@ -72,19 +72,19 @@ __device__ __forceinline__ void fetch(const float *__restrict__ X, int64_t ld, i
 
 template <int TILE, bool KMAJOR>
 __device__ __forceinline__ void stash(float (*S)[TILE + 4], int tid, const float *v, int r0, int k0,
-                                      int R, int kend) {
+                                      int R, int kend, float scale) {
     constexpr int NV = TILE * BK / NT;
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
         int r, k;
         slot<TILE, KMAJOR>(tid, q, r, k);
         const bool in = (r0 + r < R) && (k0 + k < kend);
-        S[k][r] = __int_as_float(__float_as_int(v[q]) & (in ? ~0 : 0));
+        S[k][r] = scale * __int_as_float(__float_as_int(v[q]) & (in ? ~0 : 0));
     }
 }
 
 template <int TILE, bool TA, bool TB, int MODE>
-__global__ __launch_bounds__(NT) void k_gemm_f32(GemmF32 gin, int kchunk) {
+__global__ __launch_bounds__(NT, 2) void k_gemm_f32(GemmF32 gin, int kchunk) {
     constexpr int NV = TILE * BK / NT, BW = TILE / 32, HALF = TILE / 2;
     __shared__ float As[2][BK][TILE + 4];
     __shared__ float Bs[2][BK][TILE + 4];
@@ -104,18 +104,39 @@ __global__ __launch_bounds__(NT) void k_gemm_f32(GemmF32 gin, int kchunk) {
     const int kb = split * kchunk;
     const int ke = min(g.K, kb + kchunk);
 
+    // accumulator element (i, j, r): row m0 + wm HALF + 16 i + (l & 15),
+    // column n0 + wn HALF + 16 j + 4 (l >> 4) + r (swapped operands; see the
+    // epilogue).  With beta != 0 it starts as beta C, loaded before the K
+    // loop so that the loads overlap the first slices; alpha is applied to
+    // op(A) in LDS.
     v4f acc[BW][BW];
+    if (g.splits == 1 && g.beta != 0.f) {
 #pragma unroll
-    for (int i = 0; i < BW; ++i)
+        for (int i = 0; i < BW; ++i) {
+            const int gm = m0 + wm * HALF + i * 16 + (lane & 15);
+            const int cm = gm < g.M ? gm : g.M - 1;
 #pragma unroll
-        for (int j = 0; j < BW; ++j) acc[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+            for (int j = 0; j < BW; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int gn = n0 + wn * HALF + j * 16 + 4 * (lane >> 4) + r;
+                    const int cn = gn < g.N ? gn : g.N - 1;
+                    acc[i][j][r] = g.beta * g.C[(int64_t)cm + (int64_t)cn * g.ldc];
+                }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < BW; ++i)
+#pragma unroll
+            for (int j = 0; j < BW; ++j) acc[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+    }
 
     if (kb < ke) {
         float va[NV], vb[NV];
         fetch<TILE, TA>(g.A, g.lda, m0, kb, g.M, ke, tid, va);
         fetch<TILE, !TB>(g.B, g.ldb, n0, kb, g.N, ke, tid, vb);
-        stash<TILE, TA>(As[0], tid, va, m0, kb, g.M, ke);
-        stash<TILE, !TB>(Bs[0], tid, vb, n0, kb, g.N, ke);
+        stash<TILE, TA>(As[0], tid, va, m0, kb, g.M, ke, g.alpha);
+        stash<TILE, !TB>(Bs[0], tid, vb, n0, kb, g.N, ke, 1.f);
         __syncthreads();
         int buf = 0;
         for (int k0 = kb; k0 < ke; k0 += BK) {
@@ -176,8 +197,8 @@ __global__ __launch_bounds__(NT) void k_gemm_f32(GemmF32 gin, int kchunk) {
                     }
             }
             if (more) {
-                stash<TILE, TA>(As[buf ^ 1], tid, va, m0, k0 + BK, g.M, ke);
-                stash<TILE, !TB>(Bs[buf ^ 1], tid, vb, n0, k0 + BK, g.N, ke);
+                stash<TILE, TA>(As[buf ^ 1], tid, va, m0, k0 + BK, g.M, ke, g.alpha);
+                stash<TILE, !TB>(Bs[buf ^ 1], tid, vb, n0, k0 + BK, g.N, ke, 1.f);
             }
             __syncthreads();
             buf ^= 1;
@@ -187,7 +208,6 @@ __global__ __launch_bounds__(NT) void k_gemm_f32(GemmF32 gin, int kchunk) {
     // output row (l >> 4) * 4 + r of the transposed product, column l & 15,
     // i.e. C row = base + (l & 15), column = base + 4 (l >> 4) + r.
     float *Cout = g.C + (g.splits > 1 ? (int64_t)split * g.split_stride : 0);
-    const bool direct = g.splits > 1 || g.beta == 0.f;
     const bool interior = (m0 + TILE <= g.M) && (n0 + TILE <= g.N);
 #pragma unroll
     for (int i = 0; i < BW; ++i) {
@@ -196,14 +216,9 @@ __global__ __launch_bounds__(NT) void k_gemm_f32(GemmF32 gin, int kchunk) {
         for (int j = 0; j < BW; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gn = n0 + wn * HALF + j * 16 + (g.layout ? (lane >> 4) + 4 * r
-                                                                    : 4 * (lane >> 4) + r);
-                if (interior || (gm < g.M && gn < g.N)) {
-                    float v = g.alpha * acc[i][j][r];
-                    float *p = Cout + (int64_t)gm + (int64_t)gn * g.ldc;
-                    if (!direct) v = fmaf(g.beta, *p, v);
-                    *p = v;
-                }
+                const int gn = n0 + wn * HALF + j * 16 + 4 * (lane >> 4) + r;
+                if (interior || (gm < g.M && gn < g.N))
+                    Cout[(int64_t)gm + (int64_t)gn * g.ldc] = acc[i][j][r];
             }
     }
 }

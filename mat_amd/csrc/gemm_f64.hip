@@ -29,36 +29,45 @@ constexpr int NT = 256;
 // colbase + j*16 + (lane >> 4) + 4 r.  v_mfma_f64_16x16x4_f64 with swapped
 // operands leaves output row (l >> 4) + 4 r, column l & 15 of the transposed
 // product in lane l, register r: 16 consecutive rows of one column per
-// row group, i.e. coalesced column-major stores.
+// row group, i.e. coalesced column-major stores.  acc already holds the
+// result (alpha was applied to op(A) in LDS; beta C was loaded into acc
+// before the K loop).
 template <int BI, int BJ>
 __device__ __forceinline__ void epilogue(const GemmF64 &g, v4d (&acc)[BI][BJ], int rowbase,
                                          int colbase, bool interior, int split) {
     const int lane = threadIdx.x & 63;
     double *Cout = g.C + (g.splits > 1 ? (int64_t)split * g.split_stride : 0);
-    const bool direct = g.splits > 1 || g.beta == 0.0;
 #pragma unroll
     for (int i = 0; i < BI; ++i) {
         const int gm = rowbase + i * 16 + (lane & 15);
-        double cv[BJ][4];
-        if (!direct) {
-#pragma unroll
-            for (int j = 0; j < BJ; ++j)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int gn = colbase + j * 16 + (lane >> 4) + 4 * r;
-                    const int cm = gm < g.M ? gm : g.M - 1, cn = gn < g.N ? gn : g.N - 1;
-                    cv[j][r] = Cout[(int64_t)cm + (int64_t)cn * g.ldc];
-                }
-        }
 #pragma unroll
         for (int j = 0; j < BJ; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int gn = colbase + j * 16 + (lane >> 4) + 4 * r;
-                double v = g.alpha * acc[i][j][r];
-                if (!direct) v = fma(g.beta, cv[j][r], v);
                 if (interior || (gm < g.M && gn < g.N))
-                    Cout[(int64_t)gm + (int64_t)gn * g.ldc] = v;
+                    Cout[(int64_t)gm + (int64_t)gn * g.ldc] = acc[i][j][r];
+            }
+    }
+}
+
+// acc = beta C for the tile (clamped, always in-bounds loads), issued before
+// the K loop so that the loads of C overlap the first slices.
+template <int BI, int BJ>
+__device__ __forceinline__ void preload_c(const GemmF64 &g, v4d (&acc)[BI][BJ], int rowbase,
+                                          int colbase) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < BI; ++i) {
+        const int gm = rowbase + i * 16 + (lane & 15);
+        const int cm = gm < g.M ? gm : g.M - 1;
+#pragma unroll
+        for (int j = 0; j < BJ; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gn = colbase + j * 16 + (lane >> 4) + 4 * r;
+                const int cn = gn < g.N ? gn : g.N - 1;
+                acc[i][j][r] = g.beta * g.C[(int64_t)cm + (int64_t)cn * g.ldc];
             }
     }
 }
@@ -110,7 +119,7 @@ __device__ __forceinline__ void fetch(const double *__restrict__ X, int64_t ld, 
 // (k, r) when KMAJOR, else (r, k).
 template <int TILE, bool KMAJOR, int MASK>
 __device__ __forceinline__ void stash(double (*S)[TILE + 2], int tid, const double *v, int r0,
-                                      int k0, int R, int kend, int mask_cols) {
+                                      int k0, int R, int kend, int mask_cols, double scale) {
     constexpr int NV = TILE * BK / NT;
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
@@ -127,12 +136,12 @@ __device__ __forceinline__ void stash(double (*S)[TILE + 2], int tid, const doub
         }
         const unsigned long long bits = (__double_as_longlong(v[q]) & (keep ? ~0ull : 0ull)) |
                                         (one ? 0x3FF0000000000000ull : 0ull);
-        S[k][r] = __longlong_as_double(bits);
+        S[k][r] = scale * __longlong_as_double(bits);
     }
 }
 
 template <int TILE, bool TA, bool TB, int AM, int BMK>
-__global__ __launch_bounds__(NT) void k_gemm_f64(GemmF64 gin, int kchunk) {
+__global__ __launch_bounds__(NT, 2) void k_gemm_f64(GemmF64 gin, int kchunk) {
     constexpr int NV = TILE * BK / NT, BW = TILE / 32, HALF = TILE / 2;
     __shared__ double As[2][BK][TILE + 2];
     __shared__ double Bs[2][BK][TILE + 2];
@@ -154,18 +163,22 @@ __global__ __launch_bounds__(NT) void k_gemm_f64(GemmF64 gin, int kchunk) {
     const int ke = min(g.K, kb + kchunk);
 
     v4d acc[BW][BW];
+    if (g.splits == 1 && g.beta != 0.0) {
+        preload_c<BW, BW>(g, acc, m0 + wm * HALF, n0 + wn * HALF);
+    } else {
 #pragma unroll
-    for (int i = 0; i < BW; ++i)
+        for (int i = 0; i < BW; ++i)
 #pragma unroll
-        for (int j = 0; j < BW; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+            for (int j = 0; j < BW; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+    }
 
     if (kb < ke) {
         double va[NV], vb[NV];
         // op(A) is stored k-major when transposed; op(B)^T likewise when not
         fetch<TILE, TA>(g.A, g.lda, m0, kb, g.M, ke, tid, va);
         fetch<TILE, !TB>(g.B, g.ldb, n0, kb, g.N, ke, tid, vb);
-        stash<TILE, TA, AM>(As[0], tid, va, m0, kb, g.M, ke, INT_MAX);
-        stash<TILE, !TB, BMK>(Bs[0], tid, vb, n0, kb, g.N, ke, g.bmask_cols);
+        stash<TILE, TA, AM>(As[0], tid, va, m0, kb, g.M, ke, INT_MAX, g.alpha);
+        stash<TILE, !TB, BMK>(Bs[0], tid, vb, n0, kb, g.N, ke, g.bmask_cols, 1.0);
         __syncthreads();
         int buf = 0;
         for (int k0 = kb; k0 < ke; k0 += BK) {
@@ -191,8 +204,9 @@ __global__ __launch_bounds__(NT) void k_gemm_f64(GemmF64 gin, int kchunk) {
                             __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
             }
             if (more) {
-                stash<TILE, TA, AM>(As[buf ^ 1], tid, va, m0, k0 + BK, g.M, ke, INT_MAX);
-                stash<TILE, !TB, BMK>(Bs[buf ^ 1], tid, vb, n0, k0 + BK, g.N, ke, g.bmask_cols);
+                stash<TILE, TA, AM>(As[buf ^ 1], tid, va, m0, k0 + BK, g.M, ke, INT_MAX, g.alpha);
+                stash<TILE, !TB, BMK>(Bs[buf ^ 1], tid, vb, n0, k0 + BK, g.N, ke, g.bmask_cols,
+                                      1.0);
             }
             __syncthreads();
             buf ^= 1;

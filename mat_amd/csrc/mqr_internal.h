@@ -154,7 +154,6 @@ struct GemmF32 {
     int splits = 1;
     int64_t split_stride = 0;
     int kind = K_GEMM;
-    int layout = 0;  // accumulator layout selector (0 is the gfx950 one)
     int batch = 1;   // strided batch as in GemmF64
     int64_t strideA = 0, strideB = 0, strideC = 0;
     // 0: FP32 MFMA; 1: BF16 products (hi * hi); 3: split BF16 (hi * hi +
