@@ -242,6 +242,101 @@ def gemm_rates(M, h) -> list:
     return out
 
 
+def qr_flops(m: int, n: int) -> float:
+    """dgeqrf count 2mn^2 - 2n^3/3 (m >= n)."""
+    return 2.0 * m * n * n - 2.0 * n ** 3 / 3.0
+
+
+def orgqr_flops(m: int, n: int, k: int) -> float:
+    """Thin Q (m x n from k reflectors): 4mnk - 2(m+n)k^2 + 4k^3/3."""
+    return 4.0 * m * n * k - 2.0 * (m + n) * k * k + 4.0 * k ** 3 / 3.0
+
+
+def qr_rates(M, h, steps: int = 3, seed: int = 2024) -> list:
+    """Plain FP64 QR (mqr_dgeqrf) of square and tall matrices, iid N(0,1) and
+    graded A = U diag(logspace(0, -8)) V^T (Haar U, V); the factorisation and
+    the thin Q (mqr_dorgqr) are timed separately with events on the handle's
+    stream (torch's current stream), and checked by ||QR - A||_F / ||A||_F and
+    ||Q^T Q - I||_F."""
+    import torch
+    out = []
+    cases = [("iid", 8192, 8192, None), ("graded", 8192, 8192, 1e-8),
+             ("iid", 32768, 1024, None), ("graded", 32768, 1024, 1e-8)]
+    for kind, m, n, smin in cases:
+        if smin is None:
+            A0 = M.colmajor(torch.from_numpy(
+                np.random.default_rng(seed).standard_normal((m, n))).cuda())
+        else:
+            U = haar(M, h, m, n, seed)
+            V = haar(M, h, n, n, seed + 1)
+            sv = torch.logspace(0.0, float(np.log10(smin)), n, device="cuda",
+                                dtype=torch.float64)
+            Us = M.colmajor((U * sv[None, :]).contiguous())
+            A0 = M.empty_colmajor(m, n)
+            M.gemm(h, "N", "T", 1.0, Us, M.colmajor(V.contiguous()), 0.0, A0)
+            del U, V, Us
+        k = min(m, n)
+        Aw = M.empty_colmajor(m, n)
+        f_ms, q_ms = [], []
+        for rep in range(steps + 1):
+            Aw.copy_(A0)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            tau, T = M.geqrf(h, Aw)
+            e1.record()
+            torch.cuda.synchronize()
+            if rep:
+                f_ms.append(e0.elapsed_time(e1))
+        for rep in range(steps + 1):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            Q = M.orgqr(h, Aw, T, k, nq=k)
+            e1.record()
+            torch.cuda.synchronize()
+            if rep:
+                q_ms.append(e0.elapsed_time(e1))
+            if rep < steps:
+                del Q
+        R = torch.triu(Aw[:k, :])
+        res = float(torch.linalg.norm(Q @ R - A0) / torch.linalg.norm(A0))
+        orth = float(torch.linalg.norm(Q.T @ Q - torch.eye(k, device="cuda", dtype=torch.float64)))
+        tf = float(np.median(f_ms))
+        tq = float(np.median(q_ms))
+        rec = {"case": f"{m}x{n} {kind}", "m": m, "n": n, "seed": seed,
+               "factor_ms": round(tf, 3),
+               "factor_tflops": round(qr_flops(m, n) / (tf * 1e-3) / 1e12, 3),
+               "frac_fp64_peak": round(qr_flops(m, n) / (tf * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"], 4),
+               "thin_q_ms": round(tq, 3),
+               "thin_q_tflops": round(orgqr_flops(m, k, k) / (tq * 1e-3) / 1e12, 3),
+               "qr_residual": res, "orthogonality": orth}
+        print(f"[bench] qr {rec['case']}: factor {tf:.3f} ms {rec['factor_tflops']} TFLOP/s, "
+              f"thin Q {tq:.3f} ms {rec['thin_q_tflops']} TFLOP/s, |QR-A|/|A| {res:.2e}, "
+              f"|QtQ-I| {orth:.2e}", file=sys.stderr, flush=True)
+        out.append(rec)
+        del A0, Aw, Q, R, tau, T
+        torch.cuda.empty_cache()
+    return out
+
+
+def cpu_qr_baseline(n: int = 1024, runs: int = 5) -> dict:
+    """The oracle's QR(A, Q, R, QR_Householder) at n x n iid N(0,1): blocked
+    factorisation and the full Q, one thread, median of `runs`."""
+    from oracle.oracle import Oracle
+    o = Oracle("auto")
+    A = np.asfortranarray(np.random.default_rng(11).standard_normal((n, n)))
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        o.qr(A)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return {"value": round(qr_flops(n, n) / t / 1e12, 5), "unit": "TFLOP/s (factor flops)",
+            "cores": 1, "kind": "port",
+            "sample": f"oracle QR(A, Q, R) {n}x{n} iid N(0,1), factor + full Q, FP64, 1 thread, "
+                      f"{o.variant} blocking, median of {runs} runs ({t:.3f} s each)"}
+
+
 def batched_rates(M, h, count: int = 10000, seed: int = 4321) -> list:
     """Batched uneven QR (m ~ U{16..512}, n ~ U{8..m}, iid N(0,1)) in FP64 and
     FP32: one persistent launch replayed from its hipGraph, timed with events
@@ -329,7 +424,7 @@ def main() -> int:
     ap.add_argument("--prec", default="fp32", choices=["fp32", "bf16x3", "bf16", "none"],
                     help="mixed-precision trailing updates ('none': plain FP64 only)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernel", choices=["gemm", "batched", "adjoint"], default=None,
+    ap.add_argument("--kernel", choices=["gemm", "batched", "adjoint", "qr"], default=None,
                     help="time one kernel family instead of the solve")
     args = ap.parse_args()
     if args.kernel == "adjoint":
@@ -341,6 +436,14 @@ def main() -> int:
         import mat_amd as M
         h = M.Handle(0)
         print(json.dumps({"batched_qr": batched_rates(M, h)}))
+        return 0
+    if args.kernel == "qr":
+        import mat_amd as M
+        h = M.Handle(0)
+        rec = {"qr_fp64": qr_rates(M, h)}
+        if not args.no_cpu:
+            rec["cpu_baseline"] = cpu_qr_baseline()
+        print(json.dumps(rec))
         return 0
     if args.kernel == "gemm":
         import mat_amd as M
