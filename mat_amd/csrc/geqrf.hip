@@ -127,9 +127,11 @@ int form_y_f64(hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv, 
 // Workspace (doubles) the factorisation of an m x n matrix uses: split-K
 // partials of the trailing update and of the next-block update, two Y
 // factors, and the tall-panel scratch.
+static int64_t w_parts_doubles(int64_t m) { return ((m + 63) / 64) * (int64_t)kMaxNb * kMaxNb; }
+
 static int64_t geqrf_ws_doubles(int64_t m, int64_t n, int nb) {
     return (int64_t)16 * nb * (n + nb) + (int64_t)64 * nb * nb + (int64_t)2 * m * nb + 4096 +
-           (int64_t)(panel_cholqr_ws_bytes(m) / sizeof(double)) + 64;
+           w_parts_doubles(m) + 64 + (int64_t)(panel_cholqr_ws_bytes(m) / sizeof(double)) + 64;
 }
 
 int panel_factor(mqr_handle_t hd, int64_t h, int b, double *V, int64_t lda, double *tau,
@@ -162,22 +164,33 @@ int geqrf_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, dou
     double *P = (double *)hd->ws.ptr;
     double *P2 = P + pcap;
     double *Ybuf[2] = {P2 + pcap2, P2 + pcap2 + m * nb};
-    void *pws = (void *)(((uintptr_t)(Ybuf[1] + m * nb + 1024) + 255) & ~(uintptr_t)255);
+    double *Wp = (double *)(((uintptr_t)(Ybuf[1] + m * nb) + 255) & ~(uintptr_t)255);
+    void *pws = (void *)(((uintptr_t)(Wp + w_parts_doubles(m) + 1024) + 255) & ~(uintptr_t)255);
     hipStream_t s0 = hd->stream, s1 = hd->side_stream;
     if (k == 0) return MQR_SUCCESS;
     // the side stream runs after everything queued on the handle's stream
     MQR_HIP(hipEventRecord(hd->ev_fork, s0));
     MQR_HIP(hipStreamWaitEvent(s1, hd->ev_fork, 0));
-    bool side_pending = false;
+    bool side_pending = false;  // the side stream's last update is not yet joined
     // a tall full-width panel runs the fused CholeskyQR sequence, which also
-    // forms Y = V T^T and takes the next-block update into its first launch
+    // forms Y = V T^T, takes the next-block update into its first launch and
+    // forms the following block's W = V^T C in its last
     auto fused = [&](int64_t hp, int bp, int64_t bcols) {
         return hp >= kCholqrMinRows && bp == bcols;
     };
+    auto fused_at = [&](int64_t c) {  // the panel starting at column c
+        if (c >= k || c >= n) return false;
+        return fused(m - c, (int)std::min<int64_t>(nb, k - c), std::min<int64_t>(nb, n - c));
+    };
+    bool w_ready = false;  // Wp[0:4096] = V^T C of the current panel and the next block
     const int b0 = (int)std::min<int64_t>(nb, k);
     if (fused(m, b0, b0)) {
-        MQR_CHECK(cholqr_gram1(s0, m, b0, 0, nullptr, 0, nullptr, A, lda, 0, pws));
-        MQR_CHECK(cholqr_rest(s0, m, b0, A, lda, tau, T, ldt, b0 < n ? Ybuf[0] : nullptr, pws));
+        MQR_CHECK(cholqr_gram1(s0, m, b0, 0, nullptr, 0, nullptr, 0, A, lda, 0, pws));
+        MQR_CHECK(cholqr_rest_a(s0, m, b0, A, lda, tau, T, ldt, pws));
+        w_ready = b0 < n && fused_at(b0);
+        MQR_CHECK(cholqr_finish(s0, m, b0, A, lda, T, ldt, b0 < n ? Ybuf[0] : nullptr,
+                                A + b0 * lda, lda, (int)std::min<int64_t>(nb, n - b0),
+                                w_ready ? Wp : nullptr, pws));
     } else {
         MQR_CHECK(panel_factor(hd, m, b0, A, lda, tau, T, ldt, pws));
         if (b0 < n) MQR_CHECK(form_y(s0, m, b0, A, lda, T, ldt, true, Ybuf[0]));
@@ -192,7 +205,10 @@ int geqrf_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, dou
         const int64_t bn = std::min<int64_t>(nb, n - next);
         const int64_t rest0 = next + bn, nrest = n - rest0;
         // block j-1's update of the remaining columns covers the next block
-        if (side_pending) MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
+        if (side_pending) {
+            MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
+            side_pending = false;
+        }
         if (nrest > 0) {
             MQR_HIP(hipEventRecord(hd->ev_fork, s0));
             MQR_HIP(hipStreamWaitEvent(s1, hd->ev_fork, 0));
@@ -200,18 +216,32 @@ int geqrf_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, dou
                                   pcap));
             MQR_HIP(hipEventRecord(hd->ev_join, s1));
             side_pending = true;
-        } else {
-            side_pending = false;
         }
         const int bnext = next < k ? (int)std::min<int64_t>(nb, k - next) : 0;
         if (next < k && fused(m - next, bnext, bn)) {
             double *Cn = A + j + next * lda;
-            MQR_CHECK(block_w(hd, s0, h, b, V, lda, Cn, lda, bn, P2, pcap2, K_LOOKAHEAD));
-            MQR_CHECK(cholqr_gram1(s0, h, (int)bn, b, Y, h, P2, Cn, lda, b, pws));
-            MQR_CHECK(cholqr_rest(s0, m - next, bnext, Cn + b, lda, tau + next, T + next * ldt,
-                                  ldt, next + bnext < n ? Ybuf[(it + 1) & 1] : nullptr, pws));
+            if (!w_ready)
+                MQR_CHECK(block_w(hd, s0, h, b, V, lda, Cn, lda, bn, P2, pcap2, K_LOOKAHEAD));
+            MQR_CHECK(cholqr_gram1(s0, h, (int)bn, b, Y, h, w_ready ? Wp : P2,
+                                   w_ready ? (int64_t)kMaxNb : (int64_t)b, Cn, lda, b, pws));
+            double *Xn = Cn + b;
+            MQR_CHECK(cholqr_rest_a(s0, m - next, bnext, Xn, lda, tau + next, T + next * ldt, ldt,
+                                    pws));
+            // W of the block after next needs this step's update of the
+            // remaining columns (side stream) first
+            const int64_t next2 = next + bnext;
+            w_ready = next2 < n && fused_at(next2);
+            if (w_ready && side_pending) {
+                MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
+                side_pending = false;
+            }
+            MQR_CHECK(cholqr_finish(s0, m - next, bnext, Xn, lda, T + next * ldt, ldt,
+                                    next2 < n ? Ybuf[(it + 1) & 1] : nullptr, A + next + next2 * lda,
+                                    lda, (int)std::min<int64_t>(nb, std::max<int64_t>(n - next2, 0)),
+                                    w_ready ? Wp : nullptr, pws));
             continue;
         }
+        w_ready = false;
         MQR_CHECK(apply_block(hd, s0, h, b, V, lda, Y, A + j + next * lda, lda, bn, P2, pcap2,
                               K_LOOKAHEAD, K_LOOKAHEAD));
         if (next < k) {

@@ -181,13 +181,22 @@ int panel_householder_gated(int64_t h, int b, double *P, int64_t ld, double *tau
 size_t panel_cholqr_ws_bytes(int64_t h);
 int panel_cholqr(mqr_handle_t hd, int64_t h, int b, double *X, int64_t ld, double *tau,
                  double *T, int64_t ldt, void *ws);
-// The same in two halves, for callers that fuse the next-block update into
-// the first: cholqr_gram1 applies C (hc x bn) -= Y W (kin > 0; Y ld ldy, W
-// kin x bn, ld kin), forms the Gram matrix of rows [g0, hc) of C (the panel)
-// and its first Cholesky pass; cholqr_rest factors that panel (h = hc - g0
-// rows at X = C + g0) and, when Yf, writes Yf (h x b, ld h) = V T^T.
+// The same in pieces, for callers that fuse the next-block update into the
+// first and the next W = V^T C into the last: cholqr_gram1 applies
+// C (hc x bn) -= Y W (kin > 0; Y ld ldy, W kin x bn, ld ldw), forms the Gram
+// matrix of rows [g0, hc) of C (the panel) and its first Cholesky pass;
+// cholqr_rest_a factors that panel (h = hc - g0 rows at X = C + g0) up to V1,
+// R, tau and T; cholqr_finish writes V2 and, when Yf, Yf (h x b, ld h) =
+// V T^T and, when Wp, W = V^T Cn (b x bn, ld 64) into Wp[0 : 4096] (Wp holds
+// ceil(h / 64) x 4096 doubles).  cholqr_rest = cholqr_rest_a + cholqr_finish
+// without W.
 int cholqr_gram1(hipStream_t s, int64_t hc, int bn, int kin, const double *Y, int64_t ldy,
-                 const double *W, double *C, int64_t ldc, int64_t g0, void *ws);
+                 const double *W, int64_t ldw, double *C, int64_t ldc, int64_t g0, void *ws);
+int cholqr_rest_a(hipStream_t s, int64_t h, int b, double *X, int64_t ld, double *tau, double *T,
+                  int64_t ldt, void *ws);
+int cholqr_finish(hipStream_t s, int64_t h, int b, double *X, int64_t ld, const double *T,
+                  int64_t ldt, double *Yf, const double *Cn, int64_t ldcn, int bn, double *Wp,
+                  void *ws);
 int cholqr_rest(hipStream_t s, int64_t h, int b, double *X, int64_t ld, double *tau, double *T,
                 int64_t ldt, double *Yf, void *ws);
 // Panels at least this tall use panel_cholqr.

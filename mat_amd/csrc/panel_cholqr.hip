@@ -171,6 +171,25 @@ __device__ __forceinline__ sb::v4d tmmaT(sb::v4d acc, const small::Lds *A, int i
     return acc;
 }
 
+// acc[jb] = X[tile row q] M[:, tile jb] for the upper-triangular M (k-blocks
+// <= jb), the four column tiles interleaved.
+__device__ __forceinline__ void rowtile_upper(sb::v4d (&acc)[4], const small::Lds *X,
+                                              const small::Lds *M, int q) {
+    const int l = small::lane_id(), i = l & 15, kq = l >> 4;
+#pragma unroll
+    for (int jb = 0; jb < 4; ++jb) acc[jb] = sb::zero4();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int kk = 4 * k + kq;
+        const double av = X[16 * q + i][kk];
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb)
+            if (jb >= (k >> 2))
+                acc[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(M[kk][16 * jb + i], av, acc[jb], 0,
+                                                               0, 0);
+    }
+}
+
 __device__ __forceinline__ int lrow() { return small::lane_id() & 15; }
 __device__ __forceinline__ int lcol(int r) { return (small::lane_id() >> 4) + 4 * r; }
 
@@ -202,17 +221,22 @@ __device__ __forceinline__ void lower_tile(int u, int &i, int &j) {
     else { i = 3; j = u - 3; }
 }
 
-// g += S^T S on the upper tiles wave q owns (tiles q, q + 4, q + 8)
+// g += S^T S on the upper tiles wave q owns (tiles q, q + 4, q + 8); the
+// tiles' MFMAs are interleaved so that independent accumulators pipeline.
 __device__ __forceinline__ void gram_acc(const small::Lds *S, sb::v4d (&g)[3]) {
-    const int q = small::wave_id();
+    const int q = small::wave_id(), l = small::lane_id(), i = l & 15, kq = l >> 4;
+    int ti[3], tj[3];
 #pragma unroll
-    for (int u = 0; u < 3; ++u) {
-        const int t = q + 4 * u;
-        if (t < 10) {
-            int i, j;
-            upper_tile(t, i, j);
+    for (int u = 0; u < 3; ++u) upper_tile(q + 4 * u < 10 ? q + 4 * u : 9, ti[u], tj[u]);
+    const bool third = q + 8 < 10;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) g[u] = tmmaT<true, false>(g[u], S, i, k, S, k, j);
+    for (int k = 0; k < 16; ++k) {  // 4-row slices of the 64-row chunk
+        const int kk = 4 * k + kq;
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            if (u == 2 && !third) continue;
+            const double av = S[kk][16 * ti[u] + i], bv = S[kk][16 * tj[u] + i];
+            g[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, av, g[u], 0, 0, 0);
         }
     }
 }
@@ -238,11 +262,11 @@ __device__ __forceinline__ void gram_store(const sb::v4d (&g)[3], double *__rest
     }
 }
 
-// C (hc x bn, ldc) -= Y (hc x kin, ldy) W (kin x bn, ld kin) when kin > 0,
+// C (hc x bn, ldc) -= Y (hc x kin, ldy) W (kin x bn, ldw) when kin > 0,
 // then the Gram partial of rows [g0, hc) of C into Gp[blockIdx.x].
 __global__ __launch_bounds__(small::NT) void k_upd_gram(int64_t hc, int kin, int bn,
                                                         const double *__restrict__ Y, int64_t ldy,
-                                                        const double *__restrict__ W,
+                                                        const double *__restrict__ W, int64_t ldw,
                                                         double *__restrict__ C, int64_t ldc,
                                                         int64_t g0, double *__restrict__ Gp) {
     __shared__ small::Lds Ys[RC], Ws[PB];
@@ -251,7 +275,7 @@ __global__ __launch_bounds__(small::NT) void k_upd_gram(int64_t hc, int kin, int
     if (kin > 0) {
         for (int e = threadIdx.x; e < PB * PB; e += small::NT) {
             const int r = e & (PB - 1), c = e >> 6;
-            Ws[r][c] = (r < kin && c < bn) ? W[r + (int64_t)c * kin] : 0.0;
+            Ws[r][c] = (r < kin && c < bn) ? W[r + (int64_t)c * ldw] : 0.0;
         }
     }
     for (int64_t r0 = (int64_t)blockIdx.x * RC; r0 < hc; r0 += (int64_t)gridDim.x * RC) {
@@ -268,10 +292,15 @@ __global__ __launch_bounds__(small::NT) void k_upd_gram(int64_t hc, int kin, int
             }
         __syncthreads();
         if (kin > 0) {
+            const int l = small::lane_id(), i = l & 15, kq = l >> 4;
+            for (int k = 0; 4 * k < kin; ++k) {
+                const int kk = 4 * k + kq;
+                const double av = -Ys[16 * q + i][kk];
 #pragma unroll
-            for (int jb = 0; jb < 4; ++jb)
-                for (int k = 0; 16 * k < kin; ++k)
-                    acc[jb] = tmmaT<false, false>(acc[jb], Ys, q, k, Ws, k, jb, -1.0);
+                for (int jb = 0; jb < 4; ++jb)
+                    acc[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ws[kk][16 * jb + i], av,
+                                                                   acc[jb], 0, 0, 0);
+            }
 #pragma unroll
             for (int jb = 0; jb < 4; ++jb)
 #pragma unroll
@@ -313,10 +342,9 @@ __global__ __launch_bounds__(small::NT) void k_apply_gram(int64_t h, int b,
         __syncthreads();
         sb::v4d acc[4];
         const int64_t row = r0 + 16 * q + lrow();
+        rowtile_upper(acc, Xs, Ms, q);
 #pragma unroll
         for (int jb = 0; jb < 4; ++jb) {
-            acc[jb] = sb::zero4();
-            for (int k = 0; k <= jb; ++k) acc[jb] = tmmaT<false, false>(acc[jb], Xs, q, k, Ms, k, jb);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int col = 16 * jb + lcol(r);
@@ -335,12 +363,17 @@ __global__ __launch_bounds__(small::NT) void k_apply_gram(int64_t h, int b,
 // One 64-row chunk per workgroup: V(b:h) = Z(b:h) Minv into the panel A
 // (unless *flag: then the fallback has written V), and, when Yf, the block
 // reflector's left factor Yf (ld h) = V T^T with V = [unit-lower V1; V2].
+// With Wp, also the chunk's partial of W = V^T Cn (the next block's bn
+// columns at the panel's rows, ldcn) into Wp[blockIdx.x] (64 x 64, ld 64).
 __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
                                                         const double *__restrict__ Z,
                                                         const double *__restrict__ Minv,
                                                         double *__restrict__ A, int64_t lda,
                                                         const double *__restrict__ T, int64_t ldt,
                                                         double *__restrict__ Yf,
+                                                        const double *__restrict__ Cn,
+                                                        int64_t ldcn, int bn,
+                                                        double *__restrict__ Wp,
                                                         const int *__restrict__ flag) {
     __shared__ small::Lds Vs[RC], Ms[PB], Ts[PB];
     const int q = small::wave_id();
@@ -352,10 +385,9 @@ __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
         chunk_load(Vs, Z, h, r0, h, b);
         __syncthreads();
         sb::v4d acc[4];
+        rowtile_upper(acc, Vs, Ms, q);
 #pragma unroll
         for (int jb = 0; jb < 4; ++jb) {
-            acc[jb] = sb::zero4();
-            for (int k = 0; k <= jb; ++k) acc[jb] = tmmaT<false, false>(acc[jb], Vs, q, k, Ms, k, jb);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int col = 16 * jb + lcol(r);
@@ -384,18 +416,54 @@ __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
         const int r = e & (PB - 1), c = e >> 6;
         Ts[r][c] = (r < b && c < b && r <= c) ? T[r + (int64_t)c * ldt] : 0.0;
     }
+    if (Wp) chunk_load(Ms, Cn, ldcn, r0, h, bn);  // Minv is no longer read
     __syncthreads();
     // Yf = V T^T: op(T)(k, j) = T(j, k), nonzero for k >= j
+    sb::v4d acc[4];
+    {
+        const int l = small::lane_id(), i = l & 15, kq = l >> 4;
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb) acc[jb] = sb::zero4();
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int kk = 4 * k + kq;
+            const double av = Vs[16 * q + i][kk];
+#pragma unroll
+            for (int jb = 0; jb < 4; ++jb)
+                if (jb <= (k >> 2))
+                    acc[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ts[16 * jb + i][kk], av, acc[jb],
+                                                                   0, 0, 0);
+        }
+    }
 #pragma unroll
     for (int jb = 0; jb < 4; ++jb) {
-        sb::v4d acc = sb::zero4();
-        for (int k = jb; k < 4; ++k) acc = tmmaT<false, true>(acc, Vs, q, k, Ts, k, jb);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int col = 16 * jb + lcol(r);
-            if (row < h && col < b) Yf[row + (int64_t)col * h] = acc[r];
+            if (row < h && col < b) Yf[row + (int64_t)col * h] = acc[jb][r];
         }
     }
+    if (!Wp) return;
+    // W partial: tile row q of V^T Cn over the chunk's 64 rows
+    {
+        const int l = small::lane_id(), i = l & 15, kq = l >> 4;
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb) acc[jb] = sb::zero4();
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int kk = 4 * k + kq;
+            const double av = Vs[kk][16 * q + i];
+#pragma unroll
+            for (int jb = 0; jb < 4; ++jb)
+                acc[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ms[kk][16 * jb + i], av, acc[jb], 0,
+                                                               0, 0);
+        }
+    }
+    double *Wb = Wp + (size_t)blockIdx.x * PB * PB;
+#pragma unroll
+    for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Wb[16 * q + lrow() + (16 * jb + lcol(r)) * PB] = acc[jb][r];
 }
 
 struct CqrWs {
@@ -428,7 +496,7 @@ size_t panel_cholqr_ws_bytes(int64_t h) {
 // First pass: C (hc x bn) -= Y W when kin > 0, the Gram matrix of rows
 // [g0, hc) of C (the panel), and its Cholesky factor R1 with R1^{-1}.
 int cholqr_gram1(hipStream_t s, int64_t hc, int bn, int kin, const double *Y, int64_t ldy,
-                 const double *W, double *C, int64_t ldc, int64_t g0, void *ws) {
+                 const double *W, int64_t ldw, double *C, int64_t ldc, int64_t g0, void *ws) {
     const CqrWs w = carve(ws, hc - g0);
     const int parts = gram_parts(hc);
     {
@@ -437,7 +505,7 @@ int cholqr_gram1(hipStream_t s, int64_t hc, int bn, int kin, const double *Y, in
                      2.0 * hh * bn * kin + hh * bn * bn,
                      8.0 * hh * (kin + (kin > 0 ? 2.0 : 1.0) * bn), s);
         hipLaunchKernelGGL(k_upd_gram, dim3(parts), dim3(small::NT), 0, s, hc, kin, bn, Y, ldy, W,
-                           C, ldc, g0, w.Gp);
+                           ldw, C, ldc, g0, w.Gp);
         MQR_LAUNCH_CHECK();
     }
     MQR_CHECK(reduce_partials((int64_t)PB * PB, w.Gp, parts, (int64_t)PB * PB, s));
@@ -449,10 +517,10 @@ int cholqr_gram1(hipStream_t s, int64_t hc, int bn, int kin, const double *Y, in
     return MQR_SUCCESS;
 }
 
-// Second pass, reconstruction, gated fallback and V2 of the h x b panel X
-// after cholqr_gram1; with Yf, also Yf (h x b, ld h) = V T^T.
-int cholqr_rest(hipStream_t s, int64_t h, int b, double *X, int64_t ld, double *tau, double *T,
-                int64_t ldt, double *Yf, void *ws) {
+// Second pass, reconstruction and gated fallback of the h x b panel X
+// after cholqr_gram1 (V2 follows in cholqr_finish).
+int cholqr_rest_a(hipStream_t s, int64_t h, int b, double *X, int64_t ld, double *tau, double *T,
+                  int64_t ldt, void *ws) {
     const CqrWs w = carve(ws, h);
     const int parts = gram_parts(h);
     const double hh = (double)h, bb = b;
@@ -471,22 +539,41 @@ int cholqr_rest(hipStream_t s, int64_t h, int b, double *X, int64_t ld, double *
         MQR_LAUNCH_CHECK();
     }
     // fallback: the column-by-column kernel runs only when the flag is up
-    MQR_CHECK(panel_householder_gated(h, b, X, ld, tau, T, ldt, w.flag, s));
+    return panel_householder_gated(h, b, X, ld, tau, T, ldt, w.flag, s);
+}
+
+// V2 of the panel; with Yf, Yf (h x b, ld h) = V T^T; with Wp, also
+// W = V^T Cn (b x bn, ld 64) for the next block's columns Cn (ldcn) into
+// Wp[0 : 64 x 64] (Wp holds ceil(h / 64) partials).
+int cholqr_finish(hipStream_t s, int64_t h, int b, double *X, int64_t ld, const double *T,
+                  int64_t ldt, double *Yf, const double *Cn, int64_t ldcn, int bn, double *Wp,
+                  void *ws) {
+    const CqrWs w = carve(ws, h);
+    const double hh = (double)h, bb = b;
+    const unsigned chunks = (unsigned)((h + RC - 1) / RC);
+    if (!Yf) Wp = nullptr;
     {
-        ProfScope ps(K_PANEL, hh * bb * bb + (Yf ? hh * bb * bb : 0.0),
-                     8.0 * hh * bb * (Yf ? 3.0 : 2.0), s);
-        hipLaunchKernelGGL(k_v2_formy, dim3((unsigned)((h + RC - 1) / RC)), dim3(small::NT), 0, s,
-                           h, b, (const double *)w.Z, (const double *)w.Mi, X, ld,
-                           (const double *)T, ldt, Yf, (const int *)w.flag);
+        ProfScope ps(K_PANEL, hh * bb * bb + (Yf ? hh * bb * bb : 0.0) + (Wp ? 2.0 * hh * bb * bn : 0.0),
+                     8.0 * hh * bb * (Yf ? 3.0 : 2.0) + (Wp ? 8.0 * hh * bn : 0.0), s);
+        hipLaunchKernelGGL(k_v2_formy, dim3(chunks), dim3(small::NT), 0, s, h, b,
+                           (const double *)w.Z, (const double *)w.Mi, X, ld, T, ldt, Yf, Cn, ldcn,
+                           bn, Wp, (const int *)w.flag);
         MQR_LAUNCH_CHECK();
     }
+    if (Wp) MQR_CHECK(reduce_partials((int64_t)PB * PB, Wp, (int)chunks, (int64_t)PB * PB, s));
     return MQR_SUCCESS;
+}
+
+int cholqr_rest(hipStream_t s, int64_t h, int b, double *X, int64_t ld, double *tau, double *T,
+                int64_t ldt, double *Yf, void *ws) {
+    MQR_CHECK(cholqr_rest_a(s, h, b, X, ld, tau, T, ldt, ws));
+    return cholqr_finish(s, h, b, X, ld, T, ldt, Yf, nullptr, 0, 0, nullptr, ws);
 }
 
 // Factor the h x b panel X (ld) in place.
 int panel_cholqr(mqr_handle_t hd, int64_t h, int b, double *X, int64_t ld, double *tau,
                  double *T, int64_t ldt, void *ws) {
-    MQR_CHECK(cholqr_gram1(hd->stream, h, b, 0, nullptr, 0, nullptr, X, ld, 0, ws));
+    MQR_CHECK(cholqr_gram1(hd->stream, h, b, 0, nullptr, 0, nullptr, 0, X, ld, 0, ws));
     return cholqr_rest(hd->stream, h, b, X, ld, tau, T, ldt, nullptr, ws);
 }
 

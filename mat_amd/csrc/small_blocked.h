@@ -6,9 +6,10 @@
 // The 64 x 64 matrices live in LDS (row r at S[r]) and are processed in
 // 16 x 16 tiles.  A factorisation goes by block columns: one wave factors
 // the 16 x 16 diagonal block in registers (lane j holds column j; each
-// elimination step broadcasts with v_readlane, no barrier) and inverts its
-// triangular factors by substitution, one column per lane; then the four
-// waves update the block row/column and the trailing tiles on the FP64 MFMA.
+// elimination step broadcasts within 16-lane rows by DPP, no barrier) and
+// inverts its triangular factors by substitution, one column per lane; then
+// the four waves update the block row/column and the trailing tiles on the
+// FP64 MFMA.
 // The full triangular inverses are assembled from the diagonal-block
 // inverses by recursive doubling, X_12 = -X_11 R_12 X_22, in two levels of
 // tile products.  Every barrier is reached by all threads.
@@ -68,6 +69,73 @@ __device__ __forceinline__ double rsqrt_nr(double d) {
     return y * fma(-h * y, y, 1.5);
 }
 
+// Value of v in lane K of the calling lane's 16-lane row (DPP row_newbcast).
+// The diagonal-block routines keep column l & 15 in lane l, so every row of
+// the wave holds the same 16 columns and this is a broadcast from column K.
+template <int K>
+__device__ __forceinline__ double rbcast(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x150 + K, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x150 + K, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Elimination step K of diag_chol (the steps are instantiated one by one so
+// that the broadcast lane is a compile-time constant).
+template <int K, typename Thr>
+__device__ __forceinline__ void chol_steps(double (&a)[16], double (&y)[16], int o, int b, Thr thr,
+                                           bool &bad) {
+    if constexpr (K < 16) {
+        const double d = rbcast<K>(a[K]);
+        bad |= (o + K < b) && !(d > thr(o + K));
+        y[K] = rsqrt_nr(d);
+        a[K] *= y[K];  // row K of R (columns j >= K)
+#pragma unroll
+        for (int i = K + 1; i < 16; ++i) {
+            double r;
+            // R(K, i) from column i
+            switch (i) {
+                case 1: r = rbcast<1>(a[K]); break;
+                case 2: r = rbcast<2>(a[K]); break;
+                case 3: r = rbcast<3>(a[K]); break;
+                case 4: r = rbcast<4>(a[K]); break;
+                case 5: r = rbcast<5>(a[K]); break;
+                case 6: r = rbcast<6>(a[K]); break;
+                case 7: r = rbcast<7>(a[K]); break;
+                case 8: r = rbcast<8>(a[K]); break;
+                case 9: r = rbcast<9>(a[K]); break;
+                case 10: r = rbcast<10>(a[K]); break;
+                case 11: r = rbcast<11>(a[K]); break;
+                case 12: r = rbcast<12>(a[K]); break;
+                case 13: r = rbcast<13>(a[K]); break;
+                case 14: r = rbcast<14>(a[K]); break;
+                default: r = rbcast<15>(a[K]); break;
+            }
+            a[i] = fma(-r, a[K], a[i]);
+        }
+        chol_steps<K + 1>(a, y, o, b, thr, bad);
+    }
+}
+
+// Elimination step K of diag_lu.
+template <int K>
+__device__ __forceinline__ void lu_steps(double (&a)[16], double (&ip)[16], int j, int o,
+                                         double *sg) {
+    if constexpr (K < 16) {
+        const double akk = rbcast<K>(a[K]);
+        const double s = akk >= 0.0 ? -1.0 : 1.0;
+        const double piv = akk - s;
+        ip[K] = small::recip(piv);
+        if (small::lane_id() == 0) sg[o + K] = s;
+        if (j == K) a[K] = piv;
+#pragma unroll
+        for (int i = K + 1; i < 16; ++i) {
+            const double lik = rbcast<K>(a[i]) * ip[K];
+            a[i] = (j > K) ? fma(-lik, a[K], a[i]) : ((j == K) ? lik : a[i]);
+        }
+        lu_steps<K + 1>(a, ip, j, o, sg);
+    }
+}
+
 // One wave: upper Cholesky factor of the 16 x 16 diagonal block kb of S
 // (upper triangle read, Schur-updated by the earlier blocks) in place, zero
 // below its diagonal; its inverse into the same block of X.  A pivot of
@@ -80,15 +148,7 @@ __device__ __forceinline__ void diag_chol(Lds *S, Lds *X, int kb, int b, Thr thr
 #pragma unroll
     for (int i = 0; i < 16; ++i) a[i] = S[o + i][o + j];
     bool bad = false;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        const double d = small::bcast(a[k], k);
-        bad |= (o + k < b) && !(d > thr(o + k));
-        y[k] = rsqrt_nr(d);
-        a[k] *= y[k];  // row k of R (columns j >= k)
-#pragma unroll
-        for (int i = k + 1; i < 16; ++i) a[i] = fma(-small::bcast(a[k], i), a[k], a[i]);
-    }
+    chol_steps<0>(a, y, o, b, thr, bad);
     if (bad && small::lane_id() == 0) *fail = 1;
     if (small::lane_id() < 16) {
 #pragma unroll
@@ -121,20 +181,7 @@ __device__ __forceinline__ void diag_lu(Lds *S, Lds *XU, Lds *XL, int kb, double
     double a[16], ip[16], x[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) a[i] = S[o + i][o + j];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        const double akk = small::bcast(a[k], k);
-        const double s = akk >= 0.0 ? -1.0 : 1.0;
-        const double piv = akk - s;
-        ip[k] = small::recip(piv);
-        if (small::lane_id() == 0) sg[o + k] = s;
-        if (j == k) a[k] = piv;
-#pragma unroll
-        for (int i = k + 1; i < 16; ++i) {
-            const double lik = small::bcast(a[i], k) * ip[k];
-            a[i] = (j > k) ? fma(-lik, a[k], a[i]) : ((j == k) ? lik : a[i]);
-        }
-    }
+    lu_steps<0>(a, ip, j, o, sg);
     if (small::lane_id() < 16) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) S[o + i][o + j] = a[i];
