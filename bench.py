@@ -98,8 +98,10 @@ def backward_errors(P, x):
 ROOF_KIND = "update_gemm"
 # HBM bytes moved per algorithmic byte by the FP64 trailing update, from two
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by a 537 MB device copy
-# in the same run; profiles/trailing_update_gemm_ls16384x4096.md.
-TRAFFIC_RATIO_FP64_UPDATE = 1.229
+# in the same run (profiles/trailing_update_gemm_ls16384x4096.md).
+TRAFFIC_RATIO_FP64_UPDATE = 1.225
+# The same for the FP32 trailing update of the mixed solve.
+TRAFFIC_RATIO_FP32_UPDATE = 1.299
 
 
 def run_plain(M, h, P, steps: int, warmup: int):
@@ -172,12 +174,12 @@ def roofline_from(timed: dict, prof: dict, prec: str) -> dict:
     achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[peak_name(prec)]
     bytes_per_launch = p["bytes"] / launches
-    traffic = (round(bytes_per_launch * TRAFFIC_RATIO_FP64_UPDATE) if prec == "fp64" else None)
+    ratio = {"fp64": TRAFFIC_RATIO_FP64_UPDATE, "fp32": TRAFFIC_RATIO_FP32_UPDATE}.get(prec)
+    traffic = round(bytes_per_launch * ratio) if ratio else None
     return {"kernel": ROOF_KIND, "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
             "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
-            "traffic_source": ("measured bytes/algorithmic bytes of the FP64 update tile "
-                               f"({TRAFFIC_RATIO_FP64_UPDATE}) x this run's algorithmic bytes"
-                               if traffic else None),
+            "traffic_source": (f"measured HBM bytes / algorithmic bytes of the {prec.upper()} update "
+                               f"tile ({ratio}) x this run's algorithmic bytes" if traffic else None),
             "avg_launch_ms": round(avg_ms, 5), "launches_timed": p["launches"],
             "flops_per_launch": flops_per_launch,
             "bytes_per_launch": bytes_per_launch,

@@ -68,14 +68,15 @@ __global__ void k_cast_v(int64_t h, int b, const double *__restrict__ V, int64_t
 __global__ void k_reduce_f32(int64_t count, float *__restrict__ P, int splits, int64_t stride) {
     const int64_t e = gidx();
     if (e >= count) return;
-    double a0 = 0.0, a1 = 0.0;
+    // eight independent loads per round keep the partials' reads in flight
+    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     int s = 0;
-    for (; s + 1 < splits; s += 2) {
-        a0 += (double)P[(int64_t)s * stride + e];
-        a1 += (double)P[(int64_t)(s + 1) * stride + e];
+    for (; s + 7 < splits; s += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] += (double)P[(int64_t)(s + u) * stride + e];
     }
-    if (s < splits) a0 += (double)P[(int64_t)s * stride + e];
-    P[e] = (float)(a0 + a1);
+    for (int u = 0; s < splits; ++s, ++u) a[u] += (double)P[(int64_t)s * stride + e];
+    P[e] = (float)(((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7])));
 }
 
 // R (ldr) entries of the block rows right of the diagonal blocks:
