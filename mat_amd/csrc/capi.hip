@@ -197,6 +197,7 @@ int mqr_destroy(mqr_handle_t h) {
     if (h->user_stream != h->stream) (void)hipStreamSynchronize(h->user_stream);
     if (h->ws.ptr) (void)hipFree(h->ws.ptr);
     if (h->ws_ir.ptr) (void)hipFree(h->ws_ir.ptr);
+    if (h->ws_vec.ptr) (void)hipFree(h->ws_vec.ptr);
     if (h->host_pinned) (void)hipHostFree(h->host_pinned);
     for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out})
         if (e) (void)hipEventDestroy(e);
@@ -447,8 +448,7 @@ int mqr_dgels(mqr_handle_t h, int64_t m, int64_t n, int64_t nrhs, double *A, int
     MQR_CHECK(ws_reserve(h, h->ws_ir, sizeof(double) * ((size_t)n * (nb + 1) + 64)));
     tauT = (double *)h->ws_ir.ptr;
     double *T = tauT + n;
-    MQR_CHECK(geqrf_f64(h, m, n, A, lda, tauT, T, nb));
-    MQR_CHECK(ormqr_f64(h, true, m, nrhs, n, A, lda, T, nb, B, ldb));
+    MQR_CHECK(geqrf_rhs_f64(h, m, n, A, lda, tauT, T, nb, B, ldb, nrhs));
     return trsm_upper_f64(h, false, n, nrhs, A, lda, B, ldb);
 }
 

@@ -71,6 +71,52 @@ __global__ __launch_bounds__(256) void k_trsm_diag(int nbk, int64_t nrhs, const 
     }
 }
 
+// One TRB-row block i0 of the back substitution R x = c (one right-hand
+// side).  Every workgroup loads the bs x bs diagonal block and c[i0:i0+bs]
+// (already reduced by the later blocks) and solves it with one wave (lane i
+// holds x_i; the pivot value is broadcast by v_readlane); workgroup 0 stores
+// x to xout, and workgroup w then subtracts R[r, i0:i0+bs] x from
+// c[r] for its 256 rows r < i0.
+__global__ __launch_bounds__(256) void k_trsv_block(int64_t i0, int bs, const double *__restrict__ R,
+                                                    int64_t ldr, double *__restrict__ c,
+                                                    double *__restrict__ xout) {
+    __shared__ double Rs[TRB][TRB + 1];
+    __shared__ double xs[TRB];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < bs * bs; e += 256) {
+        const int r = e % bs, cc = e / bs;
+        Rs[r][cc] = R[i0 + r + (i0 + cc) * ldr];
+    }
+    __syncthreads();
+    if (tid < 64) {
+        double xi = tid < bs ? c[i0 + tid] : 0.0;
+        for (int j = bs - 1; j >= 0; --j) {
+            // x_j is final in lane j once the later columns are eliminated
+            const double vj = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(xi), j),
+                                               __builtin_amdgcn_readlane(__double2loint(xi), j)) /
+                              Rs[j][j];
+            if (tid == j) xi = vj;
+            if (tid < j) xi = fma(-Rs[tid][j], vj, xi);
+        }
+        if (tid < bs) {
+            xs[tid] = xi;
+            if (blockIdx.x == 0) xout[i0 + tid] = xi;
+        }
+    }
+    __syncthreads();
+    const int64_t r = (int64_t)blockIdx.x * 256 + tid;
+    if (r < i0) {
+        double acc = 0.0;
+        for (int j = 0; j < bs; ++j) acc = fma(R[r + (i0 + j) * ldr], xs[j], acc);
+        c[r] -= acc;
+    }
+}
+
+__global__ void k_copy_vec(int64_t n, const double *__restrict__ x, double *__restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = x[i];
+}
+
 int target_blocks(mqr_handle_t h) { return 4 * h->cu_count; }
 
 // Y (h x b, ld h) = V op(T) on stream st, op(T) = T^T (trans) or T: the
@@ -156,6 +202,15 @@ int panel_factor(mqr_handle_t hd, int64_t h, int b, double *V, int64_t lda, doub
 // max(rest update, next update + panel) instead of their sum.
 int geqrf_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, double *tau,
               double *T, int64_t ldt) {
+    return geqrf_rhs_f64(hd, m, n, A, lda, tau, T, ldt, nullptr, 0, 0);
+}
+
+// The same, applying every block reflector also to the m x nrhs matrix B
+// (ldb), which ends as Q^T B.  B rides along on the side stream with the
+// trailing update, so Q^T B costs no separate pass over the reflectors.
+int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, double *tau,
+                  double *T, int64_t ldt, double *B, int64_t ldb, int64_t nrhs) {
+    if (nrhs <= 0) B = nullptr;
     const int64_t k = std::min(m, n);
     const int nb = hd->nb;
     const int64_t pcap = (int64_t)16 * nb * (n + nb);
@@ -183,6 +238,7 @@ int geqrf_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, dou
         return fused(m - c, (int)std::min<int64_t>(nb, k - c), std::min<int64_t>(nb, n - c));
     };
     bool w_ready = false;  // Wp[0:4096] = V^T C of the current panel and the next block
+    int64_t b_next = 0;    // first panel column whose block is not yet applied to B
     const int b0 = (int)std::min<int64_t>(nb, k);
     if (fused(m, b0, b0)) {
         MQR_CHECK(cholqr_gram1(s0, m, b0, 0, nullptr, 0, nullptr, 0, A, lda, 0, pws));
@@ -209,11 +265,17 @@ int geqrf_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, dou
             MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
             side_pending = false;
         }
-        if (nrest > 0) {
+        if (nrest > 0 || B) {
             MQR_HIP(hipEventRecord(hd->ev_fork, s0));
             MQR_HIP(hipStreamWaitEvent(s1, hd->ev_fork, 0));
-            MQR_CHECK(apply_block(hd, s1, h, b, V, lda, Y, A + j + rest0 * lda, lda, nrest, P,
-                                  pcap));
+            if (nrest > 0)
+                MQR_CHECK(apply_block(hd, s1, h, b, V, lda, Y, A + j + rest0 * lda, lda, nrest, P,
+                                      pcap));
+            if (B) {
+                MQR_CHECK(apply_block(hd, s1, h, b, V, lda, Y, B + j, ldb, nrhs, P, pcap, K_APPLYQ,
+                                      K_APPLYQ));
+                b_next = j + b;
+            }
             MQR_HIP(hipEventRecord(hd->ev_join, s1));
             side_pending = true;
         }
@@ -254,6 +316,14 @@ int geqrf_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, dou
         }
     }
     if (side_pending) MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
+    // blocks whose step ended the loop early (the last one when m >= n)
+    for (int64_t j = b_next; B && j < k; j += nb) {
+        const int b = (int)std::min<int64_t>(nb, k - j);
+        const double *V = A + j + j * lda;
+        MQR_CHECK(form_y(s0, m - j, b, V, lda, T + j * ldt, ldt, true, Ybuf[0]));
+        MQR_CHECK(apply_block(hd, s0, m - j, b, V, lda, Ybuf[0], B + j, ldb, nrhs, P, pcap,
+                              K_APPLYQ, K_APPLYQ));
+    }
     return MQR_SUCCESS;
 }
 
@@ -303,9 +373,36 @@ int orgqr_f64(mqr_handle_t hd, int64_t m, int64_t nq, int64_t k, const double *V
     return MQR_SUCCESS;
 }
 
+// Back substitution R x = c for one right-hand side (c overwritten by x),
+// one launch per TRB-row block from the last: every workgroup solves the
+// diagonal block from c (final by then), then subtracts that block's columns
+// times x from its share of the rows above; the block's x goes to a scratch
+// vector (other workgroups of the launch still read c there) and is copied
+// into c at the end.
+int trsv_upper_f64(mqr_handle_t hd, int64_t n, const double *R, int64_t ldr, double *c) {
+    MQR_CHECK(ws_reserve(hd, hd->ws_vec, sizeof(double) * (size_t)(n + 64)));
+    double *x = (double *)hd->ws_vec.ptr;
+    const int64_t nblk = (n + TRB - 1) / TRB;
+    for (int64_t q = nblk - 1; q >= 0; --q) {
+        const int64_t i0 = q * TRB;
+        const int bs = (int)std::min<int64_t>(TRB, n - i0);
+        const unsigned grid = (unsigned)std::max<int64_t>(1, (i0 + 255) / 256);
+        ProfScope ps(K_TRSM, (double)bs * bs + 2.0 * i0 * bs, 8.0 * ((double)bs * bs + i0 * bs),
+                     hd->stream);
+        hipLaunchKernelGGL(k_trsv_block, dim3(grid), dim3(256), 0, hd->stream, i0, bs, R, ldr, c,
+                           x);
+        MQR_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_copy_vec, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, hd->stream, n,
+                       (const double *)x, c);
+    MQR_LAUNCH_CHECK();
+    return MQR_SUCCESS;
+}
+
 int trsm_upper_f64(mqr_handle_t hd, bool trans, int64_t n, int64_t nrhs, const double *R,
                    int64_t ldr, double *B, int64_t ldb) {
     if (n <= 0 || nrhs <= 0) return MQR_SUCCESS;
+    if (nrhs == 1 && !trans) return trsv_upper_f64(hd, n, R, ldr, B);
     const int64_t nblk = (n + TRB - 1) / TRB;
     const unsigned cgrid = (unsigned)((nrhs + 63) / 64);
     for (int64_t q = 0; q < nblk; ++q) {

@@ -585,9 +585,11 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
         MQR_HIP(hipMemcpy2DAsync(w.F, sizeof(double) * m, A, sizeof(double) * lda,
                                  sizeof(double) * m, n, hipMemcpyDeviceToDevice, s));
     }
-    if (fp64)
-        MQR_CHECK(geqrf_f64(hd, m, n, w.F, m, w.tau, w.T, nb));
-    else
+    if (fp64) {
+        // b rides along as Q^T b
+        MQR_HIP(hipMemcpyAsync(w.r, b, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
+        MQR_CHECK(geqrf_rhs_f64(hd, m, n, w.F, m, w.tau, w.T, nb, w.r, m, 1));
+    } else
         MQR_CHECK(factor_fp32(hd, m, n, A, lda, b, w,
                               prec == MQR_PREC_BF16X3 ? 3 : prec == MQR_PREC_BF16 ? 1 : 0));
     MQR_CHECK(tri_inverse(hd, n, w.R, w.ldr, w.X, w.tmp));
@@ -598,8 +600,6 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
     hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, s, n, w.xx, 0.0);
     MQR_LAUNCH_CHECK();
     if (fp64) {
-        MQR_HIP(hipMemcpyAsync(w.r, b, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
-        MQR_CHECK(ormqr_f64(hd, true, m, 1, n, w.F, m, w.T, nb, w.r, m));
         MQR_HIP(hipMemcpyAsync(w.y, w.r, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     } else {
         hipLaunchKernelGGL(k_vec_s2d, dim3(grid_for(n)), dim3(256), 0, s, n,
@@ -653,8 +653,7 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
         MQR_HIP(hipMemcpy2DAsync(f.F, sizeof(double) * m, A, sizeof(double) * lda,
                                  sizeof(double) * m, n, hipMemcpyDeviceToDevice, s));
         MQR_HIP(hipMemcpyAsync(f.r, b, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
-        MQR_CHECK(geqrf_f64(hd, m, n, f.F, m, f.tau, f.T, nb));
-        MQR_CHECK(ormqr_f64(hd, true, m, 1, n, f.F, m, f.T, nb, f.r, m));
+        MQR_CHECK(geqrf_rhs_f64(hd, m, n, f.F, m, f.tau, f.T, nb, f.r, m, 1));
         MQR_CHECK(trsm_upper_f64(hd, false, n, 1, f.F, m, f.r, m));
         MQR_HIP(hipMemcpyAsync(x, f.r, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         hipLaunchKernelGGL(k_sq, dim3(grid_for(n)), dim3(256), 0, s, n, (const double *)x, f.xx);

@@ -99,6 +99,7 @@ struct mqr_handle_st {
     int cu_count = 256;
     mqr::Workspace ws;             // general scratch
     mqr::Workspace ws_ir;          // mixed-precision / refinement scratch
+    mqr::Workspace ws_vec;         // vector scratch of the triangular solve
     double *host_pinned = nullptr; // small pinned buffer for device->host scalars
     mqr::Profiler prof;
 };
@@ -199,6 +200,11 @@ int cholqr_finish(hipStream_t s, int64_t h, int b, double *X, int64_t ld, const 
                   void *ws);
 int cholqr_rest(hipStream_t s, int64_t h, int b, double *X, int64_t ld, double *tau, double *T,
                 int64_t ldt, double *Yf, void *ws);
+// Blocked FP64 QR of A (m x n, lda) in place with tau and the T blocks (ldt);
+// the block reflectors are also applied to B (m x nrhs, ldb), which ends as
+// Q^T B (B may be null).
+int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, double *tau,
+                  double *T, int64_t ldt, double *B, int64_t ldb, int64_t nrhs);
 // Panels at least this tall use panel_cholqr.
 constexpr int64_t kCholqrMinRows = 384;
 // Either of the above by height.

@@ -180,6 +180,37 @@ def test_trsm_upper(h):
         np.testing.assert_allclose(host(dB), ref, rtol=0, atol=1e-12 * n * np.abs(ref).max())
 
 
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 333, 1100])
+def test_trsv_upper_one_rhs(h, n):
+    """One right-hand side takes the per-block back substitution."""
+    rng = np.random.default_rng(60 + n)
+    R = np.triu(rng.standard_normal((n, n))) + 4 * np.eye(n) * np.sign(rng.standard_normal(n))
+    b = rng.standard_normal((n, 1))
+    import scipy.linalg as sla
+    db = M.colmajor(b)
+    M.trsm_upper(h, "N", M.colmajor(R), db)
+    ref = sla.solve_triangular(R, b, lower=False)
+    np.testing.assert_allclose(host(db), ref, rtol=0, atol=1e-12 * n * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("nrhs", [1, 2])
+def test_dgels_qtb_contract(h, nrhs):
+    """mqr_dgels leaves Q^T B in B: rows [0, n) hold X and the norm of rows
+    [n, m) is the residual norm ||B - A X|| (Q orthogonal)."""
+    rng = np.random.default_rng(70 + nrhs)
+    m, n = 2000, 640
+    A = rng.standard_normal((m, n))
+    B = rng.standard_normal((m, nrhs))
+    dA, dB = M.colmajor(A), M.colmajor(B)
+    X = host(M.gels(h, dA, dB))
+    QtB = host(dB)
+    Xref = np.linalg.lstsq(A, B, rcond=None)[0]
+    np.testing.assert_allclose(X, Xref, rtol=0, atol=1e-10 * max(1, np.abs(Xref).max()))
+    for c in range(nrhs):
+        r = np.linalg.norm(B[:, c] - A @ X[:, c])
+        assert abs(np.linalg.norm(QtB[n:, c]) - r) <= 1e-12 * np.linalg.norm(B[:, c]) * m
+
+
 def test_dgels_matches_oracle(h, orc):
     rng = np.random.default_rng(7)
     for (m, n, nrhs) in [(600, 200, 1), (1000, 1000, 2), (4000, 130, 3)]:
