@@ -249,6 +249,7 @@ struct MixedWs {
     float *V32[2] = {nullptr, nullptr}, *Y32[2] = {nullptr, nullptr};
     float *P32 = nullptr, *P32b = nullptr;
     double *Pb2 = nullptr, *Yd = nullptr;
+    double *Wp = nullptr;  // partials of the next block's W (FP32 mode)
     double *X = nullptr, *T = nullptr, *tau = nullptr;
     double *r = nullptr, *z = nullptr, *y = nullptr, *part = nullptr, *sc = nullptr;
     double *xx = nullptr, *zz = nullptr, *tmp = nullptr;
@@ -284,6 +285,7 @@ MixedWs carve(char *base, bool fp64, int64_t m, int64_t n, int nb) {
         }
         w.P32 = (float *)take(sizeof(float) * kMaxSplits * nb * (n + 1));
         w.P32b = (float *)take(sizeof(float) * kMaxSplits * nb * nb);
+        w.Wp = (double *)take(sizeof(double) * ((m + 63) / 64) * 64 * 64);
     }
     w.X = (double *)take(sizeof(double) * n * n);
     w.tmp = (double *)take(sizeof(double) * ((n + 1) / 2) * n);
@@ -380,7 +382,31 @@ int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t 
     MQR_HIP(hipStreamWaitEvent(s1, hd->ev_fork, 0));
     double *Vb[2] = {w.Pb, w.Pb2};
     bool side_pending = false;
-    MQR_CHECK(panel_fp32(hd, m, 0, (int)std::min<int64_t>(nb, k), w, Vb[0], w.V32[0], w.Y32[0]));
+    // Tall full-width panels take the fused sequence of the FP64 path: the
+    // first launch updates the next block (FP32 storage, FP64 arithmetic),
+    // writes the panel's FP64 copy and its Gram matrix; the last forms V2,
+    // Y = V T^T, their FP32 copies, the panel's R block and the following
+    // block's W.
+    auto fused = [&](int64_t hp, int bp, int64_t bcols) {
+        return hp >= kCholqrMinRows && bp == bcols;
+    };
+    auto fused_at = [&](int64_t c) {
+        if (c >= k || c >= nc) return false;
+        return fused(m - c, (int)std::min<int64_t>(nb, k - c), std::min<int64_t>(nb, nc - c));
+    };
+    bool w_ready = false;  // w.Wp[0:4096] = V^T C of the current panel and the next block
+    const int b0 = (int)std::min<int64_t>(nb, k);
+    if (fused_at(0)) {
+        MQR_CHECK(cholqr_gram1(s0, m, b0, 0, nullptr, 0, nullptr, 0, w.Aw, m, 0, w.pws, true,
+                               Vb[0], m));
+        MQR_CHECK(cholqr_rest_a(s0, m, b0, Vb[0], m, w.tau, w.T, nb, w.pws));
+        w_ready = fused_at(b0);
+        MQR_CHECK(cholqr_finish(s0, m, b0, Vb[0], m, w.T, nb, w.Yd, w.Aw + b0 * m, m,
+                                (int)std::min<int64_t>(nb, nc - b0), w_ready ? w.Wp : nullptr,
+                                w.pws, true, w.V32[0], w.Y32[0], w.R, w.ldr));
+    } else {
+        MQR_CHECK(panel_fp32(hd, m, 0, b0, w, Vb[0], w.V32[0], w.Y32[0]));
+    }
     for (int64_t j = 0, it = 0; j < k; j += nb, ++it) {
         const int bj = (int)std::min<int64_t>(nb, k - j);
         const int64_t h = m - j;
@@ -388,7 +414,10 @@ int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t 
         const int64_t next = j + bj;
         const int64_t bn = std::min<int64_t>(nb, nc - next);
         const int64_t rest0 = next + bn, nrest = nc - rest0;
-        if (side_pending) MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
+        if (side_pending) {
+            MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
+            side_pending = false;
+        }
         if (nrest > 0) {
             MQR_HIP(hipEventRecord(hd->ev_fork, s0));
             MQR_HIP(hipStreamWaitEvent(s1, hd->ev_fork, 0));
@@ -396,13 +425,33 @@ int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t 
                                       w.P32, pcap, K_VTA, K_UPDATE, bf16_terms));
             MQR_HIP(hipEventRecord(hd->ev_join, s1));
             side_pending = true;
-        } else {
-            side_pending = false;
         }
+        const int bnext = next < k ? (int)std::min<int64_t>(nb, k - next) : 0;
+        if (w_ready && next < k && fused(m - next, bnext, bn)) {
+            double *Vn = Vb[(it + 1) & 1];
+            MQR_CHECK(cholqr_gram1(s0, h, (int)bn, bj, w.Yd, h, w.Wp, 64, w.Aw + j + next * m, m,
+                                   bj, w.pws, true, Vn, m));
+            MQR_CHECK(cholqr_rest_a(s0, m - next, bnext, Vn, m, w.tau + next, w.T + next * nb, nb,
+                                    w.pws));
+            // W of the block after next needs this step's update of the
+            // remaining columns (side stream) first
+            const int64_t next2 = next + bnext;
+            w_ready = fused_at(next2);
+            if (w_ready && side_pending) {
+                MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
+                side_pending = false;
+            }
+            MQR_CHECK(cholqr_finish(s0, m - next, bnext, Vn, m, w.T + next * nb, nb, w.Yd,
+                                    w.Aw + next + std::min(next2, nc - 1) * m, m,
+                                    (int)std::min<int64_t>(nb, std::max<int64_t>(nc - next2, 0)),
+                                    w_ready ? w.Wp : nullptr, w.pws, true, w.V32[(it + 1) & 1],
+                                    w.Y32[(it + 1) & 1], w.R + next + next * w.ldr, w.ldr));
+            continue;
+        }
+        w_ready = false;
         MQR_CHECK(apply_block_f32(hd, s0, h, bj, V32, Y32, w.Aw + j + next * m, m, bn, w.P32b,
                                   pcap2, K_LOOKAHEAD, K_LOOKAHEAD, bf16_terms));
         if (next < k) {
-            const int bnext = (int)std::min<int64_t>(nb, k - next);
             MQR_CHECK(panel_fp32(hd, m, next, bnext, w, Vb[(it + 1) & 1], w.V32[(it + 1) & 1],
                                  w.Y32[(it + 1) & 1]));
         }

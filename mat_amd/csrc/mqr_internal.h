@@ -191,13 +191,20 @@ int panel_cholqr(mqr_handle_t hd, int64_t h, int b, double *X, int64_t ld, doubl
 // V T^T and, when Wp, W = V^T Cn (b x bn, ld 64) into Wp[0 : 4096] (Wp holds
 // ceil(h / 64) x 4096 doubles).  cholqr_rest = cholqr_rest_a + cholqr_finish
 // without W.
+// C is FP64, or FP32 when c_f32 (then Xo, ld ldxo, receives the panel rows
+// in FP64 for the factorisation).
 int cholqr_gram1(hipStream_t s, int64_t hc, int bn, int kin, const double *Y, int64_t ldy,
-                 const double *W, int64_t ldw, double *C, int64_t ldc, int64_t g0, void *ws);
+                 const double *W, int64_t ldw, void *C, int64_t ldc, int64_t g0, void *ws,
+                 bool c_f32 = false, double *Xo = nullptr, int64_t ldxo = 0);
 int cholqr_rest_a(hipStream_t s, int64_t h, int b, double *X, int64_t ld, double *tau, double *T,
                   int64_t ldt, void *ws);
+// Cn is FP64, or FP32 when cn_f32; with Yf, V32 and Y32 (ld h) receive FP32
+// copies of V (unit-lower V1 explicit) and Yf, and Rout (ldr) the panel's
+// b x b R block (zero below the diagonal).
 int cholqr_finish(hipStream_t s, int64_t h, int b, double *X, int64_t ld, const double *T,
-                  int64_t ldt, double *Yf, const double *Cn, int64_t ldcn, int bn, double *Wp,
-                  void *ws);
+                  int64_t ldt, double *Yf, const void *Cn, int64_t ldcn, int bn, double *Wp,
+                  void *ws, bool cn_f32 = false, float *V32 = nullptr, float *Y32 = nullptr,
+                  double *Rout = nullptr, int64_t ldr = 0);
 int cholqr_rest(hipStream_t s, int64_t h, int b, double *X, int64_t ld, double *tau, double *T,
                 int64_t ldt, double *Yf, void *ws);
 // Blocked FP64 QR of A (m x n, lda) in place with tau and the T blocks (ldt);

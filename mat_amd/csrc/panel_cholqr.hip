@@ -199,11 +199,12 @@ __device__ __forceinline__ void tstoreT(small::Lds *S, int ia, int jb, sb::v4d v
 }
 
 // S[r][c] = M[r0 + r, c] for r0 + r < rows and c < cols, else 0
-__device__ __forceinline__ void chunk_load(small::Lds *S, const double *__restrict__ M, int64_t ld,
+template <typename TM>
+__device__ __forceinline__ void chunk_load(small::Lds *S, const TM *__restrict__ M, int64_t ld,
                                            int64_t r0, int64_t rows, int cols) {
     for (int e = threadIdx.x; e < RC * PB; e += small::NT) {
         const int r = e & (RC - 1), c = e >> 6;
-        S[r][c] = (r0 + r < rows && c < cols) ? M[r0 + r + (int64_t)c * ld] : 0.0;
+        S[r][c] = (r0 + r < rows && c < cols) ? (double)M[r0 + r + (int64_t)c * ld] : 0.0;
     }
 }
 
@@ -262,13 +263,17 @@ __device__ __forceinline__ void gram_store(const sb::v4d (&g)[3], double *__rest
     }
 }
 
-// C (hc x bn, ldc) -= Y (hc x kin, ldy) W (kin x bn, ldw) when kin > 0,
-// then the Gram partial of rows [g0, hc) of C into Gp[blockIdx.x].
+// C (hc x bn, ldc, FP64 or FP32) -= Y (hc x kin, ldy) W (kin x bn, ldw)
+// when kin > 0 (in FP64), then the Gram partial of rows [g0, hc) of C into
+// Gp[blockIdx.x]; with Xo, also an FP64 copy of those rows into
+// Xo[row - g0 + col * ldxo] (the panel's FP64 working copy when C is FP32).
+template <typename TC>
 __global__ __launch_bounds__(small::NT) void k_upd_gram(int64_t hc, int kin, int bn,
                                                         const double *__restrict__ Y, int64_t ldy,
                                                         const double *__restrict__ W, int64_t ldw,
-                                                        double *__restrict__ C, int64_t ldc,
-                                                        int64_t g0, double *__restrict__ Gp) {
+                                                        TC *__restrict__ C, int64_t ldc,
+                                                        int64_t g0, double *__restrict__ Gp,
+                                                        double *__restrict__ Xo, int64_t ldxo) {
     __shared__ small::Lds Ys[RC], Ws[PB];
     const int q = small::wave_id();
     sb::v4d g[3] = {sb::zero4(), sb::zero4(), sb::zero4()};
@@ -288,7 +293,7 @@ __global__ __launch_bounds__(small::NT) void k_upd_gram(int64_t hc, int kin, int
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int col = 16 * jb + lcol(r);
-                acc[jb][r] = (row < hc && col < bn) ? C[row + (int64_t)col * ldc] : 0.0;
+                acc[jb][r] = (row < hc && col < bn) ? (double)C[row + (int64_t)col * ldc] : 0.0;
             }
         __syncthreads();
         if (kin > 0) {
@@ -306,7 +311,17 @@ __global__ __launch_bounds__(small::NT) void k_upd_gram(int64_t hc, int kin, int
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int col = 16 * jb + lcol(r);
-                    if (row < hc && col < bn) C[row + (int64_t)col * ldc] = acc[jb][r];
+                    if (row < hc && col < bn) C[row + (int64_t)col * ldc] = (TC)acc[jb][r];
+                }
+        }
+        if (Xo) {
+#pragma unroll
+            for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = 16 * jb + lcol(r);
+                    if (row >= g0 && row < hc && col < bn)
+                        Xo[row - g0 + (int64_t)col * ldxo] = acc[jb][r];
                 }
         }
         __syncthreads();  // Ys no longer read by the update
@@ -365,16 +380,20 @@ __global__ __launch_bounds__(small::NT) void k_apply_gram(int64_t h, int b,
 // reflector's left factor Yf (ld h) = V T^T with V = [unit-lower V1; V2].
 // With Wp, also the chunk's partial of W = V^T Cn (the next block's bn
 // columns at the panel's rows, ldcn) into Wp[blockIdx.x] (64 x 64, ld 64).
+template <typename TC>
 __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
                                                         const double *__restrict__ Z,
                                                         const double *__restrict__ Minv,
                                                         double *__restrict__ A, int64_t lda,
                                                         const double *__restrict__ T, int64_t ldt,
                                                         double *__restrict__ Yf,
-                                                        const double *__restrict__ Cn,
+                                                        const TC *__restrict__ Cn,
                                                         int64_t ldcn, int bn,
                                                         double *__restrict__ Wp,
-                                                        const int *__restrict__ flag) {
+                                                        const int *__restrict__ flag,
+                                                        float *__restrict__ V32,
+                                                        float *__restrict__ Y32,
+                                                        double *__restrict__ Rout, int64_t ldr) {
     __shared__ small::Lds Vs[RC], Ms[PB], Ts[PB];
     const int q = small::wave_id();
     const bool fb = *flag != 0;
@@ -408,8 +427,11 @@ __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
         for (int e = threadIdx.x; e < RC * PB; e += small::NT) {
             const int r = e & (RC - 1), c = e >> 6;
             const int64_t gr = r0 + r;
-            if (gr < b)
-                Vs[r][c] = c < b ? (gr > c ? A[gr + (int64_t)c * lda] : (gr == c ? 1.0 : 0.0)) : 0.0;
+            if (gr < b) {
+                const double a = c < b ? A[gr + (int64_t)c * lda] : 0.0;
+                Vs[r][c] = c < b ? (gr > c ? a : (gr == c ? 1.0 : 0.0)) : 0.0;
+                if (Rout && c < b) Rout[gr + (int64_t)c * ldr] = gr <= c ? a : 0.0;
+            }
         }
     }
     for (int e = threadIdx.x; e < PB * PB; e += small::NT) {
@@ -418,6 +440,16 @@ __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
     }
     if (Wp) chunk_load(Ms, Cn, ldcn, r0, h, bn);  // Minv is no longer read
     __syncthreads();
+    if (V32) {
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = 16 * jb + lcol(r);
+                if (row < h && col < b)
+                    V32[row + (int64_t)col * h] = (float)Vs[16 * q + lrow()][col];
+            }
+    }
     // Yf = V T^T: op(T)(k, j) = T(j, k), nonzero for k >= j
     sb::v4d acc[4];
     {
@@ -440,7 +472,10 @@ __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int col = 16 * jb + lcol(r);
-            if (row < h && col < b) Yf[row + (int64_t)col * h] = acc[jb][r];
+            if (row < h && col < b) {
+                Yf[row + (int64_t)col * h] = acc[jb][r];
+                if (Y32) Y32[row + (int64_t)col * h] = (float)acc[jb][r];
+            }
         }
     }
     if (!Wp) return;
@@ -496,7 +531,8 @@ size_t panel_cholqr_ws_bytes(int64_t h) {
 // First pass: C (hc x bn) -= Y W when kin > 0, the Gram matrix of rows
 // [g0, hc) of C (the panel), and its Cholesky factor R1 with R1^{-1}.
 int cholqr_gram1(hipStream_t s, int64_t hc, int bn, int kin, const double *Y, int64_t ldy,
-                 const double *W, int64_t ldw, double *C, int64_t ldc, int64_t g0, void *ws) {
+                 const double *W, int64_t ldw, void *C, int64_t ldc, int64_t g0, void *ws,
+                 bool c_f32, double *Xo, int64_t ldxo) {
     const CqrWs w = carve(ws, hc - g0);
     const int parts = gram_parts(hc);
     {
@@ -504,8 +540,12 @@ int cholqr_gram1(hipStream_t s, int64_t hc, int bn, int kin, const double *Y, in
         ProfScope ps(kin > 0 ? K_LOOKAHEAD : K_PANEL,
                      2.0 * hh * bn * kin + hh * bn * bn,
                      8.0 * hh * (kin + (kin > 0 ? 2.0 : 1.0) * bn), s);
-        hipLaunchKernelGGL(k_upd_gram, dim3(parts), dim3(small::NT), 0, s, hc, kin, bn, Y, ldy, W,
-                           ldw, C, ldc, g0, w.Gp);
+        if (c_f32)
+            hipLaunchKernelGGL(k_upd_gram<float>, dim3(parts), dim3(small::NT), 0, s, hc, kin, bn,
+                               Y, ldy, W, ldw, (float *)C, ldc, g0, w.Gp, Xo, ldxo);
+        else
+            hipLaunchKernelGGL(k_upd_gram<double>, dim3(parts), dim3(small::NT), 0, s, hc, kin, bn,
+                               Y, ldy, W, ldw, (double *)C, ldc, g0, w.Gp, Xo, ldxo);
         MQR_LAUNCH_CHECK();
     }
     MQR_CHECK(reduce_partials((int64_t)PB * PB, w.Gp, parts, (int64_t)PB * PB, s));
@@ -546,18 +586,25 @@ int cholqr_rest_a(hipStream_t s, int64_t h, int b, double *X, int64_t ld, double
 // W = V^T Cn (b x bn, ld 64) for the next block's columns Cn (ldcn) into
 // Wp[0 : 64 x 64] (Wp holds ceil(h / 64) partials).
 int cholqr_finish(hipStream_t s, int64_t h, int b, double *X, int64_t ld, const double *T,
-                  int64_t ldt, double *Yf, const double *Cn, int64_t ldcn, int bn, double *Wp,
-                  void *ws) {
+                  int64_t ldt, double *Yf, const void *Cn, int64_t ldcn, int bn, double *Wp,
+                  void *ws, bool cn_f32, float *V32, float *Y32, double *Rout, int64_t ldr) {
     const CqrWs w = carve(ws, h);
     const double hh = (double)h, bb = b;
     const unsigned chunks = (unsigned)((h + RC - 1) / RC);
-    if (!Yf) Wp = nullptr;
+    if (!Yf) Wp = nullptr, V32 = nullptr, Y32 = nullptr;
     {
         ProfScope ps(K_PANEL, hh * bb * bb + (Yf ? hh * bb * bb : 0.0) + (Wp ? 2.0 * hh * bb * bn : 0.0),
                      8.0 * hh * bb * (Yf ? 3.0 : 2.0) + (Wp ? 8.0 * hh * bn : 0.0), s);
-        hipLaunchKernelGGL(k_v2_formy, dim3(chunks), dim3(small::NT), 0, s, h, b,
-                           (const double *)w.Z, (const double *)w.Mi, X, ld, T, ldt, Yf, Cn, ldcn,
-                           bn, Wp, (const int *)w.flag);
+        if (cn_f32)
+            hipLaunchKernelGGL(k_v2_formy<float>, dim3(chunks), dim3(small::NT), 0, s, h, b,
+                               (const double *)w.Z, (const double *)w.Mi, X, ld, T, ldt, Yf,
+                               (const float *)Cn, ldcn, bn, Wp, (const int *)w.flag, V32, Y32,
+                               Rout, ldr);
+        else
+            hipLaunchKernelGGL(k_v2_formy<double>, dim3(chunks), dim3(small::NT), 0, s, h, b,
+                               (const double *)w.Z, (const double *)w.Mi, X, ld, T, ldt, Yf,
+                               (const double *)Cn, ldcn, bn, Wp, (const int *)w.flag, V32, Y32,
+                               Rout, ldr);
         MQR_LAUNCH_CHECK();
     }
     if (Wp) MQR_CHECK(reduce_partials((int64_t)PB * PB, Wp, (int)chunks, (int64_t)PB * PB, s));
