@@ -12,10 +12,16 @@
 //
 // Per matrix, blocks of NB columns (NB = 8 or 16, per size bin from a tuned
 // table):
-//   panel   : the h x b panel is staged in LDS (column-major, ld = h + 1) and
-//             factored column by column with workgroup reductions
-//             (QR/__Householder_Blocked.h:62-84), T from
-//             T^{-1} = strict_upper(V^T V) + diag(1 / tau) (:86-103);
+//   panel   : the h x b panel is held in registers (thread t owns rows t and
+//             t + 256) and factored column by column
+//             (QR/__Householder_Blocked.h:62-84).  One workgroup reduction per
+//             column gives every dot product the column needs at once: the
+//             squared norm below the diagonal and x^T a_c for the columns
+//             right of it (16 sums per lane, reduced across the wave by a
+//             transposed butterfly, then across the four waves in LDS, with
+//             one barrier).  The factored panel goes to LDS (column-major,
+//             ld = h + 1) for T and the update, and back to global memory;
+//             T from T^{-1} = strict_upper(V^T V) + diag(1 / tau) (:86-103);
 //   update  : the trailing columns, in chunks of 16, one chunk per wave:
 //             W = V^T C (MFMA, K = h, C straight from global memory),
 //             W2 = T^T W (LDS), C -= V W2 (MFMA with swapped operands, so
@@ -24,6 +30,7 @@
 #include <algorithm>
 #include <cstring>
 #include <numeric>
+#include <utility>
 #include <vector>
 
 #include "mqr_internal.h"
@@ -69,11 +76,32 @@ struct Mf<float> {
     static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
 };
 
-template <typename T>
-__device__ __forceinline__ T wsum(T v) {
+// One exchange of the transposed butterfly below: a lane keeps the half of
+// v[0, 2 HALF) that lane bit log2(MASK) selects and adds its partner's copy.
+template <int HALF, int MASK, typename T>
+__device__ __forceinline__ void wsum_step(T (&v)[16], int lane) {
+    const bool hi = (lane & MASK) != 0;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    for (int k = 0; k < HALF; ++k) {
+        const T keep = hi ? v[HALF + k] : v[k];
+        const T send = hi ? v[k] : v[HALF + k];
+        v[k] = keep + __shfl_xor(send, MASK, 64);
+    }
+}
+
+// Sixteen wave sums at once: exchanges over lane bits 5, 4, 3 and 2 halve
+// the values a lane holds, then two steps sum over the four lanes left
+// holding the same value.  Lane l returns the wave sum of v[l >> 2].
+template <typename T>
+__device__ __forceinline__ T wsum16(T (&v)[16], int lane) {
+    wsum_step<8, 32>(v, lane);
+    wsum_step<4, 16>(v, lane);
+    wsum_step<2, 8>(v, lane);
+    wsum_step<1, 4>(v, lane);
+    T x = v[0];
+    x += __shfl_xor(x, 2, 64);
+    x += __shfl_xor(x, 1, 64);
+    return x;
 }
 
 template <typename T>
@@ -82,7 +110,8 @@ struct Shared {
     T W[4][16][17];       // per-wave W, then W2; partial V^T V
     T Tm[16][17];         // T of the panel
     T G[16][17];          // V^T V of the panel
-    T red[4][16];
+    T red[2][4][16];      // per-column partial sums, double-buffered
+    T rowr[2][16];        // row r of the panel while column r is reduced
     T tau[16];
     int idx;
 };
@@ -95,8 +124,89 @@ __device__ __forceinline__ T vget(const T *P, int ld, int h, int b, int r, int p
     return r > p ? x : (r == p && p < b ? T(1) : T(0));
 }
 
+// Column R of the panel (QR/__Householder_Blocked.h:62-84).  Thread t owns
+// row t in registers (a[]) and row t + BNT in the LDS panel P (column-major,
+// ld), which keeps the register count at two waves per SIMD.  With x = a_R:
+// sigma^2 = sum_{i>R} x_i^2 and d_c = sum_{i>R} x_i a_c(i) (c > R) come from
+// one workgroup reduction; then ||x|| = sqrt(x_R^2 + sigma^2),
+// v = x / (x_R + sign(x_R) ||x||) below the diagonal,
+// tau = (x_R + sign(x_R) ||x||) / (sign(x_R) ||x||), and
+// a_c -= tau (a_c(R) + v^T a_c(>R)) v, where v^T a_c(>R) = d_c / (...).
+// A zero column (||x|| = 0) is left as it is, with tau = 0 (H = I).
+template <int R, typename T>
+__device__ __forceinline__ void panel_column(T (&a)[16], Shared<T> &sh, int b, int h, int ld,
+                                             int tid) {
+    if (R >= b) return;  // uniform
+    const int lane = tid & 63, wave = tid >> 6;
+    T *P = sh.P;
+    const int i1 = tid + BNT;  // the LDS-resident row, always below row R
+    const bool own1 = i1 < h;
+    T part[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) part[c] = T(0);
+    {
+        const T x = tid > R ? a[R] : T(0);
+#pragma unroll
+        for (int c = R; c < 16; ++c) part[c] += x * a[c];
+    }
+    if (own1) {
+        const T x = P[R * ld + i1];
+#pragma unroll
+        for (int c = R; c < 16; ++c) part[c] += x * (c < b ? P[c * ld + i1] : T(0));
+    }
+    const T tot = wsum16(part, lane);
+    // double-buffered slots: the next column writes the other buffer, and the
+    // one after it only once every thread has passed the next barrier
+    T *red = &sh.red[R & 1][0][0];
+    T *rowr = sh.rowr[R & 1];
+    if ((lane & 3) == 0) red[wave * 16 + (lane >> 2)] = tot;
+    if (tid == R) {
+#pragma unroll
+        for (int c = R; c < 16; ++c) rowr[c] = a[c];
+    }
+    __syncthreads();
+    const T alpha = rowr[R];
+    const T sig2 = (red[R] + red[16 + R]) + (red[32 + R] + red[48 + R]);
+    const T nrm = sqrt(alpha * alpha + sig2);
+    if (nrm == T(0)) {
+        // uniform: every thread read the same sums
+        if (tid == 0) sh.tau[R] = T(0);
+        return;
+    }
+    const T cr = alpha < T(0) ? -nrm : nrm;
+    const T w0 = alpha + cr;
+    const T hr = T(1) / w0;
+    const T t = w0 / cr;
+    if (tid == 0) sh.tau[R] = t;
+    // the reflector's entry for each owned row: v_i below the diagonal, 1 on
+    // it, 0 above (rows above R are final and get a zero update)
+    const T v0 = a[R] * hr;
+    const T vr0 = tid > R ? v0 : (tid == R ? T(1) : T(0));
+    a[R] = tid > R ? v0 : (tid == R ? -cr : a[R]);
+    T vr1 = T(0);
+    if (own1) {
+        vr1 = P[R * ld + i1] * hr;
+        P[R * ld + i1] = vr1;
+    }
+#pragma unroll
+    for (int c = R + 1; c < 16; ++c) {
+        const T dc = (red[c] + red[16 + c]) + (red[32 + c] + red[48 + c]);
+        const T cf = -t * (rowr[c] + hr * dc);
+        a[c] += cf * vr0;
+        if (own1 && c < b) P[c * ld + i1] += cf * vr1;
+        // bound the partial sums in flight (register pressure)
+        if ((c & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+template <typename T, int... R>
+__device__ __forceinline__ void panel_columns(T (&a)[16], Shared<T> &sh, int b, int h, int ld,
+                                              int tid, std::integer_sequence<int, R...>) {
+    (panel_column<R>(a, sh, b, h, ld, tid), ...);
+}
+
 template <typename T>
-__device__ void factor_one(const BatchDesc &d, Shared<T> &sh) {
+__device__ __forceinline__ void factor_one(const BatchDesc &d, Shared<T> &sh) {
     T *A = (T *)d.A;
     T *tau = (T *)d.tau;
     const int64_t lda = d.lda;
@@ -107,71 +217,24 @@ __device__ void factor_one(const BatchDesc &d, Shared<T> &sh) {
         const int h = m - j;
         const int ld = h + 1;
         T *Aj = A + j + (int64_t)j * lda;
-        // ---- panel to LDS (coalesced along rows) ----
-        for (int e = tid; e < b * h; e += BNT) {
-            const int r = e % h, c = e / h;
+        // ---- panel: row t in registers of thread t, rows >= BNT in LDS ----
+        T a[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) a[c] = (tid < h && c < b) ? Aj[tid + (int64_t)c * lda] : T(0);
+        for (int e = tid; e < b * (h - BNT); e += BNT) {
+            const int r = BNT + e % (h - BNT), c = e / (h - BNT);
             sh.P[c * ld + r] = Aj[r + (int64_t)c * lda];
         }
         __syncthreads();
         // ---- column-by-column reflectors (QR/__Householder_Blocked.h:62-84) ----
-        for (int r = 0; r < b; ++r) {
-            T *w = sh.P + r * ld;
-            T s = 0;
-            for (int i = r + tid; i < h; i += BNT) s += w[i] * w[i];
-            s = wsum(s);
-            if (lane == 0) sh.red[wave][0] = s;
-            __syncthreads();
-            const T nrm2 = (sh.red[0][0] + sh.red[1][0]) + (sh.red[2][0] + sh.red[3][0]);
-            T cr = sqrt(nrm2);
-            const T alpha = w[r];
-            __syncthreads();
-            if (cr == T(0)) {
-                if (tid == 0) sh.tau[r] = T(0);
-                continue;  // uniform: every thread read the same sum
-            }
-            if (alpha < T(0)) cr = -cr;
-            const T w0 = alpha + cr;
-            const T hr = T(1) / w0;
-            const T t = w0 / cr;
-            for (int i = r + 1 + tid; i < h; i += BNT) w[i] *= hr;
-            if (tid == 0) {
-                w[r] = -cr;
-                sh.tau[r] = t;
-            }
-            __syncthreads();
-            // apply I - t v v^T to the panel columns c > r
-            T part[16];
-#pragma unroll
-            for (int c = 0; c < 16; ++c) part[c] = 0;
-            for (int i = r + 1 + tid; i < h; i += BNT) {
-                const T vi = w[i];
-#pragma unroll
-                for (int c = 0; c < 16; ++c)
-                    if (c > r && c < b) part[c] += vi * sh.P[c * ld + i];
-            }
+        panel_columns(a, sh, b, h, ld, tid, std::make_integer_sequence<int, 16>());
+        // ---- register rows of the factored panel to LDS ----
+        if (tid < h) {
 #pragma unroll
             for (int c = 0; c < 16; ++c)
-                if (c > r && c < b) {
-                    const T v = wsum(part[c]);
-                    if (lane == 0) sh.red[wave][c] = v;
-                }
-            __syncthreads();
-            T coef[16];
-#pragma unroll
-            for (int c = 0; c < 16; ++c)
-                coef[c] = (c > r && c < b)
-                              ? -t * (sh.P[c * ld + r] + ((sh.red[0][c] + sh.red[1][c]) +
-                                                          (sh.red[2][c] + sh.red[3][c])))
-                              : T(0);
-            __syncthreads();
-            for (int i = r + tid; i < h; i += BNT) {
-                const T vi = i == r ? T(1) : w[i];
-#pragma unroll
-                for (int c = 0; c < 16; ++c)
-                    if (c > r && c < b) sh.P[c * ld + i] += coef[c] * vi;
-            }
-            __syncthreads();
+                if (c < b) sh.P[c * ld + tid] = a[c];
         }
+        __syncthreads();
         // ---- T from G = V^T V: T^{-1} = strict_upper(G) + diag(1 / tau) ----
         // G on the MFMA: wave w sums rows [w hq, (w + 1) hq), then the four
         // partial Gram matrices are added
@@ -217,7 +280,7 @@ __device__ void factor_one(const BatchDesc &d, Shared<T> &sh) {
 #pragma unroll
             for (int i = 0; i < 16; ++i) sh.Tm[i][c] = (c < b && i < b && !zc) ? x[i] : T(0);
         }
-        // ---- panel back to global, tau ----
+        // ---- panel back to global (from LDS: the register copy is dead), tau ----
         for (int e = tid; e < b * h; e += BNT) {
             const int r = e % h, c = e / h;
             Aj[r + (int64_t)c * lda] = sh.P[c * ld + r];
