@@ -76,31 +76,100 @@ struct Mf<float> {
     static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
 };
 
-// One exchange of the transposed butterfly below: a lane keeps the half of
-// v[0, 2 HALF) that lane bit log2(MASK) selects and adds its partner's copy.
-template <int HALF, int MASK, typename T>
-__device__ __forceinline__ void wsum_step(T (&v)[16], int lane) {
-    const bool hi = (lane & MASK) != 0;
+// Cross-lane moves of 32- and 64-bit values on the VALU (no LDS round trip).
+template <typename T>
+__device__ __forceinline__ uint64_t bits64(T x) {
+    return sizeof(T) == 8 ? __builtin_bit_cast(uint64_t, (double)x)
+                          : (uint64_t)__builtin_bit_cast(uint32_t, (float)x);
+}
+template <typename T>
+__device__ __forceinline__ T from_bits(uint32_t lo, uint32_t hi) {
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+    else
+        return __builtin_bit_cast(float, lo);
+}
+// DPP lane move within rows of 16 lanes (quad_perm or row rotate controls)
+template <int CTRL, typename T>
+__device__ __forceinline__ T dpp(T x) {
+    const uint64_t u = bits64(x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+    uint32_t hi = 0;
+    if constexpr (sizeof(T) == 8)
+        hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+    return from_bits<T>(lo, hi);
+}
+constexpr int kQuadXor1 = 0xB1;  // quad_perm [1, 0, 3, 2]
+constexpr int kQuadXor2 = 0x4E;  // quad_perm [2, 3, 0, 1]
+constexpr int kRowRor4 = 0x124;  // row_ror:4
+constexpr int kRowRor8 = 0x128;  // row_ror:8, the same as lane ^ 8 in a row
+
+// a + b after v_permlane32_swap (MASK 32: lanes 32-63 of a trade places with
+// lanes 0-31 of b) or v_permlane16_swap (MASK 16: odd rows of a with even
+// rows of b): the low half of the lanes gets a summed over the lane pair
+// (l, l ^ MASK), the high half gets b summed likewise.
+template <int MASK, typename T>
+__device__ __forceinline__ T swap_sum(T a, T b) {
+    const uint64_t ua = bits64(a), ub = bits64(b);
+    uint32_t x[2], y[2];
 #pragma unroll
-    for (int k = 0; k < HALF; ++k) {
-        const T keep = hi ? v[HALF + k] : v[k];
-        const T send = hi ? v[k] : v[HALF + k];
-        v[k] = keep + __shfl_xor(send, MASK, 64);
+    for (int w = 0; w < (sizeof(T) == 8 ? 2 : 1); ++w) {
+        const uint32_t pa = (uint32_t)(ua >> (32 * w)), pb = (uint32_t)(ub >> (32 * w));
+        if constexpr (MASK == 32) {
+            const auto r = __builtin_amdgcn_permlane32_swap(pa, pb, false, false);
+            x[w] = r[0];
+            y[w] = r[1];
+        } else {
+            const auto r = __builtin_amdgcn_permlane16_swap(pa, pb, false, false);
+            x[w] = r[0];
+            y[w] = r[1];
+        }
     }
+    if constexpr (sizeof(T) == 8)
+        return from_bits<T>(x[0], x[1]) + from_bits<T>(y[0], y[1]);
+    else
+        return from_bits<T>(x[0], 0) + from_bits<T>(y[0], 0);
 }
 
-// Sixteen wave sums at once: exchanges over lane bits 5, 4, 3 and 2 halve
-// the values a lane holds, then two steps sum over the four lanes left
-// holding the same value.  Lane l returns the wave sum of v[l >> 2].
+// Sixteen wave sums at once (a transposed butterfly): the exchanges over lane
+// bits 5, 4, 3 and 2 each halve the values a lane holds, keeping the half
+// that the bit selects plus the partner's copy of it; the last two steps sum
+// over the four lanes left holding the same value.  Lane l returns the wave
+// sum of v[l >> 2].
 template <typename T>
 __device__ __forceinline__ T wsum16(T (&v)[16], int lane) {
-    wsum_step<8, 32>(v, lane);
-    wsum_step<4, 16>(v, lane);
-    wsum_step<2, 8>(v, lane);
-    wsum_step<1, 4>(v, lane);
-    T x = v[0];
-    x += __shfl_xor(x, 2, 64);
-    x += __shfl_xor(x, 1, 64);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = swap_sum<32>(v[k], v[8 + k]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = swap_sum<16>(v[k], v[4 + k]);
+    {
+        const bool hi = (lane & 8) != 0;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const T keep = hi ? v[2 + k] : v[k];
+            const T send = hi ? v[k] : v[2 + k];
+            v[k] = keep + dpp<kRowRor8>(send);
+        }
+    }
+    T x;
+    {
+        const bool hi = (lane & 4) != 0;
+        const T keep = hi ? v[1] : v[0];
+        const T send = hi ? v[0] : v[1];
+        x = keep + __shfl_xor(send, 4, 64);
+    }
+    x += dpp<kQuadXor2>(x);
+    x += dpp<kQuadXor1>(x);
+    return x;
+}
+
+// Sum over each row of 16 lanes, returned to every lane of the row.
+template <typename T>
+__device__ __forceinline__ T rowsum16(T x) {
+    x += dpp<kQuadXor1>(x);
+    x += dpp<kQuadXor2>(x);
+    x += dpp<kRowRor4>(x);
+    x += dpp<kRowRor8>(x);
     return x;
 }
 
@@ -255,30 +324,28 @@ __device__ __forceinline__ void factor_one(const BatchDesc &d, Shared<T> &sh) {
             sh.G[p][q] = (sh.W[0][p][q] + sh.W[1][p][q]) + (sh.W[2][p][q] + sh.W[3][p][q]);
         }
         __syncthreads();
-        T(*G)[17] = sh.G;
-        if (tid < 16) {
-            // column c = tid of T = M^{-1} by back substitution, M(i, i) =
-            // 1 / tau_i; a tau = 0 reflector (the identity) is removed from M
-            // (zero row and column, unit diagonal) and left zero in T, which is
-            // what the forward recurrence produces
-            const int c = tid;
-            T x[16];
+        {
+            // T = M^{-1} by T(i, c) = tau_i (delta_ic - sum_{i<l<=c} G(i, l) T(l, c)),
+            // i from 15 down.  Wave w builds columns c = 4 w + (lane >> 4);
+            // lane l = lane & 15 of a row holds T(l, c) and column l of G, and
+            // the sum over l is a row sum.  tau_i = 0 (an identity reflector,
+            // or i >= b) leaves row and column i of T zero, which is what
+            // removing the reflector from M gives.
+            const int l = lane & 15, c = 4 * wave + (lane >> 4);
+            T g[16], tv[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                g[i] = sh.G[i][l];
+                tv[i] = i < b ? sh.tau[i] : T(0);
+            }
+            T x = T(0);
 #pragma unroll
             for (int i = 15; i >= 0; --i) {
-                T s = (i == c) ? T(1) : T(0);
-                const bool zi = i < b && sh.tau[i] == T(0);
-#pragma unroll
-                for (int l = 0; l < 16; ++l)
-                    if (l > i && l <= c) {
-                        const bool zl = l < b && sh.tau[l] == T(0);
-                        s -= ((zi || zl) ? T(0) : G[i][l]) * x[l];
-                    }
-                const T ti = (i < b && !zi) ? sh.tau[i] : T(1);
-                x[i] = (i <= c) ? s * ti : T(0);
+                const T sum = rowsum16(l > i ? g[i] * x : T(0));
+                const T xi = tv[i] * ((i == c ? T(1) : T(0)) - sum);
+                if (l == i) x = xi;
             }
-            const bool zc = c < b && sh.tau[c] == T(0);
-#pragma unroll
-            for (int i = 0; i < 16; ++i) sh.Tm[i][c] = (c < b && i < b && !zc) ? x[i] : T(0);
+            sh.Tm[l][c] = x;
         }
         // ---- panel back to global (from LDS: the register copy is dead), tau ----
         for (int e = tid; e < b * h; e += BNT) {
@@ -298,24 +365,38 @@ __device__ __forceinline__ void factor_one(const BatchDesc &d, Shared<T> &sh) {
             // W = V^T C  (16 x 16, K = h)
             acc_t acc = {0, 0, 0, 0};
             {
-                // eight row groups per round: all eight loads are issued
-                // before the MFMAs that consume them
+                // eight row groups per round; the next round's eight loads are
+                // issued before this round's MFMAs, which alternate between two
+                // accumulators
                 const int col = c0 + (lane & 15);
                 const bool cin = col < n;
                 const T *Ccol = C + (int64_t)(cin ? col : c0) * lda;
-                for (int r0 = 0; r0 < hp4; r0 += 32) {
-                    T bv[8];
+                T cur[8], nxt[8];
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        const int r = r0 + 4 * u + (lane >> 4);
-                        bv[u] = (cin && r < h) ? Ccol[r] : T(0);
-                    }
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        const int r = r0 + 4 * u + (lane >> 4);
-                        acc = Mf<T>::run(vget(sh.P, ld, h, b, r, lane & 15), bv[u], acc);
-                    }
+                for (int u = 0; u < 8; ++u) {
+                    const int r = 4 * u + (lane >> 4);
+                    cur[u] = (cin && r < h) ? Ccol[r] : T(0);
                 }
+                acc_t acc1 = {0, 0, 0, 0};
+                for (int r0 = 0; r0 < hp4; r0 += 32) {
+                    if (r0 + 32 < hp4) {
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            const int r = r0 + 32 + 4 * u + (lane >> 4);
+                            nxt[u] = (cin && r < h) ? Ccol[r] : T(0);
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; u += 2) {
+                        const int r = r0 + 4 * u + (lane >> 4);
+                        acc = Mf<T>::run(vget(sh.P, ld, h, b, r, lane & 15), cur[u], acc);
+                        acc1 = Mf<T>::run(vget(sh.P, ld, h, b, r + 4, lane & 15), cur[u + 1], acc1);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) cur[u] = nxt[u];
+                }
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) acc[rr] += acc1[rr];
             }
 #pragma unroll
             for (int rr = 0; rr < 4; ++rr) sh.W[wave][Mf<T>::row(lane, rr)][lane & 15] = acc[rr];
@@ -341,18 +422,34 @@ __device__ __forceinline__ void factor_one(const BatchDesc &d, Shared<T> &sh) {
             T w2v[4];
 #pragma unroll
             for (int kk = 0; kk < 4; ++kk) w2v[kk] = sh.W[wave][4 * kk + (lane >> 4)][lane & 15];
-            // two 16-row tiles per round: their C loads are issued together
+            // two 16-row tiles per round: their C loads are issued together,
+            // one round ahead of the MFMAs and stores that use them
+            T cv[2][4], cn[2][4];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int row = 16 * t + (lane & 15);
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) {
+                    const int col = c0 + Mf<T>::row(lane, rr);
+                    cv[t][rr] = (row < h && col < n) ? C[row + (int64_t)col * lda] : T(0);
+                }
+            }
             for (int row0 = 0; row0 < h; row0 += 32) {
+                if (row0 + 32 < h) {
+#pragma unroll
+                    for (int t = 0; t < 2; ++t) {
+                        const int row = row0 + 32 + 16 * t + (lane & 15);
+#pragma unroll
+                        for (int rr = 0; rr < 4; ++rr) {
+                            const int col = c0 + Mf<T>::row(lane, rr);
+                            cn[t][rr] = (row < h && col < n) ? C[row + (int64_t)col * lda] : T(0);
+                        }
+                    }
+                }
                 acc_t d[2];
-                T cv[2][4];
 #pragma unroll
                 for (int t = 0; t < 2; ++t) {
                     const int row = row0 + 16 * t + (lane & 15);
-#pragma unroll
-                    for (int rr = 0; rr < 4; ++rr) {
-                        const int col = c0 + Mf<T>::row(lane, rr);
-                        cv[t][rr] = (row < h && col < n) ? C[row + (int64_t)col * lda] : T(0);
-                    }
                     d[t] = (acc_t){0, 0, 0, 0};
 #pragma unroll
                     for (int kk = 0; kk < 4; ++kk) {
@@ -367,6 +464,7 @@ __device__ __forceinline__ void factor_one(const BatchDesc &d, Shared<T> &sh) {
                     for (int rr = 0; rr < 4; ++rr) {
                         const int col = c0 + Mf<T>::row(lane, rr);
                         if (row < h && col < n) C[row + (int64_t)col * lda] = cv[t][rr] - d[t][rr];
+                        cv[t][rr] = cn[t][rr];
                     }
                 }
             }
@@ -376,8 +474,8 @@ __device__ __forceinline__ void factor_one(const BatchDesc &d, Shared<T> &sh) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(BNT) void k_qr_batched(const BatchDesc *__restrict__ desc, int batch,
-                                                    int *__restrict__ counters) {
+__device__ __forceinline__ void qr_batched_body(const BatchDesc *__restrict__ desc, int batch,
+                                                int *__restrict__ counters) {
     extern __shared__ char smem[];
     Shared<T> &sh = *reinterpret_cast<Shared<T> *>(smem);
     for (;;) {
@@ -397,6 +495,23 @@ __global__ __launch_bounds__(BNT) void k_qr_batched(const BatchDesc *__restrict_
             atomicExch(&counters[1], 0);
         }
     }
+}
+
+// Register budgets: FP64 fits two waves per SIMD (LDS allows two workgroups
+// per CU), FP32 three.
+template <typename T>
+__global__ void k_qr_batched(const BatchDesc *__restrict__ desc, int batch, int *__restrict__ counters);
+template <>
+__global__ __launch_bounds__(BNT, 2) void k_qr_batched<double>(const BatchDesc *__restrict__ desc,
+                                                               int batch,
+                                                               int *__restrict__ counters) {
+    qr_batched_body<double>(desc, batch, counters);
+}
+template <>
+__global__ __launch_bounds__(BNT, 3) void k_qr_batched<float>(const BatchDesc *__restrict__ desc,
+                                                              int batch,
+                                                              int *__restrict__ counters) {
+    qr_batched_body<float>(desc, batch, counters);
 }
 
 template <typename T>
