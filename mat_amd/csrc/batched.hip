@@ -194,34 +194,26 @@ __device__ __forceinline__ T vget(const T *P, int ld, int h, int b, int r, int p
 }
 
 // Column R of the panel (QR/__Householder_Blocked.h:62-84).  Thread t owns
-// row t in registers (a[]) and row t + BNT in the LDS panel P (column-major,
-// ld), which keeps the register count at two waves per SIMD.  With x = a_R:
-// sigma^2 = sum_{i>R} x_i^2 and d_c = sum_{i>R} x_i a_c(i) (c > R) come from
-// one workgroup reduction; then ||x|| = sqrt(x_R^2 + sigma^2),
-// v = x / (x_R + sign(x_R) ||x||) below the diagonal,
-// tau = (x_R + sign(x_R) ||x||) / (sign(x_R) ||x||), and
+// rows t (a0) and t + BNT (a1) in registers; rows at or past h hold zeros and
+// stay zero.  With x = a_R: sigma^2 = sum_{i>R} x_i^2 and
+// d_c = sum_{i>R} x_i a_c(i) (c > R) come from one workgroup reduction; then
+// ||x|| = sqrt(x_R^2 + sigma^2), v = x / (x_R + sign(x_R) ||x||) below the
+// diagonal, tau = (x_R + sign(x_R) ||x||) / (sign(x_R) ||x||), and
 // a_c -= tau (a_c(R) + v^T a_c(>R)) v, where v^T a_c(>R) = d_c / (...).
 // A zero column (||x|| = 0) is left as it is, with tau = 0 (H = I).
 template <int R, typename T>
-__device__ __forceinline__ void panel_column(T (&a)[16], Shared<T> &sh, int b, int h, int ld,
+__device__ __forceinline__ void panel_column(T (&a0)[16], T (&a1)[16], Shared<T> &sh, int b,
                                              int tid) {
     if (R >= b) return;  // uniform
     const int lane = tid & 63, wave = tid >> 6;
-    T *P = sh.P;
-    const int i1 = tid + BNT;  // the LDS-resident row, always below row R
-    const bool own1 = i1 < h;
     T part[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) part[c] = T(0);
     {
-        const T x = tid > R ? a[R] : T(0);
+        const T x0 = tid > R ? a0[R] : T(0);
+        const T x1 = a1[R];  // row t + BNT is always below row R
 #pragma unroll
-        for (int c = R; c < 16; ++c) part[c] += x * a[c];
-    }
-    if (own1) {
-        const T x = P[R * ld + i1];
-#pragma unroll
-        for (int c = R; c < 16; ++c) part[c] += x * (c < b ? P[c * ld + i1] : T(0));
+        for (int c = R; c < 16; ++c) part[c] += x0 * a0[c] + x1 * a1[c];
     }
     const T tot = wsum16(part, lane);
     // double-buffered slots: the next column writes the other buffer, and the
@@ -231,7 +223,7 @@ __device__ __forceinline__ void panel_column(T (&a)[16], Shared<T> &sh, int b, i
     if ((lane & 3) == 0) red[wave * 16 + (lane >> 2)] = tot;
     if (tid == R) {
 #pragma unroll
-        for (int c = R; c < 16; ++c) rowr[c] = a[c];
+        for (int c = R; c < 16; ++c) rowr[c] = a0[c];
     }
     __syncthreads();
     const T alpha = rowr[R];
@@ -249,29 +241,26 @@ __device__ __forceinline__ void panel_column(T (&a)[16], Shared<T> &sh, int b, i
     if (tid == 0) sh.tau[R] = t;
     // the reflector's entry for each owned row: v_i below the diagonal, 1 on
     // it, 0 above (rows above R are final and get a zero update)
-    const T v0 = a[R] * hr;
+    const T v0 = a0[R] * hr;
     const T vr0 = tid > R ? v0 : (tid == R ? T(1) : T(0));
-    a[R] = tid > R ? v0 : (tid == R ? -cr : a[R]);
-    T vr1 = T(0);
-    if (own1) {
-        vr1 = P[R * ld + i1] * hr;
-        P[R * ld + i1] = vr1;
-    }
+    a0[R] = tid > R ? v0 : (tid == R ? -cr : a0[R]);
+    const T vr1 = a1[R] * hr;
+    a1[R] = vr1;
 #pragma unroll
     for (int c = R + 1; c < 16; ++c) {
         const T dc = (red[c] + red[16 + c]) + (red[32 + c] + red[48 + c]);
         const T cf = -t * (rowr[c] + hr * dc);
-        a[c] += cf * vr0;
-        if (own1 && c < b) P[c * ld + i1] += cf * vr1;
+        a0[c] += cf * vr0;
+        a1[c] += cf * vr1;
         // bound the partial sums in flight (register pressure)
         if ((c & 3) == 3) __builtin_amdgcn_sched_barrier(0);
     }
 }
 
 template <typename T, int... R>
-__device__ __forceinline__ void panel_columns(T (&a)[16], Shared<T> &sh, int b, int h, int ld,
+__device__ __forceinline__ void panel_columns(T (&a0)[16], T (&a1)[16], Shared<T> &sh, int b,
                                               int tid, std::integer_sequence<int, R...>) {
-    (panel_column<R>(a, sh, b, h, ld, tid), ...);
+    (panel_column<R>(a0, a1, sh, b, tid), ...);
 }
 
 template <typename T>
@@ -286,23 +275,22 @@ __device__ __forceinline__ void factor_one(const BatchDesc &d, Shared<T> &sh) {
         const int h = m - j;
         const int ld = h + 1;
         T *Aj = A + j + (int64_t)j * lda;
-        // ---- panel: row t in registers of thread t, rows >= BNT in LDS ----
-        T a[16];
+        // ---- panel to registers: thread t owns rows t and t + BNT ----
+        T a0[16], a1[16];
 #pragma unroll
-        for (int c = 0; c < 16; ++c) a[c] = (tid < h && c < b) ? Aj[tid + (int64_t)c * lda] : T(0);
-        for (int e = tid; e < b * (h - BNT); e += BNT) {
-            const int r = BNT + e % (h - BNT), c = e / (h - BNT);
-            sh.P[c * ld + r] = Aj[r + (int64_t)c * lda];
+        for (int c = 0; c < 16; ++c) {
+            a0[c] = (tid < h && c < b) ? Aj[tid + (int64_t)c * lda] : T(0);
+            a1[c] = (tid + BNT < h && c < b) ? Aj[tid + BNT + (int64_t)c * lda] : T(0);
         }
-        __syncthreads();
         // ---- column-by-column reflectors (QR/__Householder_Blocked.h:62-84) ----
-        panel_columns(a, sh, b, h, ld, tid, std::make_integer_sequence<int, 16>());
-        // ---- register rows of the factored panel to LDS ----
-        if (tid < h) {
+        panel_columns(a0, a1, sh, b, tid, std::make_integer_sequence<int, 16>());
+        // ---- factored panel to LDS (for T and the update) ----
 #pragma unroll
-            for (int c = 0; c < 16; ++c)
-                if (c < b) sh.P[c * ld + tid] = a[c];
-        }
+        for (int c = 0; c < 16; ++c)
+            if (c < b) {
+                if (tid < h) sh.P[c * ld + tid] = a0[c];
+                if (tid + BNT < h) sh.P[c * ld + tid + BNT] = a1[c];
+            }
         __syncthreads();
         // ---- T from G = V^T V: T^{-1} = strict_upper(G) + diag(1 / tau) ----
         // G on the MFMA: wave w sums rows [w hq, (w + 1) hq), then the four
