@@ -22,11 +22,11 @@
 //             one barrier).  The factored panel goes to LDS (column-major,
 //             ld = h + 1) for T and the update, and back to global memory;
 //             T from T^{-1} = strict_upper(V^T V) + diag(1 / tau) (:86-103);
-//   update  : the trailing columns, in chunks of 16, one chunk per wave:
-//             W = V^T C (MFMA, K = h, C straight from global memory),
-//             W2 = T^T W (LDS), C -= V W2 (MFMA with swapped operands, so
-//             that 16 lanes cover 16 consecutive rows of C: coalesced
-//             read-modify-write) (:143-212).
+//   update  : the trailing columns, in chunks of 16 taken by all four waves
+//             together, each wave holding a quarter of the chunk's rows in
+//             registers: partial W = V^T C (MFMA), the four partials summed
+//             in LDS, W2 = T^T W and C -= V W2 (MFMA, from registers), so
+//             the chunk is read once and written once (:143-212).
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -54,6 +54,8 @@ constexpr int BMAXM = 512;     // largest m accepted
 constexpr int BLD = BMAXM + 1; // LDS leading dimension bound
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+template <typename T>
+using GPtr = __attribute__((address_space(1))) T *;
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 template <typename T>
@@ -66,6 +68,9 @@ struct Mf<double> {
     }
     // output row of lane l, register r (the column is l & 15)
     static __device__ __forceinline__ int row(int l, int r) { return (l >> 4) + 4 * r; }
+    // physical row (within a 16-row tile) given to tile row i, chosen so that
+    // lane l, register r lands on row 4 (l >> 4) + r: four consecutive rows
+    static __device__ __forceinline__ int phys(int i) { return ((i & 3) << 2) | (i >> 2); }
 };
 template <>
 struct Mf<float> {
@@ -74,6 +79,7 @@ struct Mf<float> {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
     static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
+    static __device__ __forceinline__ int phys(int i) { return i; }
 };
 
 // Cross-lane moves of 32- and 64-bit values on the VALU (no LDS round trip).
@@ -265,8 +271,10 @@ __device__ __forceinline__ void panel_columns(T (&a0)[16], T (&a1)[16], Shared<T
 
 template <typename T>
 __device__ __forceinline__ void factor_one(const BatchDesc &d, Shared<T> &sh) {
-    T *A = (T *)d.A;
-    T *tau = (T *)d.tau;
+    // the descriptor's pointers are global memory: global (not flat)
+    // loads and stores, which do not hold up the LDS wait counter
+    GPtr<T> A = (GPtr<T>)d.A;
+    GPtr<T> tau = (GPtr<T>)d.tau;
     const int64_t lda = d.lda;
     const int m = d.m, n = d.n, k = min(m, n), NB = d.nb;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -274,7 +282,7 @@ __device__ __forceinline__ void factor_one(const BatchDesc &d, Shared<T> &sh) {
         const int b = min(NB, k - j);
         const int h = m - j;
         const int ld = h + 1;
-        T *Aj = A + j + (int64_t)j * lda;
+        GPtr<T> Aj = A + j + (int64_t)j * lda;
         // ---- panel to registers: thread t owns rows t and t + BNT ----
         T a0[16], a1[16];
 #pragma unroll
@@ -342,117 +350,90 @@ __device__ __forceinline__ void factor_one(const BatchDesc &d, Shared<T> &sh) {
         }
         if (tid < b) tau[j + tid] = sh.tau[tid];
         __syncthreads();
-        // ---- trailing update, one 16-column chunk per wave ----
+        // ---- trailing update, one 16-column chunk at a time by all four
+        // waves: wave w holds rows [w hs, (w + 1) hs) of the chunk in
+        // registers (hs <= 128) from W = V^T C until C -= V (T^T W) is stored,
+        // so C is read once and written once per panel (QR/__Householder_Blocked.h:143-212).
+        // Register layout of a 16-row tile (the MFMA's B operand and output):
+        // lane l, register u holds row 4 (l >> 4) + u, column l & 15, the
+        // MFMA's row index Mf::row(l, u) mapped through Mf::phys.  The four
+        // partial W are summed in LDS (two barriers per chunk); every wave then
+        // forms W2 = T^T W and V W2 on the MFMA from registers.
         typedef typename Mf<T>::acc acc_t;
         const int c0base = j + b;
         const int nchunks = (n - c0base + 15) / 16;
-        const int hp4 = (h + 3) & ~3;
-        for (int q = wave; q < nchunks; q += 4) {
+        const int hs = 16 * ((h + 63) / 64);  // 4 hs >= h, multiple of 16
+        const int r0w = wave * hs;
+        const int ntile = max(0, min(hs, h - r0w) + 15) / 16;
+        T tA[4];  // T^T as the A operand of W2 = T^T W
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) tA[kk] = sh.Tm[Mf<T>::row(lane, kk)][lane & 15];
+        for (int q = 0; q < nchunks; ++q) {
             const int c0 = c0base + 16 * q;
-            T *C = A + j;  // row j of the matrix, columns absolute
-            // W = V^T C  (16 x 16, K = h)
-            acc_t acc = {0, 0, 0, 0};
-            {
-                // eight row groups per round; the next round's eight loads are
-                // issued before this round's MFMAs, which alternate between two
-                // accumulators
-                const int col = c0 + (lane & 15);
-                const bool cin = col < n;
-                const T *Ccol = C + (int64_t)(cin ? col : c0) * lda;
-                T cur[8], nxt[8];
+            const int col = c0 + (lane & 15);
+            const bool cin = col < n;
+            GPtr<T> Ccol = A + j + (int64_t)(cin ? col : c0) * lda;  // row j of the column
+            T cr[8][4];
+            // lane l, register u: row 4 (l >> 4) + u of tile t, four
+            // consecutive rows of one column per lane (one vector load)
+            const int lr = 4 * (lane >> 4);
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int r = 4 * u + (lane >> 4);
-                    cur[u] = (cin && r < h) ? Ccol[r] : T(0);
+            for (int t = 0; t < 8; ++t) {
+                const int row = r0w + 16 * t + lr;
+                if (t < ntile && cin && row + 3 < h) {
+                    __builtin_memcpy(&cr[t][0], Ccol + row, 4 * sizeof(T));
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        cr[t][u] = (t < ntile && cin && row + u < h) ? Ccol[row + u] : T(0);
                 }
-                acc_t acc1 = {0, 0, 0, 0};
-                for (int r0 = 0; r0 < hp4; r0 += 32) {
-                    if (r0 + 32 < hp4) {
-#pragma unroll
-                        for (int u = 0; u < 8; ++u) {
-                            const int r = r0 + 32 + 4 * u + (lane >> 4);
-                            nxt[u] = (cin && r < h) ? Ccol[r] : T(0);
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < 8; u += 2) {
-                        const int r = r0 + 4 * u + (lane >> 4);
-                        acc = Mf<T>::run(vget(sh.P, ld, h, b, r, lane & 15), cur[u], acc);
-                        acc1 = Mf<T>::run(vget(sh.P, ld, h, b, r + 4, lane & 15), cur[u + 1], acc1);
-                    }
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) cur[u] = nxt[u];
-                }
-#pragma unroll
-                for (int rr = 0; rr < 4; ++rr) acc[rr] += acc1[rr];
             }
+            // partial W = V^T C over this wave's rows, two accumulators
+            acc_t wa = {0, 0, 0, 0}, wb = {0, 0, 0, 0};
 #pragma unroll
-            for (int rr = 0; rr < 4; ++rr) sh.W[wave][Mf<T>::row(lane, rr)][lane & 15] = acc[rr];
-            __builtin_amdgcn_s_waitcnt(0);
-            __builtin_amdgcn_wave_barrier();
-            // W2 = T^T W (rows and columns >= b of T are zero), written over W
-            T w2[4];
+            for (int t = 0; t < 8; ++t) {
+                if (t < ntile) {
+#pragma unroll
+                    for (int u = 0; u < 4; u += 2) {
+                        const int row = r0w + 16 * t + lr + u;
+                        wa = Mf<T>::run(vget(sh.P, ld, h, b, row, lane & 15), cr[t][u], wa);
+                        wb = Mf<T>::run(vget(sh.P, ld, h, b, row + 1, lane & 15), cr[t][u + 1], wb);
+                    }
+                }
+            }
+            __syncthreads();  // the previous chunk's partials have been read
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr)
+                sh.W[wave][Mf<T>::row(lane, rr)][lane & 15] = wa[rr] + wb[rr];
+            __syncthreads();
+            acc_t w;
 #pragma unroll
             for (int rr = 0; rr < 4; ++rr) {
-                const int i = Mf<T>::row(lane, rr), jj = lane & 15;
-                T s = 0;
-                for (int p = 0; p <= i; ++p) s += sh.Tm[p][i] * sh.W[wave][p][jj];
-                w2[rr] = s;
+                const int r = Mf<T>::row(lane, rr), cc = lane & 15;
+                w[rr] = (sh.W[0][r][cc] + sh.W[1][r][cc]) + (sh.W[2][r][cc] + sh.W[3][r][cc]);
             }
-            __builtin_amdgcn_s_waitcnt(0);
-            __builtin_amdgcn_wave_barrier();
+            // W2 = T^T W (K = 16: four MFMAs, W's registers are the B operand)
+            acc_t w2 = {0, 0, 0, 0};
 #pragma unroll
-            for (int rr = 0; rr < 4; ++rr) sh.W[wave][Mf<T>::row(lane, rr)][lane & 15] = w2[rr];
-            __builtin_amdgcn_s_waitcnt(0);
-            __builtin_amdgcn_wave_barrier();
-            // C -= V W2, 16 rows at a time; operands swapped so that lane l
-            // holds row (l & 15) of the tile
-            T w2v[4];
+            for (int kk = 0; kk < 4; ++kk) w2 = Mf<T>::run(tA[kk], w[kk], w2);
+            // C -= V W2, tile by tile, and store
 #pragma unroll
-            for (int kk = 0; kk < 4; ++kk) w2v[kk] = sh.W[wave][4 * kk + (lane >> 4)][lane & 15];
-            // two 16-row tiles per round: their C loads are issued together,
-            // one round ahead of the MFMAs and stores that use them
-            T cv[2][4], cn[2][4];
+            for (int t = 0; t < 8; ++t) {
+                if (t < ntile) {
+                    acc_t d = {0, 0, 0, 0};
+                    const int vrow = r0w + 16 * t + Mf<T>::phys(lane & 15);
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const int row = 16 * t + (lane & 15);
+                    for (int kk = 0; kk < 4; ++kk)
+                        d = Mf<T>::run(vget(sh.P, ld, h, b, vrow, Mf<T>::row(lane, kk)), w2[kk], d);
+                    const int row = r0w + 16 * t + lr;
 #pragma unroll
-                for (int rr = 0; rr < 4; ++rr) {
-                    const int col = c0 + Mf<T>::row(lane, rr);
-                    cv[t][rr] = (row < h && col < n) ? C[row + (int64_t)col * lda] : T(0);
-                }
-            }
-            for (int row0 = 0; row0 < h; row0 += 32) {
-                if (row0 + 32 < h) {
+                    for (int u = 0; u < 4; ++u) cr[t][u] -= d[u];
+                    if (cin && row + 3 < h) {
+                        __builtin_memcpy(Ccol + row, &cr[t][0], 4 * sizeof(T));
+                    } else {
 #pragma unroll
-                    for (int t = 0; t < 2; ++t) {
-                        const int row = row0 + 32 + 16 * t + (lane & 15);
-#pragma unroll
-                        for (int rr = 0; rr < 4; ++rr) {
-                            const int col = c0 + Mf<T>::row(lane, rr);
-                            cn[t][rr] = (row < h && col < n) ? C[row + (int64_t)col * lda] : T(0);
-                        }
-                    }
-                }
-                acc_t d[2];
-#pragma unroll
-                for (int t = 0; t < 2; ++t) {
-                    const int row = row0 + 16 * t + (lane & 15);
-                    d[t] = (acc_t){0, 0, 0, 0};
-#pragma unroll
-                    for (int kk = 0; kk < 4; ++kk) {
-                        const T bv = vget(sh.P, ld, h, b, row, 4 * kk + (lane >> 4));
-                        d[t] = Mf<T>::run(w2v[kk], bv, d[t]);
-                    }
-                }
-#pragma unroll
-                for (int t = 0; t < 2; ++t) {
-                    const int row = row0 + 16 * t + (lane & 15);
-#pragma unroll
-                    for (int rr = 0; rr < 4; ++rr) {
-                        const int col = c0 + Mf<T>::row(lane, rr);
-                        if (row < h && col < n) C[row + (int64_t)col * lda] = cv[t][rr] - d[t][rr];
-                        cv[t][rr] = cn[t][rr];
+                        for (int u = 0; u < 4; ++u)
+                            if (cin && row + u < h) Ccol[row + u] = cr[t][u];
                     }
                 }
             }
