@@ -8,7 +8,9 @@
 // R^{-T} is applied as a blocked right triangular solve X R^T = B over
 // 64-column blocks from the last: X_J = B_J R_JJ^{-T} (the 64 x 64 diagonal
 // inverses are formed in FP64 in one workgroup each), then
-// B_<J -= X_J R_<J,J^T; every product is a batched MFMA GEMM.
+// B_<J -= X_J R_<J,J^T; every product is a batched MFMA GEMM.  B is formed
+// in a workspace and X_J is written straight into dA, so no product reads
+// and writes the same block.
 #include <algorithm>
 
 #include "mqr_internal.h"
@@ -117,8 +119,8 @@ int qr_backward_t(mqr_handle_t hd, int64_t batch, int64_t m, int64_t n, const T 
     const int nblk = (int)((n + AB - 1) / AB);
     const int B = (int)batch;
     // workspace: M (n x n), triu(dR) (n x n), the diagonal inverses,
-    // W (m x AB), per matrix
-    const int64_t sM = n * n, sI = (int64_t)nblk * AB * AB, sW = m * AB;
+    // B (m x n), per matrix
+    const int64_t sM = n * n, sI = (int64_t)nblk * AB * AB, sW = m * n;
     MQR_CHECK(ws_reserve(hd, hd->ws, sizeof(T) * (size_t)batch * (2 * sM + sI + sW) + 4096));
     T *Mw = (T *)hd->ws.ptr;
     T *Uw = Mw + batch * sM;
@@ -151,35 +153,28 @@ int qr_backward_t(mqr_handle_t hd, int64_t batch, int64_t m, int64_t n, const T 
                            sr, Iw);
         MQR_LAUNCH_CHECK();
     }
-    // dA = dQ + Q copyltu(M)
+    // B = dQ + Q copyltu(M), in the workspace (m x n, ld m)
     {
         ProfScope ps(K_OTHER, 0.0, 2.0 * sizeof(T) * batch * m * n, s);
         const unsigned g2 = (unsigned)std::min<int64_t>(4096, (m * n + 255) / 256);
-        hipLaunchKernelGGL(k_copy2d<T>, dim3(g2, B), dim3(256), 0, s, m, n, dQ, lddq, sdq, dA,
-                           ldda, sda);
+        hipLaunchKernelGGL(k_copy2d<T>, dim3(g2, B), dim3(256), 0, s, m, n, dQ, lddq, sdq, Ww,
+                           m, sW);
         MQR_LAUNCH_CHECK();
     }
     MQR_CHECK(bgemm(f64, false, false, (int)m, (int)n, (int)n, 1.0, Q, ldq, sq, Mw, n, sM, 1.0,
-                    dA, ldda, sda, B, K_GEMM, s));
-    // dA <- dA R^{-T}: blocks from the last
+                    Ww, m, sW, B, K_GEMM, s));
+    // dA = B R^{-T}: blocks from the last
     for (int J = nblk - 1; J >= 0; --J) {
         const int64_t j0 = (int64_t)J * AB;
         const int bj = (int)std::min<int64_t>(AB, n - j0);
-        {
-            ProfScope ps(K_TRSM, 0.0, 2.0 * sizeof(T) * batch * m * bj, s);
-            const unsigned g2 = (unsigned)std::min<int64_t>(4096, (m * bj + 255) / 256);
-            hipLaunchKernelGGL(k_copy2d<T>, dim3(g2, B), dim3(256), 0, s, m, (int64_t)bj,
-                               (const T *)(dA + j0 * ldda), ldda, sda, Ww, m, sW);
-            MQR_LAUNCH_CHECK();
-        }
-        // X_J = W R_JJ^{-T}
-        MQR_CHECK(bgemm(f64, false, true, (int)m, bj, bj, 1.0, Ww, m, sW,
+        // X_J = B_J R_JJ^{-T}, into dA
+        MQR_CHECK(bgemm(f64, false, true, (int)m, bj, bj, 1.0, Ww + j0 * m, m, sW,
                         Iw + (int64_t)J * AB * AB, AB, sI, 0.0, dA + j0 * ldda, ldda, sda, B,
                         K_TRSM, s));
         // B_<J -= X_J R_<J,J^T
         if (j0 > 0)
             MQR_CHECK(bgemm(f64, false, true, (int)m, (int)j0, bj, -1.0, dA + j0 * ldda, ldda,
-                            sda, R + j0 * ldr, ldr, sr, 1.0, dA, ldda, sda, B, K_TRSM, s));
+                            sda, R + j0 * ldr, ldr, sr, 1.0, Ww, m, sW, B, K_TRSM, s));
     }
     return MQR_SUCCESS;
 }
