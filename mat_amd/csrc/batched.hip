@@ -21,7 +21,8 @@
 //             transposed butterfly, then across the four waves in LDS, with
 //             one barrier).  The factored panel goes to LDS (column-major,
 //             ld = h + 1) for T and the update, and back to global memory;
-//             T from T^{-1} = strict_upper(V^T V) + diag(1 / tau) (:86-103);
+//             T = M^{-1}, M = strict_upper(V^T V) + diag(1 / tau) (:86-103),
+//             by a recurrence run 16 lanes per column with DPP row sums;
 //   update  : the trailing columns, in chunks of 16 taken by all four waves
 //             together, each wave holding a quarter of the chunk's rows in
 //             registers: partial W = V^T C (MFMA), the four partials summed
@@ -182,7 +183,7 @@ __device__ __forceinline__ T rowsum16(T x) {
 template <typename T>
 struct Shared {
     T P[16 * BLD];        // panel, column-major, ld = h + 1
-    T W[4][16][17];       // per-wave W, then W2; partial V^T V
+    T W[4][16][17];       // per-wave partials: V^T V, then V^T C of each chunk
     T Tm[16][17];         // T of the panel
     T G[16][17];          // V^T V of the panel
     T red[2][4][16];      // per-column partial sums, double-buffered

@@ -81,6 +81,33 @@ def test_batched_matches_oracle(h, orc, torch_mod, dtype, tile):
     assert not bad, bad[:5]
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_batched_layout_boundaries(h, orc, torch_mod, dtype):
+    """Shapes at the kernel's internal boundaries, against the oracle: a
+    panel's rows split between two register rows per thread at row 256, the
+    trailing chunk split into per-wave slices of up to 128 rows (16-row tiles,
+    the last one partial), and last panels narrower than 16 columns."""
+    torch = torch_mod
+    td = torch.float64 if dtype == "f64" else torch.float32
+    eps = np.finfo(np.float64 if dtype == "f64" else np.float32).eps
+    shapes = [(256, 256), (257, 100), (255, 255), (511, 511), (512, 17), (300, 16), (33, 33),
+              (64, 64), (65, 49), (129, 129), (384, 200), (17, 17), (240, 241), (500, 31)]
+    rng = np.random.default_rng(17)
+    mats = [np.asfortranarray(rng.standard_normal(s)) for s in shapes]
+    note(f"batched boundaries {dtype}: {shapes}")
+    dev = [M.colmajor(A, dtype=td) for A in mats]
+    taus = M.geqrf_batched(h, dev)
+    torch.cuda.synchronize()
+    bad = []
+    for A, d, t in zip(mats, dev, taus):
+        A_in = A.astype(np.float32).astype(np.float64) if dtype == "f32" else A
+        ok, info = check_against_oracle(orc, A_in, np.asfortranarray(d.cpu().double().numpy()),
+                                        t.cpu().double().numpy(), eps)
+        if not ok:
+            bad.append((A.shape, info))
+    assert not bad, bad
+
+
 def test_batched_special_columns(h, orc, torch_mod):
     """Zero columns keep tau = 0 and identity gives -I exactly (the
     reference's reflector conventions)."""
