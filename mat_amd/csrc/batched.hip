@@ -236,23 +236,22 @@ __device__ __forceinline__ void panel_column(T (&a0)[16], T (&a1)[16], Shared<T>
     const T alpha = rowr[R];
     const T sig2 = (red[R] + red[16 + R]) + (red[32 + R] + red[48 + R]);
     const T nrm = sqrt(alpha * alpha + sig2);
-    if (nrm == T(0)) {
-        // uniform: every thread read the same sums
-        if (tid == 0) sh.tau[R] = T(0);
-        return;
-    }
+    // a zero column (nrm = 0) keeps its entries, with tau = 0 (H = I): hr
+    // and t are selected to zero, which makes every update below add zero
+    // (branch-free, so the register panel is not copied at a join)
+    const bool zero = nrm == T(0);  // uniform: every thread read the same sums
     const T cr = alpha < T(0) ? -nrm : nrm;
     const T w0 = alpha + cr;
-    const T hr = T(1) / w0;
-    const T t = w0 / cr;
+    const T hr = zero ? T(0) : T(1) / w0;
+    const T t = zero ? T(0) : w0 / cr;
     if (tid == 0) sh.tau[R] = t;
     // the reflector's entry for each owned row: v_i below the diagonal, 1 on
     // it, 0 above (rows above R are final and get a zero update)
     const T v0 = a0[R] * hr;
     const T vr0 = tid > R ? v0 : (tid == R ? T(1) : T(0));
-    a0[R] = tid > R ? v0 : (tid == R ? -cr : a0[R]);
+    a0[R] = zero ? a0[R] : (tid > R ? v0 : (tid == R ? -cr : a0[R]));
     const T vr1 = a1[R] * hr;
-    a1[R] = vr1;
+    a1[R] = zero ? a1[R] : vr1;
 #pragma unroll
     for (int c = R + 1; c < 16; ++c) {
         const T dc = (red[c] + red[16 + c]) + (red[32 + c] + red[48 + c]);
@@ -446,8 +445,8 @@ __device__ __forceinline__ void factor_one(const BatchDesc &d, Shared<T> &sh) {
 template <typename T>
 __device__ __forceinline__ void qr_batched_body(const BatchDesc *__restrict__ desc, int batch,
                                                 int *__restrict__ counters) {
-    extern __shared__ char smem[];
-    Shared<T> &sh = *reinterpret_cast<Shared<T> *>(smem);
+    // static LDS: every address in the layout is a compile-time offset
+    __shared__ Shared<T> sh;
     for (;;) {
         if (threadIdx.x == 0) sh.idx = atomicAdd(&counters[0], 1);
         __syncthreads();
@@ -482,11 +481,6 @@ __global__ __launch_bounds__(BNT, 3) void k_qr_batched<float>(const BatchDesc *_
                                                               int batch,
                                                               int *__restrict__ counters) {
     qr_batched_body<float>(desc, batch, counters);
-}
-
-template <typename T>
-size_t smem_bytes() {
-    return sizeof(Shared<T>);
 }
 
 }  // namespace
@@ -542,9 +536,9 @@ static int batched_setup(mqr_handle_t hd, bool f64, int64_t batch, const int32_t
                            hd->stream));
     MQR_HIP(hipMemsetAsync(p.d_counters, 0, 2 * sizeof(int), hd->stream));
     MQR_HIP(hipStreamSynchronize(hd->stream));
-    p.smem = f64 ? smem_bytes<double>() : smem_bytes<float>();
+    // the kernel's LDS is static (sizeof(Shared<T>)); no dynamic LDS
+    p.smem = 0;
     const void *fn = f64 ? (const void *)k_qr_batched<double> : (const void *)k_qr_batched<float>;
-    MQR_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.smem));
     int per_cu = 0;
     MQR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BNT, p.smem));
     per_cu = std::max(per_cu, 1);
