@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "QR TFLOP/s at 16384x4096: mixed+IR vs FP64 plain at equal error; % MFMA peak"
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
 
 
 def peak_name(prec: str) -> str:
@@ -163,23 +164,37 @@ def run_mixed(M, h, P, prec: str, steps: int, warmup: int):
 
 def roofline_from(timed: dict, prof: dict, prec: str) -> dict:
     """Roofline of the trailing-update GEMM from its launches in the timed
-    region (algorithmic flops per launch / mean HIP-event duration), plus the
-    per-kind breakdown of one separately profiled solve."""
+    region (algorithmic bytes and flops per launch / mean HIP-event
+    duration), plus the per-kind breakdown of one separately profiled solve.
+    The bound is the one the tile's algorithmic intensity selects: HBM when
+    flops per byte lie below the ridge point peak_flops / 8 TB/s (at K = 64
+    they do: 8 flop/B in FP64 against 9.8, 16 in FP32 against 19.7)."""
     p = timed.get(ROOF_KIND)
     if not p:
         return None
     launches = max(p["launches"], 1)
     avg_ms = p["total_ms"] / launches
     flops_per_launch = p["flops"] / launches
-    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
-    peak = PEAK_TFLOPS[peak_name(prec)]
     bytes_per_launch = p["bytes"] / launches
+    tflops = flops_per_launch / (avg_ms * 1e-3) / 1e12
+    gbs = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    peak_flops = PEAK_TFLOPS[peak_name(prec)]
+    intensity = flops_per_launch / max(bytes_per_launch, 1.0)
+    ridge = peak_flops * 1e12 / (PEAK_HBM_GBS * 1e9)
+    hbm = intensity < ridge
     ratio = {"fp64": TRAFFIC_RATIO_FP64_UPDATE, "fp32": TRAFFIC_RATIO_FP32_UPDATE}.get(prec)
     traffic = round(bytes_per_launch * ratio) if ratio else None
-    return {"kernel": ROOF_KIND, "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
-            "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
+    return {"kernel": ROOF_KIND, "bound": "hbm" if hbm else "mfma",
+            "achieved": round(gbs, 1) if hbm else round(tflops, 3),
+            "peak": PEAK_HBM_GBS if hbm else peak_flops,
+            "unit": "GB/s" if hbm else "TFLOP/s",
+            "frac": round(gbs / PEAK_HBM_GBS if hbm else tflops / peak_flops, 4),
+            "traffic": traffic,
             "traffic_source": (f"measured HBM bytes / algorithmic bytes of the {prec.upper()} update "
                                f"tile ({ratio}) x this run's algorithmic bytes" if traffic else None),
+            "hbm_gbs_measured_traffic": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
+            "intensity_flop_per_byte": round(intensity, 3), "ridge_flop_per_byte": round(ridge, 3),
+            "achieved_tflops": round(tflops, 3), "frac_mfma_peak": round(tflops / peak_flops, 4),
             "avg_launch_ms": round(avg_ms, 5), "launches_timed": p["launches"],
             "flops_per_launch": flops_per_launch,
             "bytes_per_launch": bytes_per_launch,
