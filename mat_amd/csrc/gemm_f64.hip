@@ -159,8 +159,13 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_f64(GemmF64 gin, int kchunk) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int m0 = blockIdx.x * TILE, n0 = blockIdx.y * TILE;
-    const int kb = split * kchunk;
-    const int ke = min(g.K, kb + kchunk);
+    int kb = split * kchunk;
+    int ke = min(g.K, kb + kchunk);
+    // triangular operands: rows >= m0 of an upper op(A) are zero for k < m0,
+    // columns < n0 + TILE of an upper op(B) are zero for k >= n0 + TILE
+    // (m0 is a multiple of BK, so the slices stay aligned)
+    if (g.a_upper) kb = max(kb, m0);
+    if (g.b_upper) ke = min(ke, n0 + TILE);
 
     v4d acc[BW][BW];
     if (g.splits == 1 && g.beta != 0.0) {
@@ -220,7 +225,7 @@ constexpr int BIG = 128, SMALL = 64;
 
 template <bool TA, bool TB, int AM, int BMK>
 void launch(const GemmF64 &g, hipStream_t s) {
-    const int tile = (g.M >= BIG && g.N >= BIG) ? BIG : SMALL;
+    const int tile = (g.M >= BIG && g.N >= BIG && !g.small_tiles) ? BIG : SMALL;
     int kchunk = (g.K + g.splits - 1) / g.splits;
     kchunk = ((kchunk + BK - 1) / BK) * BK;
     dim3 grid((g.M + tile - 1) / tile, (g.N + tile - 1) / tile, g.splits * std::max(g.batch, 1));

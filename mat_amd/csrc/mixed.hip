@@ -492,13 +492,20 @@ int tri_inverse(mqr_handle_t hd, int64_t n, const double *R, int64_t ldr, double
             g.B = X + sz + sz * n; g.ldb = n; g.strideB = xstep;
             g.C = tmp; g.ldc = sz; g.strideC = sz * sz;
             g.batch = (int)full; g.kind = K_TRSM;
+            g.b_upper = true;  // X_C
+            // one pair level is a few large tiles: 64 x 64 tiles keep the
+            // GPU filled and spread the triangular (clipped) K ranges
+            const bool few = full * ((sz + 127) / 128) * ((sz + 127) / 128) < 2 * hd->cu_count;
+            g.small_tiles = few;
             MQR_CHECK(gemm_f64(g, s));
             GemmF64 h2;
+            h2.small_tiles = few;
             h2.M = (int)sz; h2.N = (int)sz; h2.K = (int)sz;
             h2.A = X; h2.lda = n; h2.strideA = xstep;
             h2.B = tmp; h2.ldb = sz; h2.strideB = sz * sz;
             h2.C = X + sz * n; h2.ldc = n; h2.strideC = xstep;
             h2.alpha = -1.0; h2.batch = (int)full; h2.kind = K_TRSM;
+            h2.a_upper = true;  // X_A
             MQR_CHECK(gemm_f64(h2, s));
         }
         // a trailing partial pair
@@ -510,11 +517,15 @@ int tri_inverse(mqr_handle_t hd, int64_t n, const double *R, int64_t ldr, double
             g.A = R + i + (i + sz) * ldr; g.lda = ldr;
             g.B = X + (i + sz) + (i + sz) * n; g.ldb = n;
             g.C = tmp; g.ldc = sz; g.kind = K_TRSM;
+            g.b_upper = true;  // X_C
+            g.small_tiles = true;
             MQR_CHECK(gemm_f64(g, s));
             GemmF64 h2;
+            h2.small_tiles = true;
             h2.M = (int)sz; h2.N = (int)sc; h2.K = (int)sz;
             h2.A = X + i + i * n; h2.lda = n; h2.B = tmp; h2.ldb = sz;
             h2.C = X + i + (i + sz) * n; h2.ldc = n; h2.alpha = -1.0; h2.kind = K_TRSM;
+            h2.a_upper = true;  // X_A
             MQR_CHECK(gemm_f64(h2, s));
         }
     }

@@ -121,6 +121,13 @@ struct GemmF64 {
     bool ta = false, tb = false;
     int amask = MASK_NONE, bmask = MASK_NONE;
     int bmask_cols = 0;  // B mask applies to stored columns < bmask_cols
+    // op(A) upper triangular (op(A)(i, k) = 0 for k < i) / op(B) upper
+    // triangular (op(B)(k, j) = 0 for k > j): each tile skips the K range
+    // whose products are zero
+    bool a_upper = false, b_upper = false;
+    // 64 x 64 output tiles even when M, N >= 128 (more workgroups for small
+    // grids)
+    bool small_tiles = false;
     const double *A = nullptr;
     int64_t lda = 0;
     const double *B = nullptr;
