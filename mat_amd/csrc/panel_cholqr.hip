@@ -154,7 +154,7 @@ __global__ __launch_bounds__(small::NT) void k_recon(int b, const double *__rest
 // contiguous 128-byte column pieces.
 
 constexpr int RC = 64;              // rows per chunk
-constexpr int kMaxGramParts = 128;  // Gram partials per pass (workgroups)
+constexpr int kMaxGramParts = 256;  // Gram partials per pass (workgroups)
 
 // acc += s op(A)[ia, ka] op(B)[kb, jb] in the row-per-lane layout
 template <bool TA, bool TB>
