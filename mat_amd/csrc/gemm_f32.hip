@@ -59,6 +59,18 @@ template <int TILE, bool KMAJOR>
 __device__ __forceinline__ void fetch(const float *__restrict__ X, int64_t ld, int r0, int k0,
                                       int R, int kend, int tid, float *v) {
     constexpr int NV = TILE * BK / NT;
+    if (KMAJOR) {
+        // a thread's NV values are consecutive k of one row: one vector load
+        // when the row and the whole run are in range
+        int r, k;
+        slot<TILE, true>(tid, 0, r, k);
+        r += r0;
+        k += k0;
+        if (r < R && k + NV <= kend) {
+            __builtin_memcpy(v, X + (int64_t)k + (int64_t)r * ld, NV * sizeof(float));
+            return;
+        }
+    }
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
         int r, k;
