@@ -70,6 +70,26 @@ __device__ __forceinline__ void fetch(const float *__restrict__ X, int64_t ld, i
             __builtin_memcpy(v, X + (int64_t)k + (int64_t)r * ld, NV * sizeof(float));
             return;
         }
+    } else {
+        // values q and q + 1 are rows r and r + 1 at the same k: one
+        // two-element load when both rows and k are in range
+        bool fast = true;
+#pragma unroll
+        for (int q = 0; q < NV; q += 2) {
+            int r, k;
+            slot<TILE, false>(tid, q, r, k);
+            fast = fast && (r0 + r + 1 < R) && (k0 + k < kend);
+        }
+        if (fast) {
+#pragma unroll
+            for (int q = 0; q < NV; q += 2) {
+                int r, k;
+                slot<TILE, false>(tid, q, r, k);
+                __builtin_memcpy(v + q, X + (int64_t)(r0 + r) + (int64_t)(k0 + k) * ld,
+                                 2 * sizeof(float));
+            }
+            return;
+        }
     }
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
