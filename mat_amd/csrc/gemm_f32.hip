@@ -41,6 +41,34 @@ __device__ __forceinline__ void split4(const float (&x)[4], v4s &hi, v4s &lo) {
     }
 }
 
+// N consecutive floats of one column to / from memory: one vector access
+// when the address is aligned to the vector, else scalar ones
+template <int N>
+__device__ __forceinline__ void load_run(float (&v)[N], const float *src) {
+    typedef float vt __attribute__((ext_vector_type(N)));
+    if ((reinterpret_cast<uintptr_t>(src) & (N * sizeof(float) - 1)) == 0) {
+        const vt x = *reinterpret_cast<const vt *>(src);
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = x[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = src[i];
+    }
+}
+template <int N>
+__device__ __forceinline__ void store_run(float *dst, const float (&v)[N]) {
+    typedef float vt __attribute__((ext_vector_type(N)));
+    if ((reinterpret_cast<uintptr_t>(dst) & (N * sizeof(float) - 1)) == 0) {
+        vt x;
+#pragma unroll
+        for (int i = 0; i < N; ++i) x[i] = v[i];
+        *reinterpret_cast<vt *>(dst) = x;
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) dst[i] = v[i];
+    }
+}
+
 template <int TILE, bool KMAJOR>
 __device__ __forceinline__ void slot(int tid, int q, int &r, int &k) {
     constexpr int NV = TILE * BK / NT;
@@ -67,7 +95,10 @@ __device__ __forceinline__ void fetch(const float *__restrict__ X, int64_t ld, i
         r += r0;
         k += k0;
         if (r < R && k + NV <= kend) {
-            __builtin_memcpy(v, X + (int64_t)k + (int64_t)r * ld, NV * sizeof(float));
+            float t[NV];
+            load_run<NV>(t, X + (int64_t)k + (int64_t)r * ld);
+#pragma unroll
+            for (int q = 0; q < NV; ++q) v[q] = t[q];
             return;
         }
     } else {
@@ -85,8 +116,10 @@ __device__ __forceinline__ void fetch(const float *__restrict__ X, int64_t ld, i
             for (int q = 0; q < NV; q += 2) {
                 int r, k;
                 slot<TILE, false>(tid, q, r, k);
-                __builtin_memcpy(v + q, X + (int64_t)(r0 + r) + (int64_t)(k0 + k) * ld,
-                                 2 * sizeof(float));
+                float t[2];
+                load_run<2>(t, X + (int64_t)(r0 + r) + (int64_t)(k0 + k) * ld);
+                v[q] = t[0];
+                v[q + 1] = t[1];
             }
             return;
         }
@@ -136,26 +169,37 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_f32(GemmF32 gin, int kchunk) {
     const int kb = split * kchunk;
     const int ke = min(g.K, kb + kchunk);
 
-    // accumulator element (i, j, r): row m0 + wm HALF + 16 i + (l & 15),
+    // accumulator element (i, j, r): row m0 + wm HALF + BW (l & 15) + i,
     // column n0 + wn HALF + 16 j + 4 (l >> 4) + r (swapped operands; see the
-    // epilogue).  With beta != 0 it starts as beta C, loaded before the K
-    // loop so that the loads overlap the first slices; alpha is applied to
-    // op(A) in LDS.
+    // epilogue).  The A operand's rows are assigned so that a lane's BW row
+    // tiles are BW consecutive rows: C moves as one vector per lane and
+    // column.  With beta != 0 it starts as beta C, loaded before the K loop
+    // so that the loads overlap the first slices; alpha is applied to op(A)
+    // in LDS.
+    const int arow = wm * HALF + BW * (lane & 15);  // first of the lane's rows
+    const bool interior = (m0 + TILE <= g.M) && (n0 + TILE <= g.N);
     v4f acc[BW][BW];
     if (g.splits == 1 && g.beta != 0.f) {
 #pragma unroll
-        for (int i = 0; i < BW; ++i) {
-            const int gm = m0 + wm * HALF + i * 16 + (lane & 15);
-            const int cm = gm < g.M ? gm : g.M - 1;
+        for (int j = 0; j < BW; ++j)
 #pragma unroll
-            for (int j = 0; j < BW; ++j)
+            for (int r = 0; r < 4; ++r) {
+                const int gn = n0 + wn * HALF + j * 16 + 4 * (lane >> 4) + r;
+                const int cn = gn < g.N ? gn : g.N - 1;
+                const float *src = g.C + (int64_t)(m0 + arow) + (int64_t)cn * g.ldc;
+                float v[BW];
+                if (interior) {
+                    load_run<BW>(v, src);
+                } else {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int gn = n0 + wn * HALF + j * 16 + 4 * (lane >> 4) + r;
-                    const int cn = gn < g.N ? gn : g.N - 1;
-                    acc[i][j][r] = g.beta * g.C[(int64_t)cm + (int64_t)cn * g.ldc];
+                    for (int i = 0; i < BW; ++i) {
+                        const int gm = m0 + arow + i;
+                        v[i] = g.C[(int64_t)(gm < g.M ? gm : g.M - 1) + (int64_t)cn * g.ldc];
+                    }
                 }
-        }
+#pragma unroll
+                for (int i = 0; i < BW; ++i) acc[i][j][r] = g.beta * v[i];
+            }
     } else {
 #pragma unroll
         for (int i = 0; i < BW; ++i)
@@ -182,8 +226,7 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_f32(GemmF32 gin, int kchunk) {
                 for (int kk = 0; kk < BK; kk += 4) {
                     float a[BW], b[BW];
 #pragma unroll
-                    for (int i = 0; i < BW; ++i)
-                        a[i] = As[buf][kk + (lane >> 4)][wm * HALF + i * 16 + (lane & 15)];
+                    for (int i = 0; i < BW; ++i) a[i] = As[buf][kk + (lane >> 4)][arow + i];
 #pragma unroll
                     for (int j = 0; j < BW; ++j)
                         b[j] = Bs[buf][kk + (lane >> 4)][wn * HALF + j * 16 + (lane & 15)];
@@ -203,7 +246,7 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_f32(GemmF32 gin, int kchunk) {
                     float x[4];
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
-                        x[q] = As[buf][4 * (lane >> 4) + q][wm * HALF + i * 16 + (lane & 15)];
+                        x[q] = As[buf][4 * (lane >> 4) + q][arow + i];
                     split4(x, ah[i], al[i]);
                 }
 #pragma unroll
@@ -238,21 +281,25 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_f32(GemmF32 gin, int kchunk) {
     }
     // v_mfma_f32_16x16x4_f32 with swapped operands: lane l, register r holds
     // output row (l >> 4) * 4 + r of the transposed product, column l & 15,
-    // i.e. C row = base + (l & 15), column = base + 4 (l >> 4) + r.
+    // i.e. C row = m0 + arow + i, column = base + 4 (l >> 4) + r.
     float *Cout = g.C + (g.splits > 1 ? (int64_t)split * g.split_stride : 0);
-    const bool interior = (m0 + TILE <= g.M) && (n0 + TILE <= g.N);
 #pragma unroll
-    for (int i = 0; i < BW; ++i) {
-        const int gm = m0 + wm * HALF + i * 16 + (lane & 15);
+    for (int j = 0; j < BW; ++j)
 #pragma unroll
-        for (int j = 0; j < BW; ++j)
+        for (int r = 0; r < 4; ++r) {
+            const int gn = n0 + wn * HALF + j * 16 + 4 * (lane >> 4) + r;
+            float v[BW];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int gn = n0 + wn * HALF + j * 16 + 4 * (lane >> 4) + r;
-                if (interior || (gm < g.M && gn < g.N))
-                    Cout[(int64_t)gm + (int64_t)gn * g.ldc] = acc[i][j][r];
+            for (int i = 0; i < BW; ++i) v[i] = acc[i][j][r];
+            float *dst = Cout + (int64_t)(m0 + arow) + (int64_t)gn * g.ldc;
+            if (interior) {
+                store_run<BW>(dst, v);
+            } else if (gn < g.N) {
+#pragma unroll
+                for (int i = 0; i < BW; ++i)
+                    if (m0 + arow + i < g.M) dst[i] = v[i];
             }
-    }
+        }
 }
 
 template <bool TA, bool TB, int MODE>
