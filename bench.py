@@ -100,9 +100,9 @@ ROOF_KIND = "update_gemm"
 # HBM bytes moved per algorithmic byte by the FP64 trailing update, from two
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by a 537 MB device copy
 # in the same run (profiles/trailing_update_gemm_ls16384x4096.md).
-TRAFFIC_RATIO_FP64_UPDATE = 1.225
+TRAFFIC_RATIO_FP64_UPDATE = 1.226
 # The same for the FP32 trailing update of the mixed solve.
-TRAFFIC_RATIO_FP32_UPDATE = 1.299
+TRAFFIC_RATIO_FP32_UPDATE = 1.187
 
 
 def run_plain(M, h, P, steps: int, warmup: int):
