@@ -124,17 +124,19 @@ __global__ __launch_bounds__(256) void k_sum(int cnt, const double *__restrict__
 }
 
 constexpr int GV_ROWS = 256, GV_COLS = 256;
+constexpr int GV_XCOLS = 64;  // column chunk for products with the n x n X
 
-// part[chunk][row] = sum over the chunk's columns of A[row, c] x[c]
+// part[chunk][row] = sum over the chunk's columns (cols wide) of A[row, c] x[c];
+// with upper set, A is upper triangular and only columns c >= row are read
 __global__ __launch_bounds__(256) void k_gemv_n(int64_t m, int64_t n, const double *__restrict__ A,
                                                 int64_t lda, const double *__restrict__ x,
-                                                double *__restrict__ part) {
+                                                double *__restrict__ part, int cols, int upper) {
     const int64_t row = (int64_t)blockIdx.x * GV_ROWS + threadIdx.x;
-    const int64_t c0 = (int64_t)blockIdx.y * GV_COLS;
-    const int64_t c1 = std::min<int64_t>(n, c0 + GV_COLS);
+    const int64_t c0 = (int64_t)blockIdx.y * cols;
+    const int64_t c1 = std::min<int64_t>(n, c0 + cols);
     if (row >= m) return;
     double s0 = 0.0, s1 = 0.0;
-    int64_t c = c0;
+    int64_t c = upper ? std::max(c0, row) : c0;
     for (; c + 1 < c1; c += 2) {
         s0 = fma(A[row + c * lda], x[c], s0);
         s1 = fma(A[row + (c + 1) * lda], x[c + 1], s1);
@@ -294,7 +296,9 @@ MixedWs carve(char *base, bool fp64, int64_t m, int64_t n, int nb) {
     w.r = (double *)take(sizeof(double) * m);
     w.z = (double *)take(sizeof(double) * n);
     w.y = (double *)take(sizeof(double) * n);
-    w.part = (double *)take(sizeof(double) * std::max<int64_t>(nchunks * std::max(m, n), 2048));
+    const int64_t nxchunks = (n + GV_XCOLS - 1) / GV_XCOLS;
+    w.part = (double *)take(sizeof(double) *
+                            std::max<int64_t>(std::max(nchunks * std::max(m, n), nxchunks * n), 2048));
     w.sc = (double *)take(sizeof(double) * 64);
     w.xx = (double *)take(sizeof(double) * n);
     w.zz = (double *)take(sizeof(double) * n);
@@ -557,7 +561,7 @@ int residual_pass(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_
     const int64_t nchunks = (n + GV_COLS - 1) / GV_COLS;
     ProfScope ps(K_RESID, 4.0 * m * n, 16.0 * m * n, s);
     hipLaunchKernelGGL(k_gemv_n, dim3((unsigned)((m + GV_ROWS - 1) / GV_ROWS), (unsigned)nchunks),
-                       dim3(256), 0, s, m, n, A, lda, x, w.part);
+                       dim3(256), 0, s, m, n, A, lda, x, w.part, GV_COLS, 0);
     hipLaunchKernelGGL(k_resid, dim3(grid_for(m, 256, 1 << 22)), dim3(256), 0, s, m, (int)nchunks,
                        b, (const double *)w.part, w.r);
     hipLaunchKernelGGL(k_gemv_t, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, m, n, A, lda,
@@ -583,10 +587,13 @@ int precondition(mqr_handle_t hd, int64_t n, MixedWs &w) {
 // x += X y, xx = x^2
 int correction(mqr_handle_t hd, int64_t n, double *x, MixedWs &w) {
     hipStream_t s = hd->stream;
-    const int64_t nchunks = (n + GV_COLS - 1) / GV_COLS;
-    ProfScope ps(K_TRSM, (double)n * n, 8.0 * n * n, s);
+    // X is upper triangular: narrow column chunks for more workgroups, and
+    // each row reads only its columns at or right of the diagonal
+    const int64_t nchunks = (n + GV_XCOLS - 1) / GV_XCOLS;
+    ProfScope ps(K_TRSM, (double)n * n, 4.0 * n * n, s);
     hipLaunchKernelGGL(k_gemv_n, dim3((unsigned)((n + GV_ROWS - 1) / GV_ROWS), (unsigned)nchunks),
-                       dim3(256), 0, s, n, n, (const double *)w.X, n, (const double *)w.y, w.part);
+                       dim3(256), 0, s, n, n, (const double *)w.X, n, (const double *)w.y, w.part,
+                       GV_XCOLS, 1);
     hipLaunchKernelGGL(k_update_x, dim3(grid_for(n, 256, 1 << 22)), dim3(256), 0, s, n,
                        (int)nchunks, (const double *)w.part, x, w.xx);
     MQR_LAUNCH_CHECK();
