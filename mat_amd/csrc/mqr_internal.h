@@ -181,11 +181,9 @@ int gemm_pick_splits(int M, int N, int K, int target_blocks);
 // Householder panel, single workgroup.
 int panel_householder(int64_t h, int b, double *P, int64_t ld, double *tau, double *T,
                       int64_t ldt, hipStream_t s);
-// Same, launched unconditionally but running only when *gate != 0.
-int panel_householder_gated(int64_t h, int b, double *P, int64_t ld, double *tau, double *T,
-                            int64_t ldt, const int *gate, hipStream_t s);
-// Shifted CholeskyQR3 + Householder reconstruction panel (falls back to the
-// gated Householder kernel on breakdown).  ws: panel_cholqr_ws_bytes(h).
+// CholeskyQR2 + Householder reconstruction panel; on breakdown the
+// reconstruction kernel factors the panel column by column itself.
+// ws: panel_cholqr_ws_bytes(h).
 size_t panel_cholqr_ws_bytes(int64_t h);
 int panel_cholqr(mqr_handle_t hd, int64_t h, int b, double *X, int64_t ld, double *tau,
                  double *T, int64_t ldt, void *ws);

@@ -28,6 +28,7 @@
 #include <climits>
 
 #include "mqr_internal.h"
+#include "panel_hh.h"
 #include "small_blocked.h"
 #include "small_dense.h"
 
@@ -70,9 +71,14 @@ __global__ __launch_bounds__(small::NT) void k_chol_inv(int b, const double *__r
 // Q1 - S = L U (S the signs), the reflectors I - V T V^T whose first b
 // columns are Q S satisfy V1 = L, V2 = Q2 U^{-1} and T = -U S L^{-T}
 // (top block: Q1 S - I = -V1 T V1^T).
-__global__ __launch_bounds__(small::NT) void k_recon(int b, const double *__restrict__ G,
-                                                     int64_t ldg, const double *__restrict__ Z,
-                                                     int64_t ldz,
+//
+// Breakdown: when the first pass raised the flag, or the second Cholesky
+// fails here, the untouched h x b panel A is factored column by column in
+// this same workgroup (the reference's panel loop), with the flag left up
+// for the V2 kernel; no separate fallback launch.
+__global__ __launch_bounds__(small::NT) void k_recon(int64_t h, int b,
+                                                     const double *__restrict__ G, int64_t ldg,
+                                                     const double *__restrict__ Z, int64_t ldz,
                                                      const double *__restrict__ R1,
                                                      double *__restrict__ A, int64_t lda,
                                                      double *__restrict__ tau,
@@ -82,7 +88,13 @@ __global__ __launch_bounds__(small::NT) void k_recon(int b, const double *__rest
     __shared__ small::Lds S0[PB], S1[PB], S2[PB], S3[PB];
     __shared__ double sg[PB];
     __shared__ int fail;
-    if (*flag) return;
+    // the fallback's LDS, carved from the reconstruction's buffers
+    const hh::PanelLds fb{reinterpret_cast<double(*)[hh::CH]>(&S2[0][0]), &S2[8][0], S0,
+                          reinterpret_cast<small::RowBuf *>(&S1[0][0]), &S3[0][0]};
+    if (*flag) {
+        hh::panel_hh_body(h, b, A, lda, tau, T, ldt, fb);
+        return;
+    }
     const int tid = threadIdx.x, q = small::wave_id();
     if (tid == 0) fail = 0;
     small::lds_load(S0, Z, ldz, b);
@@ -92,6 +104,8 @@ __global__ __launch_bounds__(small::NT) void k_recon(int b, const double *__rest
     sb::chol_inv(S1, S2, b, [](int) { return 1e-2; }, &fail);
     if (fail) {  // uniform: read after the barrier that ends chol_inv
         if (tid == 0) *flag = 1;
+        __syncthreads();  // the LDS buffers are reused by the fallback
+        hh::panel_hh_body(h, b, A, lda, tau, T, ldt, fb);
         return;
     }
     // Q1 = Z_top R2^{-1} -> S3 (wave q: tile row q)
@@ -573,13 +587,12 @@ int cholqr_rest_a(hipStream_t s, int64_t h, int b, double *X, int64_t ld, double
     MQR_CHECK(reduce_partials((int64_t)PB * PB, w.Gp, parts, (int64_t)PB * PB, s));
     {
         ProfScope ps(K_PANEL_SMALL, 8.0 * bb * bb * bb / 3.0, 8.0 * bb * bb * 6.0, s);
-        hipLaunchKernelGGL(k_recon, dim3(1), dim3(small::NT), 0, s, b, (const double *)w.Gp,
+        hipLaunchKernelGGL(k_recon, dim3(1), dim3(small::NT), 0, s, h, b, (const double *)w.Gp,
                            (int64_t)PB, (const double *)w.Z, h, (const double *)w.R1, X, ld, tau,
                            T, ldt, w.Mi, w.flag);
         MQR_LAUNCH_CHECK();
     }
-    // fallback: the column-by-column kernel runs only when the flag is up
-    return panel_householder_gated(h, b, X, ld, tau, T, ldt, w.flag, s);
+    return MQR_SUCCESS;
 }
 
 // V2 of the panel; with Yf, Yf (h x b, ld h) = V T^T; with Wp, also
