@@ -81,11 +81,17 @@ __global__ void k_reduce_f32(int64_t count, float *__restrict__ P, int splits, i
 
 // R (ldr) entries of the block rows right of the diagonal blocks:
 // R[r][c] = Aw[r][c] for c >= the end of r's nb-block (FP32 -> FP64)
+// columns c = blockIdx.y, + gridDim.y, ...; rows from blockIdx.x * 256:
+// rows above c's diagonal block copied (FP32 -> FP64), rows below the
+// diagonal zeroed; the diagonal block's upper part (written in FP64 by the
+// panel) is left
 __global__ void k_copy_rupper(int64_t n, int nb, const float *__restrict__ Aw, int64_t ldw,
                               double *__restrict__ R, int64_t ldr) {
-    for (int64_t e = gidx(); e < n * n; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = e % n, c = e / n;
-        if (c >= (r / nb + 1) * nb) R[r + c * ldr] = (double)Aw[r + c * ldw];
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= n) return;
+    for (int64_t c = blockIdx.y; c < n; c += gridDim.y) {
+        const int64_t top = (c / nb) * nb;  // first row of c's diagonal block
+        if (r < top) R[r + c * ldr] = (double)Aw[r + c * ldw];
         else if (r > c) R[r + c * ldr] = 0.0;
     }
 }
@@ -463,7 +469,9 @@ int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t 
     if (side_pending) MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
     {
         ProfScope ps(K_CAST, 0.0, 12.0 * n * n, s0);
-        hipLaunchKernelGGL(k_copy_rupper, dim3(grid_for(n * n, 256, 1 << 20)), dim3(256), 0, s0, n,
+        hipLaunchKernelGGL(k_copy_rupper,
+                           dim3((unsigned)((n + 255) / 256), (unsigned)std::min<int64_t>(n, 65535)),
+                           dim3(256), 0, s0, n,
                            nb, (const float *)w.Aw, m, w.R, w.ldr);
         MQR_LAUNCH_CHECK();
     }
