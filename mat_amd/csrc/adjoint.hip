@@ -21,16 +21,19 @@ namespace {
 
 constexpr int AB = small::N;  // block of the triangular solve
 
-// One batched GEMM on the FP32 or FP64 tile.
+// One batched GEMM on the FP32 or FP64 tile: C = alpha op(A) op(B) + beta Cin,
+// with Cin = C unless given.
 int bgemm(bool f64, bool ta, bool tb, int M, int N, int K, double alpha, const void *A,
           int64_t lda, int64_t sA, const void *B, int64_t ldb, int64_t sB, double beta, void *C,
-          int64_t ldc, int64_t sC, int batch, int kind, hipStream_t s) {
+          int64_t ldc, int64_t sC, int batch, int kind, hipStream_t s,
+          const void *Cin = nullptr, int64_t ldcin = 0, int64_t sCin = 0) {
     if (M <= 0 || N <= 0 || K <= 0) return MQR_SUCCESS;
     if (f64) {
         GemmF64 g;
         g.M = M; g.N = N; g.K = K; g.ta = ta; g.tb = tb;
         g.A = (const double *)A; g.lda = lda; g.B = (const double *)B; g.ldb = ldb;
         g.C = (double *)C; g.ldc = ldc; g.alpha = alpha; g.beta = beta;
+        g.Cin = (const double *)Cin; g.ldcin = ldcin; g.strideCin = sCin;
         g.batch = batch; g.strideA = sA; g.strideB = sB; g.strideC = sC; g.kind = kind;
         return gemm_f64(g, s);
     }
@@ -38,6 +41,7 @@ int bgemm(bool f64, bool ta, bool tb, int M, int N, int K, double alpha, const v
     g.M = M; g.N = N; g.K = K; g.ta = ta; g.tb = tb;
     g.A = (const float *)A; g.lda = lda; g.B = (const float *)B; g.ldb = ldb;
     g.C = (float *)C; g.ldc = ldc; g.alpha = (float)alpha; g.beta = (float)beta;
+    g.Cin = (const float *)Cin; g.ldcin = ldcin; g.strideCin = sCin;
     g.batch = batch; g.strideA = sA; g.strideB = sB; g.strideC = sC; g.kind = kind;
     return gemm_f32(g, s);
 }
@@ -153,16 +157,10 @@ int qr_backward_t(mqr_handle_t hd, int64_t batch, int64_t m, int64_t n, const T 
                            sr, Iw);
         MQR_LAUNCH_CHECK();
     }
-    // B = dQ + Q copyltu(M), in the workspace (m x n, ld m)
-    {
-        ProfScope ps(K_OTHER, 0.0, 2.0 * sizeof(T) * batch * m * n, s);
-        const unsigned g2 = (unsigned)std::min<int64_t>(4096, (m * n + 255) / 256);
-        hipLaunchKernelGGL(k_copy2d<T>, dim3(g2, B), dim3(256), 0, s, m, n, dQ, lddq, sdq, Ww,
-                           m, sW);
-        MQR_LAUNCH_CHECK();
-    }
-    MQR_CHECK(bgemm(f64, false, false, (int)m, (int)n, (int)n, 1.0, Q, ldq, sq, Mw, n, sM, 1.0,
-                    Ww, m, sW, B, K_GEMM, s));
+    // B = dQ + Q copyltu(M), in the workspace (m x n, ld m), dQ read as the
+    // GEMM's beta term
+    MQR_CHECK(bgemm(f64, false, false, (int)m, (int)n, (int)n, 1.0, Q, ldq, sq, Mw, n, sM,
+                    dQ ? 1.0 : 0.0, Ww, m, sW, B, K_GEMM, s, dQ, lddq, sdq));
     // dA = B R^{-T}: blocks from the last
     for (int J = nblk - 1; J >= 0; --J) {
         const int64_t j0 = (int64_t)J * AB;

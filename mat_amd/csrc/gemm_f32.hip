@@ -162,6 +162,7 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_f32(GemmF32 gin, int kchunk) {
         g.A += bz * g.strideA;
         g.B += bz * g.strideB;
         g.C += bz * g.strideC;
+        if (g.Cin) g.Cin += bz * g.strideCin;
     }
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -186,7 +187,9 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_f32(GemmF32 gin, int kchunk) {
             for (int r = 0; r < 4; ++r) {
                 const int gn = n0 + wn * HALF + j * 16 + 4 * (lane >> 4) + r;
                 const int cn = gn < g.N ? gn : g.N - 1;
-                const float *src = g.C + (int64_t)(m0 + arow) + (int64_t)cn * g.ldc;
+                const float *Cb = g.Cin ? g.Cin : g.C;
+                const int64_t ldb_c = g.Cin ? g.ldcin : g.ldc;
+                const float *src = Cb + (int64_t)(m0 + arow) + (int64_t)cn * ldb_c;
                 float v[BW];
                 if (interior) {
                     load_run<BW>(v, src);
@@ -194,7 +197,7 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_f32(GemmF32 gin, int kchunk) {
 #pragma unroll
                     for (int i = 0; i < BW; ++i) {
                         const int gm = m0 + arow + i;
-                        v[i] = g.C[(int64_t)(gm < g.M ? gm : g.M - 1) + (int64_t)cn * g.ldc];
+                        v[i] = Cb[(int64_t)(gm < g.M ? gm : g.M - 1) + (int64_t)cn * ldb_c];
                     }
                 }
 #pragma unroll

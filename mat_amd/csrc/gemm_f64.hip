@@ -67,7 +67,8 @@ __device__ __forceinline__ void preload_c(const GemmF64 &g, v4d (&acc)[BI][BJ], 
             for (int r = 0; r < 4; ++r) {
                 const int gn = colbase + j * 16 + (lane >> 4) + 4 * r;
                 const int cn = gn < g.N ? gn : g.N - 1;
-                acc[i][j][r] = g.beta * g.C[(int64_t)cm + (int64_t)cn * g.ldc];
+                acc[i][j][r] = g.beta * (g.Cin ? g.Cin[(int64_t)cm + (int64_t)cn * g.ldcin]
+                                               : g.C[(int64_t)cm + (int64_t)cn * g.ldc]);
             }
     }
 }
@@ -187,6 +188,7 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_f64(GemmF64 gin, int kchunk) {
         g.A += bz * g.strideA;
         g.B += bz * g.strideB;
         g.C += bz * g.strideC;
+        if (g.Cin) g.Cin += bz * g.strideCin;
     }
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;

@@ -134,6 +134,10 @@ struct GemmF64 {
     int64_t ldb = 0;
     double *C = nullptr;
     int64_t ldc = 0;
+    // when set, beta C is read from Cin (ld ldcin, batch stride strideCin)
+    // instead of C: C = alpha op(A) op(B) + beta Cin
+    const double *Cin = nullptr;
+    int64_t ldcin = 0, strideCin = 0;
     double alpha = 1.0, beta = 0.0;
     // split-K: when splits > 1, partial products are written (not accumulated)
     // to C + s * split_stride for split s; beta is ignored.
@@ -158,6 +162,9 @@ struct GemmF32 {
     int64_t ldb = 0;
     float *C = nullptr;
     int64_t ldc = 0;
+    // when set, beta C is read from Cin (ld ldcin, batch stride strideCin)
+    const float *Cin = nullptr;
+    int64_t ldcin = 0, strideCin = 0;
     float alpha = 1.f, beta = 0.f;
     int splits = 1;
     int64_t split_stride = 0;
