@@ -233,8 +233,9 @@ int panel_factor(mqr_handle_t hd, int64_t h, int b, double *V, int64_t lda, doub
 // the block reflector I - V T V^T = I - (V T) V^T.
 int form_y_f64(hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv, const double *Tp,
                int64_t ldt, bool trans, double *Y);
-// P[0:count] += P[s*stride + 0:count] for s = 1..splits-1.
-int reduce_partials(int64_t count, double *P, int splits, int64_t stride, hipStream_t s);
+// out[0:count] = sum over s < splits of P[s*stride + 0:count]; out = P when null.
+int reduce_partials(int64_t count, double *P, int splits, int64_t stride, hipStream_t s,
+                    double *out = nullptr);
 // Largest block size the LDS-resident small kernels accept.
 constexpr int kMaxNb = 64;
 

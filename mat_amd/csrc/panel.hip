@@ -34,14 +34,14 @@ int panel_householder(int64_t h, int b, double *P, int64_t ld, double *tau, doub
     return MQR_SUCCESS;
 }
 
-// P[0:count] = sum over s of P[s * stride + 0:count]
-// P[0:count] = sum of the splits partials P[s * stride : s * stride + count].
+// out[0:count] = sum of the splits partials P[s * stride : s * stride + count].
 // A workgroup covers 256 / G entries with G threads per entry (thread group
 // g sums partials g, g + G, ...); the G sums meet in LDS.  Spreading one
 // entry over G threads keeps many loads in flight when there are few entries
 // and many partials (a 64 x 64 Gram matrix from 128 row chunks).
-__global__ __launch_bounds__(256) void k_reduce_partials(int64_t count, double *__restrict__ P,
-                                                         int splits, int64_t stride, int G) {
+__global__ __launch_bounds__(256) void k_reduce_partials(int64_t count, const double *P,
+                                                         int splits, int64_t stride, int G,
+                                                         double *out) {
     __shared__ double part[256];
     const int E = 256 / G;
     const int t = threadIdx.x, g = t / E;
@@ -60,19 +60,21 @@ __global__ __launch_bounds__(256) void k_reduce_partials(int64_t count, double *
     if (g == 0 && e < count) {
         double sum = part[t];
         for (int k = 1; k < G; ++k) sum += part[t + k * E];
-        P[e] = sum;
+        out[e] = sum;
     }
 }
 
-int reduce_partials(int64_t count, double *P, int splits, int64_t stride, hipStream_t s) {
-    if (splits <= 1 || count <= 0) return MQR_SUCCESS;
+int reduce_partials(int64_t count, double *P, int splits, int64_t stride, hipStream_t s,
+                    double *out) {
+    if (!out) out = P;
+    if (count <= 0 || (splits <= 1 && out == P)) return MQR_SUCCESS;
     ProfScope ps(K_OTHER, (double)count * (splits - 1), 8.0 * count * (splits + 1), s);
     // threads per entry: enough workgroups to spread the loads over the chip
     int G = 1;
     while (G < 16 && G * 2 <= splits && (count * G) / 256 < 512) G *= 2;
     const int E = 256 / G;
     hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)((count + E - 1) / E)), dim3(256), 0, s,
-                       count, P, splits, stride, G);
+                       count, (const double *)P, splits, stride, G, out);
     MQR_LAUNCH_CHECK();
     return MQR_SUCCESS;
 }

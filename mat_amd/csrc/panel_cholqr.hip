@@ -4,9 +4,10 @@
 // (QR/__Householder_Blocked.h:62-84): b dependent reductions over h rows.  On
 // the GPU every such reduction is a grid-wide synchronisation, so the tall
 // panel is factored instead by CholeskyQR2: two passes of
-//   Gram (MFMA, split-K) -> Cholesky (+ triangular inverse) in one workgroup
-//   -> Y = X R^{-1} (MFMA)
-// (orthonormal to working accuracy when kappa(X) is below ~1e6).  The
+//   Gram (MFMA, split-K) -> Cholesky (+ triangular inverse) -> Y = X R^{-1}
+// (orthonormal to working accuracy when kappa(X) is below ~1e6).  Every
+// workgroup of the kernel forming the first Y repeats the first Cholesky;
+// the second runs in the single-workgroup reconstruction.  The
 // Householder vectors are then reconstructed from the orthonormal factor
 // Q = Y R2^{-1} by an LU factorisation of Q(0:b, :) - S without pivoting,
 // S = diag(+-1) chosen during the elimination (S_j = -sign of the current
@@ -18,8 +19,9 @@
 // A first-pass pivot d_i <= 1e-12 G_ii (a zero, dependent or nearly
 // dependent column), a second-pass pivot below 1e-2, or a non-finite value
 // raises a device flag; the rest of the sequence then leaves the panel
-// untouched and the column-by-column Householder kernel (panel.hip) factors
-// it instead, which also keeps the reference's tau = 0 for zero columns.
+// untouched and the reconstruction kernel factors it column by column
+// instead (panel_hh.h), which also keeps the reference's tau = 0 for zero
+// columns.
 //
 // The b x b factorisations run register-resident in one workgroup
 // (small_dense.h); every barrier is executed by all of its waves.  The
@@ -36,30 +38,6 @@ namespace mqr {
 namespace {
 
 constexpr int PB = kMaxNb;  // padded panel width (small::N)
-
-// Cholesky + inverse of the first pass.  G: reduced Gram (b x b, ld b).
-// A pivot d_i <= 1e-12 G_ii (column i nearly in the span of the previous
-// ones: kappa(X) beyond ~1e6, where two CholeskyQR passes no longer give an
-// orthonormal factor to working accuracy), a zero column or a non-finite
-// value sets the flag; the rest of the sequence then skips the panel.
-__global__ __launch_bounds__(small::NT) void k_chol_inv(int b, const double *__restrict__ G,
-                                                        int64_t ldg, double *__restrict__ R,
-                                                        double *__restrict__ Rinv,
-                                                        int *__restrict__ flag) {
-    __shared__ small::Lds S[PB], X[PB];
-    __shared__ double thr[PB];
-    __shared__ int fail;
-    const int tid = threadIdx.x;
-    if (tid == 0) fail = 0;
-    small::lds_load(S, G, ldg, b);
-    __syncthreads();
-    if (tid < PB) thr[tid] = tid < b ? 1e-12 * S[tid][tid] : 0.0;
-    __syncthreads();
-    sb::chol_inv(S, X, b, [&](int i) { return thr[i]; }, &fail);
-    small::lds_store(S, R, PB, PB, PB);
-    small::lds_store(X, Rinv, PB, PB, PB);
-    if (tid == 0) *flag = fail;
-}
 
 // Second pass and reconstruction.  Inputs: reduced Gram G2 of Z = X R1^{-1}
 // (b x b, ld b), the top b rows of Z (ld ldz), R1 (padded, ld PB).  Outputs:
@@ -351,20 +329,42 @@ __global__ __launch_bounds__(small::NT) void k_upd_gram(int64_t hc, int kin, int
     gram_store(g, Gp + (size_t)blockIdx.x * PB * PB);
 }
 
-// Z (h x b, ldz) = X (h x b, ldx) Rinv (upper, ld PB) and the Gram partial
-// of Z into Gp[blockIdx.x]; nothing when *flag is set.
+// First-pass Cholesky and second-pass Gram.  Every workgroup factors the
+// reduced Gram matrix G1 (b x b, ld PB) = R1^T R1 and inverts R1 itself:
+// the same operations on the same data, so all workgroups hold the same R1
+// bit for bit and no single-workgroup launch sits between the two passes.
+// Each then forms Z = X R1^{-1} for its row chunks (h x b, ldz) and the
+// Gram partial of Z into Gp[blockIdx.x]; workgroup 0 stores R1 (padded,
+// ld PB) and the breakdown flag.
+// A pivot d_i <= 1e-12 G_ii (column i nearly in the span of the previous
+// ones: kappa(X) beyond ~1e6, where two CholeskyQR passes no longer give an
+// orthonormal factor to working accuracy), a zero column or a non-finite
+// value raises the flag; every workgroup then stops, and the rest of the
+// sequence leaves the panel to the column-by-column fallback.
 __global__ __launch_bounds__(small::NT) void k_apply_gram(int64_t h, int b,
+                                                          const double *__restrict__ G1,
                                                           const double *__restrict__ X,
-                                                          int64_t ldx,
-                                                          const double *__restrict__ Rinv,
+                                                          int64_t ldx, double *__restrict__ R1,
                                                           double *__restrict__ Z, int64_t ldz,
                                                           double *__restrict__ Gp,
-                                                          const int *__restrict__ flag) {
-    if (*flag) return;
+                                                          int *__restrict__ flag) {
     __shared__ small::Lds Xs[RC], Ms[PB];
-    const int q = small::wave_id();
+    __shared__ double thr[PB];
+    __shared__ int fail;
+    const int tid = threadIdx.x, q = small::wave_id();
+    if (tid == 0) fail = 0;
+    small::lds_load(Xs, G1, PB, b);
+    __syncthreads();
+    if (tid < PB) thr[tid] = tid < b ? 1e-12 * Xs[tid][tid] : 0.0;
+    __syncthreads();
+    // R1 -> Xs, R1^{-1} -> Ms; ends with a barrier
+    sb::chol_inv(Xs, Ms, b, [&](int i) { return thr[i]; }, &fail);
+    if (blockIdx.x == 0) {
+        small::lds_store(Xs, R1, PB, PB, PB);
+        if (tid == 0) *flag = fail;
+    }
+    if (fail) return;  // uniform: written before chol_inv's last barrier
     sb::v4d g[3] = {sb::zero4(), sb::zero4(), sb::zero4()};
-    small::lds_load(Ms, Rinv, PB, PB);
     for (int64_t r0 = (int64_t)blockIdx.x * RC; r0 < h; r0 += (int64_t)gridDim.x * RC) {
         __syncthreads();
         chunk_load(Xs, X, ldx, r0, h, b);
@@ -516,7 +516,7 @@ __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
 }
 
 struct CqrWs {
-    double *Z, *R1, *I1, *Mi, *Gp;
+    double *Z, *R1, *G1, *Mi, *Gp;
     int *flag;
 };
 
@@ -524,8 +524,8 @@ CqrWs carve(void *ws, int64_t h) {
     CqrWs w;
     w.Z = (double *)ws;
     w.R1 = w.Z + (size_t)h * PB;
-    w.I1 = w.R1 + PB * PB;
-    w.Mi = w.I1 + PB * PB;
+    w.G1 = w.R1 + PB * PB;
+    w.Mi = w.G1 + PB * PB;
     w.Gp = w.Mi + PB * PB;
     w.flag = (int *)(w.Gp + (size_t)kMaxGramParts * PB * PB);
     return w;
@@ -537,13 +537,14 @@ int gram_parts(int64_t rows) {
 
 }  // namespace
 
-// Workspace: Z (h x PB), R1, R1^{-1}, Minv (PB x PB), the Gram partials, flag.
+// Workspace: Z (h x PB), R1, G1, Minv (PB x PB), the Gram partials, flag.
 size_t panel_cholqr_ws_bytes(int64_t h) {
     return sizeof(double) * ((size_t)h * PB + 3 * PB * PB + (size_t)kMaxGramParts * PB * PB) + 256;
 }
 
-// First pass: C (hc x bn) -= Y W when kin > 0, the Gram matrix of rows
-// [g0, hc) of C (the panel), and its Cholesky factor R1 with R1^{-1}.
+// First pass: C (hc x bn) -= Y W when kin > 0, and the Gram matrix G1 of
+// rows [g0, hc) of C (the panel), whose Cholesky factor the second pass
+// forms.
 int cholqr_gram1(hipStream_t s, int64_t hc, int bn, int kin, const double *Y, int64_t ldy,
                  const double *W, int64_t ldw, void *C, int64_t ldc, int64_t g0, void *ws,
                  bool c_f32, double *Xo, int64_t ldxo) {
@@ -562,26 +563,23 @@ int cholqr_gram1(hipStream_t s, int64_t hc, int bn, int kin, const double *Y, in
                                Y, ldy, W, ldw, (double *)C, ldc, g0, w.Gp, Xo, ldxo);
         MQR_LAUNCH_CHECK();
     }
-    MQR_CHECK(reduce_partials((int64_t)PB * PB, w.Gp, parts, (int64_t)PB * PB, s));
-    const double bb = bn;
-    ProfScope ps(K_PANEL_SMALL, 2.0 * bb * bb * bb / 3.0, 8.0 * bb * bb * 3.0, s);
-    hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(small::NT), 0, s, bn, w.Gp, (int64_t)PB, w.R1,
-                       w.I1, w.flag);
-    MQR_LAUNCH_CHECK();
-    return MQR_SUCCESS;
+    // G1 goes to its own buffer: the next launch overwrites the partials
+    return reduce_partials((int64_t)PB * PB, w.Gp, parts, (int64_t)PB * PB, s, w.G1);
 }
 
-// Second pass, reconstruction and gated fallback of the h x b panel X
-// after cholqr_gram1 (V2 follows in cholqr_finish).
+// First Cholesky, second pass, reconstruction and gated fallback of the
+// h x b panel X after cholqr_gram1 (V2 follows in cholqr_finish).
 int cholqr_rest_a(hipStream_t s, int64_t h, int b, double *X, int64_t ld, double *tau, double *T,
                   int64_t ldt, void *ws) {
     const CqrWs w = carve(ws, h);
     const int parts = gram_parts(h);
     const double hh = (double)h, bb = b;
     {
-        ProfScope ps(K_PANEL, hh * bb * bb + hh * bb * bb, 16.0 * hh * bb, s);
-        hipLaunchKernelGGL(k_apply_gram, dim3(parts), dim3(small::NT), 0, s, h, b, (const double *)X,
-                           ld, (const double *)w.I1, w.Z, h, w.Gp, (const int *)w.flag);
+        ProfScope ps(K_PANEL, hh * bb * bb + hh * bb * bb + 2.0 * bb * bb * bb / 3.0,
+                     16.0 * hh * bb + 8.0 * bb * bb * 2.0, s);
+        hipLaunchKernelGGL(k_apply_gram, dim3(parts), dim3(small::NT), 0, s, h, b,
+                           (const double *)w.G1, (const double *)X, ld, w.R1, w.Z, h, w.Gp,
+                           w.flag);
         MQR_LAUNCH_CHECK();
     }
     MQR_CHECK(reduce_partials((int64_t)PB * PB, w.Gp, parts, (int64_t)PB * PB, s));
