@@ -224,8 +224,11 @@ int cholqr_rest(hipStream_t s, int64_t h, int b, double *X, int64_t ld, double *
 // Q^T B (B may be null).
 int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, double *tau,
                   double *T, int64_t ldt, double *B, int64_t ldb, int64_t nrhs);
-// Panels at least this tall use panel_cholqr.
-constexpr int64_t kCholqrMinRows = 384;
+// Panels at least this tall use panel_cholqr (a few launches, ~0.2 ms); the
+// column-by-column kernel takes ~1.4 ms on a 64-wide panel of a few hundred
+// rows (two barriers and a global round trip per 8-column round), so it is
+// kept for panels under one 64-row chunk and for breakdown.
+constexpr int64_t kCholqrMinRows = 64;
 // Either of the above by height.
 int panel_factor(mqr_handle_t hd, int64_t h, int b, double *V, int64_t lda, double *tau,
                  double *T, int64_t ldt, void *ws);
