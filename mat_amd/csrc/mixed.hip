@@ -188,16 +188,29 @@ __global__ __launch_bounds__(256) void k_gemv_t(int64_t m, int64_t n, const doub
     }
 }
 
-// x[i] += sum_p part[p * n + i]; xx[i] = x[i]^2
-__global__ void k_update_x(int64_t n, int nparts, const double *__restrict__ part,
-                           double *__restrict__ x, double *__restrict__ xx) {
-    const int64_t i = gidx();
-    if (i >= n) return;
+// x[i] += sum_p part[p * n + i]; xx[i] = x[i]^2.  A workgroup covers
+// UX_E entries with UX_G threads each (thread group g sums parts g,
+// g + UX_G, ...), so that the few entries still keep many loads in flight.
+constexpr int UX_G = 8, UX_E = 256 / UX_G;
+__global__ __launch_bounds__(256) void k_update_x(int64_t n, int nparts,
+                                                  const double *__restrict__ part,
+                                                  double *__restrict__ x,
+                                                  double *__restrict__ xx) {
+    __shared__ double red[256];
+    const int t = threadIdx.x, g = t / UX_E;
+    const int64_t i = (int64_t)blockIdx.x * UX_E + (t % UX_E);
     double s = 0.0;
-    for (int p = 0; p < nparts; ++p) s += part[(int64_t)p * n + i];
-    const double v = x[i] + s;
-    x[i] = v;
-    xx[i] = v * v;
+    if (i < n)
+        for (int p = g; p < nparts; p += UX_G) s += part[(int64_t)p * n + i];
+    red[t] = s;
+    __syncthreads();
+    if (g == 0 && i < n) {
+        double tot = red[t];
+        for (int k = 1; k < UX_G; ++k) tot += red[t + k * UX_E];
+        const double v = x[i] + tot;
+        x[i] = v;
+        xx[i] = v * v;
+    }
 }
 
 __global__ void k_sq(int64_t n, const double *__restrict__ x, double *__restrict__ xx) {
@@ -602,7 +615,7 @@ int correction(mqr_handle_t hd, int64_t n, double *x, MixedWs &w) {
     hipLaunchKernelGGL(k_gemv_n, dim3((unsigned)((n + GV_ROWS - 1) / GV_ROWS), (unsigned)nchunks),
                        dim3(256), 0, s, n, n, (const double *)w.X, n, (const double *)w.y, w.part,
                        GV_XCOLS, 1);
-    hipLaunchKernelGGL(k_update_x, dim3(grid_for(n, 256, 1 << 22)), dim3(256), 0, s, n,
+    hipLaunchKernelGGL(k_update_x, dim3((unsigned)((n + UX_E - 1) / UX_E)), dim3(256), 0, s, n,
                        (int)nchunks, (const double *)w.part, x, w.xx);
     MQR_LAUNCH_CHECK();
     return MQR_SUCCESS;
