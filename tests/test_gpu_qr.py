@@ -305,6 +305,35 @@ def test_tall_panel_fallback_on_dependent_columns(h, orc):
     assert abs(VR[70, 70]) < 1e-12 * np.linalg.norm(A) and abs(VR[200, 200]) < 1e-12 * np.linalg.norm(A)
 
 
+def test_short_panel_fallback_on_dependent_columns(h, orc):
+    """The same hand-over in the last panels of a square matrix (64 to 128
+    rows left): a zero column stays exactly zero under the earlier block
+    reflectors and keeps tau = 0; a duplicated column leaves a zero pivot.
+    R agrees with the oracle in its rows above that pivot; below it the
+    reflectors follow rounding noise, and any rotation of those rows is an
+    equally valid factorisation."""
+    rng = np.random.default_rng(12)
+    m = n = 320
+    A = rng.standard_normal((m, n))
+    A[:, 200] = 0.0               # panel at column 192: 128 rows left
+    A[:, 270] = A[:, 260]         # last panel: 64 rows left
+    dA, tau, T = gpu_factor(h, A)
+    tau = tau.cpu().numpy()
+    assert tau[200] == 0.0
+    VR = host(dA)
+    assert np.all(VR[200:, 200] == 0.0)
+    dQ = M.orgqr(h, dA, T, n, nq=m)
+    Q = host(dQ)
+    nA = np.linalg.norm(A)
+    R = np.triu(VR)
+    assert np.linalg.norm(Q @ R - A) / nA < 1e-14 * m
+    assert np.linalg.norm(Q.T @ Q - np.eye(m)) < 1e-13 * m
+    VRo, _, tau_o = orc.geqrf_blocked(A)
+    assert tau_o[200] == 0.0
+    assert abs(VR[270, 270]) < 1e-12 * nA
+    np.testing.assert_allclose(R[:270], np.triu(VRo)[:270], rtol=0, atol=1e-12 * nA)
+
+
 def test_host_qr_api_matches_oracle(orc):
     rng = np.random.default_rng(9)
     for (m, n) in [(7, 4), (300, 200), (257, 300)]:
