@@ -346,30 +346,148 @@ int ormqr_f64(mqr_handle_t hd, bool trans, int64_t m, int64_t nrhs, int64_t k, c
     return MQR_SUCCESS;
 }
 
+namespace {
+
+// Stored 64-column blocks applied together by orgqr_f64.
+constexpr int kMergeBlocks = 4;
+
+// Tb (nbig x nbig, ld nbig): the stored T blocks (T(0:b, j0 + c) for the
+// block holding column j0 + c) on the diagonal 64-blocks, zero elsewhere.
+__global__ void k_merged_t_init(int nbig, const double *__restrict__ T, int64_t ldt, int64_t j0,
+                                double *__restrict__ Tb) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nbig * nbig) return;
+    const int r = (int)(e % nbig), c = (int)(e / nbig);
+    Tb[e] = ((r >> 6) == (c >> 6) && r <= c) ? T[(r & 63) + (j0 + c) * ldt] : 0.0;
+}
+
+// Tb (b x b, ld b): the compact-WY factor of the stored 64-column blocks
+// j0, j0 + 64, ... (b columns in all) taken as one block reflector,
+// I - V Tb V^T = H_j0 H_j0+64 ... with V the (h x b) unit-lower trapezoid.
+// inv(Tb) = strict_upper(V^T V) + diag(1 / tau): its diagonal 64-blocks are
+// the inverses of the stored T blocks and its off-diagonal blocks those of
+// G = V^T V, so Tb's off-diagonal blocks follow by recursive doubling,
+// Tb_12 = -T_1 G_12 T_2.  A reflector with tau = 0 has a zero row and column
+// in its stored block, which the products carry into Tb.  G, tmp: b x b and
+// b/2 x b doubles; P (pcap doubles) holds split-K partials.
+int merged_t(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double *V, int64_t ldv,
+             const double *T, int64_t ldt, int64_t j0, double *G, double *Tb, double *tmp,
+             double *P, int64_t pcap) {
+    int splits = gemm_pick_splits(b, b, (int)h, target_blocks(hd));
+    while (splits > 1 && (int64_t)splits * b * b > pcap) --splits;
+    GemmF64 g;
+    g.M = b; g.N = b; g.K = (int)h;
+    g.ta = true; g.amask = MASK_VLOWER; g.bmask = MASK_VLOWER; g.bmask_cols = b;
+    g.A = V; g.lda = ldv; g.B = V; g.ldb = ldv;
+    g.C = P; g.ldc = b; g.alpha = 1.0; g.beta = 0.0;
+    g.splits = splits; g.split_stride = (int64_t)b * b; g.kind = K_APPLYQ;
+    MQR_CHECK(gemm_f64(g, st));
+    MQR_CHECK(reduce_partials((int64_t)b * b, P, splits, (int64_t)b * b, st, G));
+    hipLaunchKernelGGL(k_merged_t_init, dim3((unsigned)(((int64_t)b * b + 255) / 256)), dim3(256),
+                       0, st, b, T, ldt, j0, Tb);
+    MQR_LAUNCH_CHECK();
+    return upper_inverse_levels(hd, st, b, G, b, Tb, tmp);
+}
+
+// Events of one orgqr call, destroyed on every return path
+struct LocalEvents {
+    hipEvent_t e[4] = {};
+    ~LocalEvents() {
+        for (hipEvent_t &x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+
+}  // namespace
+
+// Q = H_0 H_1 ... applied to the first nq columns of I, from the last block
+// to the first (QR/__Householder_Blocked.h:247-333), skipping the columns of
+// Q that are still zero.  With 64-column blocks, four stored blocks are
+// applied as one block reflector (merged_t), so that both products run with
+// K = 256 instead of 64; the side stream forms the merged T and Y = V T of
+// the next block while the handle's stream applies the current one (two
+// buffers, ordered by events).
 int orgqr_f64(mqr_handle_t hd, int64_t m, int64_t nq, int64_t k, const double *VR, int64_t ldvr,
               const double *T, int64_t ldt, double *Q, int64_t ldq) {
     const int nb = hd->nb;
+    hipStream_t s0 = hd->stream, s1 = hd->side_stream;
     const int64_t total = m * nq;
     if (total > 0) {
-        hipLaunchKernelGGL(k_set_identity, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                           hd->stream, m, nq, Q, ldq);
+        hipLaunchKernelGGL(k_set_identity, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s0,
+                           m, nq, Q, ldq);
         MQR_LAUNCH_CHECK();
     }
-    const int64_t pcap = (int64_t)16 * nb * (nq + nb);
-    MQR_CHECK(ws_reserve(hd, hd->ws, sizeof(double) * (pcap + (int64_t)m * nb + 1024)));
-    double *P = (double *)hd->ws.ptr;
-    double *Y = P + pcap;
     const int64_t kk = std::min(k, nq);
-    const int64_t nblk = (kk + nb - 1) / nb;
-    for (int64_t q = nblk - 1; q >= 0; --q) {
-        const int64_t j = q * nb;
-        const int b = (int)std::min<int64_t>(nb, kk - j);
-        const double *V = VR + j + j * ldvr;
-        // columns < j of Q[j:m, :] are still zero: skip them
-        MQR_CHECK(form_y(hd->stream, m - j, b, V, ldvr, T + j * ldt, ldt, false, Y));
-        MQR_CHECK(apply_block(hd, hd->stream, m - j, b, V, ldvr, Y, Q + j + j * ldq, ldq, nq - j,
-                              P, pcap, K_APPLYQ, K_APPLYQ));
+    if (nb != 64 || kk <= nb) {
+        const int64_t pcap = (int64_t)16 * nb * (nq + nb);
+        MQR_CHECK(ws_reserve(hd, hd->ws, sizeof(double) * (pcap + (int64_t)m * nb + 1024)));
+        double *P = (double *)hd->ws.ptr;
+        double *Y = P + pcap;
+        const int64_t nblk = (kk + nb - 1) / nb;
+        for (int64_t q = nblk - 1; q >= 0; --q) {
+            const int64_t j = q * nb;
+            const int b = (int)std::min<int64_t>(nb, kk - j);
+            const double *V = VR + j + j * ldvr;
+            // columns < j of Q[j:m, :] are still zero: skip them
+            MQR_CHECK(form_y(s0, m - j, b, V, ldvr, T + j * ldt, ldt, false, Y));
+            MQR_CHECK(apply_block(hd, s0, m - j, b, V, ldvr, Y, Q + j + j * ldq, ldq, nq - j, P,
+                                  pcap, K_APPLYQ, K_APPLYQ));
+        }
+        return MQR_SUCCESS;
     }
+    const int nbm = kMergeBlocks * nb;
+    const int64_t sq = (int64_t)nbm * nbm;
+    // split-K partials: a 256-row W = V^T Q over a tall, narrow Q needs up to
+    // 64 splits to fill the GPU; the side stream's G = V^T V as many
+    const int64_t pcap = std::max<int64_t>((int64_t)16 * nb * (nq + nb),
+                                           std::min<int64_t>((int64_t)64 * nbm * nq, 1 << 24));
+    const int64_t p2cap = 64 * sq;
+    MQR_CHECK(ws_reserve(hd, hd->ws, sizeof(double) * (pcap + p2cap + 2 * (int64_t)m * nbm +
+                                                       3 * sq + sq / 2 + 1024)));
+    double *P = (double *)hd->ws.ptr;
+    double *P2 = P + pcap;
+    double *Ys[2] = {P2 + p2cap, P2 + p2cap + (int64_t)m * nbm};
+    double *Tbs[2] = {Ys[1] + (int64_t)m * nbm, Ys[1] + (int64_t)m * nbm + sq};
+    double *G = Tbs[1] + sq, *tmp = G + sq;
+    LocalEvents ev;
+    for (hipEvent_t &x : ev.e) MQR_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+    hipEvent_t *ready = ev.e, *freed = ev.e + 2;
+    // the side stream runs after everything queued on the handle's stream
+    MQR_HIP(hipEventRecord(hd->ev_fork, s0));
+    MQR_HIP(hipStreamWaitEvent(s1, hd->ev_fork, 0));
+    const int64_t nblk = (kk + nbm - 1) / nbm;
+    // side stream: merged T and Y of block q into buffer q & 1, once the
+    // update of block q + 2 has read that buffer
+    auto prepare = [&](int64_t q) -> int {
+        const int64_t j = q * nbm;
+        const int b = (int)std::min<int64_t>(nbm, kk - j);
+        const double *V = VR + j + j * ldvr;
+        const int u = (int)(q & 1);
+        if (q + 2 < nblk) MQR_HIP(hipStreamWaitEvent(s1, freed[u], 0));
+        const double *Tj = T + j * ldt;
+        int64_t ldtj = ldt;
+        if (b > nb) {
+            MQR_CHECK(merged_t(hd, s1, m - j, b, V, ldvr, T, ldt, j, G, Tbs[u], tmp, P2, p2cap));
+            Tj = Tbs[u];
+            ldtj = b;
+        }
+        MQR_CHECK(form_y(s1, m - j, b, V, ldvr, Tj, ldtj, false, Ys[u]));
+        MQR_HIP(hipEventRecord(ready[u], s1));
+        return MQR_SUCCESS;
+    };
+    MQR_CHECK(prepare(nblk - 1));
+    for (int64_t q = nblk - 1; q >= 0; --q) {
+        if (q > 0) MQR_CHECK(prepare(q - 1));
+        const int64_t j = q * nbm;
+        const int b = (int)std::min<int64_t>(nbm, kk - j);
+        const int u = (int)(q & 1);
+        MQR_HIP(hipStreamWaitEvent(s0, ready[u], 0));
+        // columns < j of Q[j:m, :] are still zero: skip them
+        MQR_CHECK(apply_block(hd, s0, m - j, b, VR + j + j * ldvr, ldvr, Ys[u], Q + j + j * ldq,
+                              ldq, nq - j, P, pcap, K_APPLYQ, K_APPLYQ));
+        MQR_HIP(hipEventRecord(freed[u], s0));
+    }
+    // the side stream's last work (block 0) was waited for above
     return MQR_SUCCESS;
 }
 

@@ -505,6 +505,18 @@ int tri_inverse(mqr_handle_t hd, int64_t n, const double *R, int64_t ldr, double
                            ldr, X, n);
         MQR_LAUNCH_CHECK();
     }
+    return upper_inverse_levels(hd, s, n, R, ldr, X, tmp);
+}
+
+}  // namespace
+
+// The off-diagonal blocks of X = M^{-1} (ld n) for the upper n x n M whose
+// strictly upper 64-blocks are read from R (ld ldr), given the inverses of
+// M's diagonal 64-blocks in X's diagonal blocks (X zero below them):
+// recursive doubling over pairs of 64, 128, ... blocks on stream s.
+// tmp: n/2 x n.
+int upper_inverse_levels(mqr_handle_t hd, hipStream_t s, int64_t n, const double *R, int64_t ldr,
+                         double *X, double *tmp) {
     for (int64_t sz = 64; sz < n; sz *= 2) {
         // all full pairs of one level as one batched product each:
         // tmp = R[i:i+sz, i+sz:i+2sz] X_C,  X[i:i+sz, i+sz:i+2sz] = -X_A tmp
@@ -556,6 +568,8 @@ int tri_inverse(mqr_handle_t hd, int64_t n, const double *R, int64_t ldr, double
     }
     return MQR_SUCCESS;
 }
+
+namespace {
 
 int read_scalars(mqr_handle_t hd, const double *dev, int cnt, double *host) {
     MQR_HIP(hipMemcpyAsync(hd->host_pinned, dev, sizeof(double) * cnt, hipMemcpyDeviceToHost,

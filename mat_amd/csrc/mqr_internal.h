@@ -236,6 +236,12 @@ int panel_factor(mqr_handle_t hd, int64_t h, int b, double *V, int64_t lda, doub
 // the block reflector I - V T V^T = I - (V T) V^T.
 int form_y_f64(hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv, const double *Tp,
                int64_t ldt, bool trans, double *Y);
+// The off-diagonal blocks of X = M^{-1} (ld n) for an upper n x n M whose
+// strictly upper 64-blocks are read from R (ld ldr), given the inverses of
+// M's diagonal 64-blocks in X's diagonal blocks (X zero below them), by
+// recursive doubling on stream s.  tmp: n/2 x n doubles.
+int upper_inverse_levels(mqr_handle_t hd, hipStream_t s, int64_t n, const double *R, int64_t ldr,
+                         double *X, double *tmp);
 // out[0:count] = sum over s < splits of P[s*stride + 0:count]; out = P when null.
 int reduce_partials(int64_t count, double *P, int splits, int64_t stride, hipStream_t s,
                     double *out = nullptr);
