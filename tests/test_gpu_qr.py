@@ -305,6 +305,32 @@ def test_tall_panel_fallback_on_dependent_columns(h, orc):
     assert abs(VR[70, 70]) < 1e-12 * np.linalg.norm(A) and abs(VR[200, 200]) < 1e-12 * np.linalg.norm(A)
 
 
+@pytest.mark.parametrize("nb", [16, 32, 48])
+def test_smaller_block_sizes_match_oracle(torch_mod, orc, nb):
+    """Block widths below 64 (mqr_set_block_size): the same R and Q as the
+    oracle, and each stored T block equal to the larft recurrence of its
+    reflectors (Q formation then applies the stored blocks one by one)."""
+    note(f"block size {nb}")
+    with M.Handle(0, nb=nb) as hn:
+        for (m, n) in [(300, 200), (700, 150)]:
+            rng = np.random.default_rng(m + n + nb)
+            A = rng.standard_normal((m, n))
+            dA, tau, T = gpu_factor(hn, A)
+            dQ = M.orgqr(hn, dA, T, n, nq=n)
+            hn.synchronize()
+            VR, Q, T, tau = host(dA), host(dQ), host(T), tau.cpu().numpy()
+            Qo, Ro, _ = orc.qr(A)
+            nA = np.linalg.norm(A)
+            R = np.triu(VR)[:n]
+            np.testing.assert_allclose(R, Ro[:n], rtol=0, atol=1e-12 * nA)
+            np.testing.assert_allclose(Q, Qo[:, :n], rtol=0, atol=1e-12 * m)
+            for j in range(0, n, nb):
+                b = min(nb, n - j)
+                Tref = orc.larft(unit_lower(VR[j:, j:j + b], b), tau[j:j + b])
+                np.testing.assert_allclose(np.triu(T[:b, j:j + b]), Tref, rtol=0,
+                                           atol=64 * m * EPS * max(1.0, np.abs(Tref).max()))
+
+
 def test_short_panel_fallback_on_dependent_columns(h, orc):
     """The same hand-over in the last panels of a square matrix (64 to 128
     rows left): a zero column stays exactly zero under the earlier block
