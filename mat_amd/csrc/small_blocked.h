@@ -174,11 +174,11 @@ __device__ __forceinline__ void diag_chol(Lds *S, Lds *X, int kb, int b, Thr thr
 
 // One wave: LU without pivoting of the diagonal block kb of S - diag(sg),
 // sg chosen step by step as -sign of the current diagonal (sign(0) = +), so
-// |U_ii| >= 1.  L (unit, strictly below) and U in place; U^{-1} into the
-// block of XU, L^{-1} (unit lower, ones on the diagonal) into XL.
-__device__ __forceinline__ void diag_lu(Lds *S, Lds *XU, Lds *XL, int kb, double *sg) {
+// |U_ii| >= 1.  L (unit, strictly below) and U in place; ip[i] = 1 / U_ii
+// stays in the wave's registers for diag_inv_u.
+__device__ __forceinline__ void diag_lu(Lds *S, int kb, double *sg, double (&ip)[16]) {
     const int j = small::lane_id() & 15, o = 16 * kb;
-    double a[16], ip[16], x[16];
+    double a[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) a[i] = S[o + i][o + j];
     lu_steps<0>(a, ip, j, o, sg);
@@ -186,8 +186,14 @@ __device__ __forceinline__ void diag_lu(Lds *S, Lds *XU, Lds *XL, int kb, double
 #pragma unroll
         for (int i = 0; i < 16; ++i) S[o + i][o + j] = a[i];
     }
-    __builtin_amdgcn_wave_barrier();
-    // column j of U^{-1} (U read back as LDS broadcasts)
+}
+
+// One wave: U^{-1} of the diagonal block kb of S (after diag_lu, ip from
+// it) into the block of XU, column j by back substitution in lane j (U read
+// back as LDS broadcasts).
+__device__ __forceinline__ void diag_inv_u(const Lds *S, Lds *XU, int kb, const double (&ip)[16]) {
+    const int j = small::lane_id() & 15, o = 16 * kb;
+    double x[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) x[i] = (i == j) ? 1.0 : 0.0;
 #pragma unroll
@@ -200,7 +206,14 @@ __device__ __forceinline__ void diag_lu(Lds *S, Lds *XU, Lds *XL, int kb, double
 #pragma unroll
         for (int i = 0; i < 16; ++i) XU[o + i][o + j] = (i <= j) ? x[i] : 0.0;
     }
-    // column j of L^{-1}
+}
+
+// One wave: L^{-1} (unit lower, ones on the diagonal) of the diagonal block
+// kb of S (after diag_lu) into the block of XL, column j by forward
+// substitution in lane j.
+__device__ __forceinline__ void diag_inv_l(const Lds *S, Lds *XL, int kb) {
+    const int j = small::lane_id() & 15, o = 16 * kb;
+    double x[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) x[i] = (i == j) ? 1.0 : 0.0;
 #pragma unroll
@@ -336,7 +349,13 @@ __device__ __forceinline__ void lu_sign_inv(Lds *S, Lds *XU, Lds *XL, double *sg
     const int q = small::wave_id();
 #pragma unroll 1
     for (int kb = 0; kb < 4; ++kb) {
-        if (q == 0) diag_lu(S, XU, XL, kb, sg);
+        // wave 0 factors the diagonal block; then waves 0 and 1 invert its
+        // two factors side by side
+        double ip[16];
+        if (q == 0) diag_lu(S, kb, sg, ip);
+        __syncthreads();
+        if (q == 0) diag_inv_u(S, XU, kb, ip);
+        else if (q == 1) diag_inv_l(S, XL, kb);
         __syncthreads();
         // L tiles S[ib][kb] <- S[ib][kb] U_kk^{-1}; U tiles S[kb][jb] <- L_kk^{-1} S[kb][jb]
         const int nt = 3 - kb;
