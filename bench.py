@@ -354,15 +354,75 @@ def cpu_qr_baseline(n: int = 1024, runs: int = 5) -> dict:
                       f"{o.variant} blocking, median of {runs} runs ({t:.3f} s each)"}
 
 
+def batched_shapes(count: int = 10000, seed: int = 4321):
+    """(m, n) of the batched workload: m ~ U{16..512}, n ~ U{8..m}, seeded."""
+    rng = np.random.default_rng(seed)
+    ms = rng.integers(16, 513, count)
+    ns = np.array([rng.integers(8, m + 1) for m in ms])
+    return ms, ns
+
+
+def _cpu_factor_share(job) -> float:
+    """Seconds the oracle's blocked factorisation takes over the given shapes
+    (iid N(0,1) matrices generated before the clock starts)."""
+    shapes, seed = job
+    from oracle.oracle import Oracle
+    o = Oracle("auto")
+    rng = np.random.default_rng(seed)
+    mats = [np.asfortranarray(rng.standard_normal((m, n))) for m, n in shapes]
+    t0 = time.perf_counter()
+    for A in mats:
+        o.geqrf_blocked(A)
+    return time.perf_counter() - t0
+
+
+def cpu_batched_baseline(sample: int = 1000) -> dict:
+    """The oracle's blocked factorisation, one call per matrix (the reference's
+    QR loop), over the first `sample` matrices of the batched workload: one
+    thread, and one process per core of the box's share (at most 16) with
+    the sample dealt out round-robin, whose slowest process sets the time.
+    Call it before anything touches the GPU: the pool starts new processes."""
+    import multiprocessing as mp
+    ms, ns = batched_shapes()
+    shapes = [(int(m), int(n)) for m, n in zip(ms[:sample], ns[:sample])]
+    flops = float(sum(2.0 * m * n * n - 2.0 * n ** 3 / 3.0 for m, n in shapes))
+    t1 = _cpu_factor_share((shapes, 1))
+    procs = max(1, min(16, os.cpu_count() or 1))
+    with mp.get_context("spawn").Pool(procs) as pool:
+        tp = max(pool.map(_cpu_factor_share, [(shapes[i::procs], 2 + i) for i in range(procs)]))
+    sample_desc = (f"oracle blocked factorisation (geqrf) of the first {sample} matrices of the "
+                   f"batch (same seeded shapes, iid N(0,1)), FP64")
+    return {"single_thread": {"value": round(flops / t1 / 1e12, 5), "unit": "TFLOP/s",
+                              "cores": 1, "kind": "port", "seconds": round(t1, 3),
+                              "sample": sample_desc + ", 1 thread"},
+            "per_core": {"value": round(flops / tp / 1e12, 5), "unit": "TFLOP/s",
+                         "cores": procs, "kind": "port", "seconds": round(tp, 3),
+                         "sample": sample_desc + f", {procs} processes"}}
+
+
+def cpu_qr_large_baseline(n: int = 4096) -> dict:
+    """The oracle's blocked factorisation of one n x n iid N(0,1) matrix, one
+    thread, timed once: a bounded sample of the 8192^2 configuration (an
+    8192^2 factorisation would take 8x as long)."""
+    from oracle.oracle import Oracle
+    o = Oracle("auto")
+    A = np.asfortranarray(np.random.default_rng(12).standard_normal((n, n)))
+    t0 = time.perf_counter()
+    o.geqrf_blocked(A)
+    t = time.perf_counter() - t0
+    return {"value": round(qr_flops(n, n) / t / 1e12, 5), "unit": "TFLOP/s (factor flops)",
+            "cores": 1, "kind": "port", "seconds": round(t, 2),
+            "sample": f"oracle blocked factorisation {n}x{n} iid N(0,1), FP64, 1 thread, "
+                      f"{o.variant} blocking, one run"}
+
+
 def batched_rates(M, h, count: int = 10000, seed: int = 4321) -> list:
     """Batched uneven QR (m ~ U{16..512}, n ~ U{8..m}, iid N(0,1)) in FP64 and
     FP32: one persistent launch replayed from its hipGraph, timed with events
     on the handle's stream around the replay (inputs restored before each)."""
     import numpy as np
     import torch
-    rng = np.random.default_rng(seed)
-    ms = rng.integers(16, 513, count)
-    ns = np.array([rng.integers(8, m + 1) for m in ms])
+    ms, ns = batched_shapes(count, seed)
     flops = float(np.sum(2.0 * ms * ns * ns - 2.0 * ns ** 3 / 3.0))
     out = []
     g = torch.Generator(device="cuda").manual_seed(seed)
@@ -450,9 +510,14 @@ def main() -> int:
         print(json.dumps({"qr_adjoint": adjoint_rates(M, h)}))
         return 0
     if args.kernel == "batched":
+        # the CPU pool first: nothing has touched the GPU yet
+        cpu = None if args.no_cpu else cpu_batched_baseline()
         import mat_amd as M
         h = M.Handle(0)
-        print(json.dumps({"batched_qr": batched_rates(M, h)}))
+        rec = {"batched_qr": batched_rates(M, h)}
+        if cpu is not None:
+            rec["cpu_baseline"] = cpu
+        print(json.dumps(rec))
         return 0
     if args.kernel == "qr":
         import mat_amd as M
@@ -460,6 +525,7 @@ def main() -> int:
         rec = {"qr_fp64": qr_rates(M, h)}
         if not args.no_cpu:
             rec["cpu_baseline"] = cpu_qr_baseline()
+            rec["cpu_baseline_large"] = cpu_qr_large_baseline()
         print(json.dumps(rec))
         return 0
     if args.kernel == "gemm":
