@@ -203,6 +203,21 @@ def roofline_from(timed: dict, prof: dict, prec: str) -> dict:
                                   for k, v in prof.items()}}
 
 
+def host_cpu() -> dict:
+    """The host's CPU model and logical CPU count (from /proc/cpuinfo and
+    os.cpu_count()), stated beside every CPU baseline."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
+
+
 def cpu_baseline(seconds: float = 20.0) -> dict:
     """The oracle's blocked Householder least squares (single thread, the
     reference's AVX-512 or AVX2 blocking), timed on a bounded sample."""
@@ -226,7 +241,7 @@ def cpu_baseline(seconds: float = 20.0) -> dict:
             "sample": f"oracle blocked Householder LS {m}x{n} iid N(0,1), FP64, 1 thread, "
                       f"{'AVX-512 blocking (BS 48, 16x12)' if o.variant == 'avx512' else 'AVX2 blocking (BS 96, 8x6)'}, "
                       f"median of {len(times)} runs ({t:.2f} s each)",
-            "cpu_avx512": bool(cpu_has_avx512())}
+            "cpu_avx512": bool(cpu_has_avx512()), "host": host_cpu()}
 
 
 def gemm_rates(M, h) -> list:
@@ -351,7 +366,8 @@ def cpu_qr_baseline(n: int = 1024, runs: int = 5) -> dict:
     return {"value": round(qr_flops(n, n) / t / 1e12, 5), "unit": "TFLOP/s (factor flops)",
             "cores": 1, "kind": "port",
             "sample": f"oracle QR(A, Q, R) {n}x{n} iid N(0,1), factor + full Q, FP64, 1 thread, "
-                      f"{o.variant} blocking, median of {runs} runs ({t:.3f} s each)"}
+                      f"{o.variant} blocking, median of {runs} runs ({t:.3f} s each)",
+            "host": host_cpu()}
 
 
 def batched_shapes(count: int = 10000, seed: int = 4321):
@@ -397,7 +413,8 @@ def cpu_batched_baseline(sample: int = 1000) -> dict:
                               "sample": sample_desc + ", 1 thread"},
             "per_core": {"value": round(flops / tp / 1e12, 5), "unit": "TFLOP/s",
                          "cores": procs, "kind": "port", "seconds": round(tp, 3),
-                         "sample": sample_desc + f", {procs} processes"}}
+                         "sample": sample_desc + f", {procs} processes"},
+            "host": host_cpu()}
 
 
 def cpu_qr_large_baseline(n: int = 4096) -> dict:
@@ -413,7 +430,7 @@ def cpu_qr_large_baseline(n: int = 4096) -> dict:
     return {"value": round(qr_flops(n, n) / t / 1e12, 5), "unit": "TFLOP/s (factor flops)",
             "cores": 1, "kind": "port", "seconds": round(t, 2),
             "sample": f"oracle blocked factorisation {n}x{n} iid N(0,1), FP64, 1 thread, "
-                      f"{o.variant} blocking, one run"}
+                      f"{o.variant} blocking, one run", "host": host_cpu()}
 
 
 def batched_rates(M, h, count: int = 10000, seed: int = 4321) -> list:
