@@ -255,3 +255,16 @@ def test_mixed_bf16_refinement(h, orc, prec, cond, converges):
     assert (info.converged == 1) == converges
     assert info.fell_back == (0 if converges else 1)
     assert ls_backward_error(A, b, x)["kw"] <= 1e-14
+
+
+def test_mixed_nonfinite_input_falls_back(h):
+    """A NaN in A: refinement stops at the first non-finite estimate instead of
+    iterating, the FP64 solve takes over, and its result carries the NaN as
+    the reference's arithmetic would."""
+    import torch
+    A, b = ls_problem(600, 150, 1e3, seed=21)
+    A[5, 7] = np.nan
+    x, info = M.gels_mixed(h, "fp32", M.colmajor(A), torch.from_numpy(b).cuda())
+    h.synchronize()
+    assert info.converged == 0 and info.fell_back == 1
+    assert not np.all(np.isfinite(x.cpu().numpy()))

@@ -385,3 +385,25 @@ def test_argument_errors_are_lapack_style(h):
     assert L.mqr_set_block_size(h.h, 24) == -2
     assert L.mqr_dorgqr(h.h, 4, 5, 4, M._ptr(A), 4, M._ptr(A), 64, M._ptr(A), 4) == -3
     assert L.mqr_dgels(h.h, 3, 4, 1, M._ptr(A), 3, M._ptr(A), 3) == -3
+
+
+@pytest.mark.parametrize("bad", [np.nan, np.inf])
+def test_nonfinite_entry_propagates_like_the_reference(h, orc, bad):
+    """A NaN or Inf in column 70 (the reference checks nothing and lets it run
+    through its arithmetic): the factorisation of the first 70 columns is
+    untouched, R and tau there equal the oracle's, and the value reaches R
+    from row 70 on in the later columns.  The fast panel holding it hands over
+    to the column-by-column factorisation."""
+    note(f"non-finite entry {bad}")
+    rng = np.random.default_rng(13)
+    m, n = 300, 200
+    A = rng.standard_normal((m, n))
+    A[10, 70] = bad
+    dA, tau, T = gpu_factor(h, A)
+    VR, tau = host(dA), tau.cpu().numpy()
+    R = np.triu(VR)[:n]
+    assert np.all(np.isfinite(R[:, :70])) and np.all(np.isfinite(tau[:70]))
+    assert not np.all(np.isfinite(R[70:, 70:]))
+    _, Ro, _ = orc.qr(A[:, :70])
+    np.testing.assert_allclose(R[:70, :70], np.triu(Ro)[:70], rtol=0,
+                               atol=1e-12 * np.linalg.norm(A[:, :70]))
