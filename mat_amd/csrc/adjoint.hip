@@ -70,14 +70,33 @@ __global__ void k_triu(int64_t n, const T *__restrict__ X, int64_t ldx, int64_t 
     }
 }
 
-// M_b <- copyltu(M_b) = tril(M_b) + tril(M_b, -1)^T (n x n, ld n), in place
+// M_b <- copyltu(M_b) = tril(M_b) + tril(M_b, -1)^T (n x n, ld n), in place,
+// by 32 x 32 tiles through LDS: workgroup t of grid.x takes the tile at
+// block row I >= block column J (t = I (I + 1) / 2 + J), reads it with
+// column-contiguous loads and writes its transpose into tile (J, I) above
+// the diagonal, so both sides move whole column pieces.  Only tiles on or
+// below the diagonal are read and only entries above it written, so the
+// workgroups never touch each other's data.
+constexpr int CL_T = 32;
 template <typename T>
-__global__ void k_copyltu(int64_t n, T *__restrict__ M, int64_t sm) {
-    const int64_t b = blockIdx.y;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * n;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = e % n, c = e / n;
-        if (r < c) M[b * sm + r + c * n] = M[b * sm + c + r * n];
+__global__ __launch_bounds__(256) void k_copyltu(int64_t n, T *__restrict__ M, int64_t sm) {
+    __shared__ T tile[CL_T][CL_T + 1];  // tile[local column][local row]
+    const int64_t t = blockIdx.x;
+    int64_t I = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while (I * (I + 1) / 2 > t) --I;
+    while ((I + 1) * (I + 2) / 2 <= t) ++I;
+    const int64_t J = t - I * (I + 1) / 2;
+    T *Mb = M + (int64_t)blockIdx.y * sm;
+    const int tx = threadIdx.x & (CL_T - 1), ty = threadIdx.x / CL_T;  // 32 x 8
+    for (int k = 0; k < CL_T; k += 256 / CL_T) {
+        const int64_t r = I * CL_T + tx, c = J * CL_T + ty + k;
+        if (r < n && c < n) tile[ty + k][tx] = Mb[r + c * n];
+    }
+    __syncthreads();
+    // (r2, c2) above the diagonal takes M(c2, r2) = tile[r2 - J 32][c2 - I 32]
+    for (int k = 0; k < CL_T; k += 256 / CL_T) {
+        const int64_t r2 = J * CL_T + tx, c2 = I * CL_T + ty + k;
+        if (r2 < n && c2 < n && r2 < c2) Mb[r2 + c2 * n] = tile[tx][ty + k];
     }
 }
 
@@ -152,7 +171,9 @@ int qr_backward_t(mqr_handle_t hd, int64_t batch, int64_t m, int64_t n, const T 
                         1.0, Mw, n, sM, B, K_GEMM, s));
     {
         ProfScope ps(K_OTHER, 0.0, 8.0 * batch * n * n, s);
-        hipLaunchKernelGGL(k_copyltu<T>, dim3(gx, B), dim3(256), 0, s, n, Mw, sM);
+        const int64_t nt = (n + CL_T - 1) / CL_T;
+        hipLaunchKernelGGL(k_copyltu<T>, dim3((unsigned)(nt * (nt + 1) / 2), B), dim3(256), 0, s,
+                           n, Mw, sM);
         hipLaunchKernelGGL(k_inv_diag_batched<T>, dim3(nblk, B), dim3(small::NT), 0, s, n, R, ldr,
                            sr, Iw);
         MQR_LAUNCH_CHECK();
