@@ -159,13 +159,19 @@ int mqr_dgeqrf(mqr_handle_t handle, int64_t m, int64_t n, double *A, int64_t lda
 /* Q(:, 0:nq) = H_0 ... H_{k-1} applied to the first nq columns of I_m
  * (nq = k: thin Q, nq = m: full Q), from a factorisation by mqr_dgeqrf.
  * Replaces __QR_HouseHolder_Blocked_CalculateQ (QR/__Householder_Blocked.h:247-333);
- * the known-zero part of Q is skipped. */
+ * the known-zero part of Q is skipped.
+ * T holds one block per nb columns, with the handle's block size nb at the
+ * time of mqr_dgeqrf: keep nb unchanged between the factorisation and the Q
+ * formation or application.  A T that one of the handle's last 16
+ * factorisations wrote with another nb is rejected (-7 here, -8 for
+ * mqr_dormqr). */
 int mqr_dorgqr(mqr_handle_t handle, int64_t m, int64_t nq, int64_t k, const double *VR,
                int64_t ldvr, const double *T, int64_t ldt, double *Q, int64_t ldq);
 
 /* C <- Q^T C (trans = 'T') or Q C (trans = 'N') for the m x nrhs matrix C,
  * with Q held as k reflectors by mqr_dgeqrf.  The block-reflector pattern of
- * QR/__Householder_Blocked.h:143-212 with C as the right-hand side. */
+ * QR/__Householder_Blocked.h:143-212 with C as the right-hand side; T as
+ * for mqr_dorgqr (same block size as the factorisation). */
 int mqr_dormqr(mqr_handle_t handle, char trans, int64_t m, int64_t nrhs, int64_t k,
                const double *VR, int64_t ldvr, const double *T, int64_t ldt, double *C,
                int64_t ldc);

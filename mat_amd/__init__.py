@@ -332,11 +332,26 @@ def gels_mixed(h: Handle, prec: str, A, b, max_iter: int = 30, tol: float = 0.0,
     return x, info
 
 
-def _batch_args(mats, taus):
+def _batch_args(mats, taus, device: int):
     import numpy as np
     import torch
     b = len(mats)
     f64 = mats[0].dtype == torch.float64 if b else True
+    # the kernel indexes every buffer with the batch's one element type: a
+    # mixed list, a matrix on another device or a row-major view would be
+    # read and written out of bounds
+    for i, (t, tau) in enumerate(zip(mats, taus)):
+        if t.dtype not in (torch.float64, torch.float32) or t.dtype != mats[0].dtype:
+            raise TypeError(f"matrix {i}: dtype {t.dtype}, expected {mats[0].dtype} like matrix 0")
+        if tau.dtype != t.dtype:
+            raise TypeError(f"tau {i}: dtype {tau.dtype}, expected {t.dtype}")
+        if (t.device.type != "cuda" or t.device.index != device or tau.device != t.device):
+            raise ValueError(f"matrix {i}: on {t.device} (tau on {tau.device}), expected the "
+                             f"handle's device cuda:{device}")
+        if t.dim() != 2 or (t.shape[0] > 1 and t.shape[1] > 1 and t.stride(0) != 1):
+            raise ValueError(f"matrix {i}: expected a column-major (m, n) tensor")
+        if tau.numel() < min(t.shape):
+            raise ValueError(f"tau {i}: {tau.numel()} entries, expected at least {min(t.shape)}")
     m = np.array([t.shape[0] for t in mats], dtype=np.int32)
     n = np.array([t.shape[1] for t in mats], dtype=np.int32)
     lda = np.array([ld(t) for t in mats], dtype=np.int64)
@@ -353,7 +368,7 @@ def geqrf_batched(h: Handle, mats, tile: int = -1):
     taus = [torch.zeros(max(min(t.shape), 1), device=t.device, dtype=t.dtype) for t in mats]
     if not mats:
         return taus
-    dt, b, (m, n, lda, A, T) = _batch_args(mats, taus)
+    dt, b, (m, n, lda, A, T) = _batch_args(mats, taus, h.device)
     check(lib().mqr_geqrf_batched_uneven(h.h, dt, b, m.ctypes.data_as(C.c_void_p),
                                          n.ctypes.data_as(C.c_void_p),
                                          lda.ctypes.data_as(C.c_void_p), C.cast(A, C.c_void_p),
@@ -372,7 +387,7 @@ class BatchedPlan:
         self.taus = [torch.zeros(max(min(t.shape), 1), device=t.device, dtype=t.dtype)
                      for t in mats]
         self.plan = C.c_void_p()
-        dt, b, (m, n, lda, A, T) = _batch_args(mats, self.taus)
+        dt, b, (m, n, lda, A, T) = _batch_args(mats, self.taus, h.device)
         check(lib().mqr_batched_plan_create(h.h, dt, b, m.ctypes.data_as(C.c_void_p),
                                             n.ctypes.data_as(C.c_void_p),
                                             lda.ctypes.data_as(C.c_void_p),

@@ -356,6 +356,28 @@ int mqr_sgemm_bf16(mqr_handle_t h, int terms, char transa, char transb, int64_t 
     return st < 0 ? st - 1 : st;  // argument positions shift by one
 }
 
+} // extern "C"
+
+// Records the block size T is written with; a later mqr_dorgqr / mqr_dormqr
+// reading the same T with another block size is rejected.
+static void remember_t_layout(mqr_handle_t h, const double *T) {
+    for (auto &e : h->t_layout)
+        if (e.T == T) {
+            e.nb = h->nb;
+            return;
+        }
+    h->t_layout[h->t_layout_next] = {T, h->nb};
+    h->t_layout_next = (h->t_layout_next + 1) % (int)(sizeof(h->t_layout) / sizeof(h->t_layout[0]));
+}
+
+static bool t_layout_mismatch(mqr_handle_t h, const double *T) {
+    for (const auto &e : h->t_layout)
+        if (e.T == T) return e.nb != h->nb;
+    return false;
+}
+
+extern "C" {
+
 int mqr_dgeqrf(mqr_handle_t h, int64_t m, int64_t n, double *A, int64_t lda, double *tau,
                double *T, int64_t ldt) {
     if (!h) return -1;
@@ -371,6 +393,7 @@ int mqr_dgeqrf(mqr_handle_t h, int64_t m, int64_t n, double *A, int64_t lda, dou
     if (m > INT32_MAX || n > INT32_MAX) return MQR_ERR_UNSUPPORTED;
     MQR_GUARD(h);
     MQR_BRIDGE(h);
+    remember_t_layout(h, T);
     return geqrf_f64(h, m, n, A, lda, tau, T, ldt);
 }
 
@@ -384,6 +407,7 @@ int mqr_dorgqr(mqr_handle_t h, int64_t m, int64_t nq, int64_t k, const double *V
     if (ldvr < std::max<int64_t>(1, m)) return -6;
     if (k > 0 && !T) return -7;
     if (ldt < std::min<int64_t>(h->nb, std::max<int64_t>(k, 1))) return -8;
+    if (k > 0 && t_layout_mismatch(h, T)) return -7;
     if (m * nq > 0 && !Q) return -9;
     if (ldq < std::max<int64_t>(1, m)) return -10;
     if (m * nq == 0) return MQR_SUCCESS;
@@ -404,6 +428,7 @@ int mqr_dormqr(mqr_handle_t h, char trans, int64_t m, int64_t nrhs, int64_t k, c
     if (ldvr < std::max<int64_t>(1, m)) return -7;
     if (k > 0 && !T) return -8;
     if (ldt < std::min<int64_t>(h->nb, std::max<int64_t>(k, 1))) return -9;
+    if (k > 0 && t_layout_mismatch(h, T)) return -8;
     if (m * nrhs > 0 && !C) return -10;
     if (ldc < std::max<int64_t>(1, m)) return -11;
     if (m * nrhs == 0 || k == 0) return MQR_SUCCESS;

@@ -139,6 +139,8 @@ int form_y(hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv, cons
 // holds the split-K partials.
 int block_w(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv,
             const double *C, int64_t ldc, int64_t nc, double *P, int64_t pcap, int kind_w) {
+    // W itself (splits = 1) must fit the partial buffer; callers chunk C
+    if ((int64_t)b * nc > pcap) return MQR_ERR_INTERNAL;
     int splits = gemm_pick_splits(b, (int)nc, (int)h, target_blocks(hd));
     while (splits > 1 && (int64_t)splits * b * nc > pcap) --splits;
     GemmF64 g;
@@ -155,12 +157,21 @@ int apply_block(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double 
                 const double *Y, double *C, int64_t ldc, int64_t nc, double *P, int64_t pcap,
                 int kind_w = K_VTA, int kind_u = K_UPDATE) {
     if (nc <= 0) return MQR_SUCCESS;
-    MQR_CHECK(block_w(hd, st, h, b, Vp, ldv, C, ldc, nc, P, pcap, kind_w));
-    GemmF64 u;
-    u.M = (int)h; u.N = (int)nc; u.K = b;
-    u.A = Y; u.lda = h; u.B = P; u.ldb = b;
-    u.C = C; u.ldc = ldc; u.alpha = -1.0; u.beta = 1.0; u.kind = kind_u;
-    return gemm_f64(u, st);
+    // column chunks whose W (b x chunk) fits the partial buffer P, so that a
+    // right-hand side wider than the matrix (least squares with many
+    // right-hand sides) never writes past P
+    const int64_t chunk = std::max<int64_t>(1, pcap / b);
+    for (int64_t c0 = 0; c0 < nc; c0 += chunk) {
+        const int64_t ncc = std::min<int64_t>(chunk, nc - c0);
+        double *Cc = C + c0 * ldc;
+        MQR_CHECK(block_w(hd, st, h, b, Vp, ldv, Cc, ldc, ncc, P, pcap, kind_w));
+        GemmF64 u;
+        u.M = (int)h; u.N = (int)ncc; u.K = b;
+        u.A = Y; u.lda = h; u.B = P; u.ldb = b;
+        u.C = Cc; u.ldc = ldc; u.alpha = -1.0; u.beta = 1.0; u.kind = kind_u;
+        MQR_CHECK(gemm_f64(u, st));
+    }
+    return MQR_SUCCESS;
 }
 
 }  // namespace

@@ -96,6 +96,14 @@ struct mqr_handle_st {
     hipStream_t side_stream = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int nb = 64;                   // factorisation block size
+    // block size each recent mqr_dgeqrf wrote its T blocks with (keyed by
+    // the T pointer), so that a Q application after mqr_set_block_size is
+    // rejected instead of reading T with the wrong block layout
+    struct TLayout {
+        const void *T = nullptr;
+        int nb = 0;
+    } t_layout[16];
+    int t_layout_next = 0;
     int cu_count = 256;
     mqr::Workspace ws;             // general scratch
     mqr::Workspace ws_ir;          // mixed-precision / refinement scratch

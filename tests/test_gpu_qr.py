@@ -407,3 +407,24 @@ def test_nonfinite_entry_propagates_like_the_reference(h, orc, bad):
     _, Ro, _ = orc.qr(A[:, :70])
     np.testing.assert_allclose(R[:70, :70], np.triu(Ro)[:70], rtol=0,
                                atol=1e-12 * np.linalg.norm(A[:, :70]))
+
+
+def test_t_blocks_rejected_after_block_size_change(torch_mod):
+    """T is laid out for the block size of its factorisation: forming or
+    applying Q after mqr_set_block_size changed it is refused (LAPACK-style
+    position of T), and accepted again once the block size is restored."""
+    rng = np.random.default_rng(14)
+    A = rng.standard_normal((200, 150))
+    with M.Handle(0, nb=64) as hn:
+        dA, tau, T = gpu_factor(hn, A)
+        L = M.lib()
+        Q = M.empty_colmajor(200, 150)
+        M.check(L.mqr_set_block_size(hn.h, 32), "mqr_set_block_size")
+        assert L.mqr_dorgqr(hn.h, 200, 150, 150, M._ptr(dA), 200, M._ptr(T), M.ld(T),
+                            M._ptr(Q), 200) == -7
+        assert L.mqr_dormqr(hn.h, b"T", 200, 150, 150, M._ptr(dA), 200, M._ptr(T), M.ld(T),
+                            M._ptr(Q), 200) == -8
+        M.check(L.mqr_set_block_size(hn.h, 64), "mqr_set_block_size")
+        Q = M.orgqr(hn, dA, T, 150, nq=150)
+        hn.synchronize()
+        assert np.linalg.norm(host(Q) @ np.triu(host(dA))[:150] - A) < 1e-12 * np.linalg.norm(A)
