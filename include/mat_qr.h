@@ -48,6 +48,7 @@ extern "C" {
 #define MQR_ERR_UNSUPPORTED 1005   /* valid request this build does not support */
 #define MQR_ERR_NOT_INITIALIZED 1006
 #define MQR_ERR_INTERNAL 1007
+#define MQR_ERR_TOO_LARGE 1008     /* request exceeds a fixed-size device scratch array */
 
 typedef struct mqr_handle_st *mqr_handle_t;
 
@@ -230,6 +231,21 @@ int mqr_gels_mixed(mqr_handle_t handle, mqr_precision_t prec, int64_t m, int64_t
 int mqr_geqrf_batched_uneven(mqr_handle_t handle, mqr_dtype_t dtype, int64_t batch,
                              const int32_t *m, const int32_t *n, const int64_t *lda,
                              void *const *A, void *const *tau, int tile);
+
+/* Allocation-free variant of mqr_geqrf_batched_uneven.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and blocking behaviour; the kernel's
+ * descriptor queue and counters live in a fixed device array of the library
+ * (about 1.3 MB per GPU, held until the process ends) instead of per-call
+ * allocations, and no hipMalloc, hipFree or host heap allocation is made.
+ * batch <= MQR_NOALLOC_MAX_BATCH, else MQR_ERR_TOO_LARGE before any work.
+ * Every call on one GPU shares that array: calls are serialised within a
+ * process, and calls from several processes that overlap on one GPU race. */
+#define MQR_NOALLOC_MAX_BATCH 32768
+int mqr_geqrf_batched_uneven_noalloc(mqr_handle_t handle, mqr_dtype_t dtype, int64_t batch,
+                                     const int32_t *m, const int32_t *n, const int64_t *lda,
+                                     void *const *A, void *const *tau, int tile);
 
 /* The same work captured once in a hipGraph: create uploads the descriptors
  * (the device pointers are fixed from then on) and instantiates the graph;

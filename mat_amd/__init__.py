@@ -31,6 +31,8 @@ MQR_ERR_NOT_CONVERGED = 1004
 MQR_ERR_UNSUPPORTED = 1005
 MQR_ERR_NOT_INITIALIZED = 1006
 MQR_ERR_INTERNAL = 1007
+MQR_ERR_TOO_LARGE = 1008
+NOALLOC_MAX_BATCH = 32768
 
 MQR_F64, MQR_F32 = 0, 1
 PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2, "bf16": 3}
@@ -67,6 +69,7 @@ SIGNATURES = {
     "mqr_dgels": (_i32, [_vp, _i64, _i64, _i64, _vp, _i64, _vp, _i64]),
     "mqr_gels_mixed": (_i32, [_vp, _i32, _i64, _i64, _vp, _i64, _vp, _vp, _vp, _vp]),
     "mqr_geqrf_batched_uneven": (_i32, [_vp, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _i32]),
+    "mqr_geqrf_batched_uneven_noalloc": (_i32, [_vp, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _i32]),
     "mqr_batched_plan_create": (_i32, [_vp, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _i32, _vp]),
     "mqr_batched_plan_run": (_i32, [_vp, _vp]),
     "mqr_batched_plan_destroy": (_i32, [_vp]),
@@ -373,6 +376,20 @@ def geqrf_batched(h: Handle, mats, tile: int = -1):
                                          n.ctypes.data_as(C.c_void_p),
                                          lda.ctypes.data_as(C.c_void_p), C.cast(A, C.c_void_p),
                                          C.cast(T, C.c_void_p), tile), "mqr_geqrf_batched_uneven")
+    return taus
+
+
+def geqrf_batched_noalloc(h: Handle, mats, taus, tile: int = -1):
+    """The same factorisation through mqr_geqrf_batched_uneven_noalloc: the
+    caller supplies the tau vectors (nothing is allocated on the device)."""
+    if not mats:
+        return taus
+    dt, b, (m, n, lda, A, T) = _batch_args(mats, taus, h.device)
+    check(lib().mqr_geqrf_batched_uneven_noalloc(h.h, dt, b, m.ctypes.data_as(C.c_void_p),
+                                                 n.ctypes.data_as(C.c_void_p),
+                                                 lda.ctypes.data_as(C.c_void_p),
+                                                 C.cast(A, C.c_void_p), C.cast(T, C.c_void_p),
+                                                 tile), "mqr_geqrf_batched_uneven_noalloc")
     return taus
 
 

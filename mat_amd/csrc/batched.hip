@@ -30,6 +30,7 @@
 //             the chunk is read once and written once (:143-212).
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <utility>
 #include <vector>
@@ -637,5 +638,115 @@ void batched_plan_destroy(mqr_batched_plan_st *pl) {
 }
 
 int batched_plan_device(const mqr_batched_plan_st *pl) { return pl->device; }
+
+}  // namespace mqr
+
+// ---- allocation-free variant ----------------------------------------------
+// mqr_geqrf_batched_uneven_noalloc runs the same persistent kernel with its
+// descriptor queue and counters in the fixed device array below instead of
+// per-call allocations.  Every call on a GPU shares that GPU's copy of the
+// array, so calls that overlap on one GPU race; a process-wide mutex
+// serialises the calls of one process (each call blocks until its kernel is
+// done), and each call clears the counters before its launch.
+namespace mqr {
+namespace {
+
+constexpr size_t kNoAllocDescBytes = sizeof(BatchDesc) * (size_t)MQR_NOALLOC_MAX_BATCH;
+constexpr size_t kNoAllocBytes = kNoAllocDescBytes + 256;  // + the two queue counters
+constexpr int kMaxDevices = 64;
+
+// One copy per GPU, alive until the process ends.  Host code never passes
+// this name to a kernel or a copy: it uses the per-device address that
+// hipGetSymbolAddress returned (g_scratch_ptr).
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+struct NoAllocDevice {
+    unsigned char *ptr = nullptr;  // device address of `scratch` on this GPU
+    unsigned grid64 = 0, grid32 = 0;
+};
+NoAllocDevice g_scratch_ptr[kMaxDevices];
+bool g_scratch_ready[kMaxDevices] = {};
+std::mutex g_noalloc_mutex;
+// host staging of the descriptors (static: no allocation at call time)
+BatchDesc g_stage[MQR_NOALLOC_MAX_BATCH];
+
+__global__ void k_clear_counters(int *c) {
+    if (threadIdx.x < 2) c[threadIdx.x] = 0;
+}
+
+}  // namespace
+
+// Looks up this device's address of the scratch array (the device must be
+// current) and the persistent grids; once per device, from mqr_create.
+int batched_noalloc_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return MQR_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (g_scratch_ready[device]) return MQR_SUCCESS;
+    void *p = nullptr;
+    MQR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return MQR_ERR_INTERNAL;
+    int cus = 0, pc64 = 0, pc32 = 0;
+    MQR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    MQR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc64, (const void *)k_qr_batched<double>,
+                                                         BNT, 0));
+    MQR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc32, (const void *)k_qr_batched<float>,
+                                                         BNT, 0));
+    NoAllocDevice &d = g_scratch_ptr[device];
+    d.ptr = (unsigned char *)p;
+    d.grid64 = (unsigned)(std::max(pc64, 1) * std::max(cus, 1));
+    d.grid32 = (unsigned)(std::max(pc32, 1) * std::max(cus, 1));
+    g_scratch_ready[device] = true;
+    return MQR_SUCCESS;
+}
+
+// The batched factorisation without any device or host allocation: the
+// descriptors are built in static host memory, copied into this GPU's
+// scratch, the counters cleared by a kernel, and the persistent kernel run;
+// returns once it is done.  batch <= MQR_NOALLOC_MAX_BATCH (checked by the
+// caller).
+int geqrf_batched_noalloc(mqr_handle_t hd, bool f64, int64_t batch, const int32_t *m,
+                          const int32_t *n, const int64_t *lda, void *const *A, void *const *tau,
+                          int tile) {
+    if (batch > MQR_NOALLOC_MAX_BATCH) return MQR_ERR_TOO_LARGE;
+    if (hd->device < 0 || hd->device >= kMaxDevices) return MQR_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (!g_scratch_ready[hd->device]) return MQR_ERR_NOT_INITIALIZED;
+    const NoAllocDevice &d = g_scratch_ptr[hd->device];
+    for (int64_t i = 0; i < batch; ++i) {
+        BatchDesc &e = g_stage[i];
+        e.A = A[i];
+        e.tau = tau[i];
+        e.lda = lda[i];
+        e.m = m[i];
+        e.n = n[i];
+        e.nb = batched_nb(f64, m[i], tile);
+        e.pad = 0;
+    }
+    // longest first, by the same flop count as the allocating path (the
+    // order only changes which workgroup takes a matrix, not its factors)
+    auto cost = [](const BatchDesc &x) {
+        const double mm = x.m, nn = std::min(x.m, x.n);
+        return 2.0 * mm * nn * x.n - 2.0 * nn * nn * nn / 3.0 + mm;
+    };
+    std::sort(g_stage, g_stage + batch,
+              [&](const BatchDesc &a, const BatchDesc &b) { return cost(a) > cost(b); });
+    BatchDesc *ddesc = (BatchDesc *)d.ptr;
+    int *dcount = (int *)(d.ptr + kNoAllocDescBytes);
+    MQR_HIP(hipMemcpyAsync(ddesc, g_stage, sizeof(BatchDesc) * (size_t)batch,
+                           hipMemcpyHostToDevice, hd->stream));
+    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(64), 0, hd->stream, dcount);
+    MQR_LAUNCH_CHECK();
+    const unsigned grid = (unsigned)std::min<int64_t>(batch, f64 ? d.grid64 : d.grid32);
+    if (f64)
+        hipLaunchKernelGGL(k_qr_batched<double>, dim3(grid), dim3(BNT), 0, hd->stream,
+                           (const BatchDesc *)ddesc, (int)batch, dcount);
+    else
+        hipLaunchKernelGGL(k_qr_batched<float>, dim3(grid), dim3(BNT), 0, hd->stream,
+                           (const BatchDesc *)ddesc, (int)batch, dcount);
+    MQR_LAUNCH_CHECK();
+    // the staging array and the scratch are reused by the next call
+    MQR_HIP(hipStreamSynchronize(hd->stream));
+    return MQR_SUCCESS;
+}
 
 }  // namespace mqr

@@ -29,6 +29,10 @@ int qr_backward(mqr_handle_t hd, bool f64, int64_t batch, int64_t m, int64_t n, 
                 int64_t ldq, int64_t sq, const void *R, int64_t ldr, int64_t sr, const void *dQ,
                 int64_t lddq, int64_t sdq, const void *dR, int64_t lddr, int64_t sdr, void *dA,
                 int64_t ldda, int64_t sda);
+int batched_noalloc_init(int device);
+int geqrf_batched_noalloc(mqr_handle_t hd, bool f64, int64_t batch, const int32_t *m,
+                          const int32_t *n, const int64_t *lda, void *const *A, void *const *tau,
+                          int tile);
 int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, const double *A,
                int64_t lda, const double *b, double *x, const mqr_ir_opts_t *opts,
                mqr_ir_info_t *info);
@@ -124,6 +128,9 @@ const char *mqr_strerror(int status) {
         case MQR_ERR_UNSUPPORTED: return "request not supported by this build";
         case MQR_ERR_NOT_INITIALIZED: return "handle not initialised";
         case MQR_ERR_INTERNAL: return "internal error";
+        case MQR_ERR_TOO_LARGE:
+            return "request exceeds the fixed device scratch of the allocation-free entry point "
+                   "(at most MQR_NOALLOC_MAX_BATCH = 32768 matrices)";
         default: return "unknown status";
     }
 }
@@ -185,6 +192,16 @@ int mqr_create(mqr_handle_t *handle, int device) {
         (void)hipStreamDestroy(h->own_stream);
         delete h;
         return MQR_ERR_ALLOC;
+    }
+    // this device's address of the allocation-free entry point's scratch
+    if (batched_noalloc_init(device) != MQR_SUCCESS) {
+        (void)hipGetLastError();
+        (void)hipHostFree(h->host_pinned);
+        for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out}) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(h->side_stream);
+        (void)hipStreamDestroy(h->own_stream);
+        delete h;
+        return MQR_ERR_HIP;
     }
     *handle = h;
     return MQR_SUCCESS;
@@ -523,6 +540,19 @@ int mqr_geqrf_batched_uneven(mqr_handle_t h, mqr_dtype_t dtype, int64_t batch, c
     MQR_GUARD(h);
     MQR_BRIDGE(h);
     return geqrf_batched(h, dtype == MQR_F64, batch, m, n, lda, A, tau, tile);
+}
+
+int mqr_geqrf_batched_uneven_noalloc(mqr_handle_t h, mqr_dtype_t dtype, int64_t batch,
+                                     const int32_t *m, const int32_t *n, const int64_t *lda,
+                                     void *const *A, void *const *tau, int tile) {
+    if (!h) return -1;
+    if (batch > MQR_NOALLOC_MAX_BATCH) return MQR_ERR_TOO_LARGE;
+    const int a = check_batched(h, dtype, batch, m, n, lda, A, tau, tile);
+    if (a) return a;
+    if (batch == 0) return MQR_SUCCESS;
+    MQR_GUARD(h);
+    MQR_BRIDGE(h);
+    return geqrf_batched_noalloc(h, dtype == MQR_F64, batch, m, n, lda, A, tau, tile);
 }
 
 int mqr_batched_plan_create(mqr_handle_t h, mqr_dtype_t dtype, int64_t batch, const int32_t *m,
