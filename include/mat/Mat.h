@@ -7,6 +7,10 @@
 //   QR(A, Q, R, QR_Householder) QR/QR.h:7-46
 //   QR<QR_Householder>(A, Q, R) QR/QR.h:48-71
 //   LeastSquares(A, b, x)       no reference counterpart (new on the same path)
+//   operator*(MAT, MAT)         Mul/Mul.h:205-235, on the GPU (MFMA GEMM)
+//   _T, _H, scalar * / + -      Base/__Matrix.h:177-259 (host)
+//   __BlockMat<T>               Base/__Block.h:2-47 (host view)
+//   load / save / size          Mat-5.2/_extra/__File.h:1-66 (binary MAT files)
 //
 // Every Householder QR call runs on the GPU through the C ABI of
 // include/mat_qr.h (libmatqr.so): A is copied to device memory, factored in
@@ -31,6 +35,7 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../mat_qr.h"
@@ -124,6 +129,29 @@ public:
     }
     void fill(T a) { std::fill(data, data + (size_t)rows * cols, a); }
 
+    // zero matrix with a on the diagonal (Base/__Matrix.h:97-103): the
+    // reference writes rows entries; min(rows, cols) are written here, so
+    // a non-square matrix is not written past its end
+    void diagonal(T *a) {
+        std::fill(data, data + (size_t)rows * cols, T(0));
+        for (int i = 0; i < std::min(rows, cols); ++i) data[(size_t)i * rows + i] = a[i];
+    }
+    // size x size, a on the diagonal, converted to T (:104-112)
+    template <typename t>
+    void diagonal(t *a, int size) {
+        init(size, size);
+        std::fill(data, data + (size_t)size * size, T(0));
+        for (int i = 0; i < size; ++i) data[(size_t)i * rows + i] = (T)a[i];
+    }
+    // exchange two rows / two columns in place (:117-135)
+    void swap_rows(int r1, int r2) {
+        for (int c = 0; c < cols; ++c) std::swap(data[(size_t)c * rows + r1], data[(size_t)c * rows + r2]);
+    }
+    void swap_cols(int c1, int c2) {
+        std::swap_ranges(data + (size_t)rows * c1, data + (size_t)rows * (c1 + 1),
+                         data + (size_t)rows * c2);
+    }
+
     T *operator[](int c) const { return data + (size_t)rows * c; }
     T &operator()(int r, int c) const { return data[(size_t)rows * c + r]; }
 
@@ -138,11 +166,85 @@ public:
         std::swap(data, B.data);
     }
 
-    MAT transpose() const {
-        MAT t(cols, rows);
-        for (int c = 0; c < cols; ++c)
-            for (int r = 0; r < rows; ++r) t.data[(size_t)t.rows * r + c] = data[(size_t)rows * c + r];
-        return t;
+    MAT transpose() const;
+    // conjugate transpose; for the real types here the transpose (:203-206)
+    MAT dagger() const;
+};
+
+// _T(A): a new cols x rows matrix holding A^T (Base/__Matrix.h:177-186)
+template <typename t>
+MAT<t> _T(const MAT<t> &A) {
+    MAT<t> n(A.cols, A.rows);
+    for (int c = 0; c < A.cols; ++c)
+        for (int r = 0; r < A.rows; ++r) n.data[(size_t)n.rows * r + c] = A.data[(size_t)A.rows * c + r];
+    return n;
+}
+// _H(A): the conjugate transpose, the transpose for real element types (:191-200)
+template <typename t>
+MAT<t> _H(const MAT<t> &A) {
+    return _T(A);
+}
+template <typename T>
+MAT<T> MAT<T>::transpose() const {
+    return _T(*this);
+}
+template <typename T>
+MAT<T> MAT<T>::dagger() const {
+    return _H(*this);
+}
+
+// Elementwise scalar and matrix arithmetic (Base/__Matrix.h:209-259).  Host
+// loops: these are O(mn) data-model operations next to the GPU products.
+template <class t>
+MAT<t> operator*(t a, const MAT<t> &A) {
+    MAT<t> n(A.rows, A.cols);
+    for (size_t i = 0; i < (size_t)A.rows * A.cols; ++i) n.data[i] = A.data[i] * a;
+    return n;
+}
+template <class t>
+MAT<t> operator*(const MAT<t> &A, t a) {
+    return a * A;
+}
+template <class t>
+MAT<t> operator/(const MAT<t> &A, t a) {
+    return (t(1) / a) * A;
+}
+template <typename t1, typename t2>
+auto operator+(const MAT<t1> &A, const MAT<t2> &B) {
+    MAT<decltype(std::declval<t1>() + std::declval<t2>())> n(A.rows, A.cols);
+    for (size_t i = 0; i < (size_t)A.rows * A.cols; ++i) n.data[i] = A.data[i] + B.data[i];
+    return n;
+}
+template <typename t1, typename t2>
+auto operator-(const MAT<t1> &A, const MAT<t2> &B) {
+    MAT<decltype(std::declval<t1>() - std::declval<t2>())> n(A.rows, A.cols);
+    for (size_t i = 0; i < (size_t)A.rows * A.cols; ++i) n.data[i] = A.data[i] - B.data[i];
+    return n;
+}
+
+// __BlockMat<T>: non-owning r x c view of a MAT at (m, n), leading dimension
+// A.rows (Base/__Block.h:2-47).
+template <typename T>
+class __BlockMat {
+public:
+    int rows;
+    int cols;
+    int _rows;
+    T *data;
+    __BlockMat() : rows(0), cols(0), _rows(0), data(nullptr) {}
+    __BlockMat(const MAT<T> &A, int m, int n, int r, int c)
+        : rows(r), cols(c), _rows(A.rows), data(A.data + (size_t)A.rows * n + m) {}
+    void set(const MAT<T> &A, int m, int n, int r, int c) {
+        rows = r;
+        cols = c;
+        _rows = A.rows;
+        data = A.data + (size_t)A.rows * n + m;
+    }
+    T *operator[](int c) { return data + (size_t)_rows * c; }
+    T &operator()(int r, int c) { return data[(size_t)_rows * c + r]; }
+    // exchange rows r1 and r2 over the columns [c1, c2) (:36-46)
+    void swap_rows(int r1, int r2, int c1, int c2) {
+        for (int c = c1; c < c2; ++c) std::swap(data[(size_t)c * _rows + r1], data[(size_t)c * _rows + r2]);
     }
 };
 
@@ -200,6 +302,71 @@ void qr_householder(const MAT<t1> &A, MAT<t2> &Q, MAT<t2> &R) {
 
 }  // namespace detail
 }  // namespace mat
+
+// Mul/Mul.h:205-235: C = A * B (A.rows x B.cols) on the GPU's MFMA GEMM
+// (mqr_dgemm_host / mqr_sgemm_host).  float x float multiplies in FP32;
+// every other element type through FP64 and converted to the result type.
+template <typename t1, typename t2>
+auto operator*(const MAT<t1> &A, const MAT<t2> &B) {
+    using t3 = decltype(std::declval<t1>() * std::declval<t2>());
+    MAT<t3> C(A.rows, B.cols);
+    if (A.cols != B.rows) throw mat::Error(-3, "operator*: A.cols != B.rows");
+    if constexpr (std::is_same<t1, float>::value && std::is_same<t2, float>::value) {
+        mat::check(mqr_sgemm_host(mat::handle(), A.rows, B.cols, A.cols, A.data, B.data, C.data),
+                   "mqr_sgemm_host");
+    } else if constexpr (std::is_same<t1, double>::value && std::is_same<t2, double>::value) {
+        mat::check(mqr_dgemm_host(mat::handle(), A.rows, B.cols, A.cols, A.data, B.data, C.data),
+                   "mqr_dgemm_host");
+    } else {
+        std::vector<double> a, b, c((size_t)A.rows * B.cols);
+        const double *pa = mat::detail::as_f64(A, a), *pb = mat::detail::as_f64(B, b);
+        mat::check(mqr_dgemm_host(mat::handle(), A.rows, B.cols, A.cols, pa, pb, c.data()),
+                   "mqr_dgemm_host");
+        mat::detail::from_f64(C, c);
+    }
+    return C;
+}
+
+// ---------------------------------------------------------------------------
+// Binary MAT files (Mat-5.2/_extra/__File.h:1-66): int32 rows, int32 cols,
+// then the elements in storage order (column-major here), no type tag.
+// Failures raise mat::Error(MQR_ERR_IO) where the reference reads garbage.
+// ---------------------------------------------------------------------------
+// file size in bytes (:3-10)
+inline long size(const std::string &name) {
+    int64_t b = -1;
+    mat::check(mqr_file_size(name.c_str(), &b), "size");
+    return (long)b;
+}
+// len elements without a header (:12-19)
+template <class T>
+void load(const std::string &name, T *data, int len) {
+    mat::check(mqr_raw_load(name.c_str(), data, sizeof(T) * (size_t)len), "load");
+}
+// header and elements; A takes the file's shape (:20-34)
+template <class T>
+void load(const std::string &name, MAT<T> &A) {
+    int32_t r = 0, c = 0;
+    mat::check(mqr_mat_load_header(name.c_str(), &r, &c), "load");
+    A.init(r, c);
+    mat::check(mqr_mat_load(name.c_str(), r, c, A.data, sizeof(T)), "load");
+}
+// r x c elements without a header (:35-44)
+template <class T>
+void load(const std::string &name, MAT<T> &A, int r, int c) {
+    A.init(r, c);
+    mat::check(mqr_raw_load(name.c_str(), A.data, sizeof(T) * (size_t)r * c), "load");
+}
+// len elements without a header (:48-55)
+template <class T>
+void save(const std::string &name, T *data, int len) {
+    mat::check(mqr_raw_save(name.c_str(), data, sizeof(T) * (size_t)len), "save");
+}
+// header and elements (:56-66)
+template <class T>
+void save(const std::string &name, MAT<T> &A) {
+    mat::check(mqr_mat_save(name.c_str(), A.rows, A.cols, A.data, sizeof(T)), "save");
+}
 
 // QR/QR.h:7-46.  Q is (re)sized to m x m and R to m x n.
 template <typename t1, typename t2>

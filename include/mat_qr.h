@@ -49,6 +49,8 @@ extern "C" {
 #define MQR_ERR_NOT_INITIALIZED 1006
 #define MQR_ERR_INTERNAL 1007
 #define MQR_ERR_TOO_LARGE 1008     /* request exceeds a fixed-size device scratch array */
+#define MQR_ERR_IO 1009            /* a file could not be opened / read / written, or its
+                                      size or header disagrees with the request */
 
 typedef struct mqr_handle_st *mqr_handle_t;
 
@@ -147,6 +149,15 @@ int mqr_sgemm(mqr_handle_t handle, char transa, char transb, int64_t m, int64_t 
 int mqr_sgemm_bf16(mqr_handle_t handle, int terms, char transa, char transb, int64_t m, int64_t n,
                    int64_t k, float alpha, const float *A, int64_t lda, const float *B,
                    int64_t ldb, float beta, float *C, int64_t ldc);
+
+/* Host-memory products behind MAT<T>::operator* (Mul/Mul.h:205-235):
+ * C (m x n) = A (m x k) * B (k x n), all column-major with leading dimensions
+ * m, k, m; the operands are copied to the device, multiplied on the MFMA
+ * tiles above and C is copied back.  Blocks until C is written. */
+int mqr_dgemm_host(mqr_handle_t handle, int64_t m, int64_t n, int64_t k, const double *A,
+                   const double *B, double *C);
+int mqr_sgemm_host(mqr_handle_t handle, int64_t m, int64_t n, int64_t k, const float *A,
+                   const float *B, float *C);
 
 /* ---- FP64 blocked Householder QR (device pointers) ------------------------ */
 /* Factor the m x n matrix A in place: R on and above the diagonal, the
@@ -285,6 +296,22 @@ int mqr_qr_householder_host(mqr_handle_t handle, int64_t m, int64_t n, const dou
  * above (MQR_PREC_FP64 = plain solve, no refinement).  Blocks. */
 int mqr_gels_host(mqr_handle_t handle, mqr_precision_t prec, int64_t m, int64_t n,
                   const double *A, const double *b, double *x, mqr_ir_info_t *info);
+
+/* ---- binary MAT files (host memory; no device involved) ------------------ */
+/* The reference's file format (Mat-5.2/_extra/__File.h:1-66): int32 rows,
+ * int32 cols, then rows * cols elements of elem_size bytes each in the
+ * matrix's storage order (column-major for MAT<T> here), no type tag.
+ * mqr_mat_load checks that the header equals (rows, cols) and that the file
+ * holds exactly that many elements; failures are MQR_ERR_IO.  The raw forms
+ * hold elements only (the reference's save / load of a pointer and a length). */
+int mqr_file_size(const char *path, int64_t *bytes);                       /* :3-10 */
+int mqr_mat_save(const char *path, int32_t rows, int32_t cols, const void *data,
+                 size_t elem_size);                                         /* :56-66 */
+int mqr_mat_load_header(const char *path, int32_t *rows, int32_t *cols);   /* :21-34 */
+int mqr_mat_load(const char *path, int32_t rows, int32_t cols, void *data,
+                 size_t elem_size);                                         /* :21-44 */
+int mqr_raw_save(const char *path, const void *data, size_t bytes);        /* :48-55 */
+int mqr_raw_load(const char *path, void *data, size_t bytes);              /* :12-19 */
 
 #ifdef __cplusplus
 }

@@ -32,6 +32,7 @@ MQR_ERR_UNSUPPORTED = 1005
 MQR_ERR_NOT_INITIALIZED = 1006
 MQR_ERR_INTERNAL = 1007
 MQR_ERR_TOO_LARGE = 1008
+MQR_ERR_IO = 1009
 NOALLOC_MAX_BATCH = 32768
 
 MQR_F64, MQR_F32 = 0, 1
@@ -75,6 +76,14 @@ SIGNATURES = {
     "mqr_batched_plan_destroy": (_i32, [_vp]),
     "mqr_qr_backward": (_i32, [_vp, _i32, _i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64,
                                _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64]),
+    "mqr_dgemm_host": (_i32, [_vp, _i64, _i64, _i64, _vp, _vp, _vp]),
+    "mqr_sgemm_host": (_i32, [_vp, _i64, _i64, _i64, _vp, _vp, _vp]),
+    "mqr_file_size": (_i32, [C.c_char_p, _vp]),
+    "mqr_mat_save": (_i32, [C.c_char_p, C.c_int32, C.c_int32, _vp, C.c_size_t]),
+    "mqr_mat_load_header": (_i32, [C.c_char_p, _vp, _vp]),
+    "mqr_mat_load": (_i32, [C.c_char_p, C.c_int32, C.c_int32, _vp, C.c_size_t]),
+    "mqr_raw_save": (_i32, [C.c_char_p, _vp, C.c_size_t]),
+    "mqr_raw_load": (_i32, [C.c_char_p, _vp, C.c_size_t]),
     "mqr_qr_householder_host": (_i32, [_vp, _i64, _i64, _vp, _vp, _vp]),
     "mqr_gels_host": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp]),
 }
@@ -567,3 +576,47 @@ def LeastSquares(A, b, prec: str = "fp64", handle: Optional[Handle] = None):
                               b.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p),
                               C.byref(info) if prec != "fp64" else None), "mqr_gels_host")
     return x, info
+
+
+# ---------------------------------------------------------------------------
+# binary MAT files (Mat-5.2/_extra/__File.h:1-66) through the library
+# ---------------------------------------------------------------------------
+
+def mat_save(path: str, A) -> None:
+    """Write a 2-D numpy array as a MAT file: int32 rows, int32 cols, then the
+    elements column-major (the storage order of MAT<T>), no type tag."""
+    import numpy as np
+    A = np.asfortranarray(A)
+    assert A.ndim == 2
+    check(lib().mqr_mat_save(os.fsencode(path), A.shape[0], A.shape[1],
+                             A.ctypes.data_as(C.c_void_p), A.itemsize), "mqr_mat_save")
+
+
+def mat_load(path: str, dtype="float64"):
+    """Read a MAT file written with elements of `dtype` (the file carries
+    no type tag); returns a Fortran-ordered numpy array."""
+    import numpy as np
+    r, c = C.c_int32(0), C.c_int32(0)
+    check(lib().mqr_mat_load_header(os.fsencode(path), C.byref(r), C.byref(c)),
+          "mqr_mat_load_header")
+    A = np.empty((r.value, c.value), dtype=dtype, order="F")
+    check(lib().mqr_mat_load(os.fsencode(path), r.value, c.value, A.ctypes.data_as(C.c_void_p),
+                             A.itemsize), "mqr_mat_load")
+    return A
+
+
+def gemm_host(A, B):
+    """C = A @ B for host float64 or float32 numpy matrices through the GPU
+    GEMM behind MAT<T>::operator* (mqr_dgemm_host / mqr_sgemm_host)."""
+    import numpy as np
+    f32 = A.dtype == np.float32 and B.dtype == np.float32
+    dt = np.float32 if f32 else np.float64
+    A = np.asfortranarray(A, dtype=dt)
+    B = np.asfortranarray(B, dtype=dt)
+    assert A.shape[1] == B.shape[0]
+    Cm = np.zeros((A.shape[0], B.shape[1]), dtype=dt, order="F")
+    fn = lib().mqr_sgemm_host if f32 else lib().mqr_dgemm_host
+    check(fn(default_handle().h, A.shape[0], B.shape[1], A.shape[1], A.ctypes.data_as(C.c_void_p),
+             B.ctypes.data_as(C.c_void_p), Cm.ctypes.data_as(C.c_void_p)),
+          "mqr_sgemm_host" if f32 else "mqr_dgemm_host")
+    return Cm

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cstring>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "mqr_internal.h"
@@ -128,6 +129,9 @@ const char *mqr_strerror(int status) {
         case MQR_ERR_UNSUPPORTED: return "request not supported by this build";
         case MQR_ERR_NOT_INITIALIZED: return "handle not initialised";
         case MQR_ERR_INTERNAL: return "internal error";
+        case MQR_ERR_IO:
+            return "file could not be opened, read or written, or its size or header does not "
+                   "match the request";
         case MQR_ERR_TOO_LARGE:
             return "request exceeds the fixed device scratch of the allocation-free entry point "
                    "(at most MQR_NOALLOC_MAX_BATCH = 32768 matrices)";
@@ -375,6 +379,58 @@ int mqr_sgemm_bf16(mqr_handle_t h, int terms, char transa, char transb, int64_t 
 
 } // extern "C"
 
+// C = A B on host buffers: device copies of A, B and C, one MFMA GEMM, C back.
+template <typename T>
+static int gemm_host(mqr_handle_t h, int64_t m, int64_t n, int64_t k, const T *A, const T *B,
+                     T *C) {
+    if (!h) return -1;
+    if (m < 0 || m > INT32_MAX) return -2;
+    if (n < 0 || n > INT32_MAX) return -3;
+    if (k < 0 || k > INT32_MAX) return -4;
+    if (m * k > 0 && !A) return -5;
+    if (k * n > 0 && !B) return -6;
+    if (m * n > 0 && !C) return -7;
+    if (m == 0 || n == 0) return MQR_SUCCESS;
+    MQR_GUARD(h);
+    MQR_BRIDGE(h);
+    T *d = nullptr;
+    const size_t na = (size_t)(m * k), nbb = (size_t)(k * n), nc = (size_t)(m * n);
+    if (hipMalloc(&d, sizeof(T) * (na + nbb + nc)) != hipSuccess) {
+        (void)hipGetLastError();
+        return MQR_ERR_ALLOC;
+    }
+    T *dA = d, *dB = d + na, *dC = d + na + nbb;
+    int st = MQR_SUCCESS;
+    if ((na && hipMemcpyAsync(dA, A, sizeof(T) * na, hipMemcpyHostToDevice, h->stream) != hipSuccess) ||
+        (nbb && hipMemcpyAsync(dB, B, sizeof(T) * nbb, hipMemcpyHostToDevice, h->stream) != hipSuccess))
+        st = MQR_ERR_HIP;
+    if (st == MQR_SUCCESS) {
+        if (k == 0) {
+            st = hipMemsetAsync(dC, 0, sizeof(T) * nc, h->stream) == hipSuccess ? MQR_SUCCESS
+                                                                             : MQR_ERR_HIP;
+        } else if constexpr (std::is_same<T, double>::value) {
+            GemmF64 g;
+            g.M = (int)m; g.N = (int)n; g.K = (int)k;
+            g.A = dA; g.lda = m; g.B = dB; g.ldb = k; g.C = dC; g.ldc = m;
+            g.alpha = 1.0; g.beta = 0.0;
+            st = gemm_f64(g, h->stream);
+        } else {
+            GemmF32 g;
+            g.M = (int)m; g.N = (int)n; g.K = (int)k;
+            g.A = dA; g.lda = m; g.B = dB; g.ldb = k; g.C = dC; g.ldc = m;
+            g.alpha = 1.f; g.beta = 0.f;
+            st = gemm_f32(g, h->stream);
+        }
+    }
+    if (st == MQR_SUCCESS &&
+        (hipMemcpyAsync(C, dC, sizeof(T) * nc, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+         hipStreamSynchronize(h->stream) != hipSuccess))
+        st = MQR_ERR_HIP;
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
+    return st;
+}
+
 // Records the block size T is written with; a later mqr_dorgqr / mqr_dormqr
 // reading the same T with another block size is rejected.
 static void remember_t_layout(mqr_handle_t h, const double *T) {
@@ -394,6 +450,16 @@ static bool t_layout_mismatch(mqr_handle_t h, const double *T) {
 }
 
 extern "C" {
+
+int mqr_dgemm_host(mqr_handle_t h, int64_t m, int64_t n, int64_t k, const double *A,
+                   const double *B, double *C) {
+    return gemm_host<double>(h, m, n, k, A, B, C);
+}
+
+int mqr_sgemm_host(mqr_handle_t h, int64_t m, int64_t n, int64_t k, const float *A,
+                   const float *B, float *C) {
+    return gemm_host<float>(h, m, n, k, A, B, C);
+}
 
 int mqr_dgeqrf(mqr_handle_t h, int64_t m, int64_t n, double *A, int64_t lda, double *tau,
                double *T, int64_t ldt) {

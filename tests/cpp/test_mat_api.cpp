@@ -3,7 +3,10 @@
 // in place and out of place, and LeastSquares.  Prints one line per check and
 // exits non-zero on the first failure.  Inputs: mt19937_64 with a Box-Muller
 // normal (bit-identical across standard libraries).
+#include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <string>
 #include <cstdio>
 #include <cstdint>
 #include <random>
@@ -156,6 +159,105 @@ int main() {
             threw = e.status == MQR_ERR_UNSUPPORTED;
         }
         expect(threw, "QR_Givens raises mat::Error (unsupported)", 0.0, 0.0);
+    }
+    {
+        // operator* on the GPU: the reference's own reconstruction check
+        // Q * R - A (Mul/Mul.h:205-235), exact on integer-valued operands
+        MAT<double> A = randn<double>(300, 170, 8), Q, R;
+        QR(A, Q, R, QR_Householder);
+        MAT<double> QR_ = Q * R;
+        MAT<double> D = QR_ - A;
+        double num = 0, den = 0;
+        for (int64_t i = 0; i < (int64_t)A.rows * A.cols; ++i) {
+            num += D.data[i] * D.data[i];
+            den += A.data[i] * A.data[i];
+        }
+        expect(QR_.rows == 300 && QR_.cols == 170, "operator* shape", 0.0, 0.0);
+        expect(std::sqrt(num / den) < 64 * 300 * u, "||Q*R - A|| / ||A|| via operator*",
+               std::sqrt(num / den), 64 * 300 * u);
+        MAT<double> I1(37, 53), I2(53, 29);
+        MAT<float> F1(37, 53), F2(53, 29);
+        std::mt19937_64 g(9);
+        for (int i = 0; i < 37 * 53; ++i) F1.data[i] = (float)(I1.data[i] = (double)(int)(g() % 17) - 8);
+        for (int i = 0; i < 53 * 29; ++i) F2.data[i] = (float)(I2.data[i] = (double)(int)(g() % 17) - 8);
+        MAT<double> P = I1 * I2;
+        MAT<float> PF = F1 * F2;
+        MAT<double> PM = I1 * F2;  // mixed element types: FP64, result double
+        double worst = 0;
+        for (int c = 0; c < 29; ++c)
+            for (int r = 0; r < 37; ++r) {
+                double s = 0;
+                for (int k = 0; k < 53; ++k) s += I1(r, k) * I2(k, c);
+                worst = std::max({worst, std::fabs(P(r, c) - s), std::fabs((double)PF(r, c) - s),
+                                  std::fabs(PM(r, c) - s)});
+            }
+        expect(worst == 0.0, "operator* exact on integers (FP64, FP32, mixed)", worst, 0.0);
+    }
+    {
+        // data model: _T, dagger, diagonal, swap_rows / swap_cols, __BlockMat,
+        // scalar arithmetic (Base/__Matrix.h, Base/__Block.h)
+        MAT<double> A = randn<double>(5, 3, 10);
+        MAT<double> T = _T(A), H = A.dagger(), T2 = A.transpose();
+        bool ok = T.rows == 3 && T.cols == 5;
+        for (int r = 0; r < 5; ++r)
+            for (int c = 0; c < 3; ++c) ok = ok && T(c, r) == A(r, c) && H(c, r) == A(r, c) && T2(c, r) == A(r, c);
+        expect(ok, "_T / dagger / transpose", 0.0, 0.0);
+        double d[4] = {1, 2, 3, 4};
+        MAT<double> Dg;
+        Dg.diagonal(d, 4);
+        ok = Dg.rows == 4 && Dg.cols == 4;
+        for (int r = 0; r < 4; ++r)
+            for (int c = 0; c < 4; ++c) ok = ok && Dg(r, c) == (r == c ? d[r] : 0.0);
+        MAT<double> S(A);
+        S.swap_rows(0, 4);
+        S.swap_cols(0, 2);
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 5; ++r) {
+                const int rr = r == 0 ? 4 : r == 4 ? 0 : r, cc = c == 0 ? 2 : c == 2 ? 0 : c;
+                ok = ok && S(r, c) == A(rr, cc);
+            }
+        expect(ok, "diagonal / swap_rows / swap_cols", 0.0, 0.0);
+        MAT<double> B = randn<double>(6, 6, 11), B0(B);
+        __BlockMat<double> V(B, 1, 2, 4, 3);
+        ok = V(0, 0) == B(1, 2) && V[2][3] == B(4, 4) && V._rows == 6;
+        V.swap_rows(0, 3, 0, 2);
+        for (int c = 0; c < 6; ++c)
+            for (int r = 0; r < 6; ++r) {
+                double want = B0(r, c);
+                if ((c == 2 || c == 3) && r == 1) want = B0(4, c);
+                if ((c == 2 || c == 3) && r == 4) want = B0(1, c);
+                ok = ok && B(r, c) == want;
+            }
+        MAT<double> E = 2.0 * A + A / 2.0 - A * 0.5;
+        for (int i = 0; i < 15; ++i) ok = ok && E.data[i] == 2.0 * A.data[i] + A.data[i] / 2.0 - A.data[i] * 0.5;
+        expect(ok, "__BlockMat view / swap_rows, scalar * / + -", 0.0, 0.0);
+    }
+    {
+        // binary MAT files (Mat-5.2/_extra/__File.h): int32 rows, int32 cols,
+        // then the elements; round trips through every load / save form
+        const std::string f = "/tmp/mat_api_test.mat", fr = "/tmp/mat_api_test.raw";
+        MAT<double> A = randn<double>(7, 4, 12), B, C;
+        save(f, A);
+        bool ok = size(f) == 8 + 8 * 28;
+        load(f, B);
+        ok = ok && B.rows == 7 && B.cols == 4 && std::memcmp(A.data, B.data, 8 * 28) == 0;
+        save(fr, A.data, 28);
+        load(fr, C, 7, 4);
+        ok = ok && size(fr) == 8 * 28 && std::memcmp(A.data, C.data, 8 * 28) == 0;
+        float fx[3] = {1.5f, -2.f, 3.25f}, fy[3] = {0, 0, 0};
+        save(fr, fx, 3);
+        load(fr, fy, 3);
+        ok = ok && fy[0] == 1.5f && fy[1] == -2.f && fy[2] == 3.25f;
+        bool threw = false;
+        try {
+            MAT<double> Z;
+            load(std::string("/tmp/does_not_exist_mat_api.mat"), Z);
+        } catch (const mat::Error &e) {
+            threw = e.status == MQR_ERR_IO;
+        }
+        std::remove(f.c_str());
+        std::remove(fr.c_str());
+        expect(ok && threw, "MAT file save / load / size round trips", 0.0, 0.0);
     }
     std::printf("%s\n", failures ? "FAILED" : "ALL PASSED");
     return failures ? 1 : 0;

@@ -8,6 +8,9 @@ The reference follows LAPACK's dlarfg sign convention except where the part of
 a column below the diagonal is exactly zero (it still reflects, tau = 2); the
 tests handle that case separately.
 
+It also writes three of these matrices as binary MAT files (A_17x9.mat,
+VR_17x9.mat, A_17x9_f32.mat) for the file-format tests.
+
 Run:  python tests/golden/make_golden.py
 """
 import os
@@ -41,6 +44,26 @@ def main() -> None:
         out[f"Q_{key}"] = Q
     here = os.path.dirname(os.path.abspath(__file__))
     np.savez_compressed(os.path.join(here, "lapack_qr.npz"), **out)
+    write_mat_files(out, here)
+
+
+def write_mat(path: str, A: np.ndarray) -> None:
+    """The reference's binary MAT layout (Mat-5.2/_extra/__File.h:56-66):
+    int32 rows, int32 cols, then the elements in storage order (column-major
+    for MAT<T>), written here with numpy alone so that the library's reader
+    is checked against an independent writer."""
+    A = np.asfortranarray(A)
+    with open(path, "wb") as f:
+        f.write(np.array(A.shape, dtype="<i4").tobytes())
+        f.write(A.tobytes(order="F"))
+
+
+def write_mat_files(out: dict, here: str) -> None:
+    """A few of the LAPACK fixtures also as MAT files: the 17x9 input and
+    its dgeqrf output (FP64), and the input rounded to FP32."""
+    write_mat(os.path.join(here, "A_17x9.mat"), out["A_17x9"])
+    write_mat(os.path.join(here, "VR_17x9.mat"), out["VR_17x9"])
+    write_mat(os.path.join(here, "A_17x9_f32.mat"), out["A_17x9"].astype(np.float32))
 
 
 if __name__ == "__main__":

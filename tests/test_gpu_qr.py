@@ -428,3 +428,25 @@ def test_t_blocks_rejected_after_block_size_change(torch_mod):
         Q = M.orgqr(hn, dA, T, 150, nq=150)
         hn.synchronize()
         assert np.linalg.norm(host(Q) @ np.triu(host(dA))[:150] - A) < 1e-12 * np.linalg.norm(A)
+
+
+def test_host_gemm_behind_operator_star(orc):
+    """mqr_dgemm_host / mqr_sgemm_host (the product MAT<T>::operator* runs):
+    exact on integer-valued operands against the oracle's packed GEMM
+    (Mul/Mul.h:239-263), within the FP64 / FP32 error bound on Gaussian
+    ones; k = 0 gives zeros."""
+    rng = np.random.default_rng(15)
+    for (m, n, k) in [(1, 1, 1), (5, 3, 4), (130, 70, 33), (300, 257, 129)]:
+        A = rng.integers(-8, 9, (m, k)).astype(np.float64)
+        B = rng.integers(-8, 9, (k, n)).astype(np.float64)
+        exact = orc.mulAB_packed_n(A, B, np.zeros((m, n)))
+        assert np.array_equal(M.gemm_host(A, B), -exact)
+        assert np.array_equal(M.gemm_host(A.astype(np.float32), B.astype(np.float32)),
+                              (-exact).astype(np.float32))
+    A = rng.standard_normal((400, 200))
+    B = rng.standard_normal((200, 90))
+    ref = orc.gemm(A, B)
+    np.testing.assert_allclose(M.gemm_host(A, B), ref, rtol=0, atol=200 * 4 * EPS * 20)
+    np.testing.assert_allclose(M.gemm_host(A.astype(np.float32), B.astype(np.float32)), ref,
+                               rtol=0, atol=200 * 4 * 6e-8 * 20)
+    assert np.array_equal(M.gemm_host(np.zeros((4, 0)), np.zeros((0, 3))), np.zeros((4, 3)))
