@@ -18,8 +18,8 @@
 // conventions), Q (m x m) and R (m x n, zero below the diagonal) are copied
 // back.  There is no CPU fallback and no runtime backend choice: a failure of
 // the device library is reported as a mat::Error exception from these host
-// helpers (the C ABI itself never throws).  QR_Givens and QR_GramSchmidt are
-// not part of this library and raise mat::Error.
+// helpers (the C ABI itself never throws).  QR_Givens and QR_GramSchmidt run
+// on the GPU as well (Q m x m for Givens, m x n for Gram-Schmidt).
 //
 // Element type: the factorisation computes in FP64; MAT<float> inputs and
 // outputs are converted on the host, like the reference's type-converting
@@ -300,6 +300,23 @@ void qr_householder(const MAT<t1> &A, MAT<t2> &Q, MAT<t2> &R) {
     }
 }
 
+// QR_Givens (Q m x m, R m x n) and QR_GramSchmidt (Q m x n, R m x n) on the
+// GPU through mqr_qr_host; A may alias R or Q (it is read before either is
+// written).
+template <typename t1, typename t2>
+void qr_other(const MAT<t1> &A, MAT<t2> &Q, MAT<t2> &R, mqr_qr_type_t type) {
+    const int m = A.rows, n = A.cols;
+    const int qc = type == MQR_QR_GIVENS ? m : n;
+    std::vector<double> abuf;
+    const double *a = as_f64(A, abuf);
+    std::vector<double> q((size_t)m * qc), r((size_t)m * n);
+    check(mqr_qr_host(handle(), type, m, n, a, q.data(), r.data()), "mqr_qr_host");
+    Q.init(m, qc);
+    R.init(m, n);
+    from_f64(Q, q);
+    from_f64(R, r);
+}
+
 }  // namespace detail
 }  // namespace mat
 
@@ -375,16 +392,21 @@ inline void QR(MAT<t1> &A, MAT<t2> &Q, MAT<t2> &R, QR_type type) {
         case QR_Householder:
             mat::detail::qr_householder(A, Q, R);
             break;
+        case QR_Givens:
+            mat::detail::qr_other(A, Q, R, MQR_QR_GIVENS);
+            break;
+        case QR_GramSchmidt:
+            mat::detail::qr_other(A, Q, R, MQR_QR_GRAMSCHMIDT);
+            break;
         default:
-            throw mat::Error(MQR_ERR_UNSUPPORTED, "QR: only QR_Householder runs on the GPU");
+            throw mat::Error(MQR_ERR_UNSUPPORTED, "QR: unknown QR_type");
     }
 }
 
 // QR/QR.h:48-71
 template <QR_type type, typename t1, typename t2>
 inline void QR(MAT<t1> &A, MAT<t2> &Q, MAT<t2> &R) {
-    static_assert(type == QR_Householder, "only QR_Householder runs on the GPU");
-    mat::detail::qr_householder(A, Q, R);
+    QR(A, Q, R, type);
 }
 
 // ---------------------------------------------------------------------------

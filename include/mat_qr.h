@@ -200,6 +200,28 @@ int mqr_dtrsm_upper(mqr_handle_t handle, char trans, int64_t n, int64_t nrhs, co
 int mqr_dgels(mqr_handle_t handle, int64_t m, int64_t n, int64_t nrhs, double *A, int64_t lda,
               double *B, int64_t ldb);
 
+/* ---- the other QR_type values of QR(A, Q, R, type) (QR/QR.h:26-42) ------- */
+/* Givens rotations (QR/__Givens.h:45-63): A m x n (lda) is read, Q (m x m,
+ * ldq) and R (m x n, ldr) are written; R may be A (in place).  The
+ * reference's rotation order and arithmetic, so Q and R equal the
+ * reference's bit for bit; R keeps the rotations' rounding residue below the
+ * diagonal like the reference.  Where the reference divides by a zero pivot
+ * (filling R and Q with NaN) the rotation takes its limit instead (c = 0,
+ * s = sign of the entry below; the identity when both are zero).  One step of
+ * the sweep holds at most 8192 rotations: min(n, m / 2 + 1) <= 8192, else
+ * MQR_ERR_UNSUPPORTED. */
+int mqr_dgeqr_givens(mqr_handle_t handle, int64_t m, int64_t n, const double *A, int64_t lda,
+                     double *Q, int64_t ldq, double *R, int64_t ldr);
+/* Gram-Schmidt (QR/__GramSchmidt.h:2-18), m >= n: the thin QR with a
+ * positive diagonal, A = Q R with Q m x n (ldq) orthonormal and R n x n
+ * (ldr >= n) upper triangular, zero below the diagonal.  Q may be A (in
+ * place).  Computed by block Gram-Schmidt with reorthogonalisation, so Q is
+ * orthogonal to working precision (the reference's modified Gram-Schmidt
+ * loses orthogonality in proportion to cond(A)); both give the same unique
+ * factors up to that rounding. */
+int mqr_dgeqr_gramschmidt(mqr_handle_t handle, int64_t m, int64_t n, const double *A, int64_t lda,
+                          double *Q, int64_t ldq, double *R, int64_t ldr);
+
 /* ---- mixed precision least squares with iterative refinement ------------ */
 typedef struct {
     int max_iter;    /* refinement steps allowed (default 30) */
@@ -292,6 +314,14 @@ int mqr_qr_backward(mqr_handle_t handle, mqr_dtype_t dtype, int64_t batch, int64
  * R m x n (R zero below the diagonal) are written; all host memory.  Blocks. */
 int mqr_qr_householder_host(mqr_handle_t handle, int64_t m, int64_t n, const double *A,
                             double *Q, double *R);
+/* QR(A, Q, R, type) on host memory for every QR_type (Mat.h:43-47):
+ * A m x n (lda = m) read; R m x n written; Q m x m for MQR_QR_HOUSEHOLDER
+ * (R zero below the diagonal) and MQR_QR_GIVENS (R with the rotation
+ * residue), m x n for MQR_QR_GRAMSCHMIDT (R zero outside its n x n upper
+ * triangle, m >= n).  Blocks. */
+typedef enum { MQR_QR_HOUSEHOLDER = 0, MQR_QR_GIVENS = 1, MQR_QR_GRAMSCHMIDT = 2 } mqr_qr_type_t;
+int mqr_qr_host(mqr_handle_t handle, mqr_qr_type_t type, int64_t m, int64_t n, const double *A,
+                double *Q, double *R);
 /* Host least squares: A m x n (lda = m), b (m), x (n) host memory; prec as
  * above (MQR_PREC_FP64 = plain solve, no refinement).  Blocks. */
 int mqr_gels_host(mqr_handle_t handle, mqr_precision_t prec, int64_t m, int64_t n,

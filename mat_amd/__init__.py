@@ -84,6 +84,9 @@ SIGNATURES = {
     "mqr_mat_load": (_i32, [C.c_char_p, C.c_int32, C.c_int32, _vp, C.c_size_t]),
     "mqr_raw_save": (_i32, [C.c_char_p, _vp, C.c_size_t]),
     "mqr_raw_load": (_i32, [C.c_char_p, _vp, C.c_size_t]),
+    "mqr_dgeqr_givens": (_i32, [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64]),
+    "mqr_dgeqr_gramschmidt": (_i32, [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64]),
+    "mqr_qr_host": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp]),
     "mqr_qr_householder_host": (_i32, [_vp, _i64, _i64, _vp, _vp, _vp]),
     "mqr_gels_host": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp]),
 }
@@ -546,21 +549,45 @@ def default_handle() -> Handle:
 
 
 def QR(A, Q, R, qr_type: int = QR_Householder, handle: Optional[Handle] = None) -> None:
-    """QR(A, Q, R, QR_Householder) on host float64 Fortran-ordered numpy
-    arrays, like QR/QR.h:7-24: Q (m x m) and R (m x n, zero below the
-    diagonal) are written in place.  Only QR_Householder runs on the GPU."""
+    """QR(A, Q, R, type) on host float64 Fortran-ordered numpy arrays, like
+    QR/QR.h:7-46, every type on the GPU: Q (m x m; m x n for QR_GramSchmidt)
+    and R (m x n) are written in place."""
     import numpy as np
-    if qr_type != QR_Householder:
-        raise MqrError(MQR_ERR_UNSUPPORTED, "QR")
+    if qr_type not in (QR_Householder, QR_Givens, QR_GramSchmidt):
+        raise MqrError(-2, "QR")
     h = handle or default_handle()
     A = np.asfortranarray(A, dtype=np.float64)
     m, n = A.shape
-    assert Q.shape == (m, m) and R.shape == (m, n)
+    qc = n if qr_type == QR_GramSchmidt else m
+    assert Q.shape == (m, qc) and R.shape == (m, n)
     assert Q.flags.f_contiguous and R.flags.f_contiguous
     assert Q.dtype == np.float64 and R.dtype == np.float64
-    check(lib().mqr_qr_householder_host(h.h, m, n, A.ctypes.data_as(C.c_void_p),
-                                        Q.ctypes.data_as(C.c_void_p), R.ctypes.data_as(C.c_void_p)),
-          "mqr_qr_householder_host")
+    check(lib().mqr_qr_host(h.h, qr_type, m, n, A.ctypes.data_as(C.c_void_p),
+                            Q.ctypes.data_as(C.c_void_p), R.ctypes.data_as(C.c_void_p)),
+          "mqr_qr_host")
+
+
+def qr_givens(h: Handle, A):
+    """QR by Givens rotations of a column-major float64 device matrix:
+    returns new (Q m x m, R m x n) device tensors (mqr_dgeqr_givens)."""
+    m, n = A.shape
+    Q = empty_colmajor(m, m, device=A.device)
+    R = empty_colmajor(m, n, device=A.device)
+    check(lib().mqr_dgeqr_givens(h.h, m, n, _ptr(A), ld(A), _ptr(Q), ld(Q), _ptr(R), ld(R)),
+          "mqr_dgeqr_givens")
+    return Q, R
+
+
+def qr_gramschmidt(h: Handle, A):
+    """Gram-Schmidt thin QR (positive diagonal) of a column-major float64
+    device matrix, m >= n: returns new (Q m x n, R n x n) device tensors
+    (mqr_dgeqr_gramschmidt)."""
+    m, n = A.shape
+    Q = empty_colmajor(m, n, device=A.device)
+    R = empty_colmajor(n, n, device=A.device)
+    check(lib().mqr_dgeqr_gramschmidt(h.h, m, n, _ptr(A), ld(A), _ptr(Q), ld(Q), _ptr(R),
+                                      ld(R)), "mqr_dgeqr_gramschmidt")
+    return Q, R
 
 
 def LeastSquares(A, b, prec: str = "fp64", handle: Optional[Handle] = None):

@@ -30,6 +30,12 @@ int qr_backward(mqr_handle_t hd, bool f64, int64_t batch, int64_t m, int64_t n, 
                 int64_t ldq, int64_t sq, const void *R, int64_t ldr, int64_t sr, const void *dQ,
                 int64_t lddq, int64_t sdq, const void *dR, int64_t lddr, int64_t sdr, void *dA,
                 int64_t ldda, int64_t sda);
+size_t givens_ws_bytes(int64_t m, int64_t n);
+bool givens_supported(int64_t m, int64_t n);
+int qr_givens_f64(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t lda, double *Q,
+                  int64_t ldq, double *R, int64_t ldr);
+int qr_gramschmidt_f64(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t lda,
+                       double *Q, int64_t ldq, double *R, int64_t ldr);
 int batched_noalloc_init(int device);
 int geqrf_batched_noalloc(mqr_handle_t hd, bool f64, int64_t batch, const int32_t *m,
                           const int32_t *n, const int64_t *lda, void *const *A, void *const *tau,
@@ -719,6 +725,93 @@ int mqr_qr_householder_host(mqr_handle_t h, int64_t m, int64_t n, const double *
     if (dA) (void)hipFree(dA);
     if (dQ) (void)hipFree(dQ);
     if (dTT) (void)hipFree(dTT);
+    return st;
+}
+
+int mqr_dgeqr_givens(mqr_handle_t h, int64_t m, int64_t n, const double *A, int64_t lda,
+                     double *Q, int64_t ldq, double *R, int64_t ldr) {
+    if (!h) return -1;
+    if (m < 0 || m > INT32_MAX) return -2;
+    if (n < 0 || n > INT32_MAX) return -3;
+    if (m * n > 0 && !A) return -4;
+    if (lda < std::max<int64_t>(1, m)) return -5;
+    if (m > 0 && !Q) return -6;
+    if (ldq < std::max<int64_t>(1, m)) return -7;
+    if (m * n > 0 && !R) return -8;
+    if (ldr < std::max<int64_t>(1, m)) return -9;
+    if (m == 0) return MQR_SUCCESS;
+    if (!givens_supported(m, n)) return MQR_ERR_UNSUPPORTED;
+    MQR_GUARD(h);
+    MQR_BRIDGE(h);
+    return qr_givens_f64(h, m, n, A, lda, Q, ldq, R, ldr);
+}
+
+int mqr_dgeqr_gramschmidt(mqr_handle_t h, int64_t m, int64_t n, const double *A, int64_t lda,
+                          double *Q, int64_t ldq, double *R, int64_t ldr) {
+    if (!h) return -1;
+    if (m < 0 || m > INT32_MAX) return -2;
+    if (n < 0 || n > m) return -3;
+    if (m * n > 0 && !A) return -4;
+    if (lda < std::max<int64_t>(1, m)) return -5;
+    if (m * n > 0 && !Q) return -6;
+    if (ldq < std::max<int64_t>(1, m)) return -7;
+    if (n > 0 && !R) return -8;
+    if (ldr < std::max<int64_t>(1, n)) return -9;
+    if (Q == A && ldq != lda) return -7;
+    if (n == 0) return MQR_SUCCESS;
+    MQR_GUARD(h);
+    MQR_BRIDGE(h);
+    return qr_gramschmidt_f64(h, m, n, A, lda, Q, ldq, R, ldr);
+}
+
+int mqr_qr_host(mqr_handle_t h, mqr_qr_type_t type, int64_t m, int64_t n, const double *A,
+                double *Q, double *R) {
+    if (!h) return -1;
+    if (type == MQR_QR_HOUSEHOLDER) {
+        const int st = mqr_qr_householder_host(h, m, n, A, Q, R);
+        return st < -1 ? st - 1 : st;  // argument positions shift by one
+    }
+    if (type != MQR_QR_GIVENS && type != MQR_QR_GRAMSCHMIDT) return -2;
+    if (m < 0 || m > INT32_MAX) return -3;
+    if (n < 0 || n > INT32_MAX || (type == MQR_QR_GRAMSCHMIDT && n > m)) return -4;
+    if (m * n > 0 && !A) return -5;
+    if (m > 0 && !Q) return -6;
+    if (m * n > 0 && !R) return -7;
+    if (m == 0) return MQR_SUCCESS;
+    if (type == MQR_QR_GIVENS && !givens_supported(m, n)) return MQR_ERR_UNSUPPORTED;
+    MQR_GUARD(h);
+    MQR_BRIDGE(h);
+    const int64_t qcols = type == MQR_QR_GIVENS ? m : n;
+    double *d = nullptr;
+    const size_t na = (size_t)(m * n), nq = (size_t)(m * qcols);
+    if (hipMalloc(&d, sizeof(double) * std::max<size_t>(na + nq + na, 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        return MQR_ERR_ALLOC;
+    }
+    double *dA = d, *dQ = d + na, *dR = d + na + nq;
+    int st = MQR_SUCCESS;
+    if (na && hipMemcpyAsync(dA, A, sizeof(double) * na, hipMemcpyHostToDevice, h->stream) !=
+                  hipSuccess)
+        st = MQR_ERR_HIP;
+    if (st == MQR_SUCCESS) {
+        if (type == MQR_QR_GIVENS) {
+            st = qr_givens_f64(h, m, n, dA, m, dQ, m, dR, m);
+        } else if (n > 0) {
+            // R is m x n here: its rows n.. are zero, the n x n upper part
+            // comes from the device routine (ld m)
+            if (hipMemsetAsync(dR, 0, sizeof(double) * na, h->stream) != hipSuccess) st = MQR_ERR_HIP;
+            if (st == MQR_SUCCESS) st = qr_gramschmidt_f64(h, m, n, dA, m, dQ, m, dR, m);
+        }
+    }
+    if (st == MQR_SUCCESS &&
+        ((nq && hipMemcpyAsync(Q, dQ, sizeof(double) * nq, hipMemcpyDeviceToHost, h->stream) !=
+                    hipSuccess) ||
+         (na && hipMemcpyAsync(R, dR, sizeof(double) * na, hipMemcpyDeviceToHost, h->stream) !=
+                    hipSuccess) ||
+         hipStreamSynchronize(h->stream) != hipSuccess))
+        st = MQR_ERR_HIP;
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
     return st;
 }
 

@@ -625,3 +625,79 @@ void orc_gels(int m, int n, const double *A, const double *b, double *x) {
     orc_trsv_upper(n, R, m, x);
     free(R); free(c); free(T_all); free(tau); free(work);
 }
+
+/* ------------------------------------------------------------------ */
+/* QR(A, Q, R, QR_Givens): QR/QR.h:26-35 with QR/__Givens.h:2-21,45-63 */
+/* ------------------------------------------------------------------ */
+/* R = A, Q = I_m; for each column j and i = m-1 down to j+1 a rotation of
+ * rows (i-1, i) built from t = R(i,j) / R(i-1,j): c = 1 / sqrt(t t + 1),
+ * s = t c (:50-55); rows i-1, i of R from column j on become
+ * (a c + b s, -a s + b c) (__ApplyGivensOnTheLeft, :13-23) and columns
+ * i-1, i of Q the same (__ApplyGivensOnTheRight, :2-11).  The source's
+ * plain products and sums, unfused (this file is built with
+ * -ffp-contract=off).  R keeps the rotations' rounding residue below the
+ * diagonal, as in the reference.
+ * fix_zero_pivots = 0 follows the source literally: a zero R(i-1,j) divides
+ * by zero (NaN / Inf spread).  fix_zero_pivots = 1 is the documented
+ * deviation of the GPU path: R(i-1,j) = 0 takes c = 0, s = copysign(1,
+ * R(i,j)) (the rotation's limit), and both zero takes c = 1, s = 0. */
+void orc_qr_givens(int m, int n, const double *A, double *Q, double *R, int fix_zero_pivots) {
+    memcpy(R, A, sizeof(double) * (size_t)m * n);
+    for (int c = 0; c < m; ++c) {
+        memset(Q + (size_t)c * m, 0, sizeof(double) * (size_t)m);
+        Q[(size_t)c * m + c] = 1.0;
+    }
+    for (int j = 0; j < n; ++j) {
+        for (int i = m - 1; i > j; --i) {
+            const double top = R[(size_t)j * m + i - 1], bot = R[(size_t)j * m + i];
+            double c, s;
+            if (fix_zero_pivots && top == 0.0) {
+                c = bot == 0.0 ? 1.0 : 0.0;
+                s = bot == 0.0 ? 0.0 : copysign(1.0, bot);
+            } else {
+                const double t = bot / top;
+                c = 1.0 / sqrt(t * t + 1.0);
+                s = t * c;
+            }
+            for (int k = j; k < n; ++k) {
+                double *col = R + (size_t)k * m;
+                const double a = col[i - 1], b = col[i];
+                col[i - 1] = a * c + b * s;
+                col[i] = -a * s + b * c;
+            }
+            double *qa = Q + (size_t)(i - 1) * m, *qb = Q + (size_t)i * m;
+            for (int r = 0; r < m; ++r) {
+                const double a = qa[r], b = qb[r];
+                qa[r] = a * c + b * s;
+                qb[r] = -a * s + b * c;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* QR(A, Q, R, QR_GramSchmidt): QR/QR.h:36-42 with QR/__GramSchmidt.h:2-18 */
+/* ------------------------------------------------------------------ */
+/* Modified Gram-Schmidt on Q = A (m x n): for column j, for i < j,
+ * R(i,j) = dot(Q_i, Q_j) with the four-way split dot, Q_j += Q_i * -R(i,j)
+ * (__mixed, fused like the AVX path); then R(j,j) = sqrt(dot(Q_j, Q_j)) and
+ * Q_j *= 1 / R(j,j) (__times).  R is m x n (ld m), zero except its upper
+ * n x n triangle.  Q holds n columns (the reference orthogonalises
+ * Q.cols columns of a Q presized by the caller). */
+void orc_qr_gramschmidt(int m, int n, const double *A, double *Q, double *R) {
+    memcpy(Q, A, sizeof(double) * (size_t)m * n);
+    memset(R, 0, sizeof(double) * (size_t)m * n);
+    for (int j = 0; j < n; ++j) {
+        double *qj = Q + (size_t)j * m;
+        for (int i = 0; i < j; ++i) {
+            const double *qi = Q + (size_t)i * m;
+            const double len = orc_dot(qi, qj, m);
+            axpy_into(qj, qj, qi, -len, m);
+            R[(size_t)j * m + i] = len;
+        }
+        const double len = sqrt(orc_dot(qj, qj, m));
+        R[(size_t)j * m + j] = len;
+        const double inv = 1.0 / len;
+        for (int r = 0; r < m; ++r) qj[r] = qj[r] * inv;
+    }
+}

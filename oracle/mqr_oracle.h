@@ -83,6 +83,16 @@ void orc_qr_unblocked(int m, int n, double *Q, double *R);
  * branch ran, 0 for the unblocked one. */
 int orc_qr_householder(int m, int n, const double *A, double *Q, double *R);
 
+/* ---- QR/QR.h:26-42: the other QR_type values ---- */
+/* Givens rotations (QR/__Givens.h:45-63): A m x n read, Q m x m and R m x n
+ * written (R keeps rotation residue below the diagonal).  fix_zero_pivots
+ * selects the GPU path's handling of a zero pivot instead of the source's
+ * division by zero. */
+void orc_qr_givens(int m, int n, const double *A, double *Q, double *R, int fix_zero_pivots);
+/* Modified Gram-Schmidt (QR/__GramSchmidt.h:2-18): A m x n read, Q m x n
+ * and R m x n (upper n x n triangle, zero elsewhere) written. */
+void orc_qr_gramschmidt(int m, int n, const double *A, double *Q, double *R);
+
 /* ---- least squares (no reference counterpart) ---- */
 /* x = argmin ||A x - b|| for m >= n via the blocked factorisation, Q^T b
  * through the compact-WY blocks, and back substitution with R

@@ -64,6 +64,8 @@ def _bind(lib: C.CDLL) -> C.CDLL:
     lib.orc_qr_householder.argtypes = [C.c_int, C.c_int, vp, vp, vp]
     lib.orc_qr_householder.restype = C.c_int
     lib.orc_gels.argtypes = [C.c_int, C.c_int, vp, vp, vp]
+    lib.orc_qr_givens.argtypes = [C.c_int, C.c_int, vp, vp, vp, C.c_int]
+    lib.orc_qr_gramschmidt.argtypes = [C.c_int, C.c_int, vp, vp, vp]
     lib.orc_trsv_upper.argtypes = [C.c_int, vp, i64, vp]
     return lib
 
@@ -212,6 +214,24 @@ class Oracle:
         R = np.zeros((m, n), order="F")
         blocked = self.lib.orc_qr_householder(m, n, _p(A), _p(Q), _p(R))
         return Q, R, bool(blocked)
+
+    def qr_givens(self, A, fix_zero_pivots: bool = False):
+        """QR(A, Q, R, QR_Givens): returns (Q m x m, R m x n)."""
+        A = _f(A)
+        m, n = A.shape
+        Q = np.zeros((m, m), order="F")
+        R = np.zeros((m, n), order="F")
+        self.lib.orc_qr_givens(m, n, _p(A), _p(Q), _p(R), 1 if fix_zero_pivots else 0)
+        return Q, R
+
+    def qr_gramschmidt(self, A):
+        """QR(A, Q, R, QR_GramSchmidt): returns (Q m x n, R m x n)."""
+        A = _f(A)
+        m, n = A.shape
+        Q = np.zeros((m, n), order="F")
+        R = np.zeros((m, n), order="F")
+        self.lib.orc_qr_gramschmidt(m, n, _p(A), _p(Q), _p(R))
+        return Q, R
 
     def gels(self, A, b) -> np.ndarray:
         A = _f(A)

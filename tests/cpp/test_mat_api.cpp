@@ -150,15 +150,41 @@ int main() {
         expect(std::sqrt(dx / nx) < 1e-12, "LeastSquares FP32+IR vs FP64 x", std::sqrt(dx / nx), 1e-12);
     }
     {
-        // QR types this library does not provide raise mat::Error
-        MAT<double> A = randn<double>(10, 5, 7), Q, R;
-        bool threw = false;
-        try {
-            QR(A, Q, R, QR_Givens);
-        } catch (const mat::Error &e) {
-            threw = e.status == MQR_ERR_UNSUPPORTED;
+        // the other QR_type values on the same switch (QR/QR.h:26-42)
+        MAT<double> A = randn<double>(60, 35, 7), Q, R;
+        QR_init(A, Q, R);
+        QR(A, Q, R, QR_Givens);
+        double rec, orth, low;
+        qr_errors(A, Q, R, &rec, &orth, &low);
+        expect(Q.rows == 60 && Q.cols == 60 && R.rows == 60 && R.cols == 35, "QR_Givens shapes",
+               0.0, 0.0);
+        expect(rec < 64 * 60 * u, "QR_Givens 60x35 ||QR-A||/||A||", rec, 64 * 60 * u);
+        expect(orth < 64 * 60 * u, "QR_Givens 60x35 ||Q^TQ-I||", orth, 64 * 60 * u);
+        MAT<double> G, S;
+        QR<QR_GramSchmidt>(A, G, S);
+        double num = 0, den = 0, o = 0, dmin = 1e300, lowgs = 0;
+        for (int c = 0; c < 35; ++c) {
+            dmin = std::min(dmin, S(c, c));
+            for (int r = 0; r < 60; ++r) {
+                double acc = 0;
+                for (int k = 0; k < 35; ++k) acc += G(r, k) * S(k, c);
+                num += (acc - A(r, c)) * (acc - A(r, c));
+                den += A(r, c) * A(r, c);
+                if (r > c) lowgs = std::max(lowgs, std::fabs(S(r, c)));
+            }
+            for (int k = 0; k < 35; ++k) {
+                double acc = 0;
+                for (int r = 0; r < 60; ++r) acc += G(r, k) * G(r, c);
+                const double e = acc - (k == c ? 1.0 : 0.0);
+                o += e * e;
+            }
         }
-        expect(threw, "QR_Givens raises mat::Error (unsupported)", 0.0, 0.0);
+        expect(G.rows == 60 && G.cols == 35 && S.rows == 60 && S.cols == 35,
+               "QR_GramSchmidt shapes (Q m x n)", 0.0, 0.0);
+        expect(std::sqrt(num / den) < 64 * 60 * u, "QR_GramSchmidt ||QR-A||/||A||",
+               std::sqrt(num / den), 64 * 60 * u);
+        expect(std::sqrt(o) < 64 * 60 * u, "QR_GramSchmidt ||Q^TQ-I||", std::sqrt(o), 64 * 60 * u);
+        expect(dmin > 0 && lowgs == 0.0, "QR_GramSchmidt R upper, positive diagonal", dmin, 0.0);
     }
     {
         // operator* on the GPU: the reference's own reconstruction check

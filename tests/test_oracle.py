@@ -376,3 +376,101 @@ def test_restatement_under_sanitizers():
                            timeout=600)
         assert r.returncode == 0, r.stdout + r.stderr
         assert "all checks passed" in r.stdout
+
+
+# ---------------------------------------------------------------------------
+# the other QR_type values: Givens (QR/__Givens.h:45-63) and modified
+# Gram-Schmidt (QR/__GramSchmidt.h:2-18)
+# ---------------------------------------------------------------------------
+
+def _row_signs(R, Rl, k):
+    d = np.sign(np.diag(R)[:k]) * np.sign(np.diag(Rl)[:k])
+    d[d == 0] = 1.0
+    return d
+
+
+@pytest.mark.parametrize("m,n", [(2, 1), (7, 4), (40, 40), (90, 33), (25, 60)])
+def test_givens_invariants_and_lapack_r(orc, m, n):
+    rng = np.random.default_rng(m * 31 + n)
+    A = rng.standard_normal((m, n))
+    Q, R = orc.qr_givens(A)
+    nA = np.linalg.norm(A)
+    k = min(m, n)
+    assert np.linalg.norm(Q @ R - A) / nA < 16 * max(m, n) * EPS
+    assert np.linalg.norm(Q.T @ Q - np.eye(m)) < 16 * m * EPS
+    # below the diagonal only the rotations' rounding residue remains
+    assert np.abs(np.tril(R, -1)).max() < 16 * max(m, n) * EPS * nA
+    # R is unique up to row signs: equal to LAPACK's after fixing them
+    Rl = np.triu(sla.qr(A, mode="r")[0])
+    d = _row_signs(R, Rl, k)
+    np.testing.assert_allclose(d[:, None] * np.triu(R)[:k], Rl[:k], rtol=0,
+                               atol=64 * max(m, n) * EPS * nA)
+    # each column's sweep keeps the sign of its pivot: R(0,0) = sign(a00) ||a_0||
+    assert np.sign(R[0, 0]) == np.sign(A[0, 0])
+    assert abs(abs(R[0, 0]) - np.linalg.norm(A[:, 0])) < 8 * m * EPS * nA
+
+
+def test_givens_zero_pivots(orc):
+    """The source divides by a zero pivot (NaN spreads); the GPU path's
+    documented fix gives exact results on triangular inputs."""
+    A = np.triu(np.random.default_rng(5).standard_normal((6, 4)))
+    Q, R = orc.qr_givens(A)
+    assert np.isnan(R).any() or np.isnan(Q).any()
+    Q, R = orc.qr_givens(A, fix_zero_pivots=True)
+    assert np.array_equal(R, A) and np.array_equal(Q, np.eye(6))
+    # a zero pivot over a nonzero entry: rows swapped with the sign chosen so
+    # that the new pivot is |b|
+    B = np.array([[0.0, 1.0], [-3.0, 2.0]])
+    Q, R = orc.qr_givens(B, fix_zero_pivots=True)
+    assert R[0, 0] == 3.0 and R[1, 0] == 0.0
+    np.testing.assert_allclose(Q @ R, B, atol=1e-15)
+    np.testing.assert_allclose(Q.T @ Q, np.eye(2), atol=1e-15)
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (7, 4), (40, 40), (300, 70)])
+def test_gramschmidt_invariants_and_positive_r(orc, m, n):
+    rng = np.random.default_rng(m * 17 + n)
+    A = rng.standard_normal((m, n))
+    Q, R = orc.qr_gramschmidt(A)
+    nA = np.linalg.norm(A)
+    assert Q.shape == (m, n) and R.shape == (m, n)
+    assert np.all(R[n:] == 0.0) and np.all(np.tril(R, -1) == 0.0)
+    assert np.all(np.diag(R) > 0.0)
+    assert np.linalg.norm(Q @ R[:n] - A) / nA < 16 * m * EPS
+    assert np.linalg.norm(Q.T @ Q - np.eye(n)) < 64 * m * EPS * np.linalg.cond(A)
+    # the thin QR with a positive diagonal is unique: LAPACK's, signs fixed
+    Rl = np.triu(sla.qr(A, mode="economic")[1])
+    Rl = np.sign(np.diag(Rl))[:, None] * Rl
+    np.testing.assert_allclose(R[:n], Rl, rtol=0, atol=64 * m * EPS * nA * np.linalg.cond(A))
+
+
+def test_gramschmidt_is_the_modified_variant(orc):
+    """Lauchli's matrix with eps^2 below the unit roundoff: classical
+    Gram-Schmidt loses orthogonality completely (q2^T q3 = 1/2), the
+    modified form of the source keeps it at O(eps)."""
+    e = 1e-9
+    A = np.array([[1.0, 1.0, 1.0], [e, 0, 0], [0, e, 0], [0, 0, e]])
+    Q, _ = orc.qr_gramschmidt(A)
+    assert abs(Q[:, 1] @ Q[:, 2]) < 1e-6
+    # the planted classical variant (every projection against the original
+    # column) is caught by the same check
+    Qc = np.zeros_like(A)
+    for j in range(3):
+        v = A[:, j] - Qc[:, :j] @ (Qc[:, :j].T @ A[:, j])
+        Qc[:, j] = v / np.linalg.norm(v)
+    assert abs(Qc[:, 1] @ Qc[:, 2]) > 0.4
+
+
+def test_planted_mistakes_in_givens_and_gs_are_caught():
+    o = Oracle("avx2")
+    A = np.random.default_rng(21).standard_normal((12, 7))
+    Q, R = o.qr_givens(A)
+    assert np.linalg.norm(Q @ R - A) < 1e-13
+    assert np.linalg.norm(Q.T @ R - A) > 1e-2          # Q transposed
+    Rb = R.copy()
+    Rb[3] = -Rb[3]
+    assert np.linalg.norm(Q @ Rb - A) > 1e-2           # one row sign flipped
+    Qg, Rg = o.qr_gramschmidt(A)
+    Rg_bad = Rg.copy()
+    Rg_bad[1, 4] = 0.0                                  # dropped projection coefficient
+    assert np.linalg.norm(Qg @ Rg_bad[:7] - A) > 1e-3
