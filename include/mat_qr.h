@@ -79,7 +79,11 @@ int mqr_destroy(mqr_handle_t handle);
 int mqr_set_stream(mqr_handle_t handle, void *stream);
 int mqr_get_stream(mqr_handle_t handle, void **stream);
 /* Block size nb of the blocked factorisation (the reference's __BlockSize_QR,
- * Mat.h:79-81,105-107); 16 <= nb <= 64, multiple of 16. */
+ * Mat.h:79-81,105-107): 16, 32, 48 or 64 (panels of nb columns), or 128,
+ * 192 or 256 (two-level blocking: panels of 64 columns, each nb-column block
+ * applied to the far columns as one block reflector; the T blocks are then
+ * nb x nb).  The default 64 also applies groups of four panels to the far
+ * columns at once, keeping its T blocks 64 wide. */
 int mqr_set_block_size(mqr_handle_t handle, int nb);
 int mqr_get_block_size(mqr_handle_t handle, int *nb);
 int mqr_synchronize(mqr_handle_t handle);
@@ -111,7 +115,8 @@ typedef enum {
     MQR_KIND_CAST = 8,        /* precision conversions */
     MQR_KIND_OTHER = 9,
     MQR_KIND_LOOKAHEAD = 10,  /* next-block update on the critical stream */
-    MQR_KIND_APPLY_Q = 11     /* Q / Q^T applications (ormqr, orgqr) */
+    MQR_KIND_APPLY_Q = 11,    /* Q / Q^T applications (ormqr, orgqr) */
+    MQR_KIND_NEAR_UPDATE = 12 /* a panel's update of the columns up to the next outer block's end */
 } mqr_kernel_kind_t;
 
 /* Turn per-launch HIP event timing on or off for calls on this handle

@@ -108,7 +108,7 @@ class KernelStat(C.Structure):
 
 # kernel kinds of the profiler, in the order of their mask bit (mqr_kernel_kind_t)
 KINDS = ("panel_gemm", "vta_gemm", "panel_small", "w2", "update_gemm", "trsm", "gemm", "resid",
-         "cast", "other", "lookahead", "apply_q")
+         "cast", "other", "lookahead", "apply_q", "near_update")
 
 
 def profile_enable(h: "Handle", on: bool = True, kinds=None) -> None:

@@ -69,6 +69,7 @@ int ws_reserve(mqr_handle_t h, Workspace &ws, size_t bytes) {
     if (ws.bytes >= bytes) return MQR_SUCCESS;
     MQR_HIP(hipStreamSynchronize(h->stream));
     MQR_HIP(hipStreamSynchronize(h->side_stream));
+    MQR_HIP(hipStreamSynchronize(h->far_stream));
     if (ws.ptr) {
         MQR_HIP(hipFree(ws.ptr));
         ws.ptr = nullptr;
@@ -179,6 +180,7 @@ int mqr_create(mqr_handle_t *handle, int device) {
     // second stream and events of the look-ahead schedule (default priority)
     h->user_stream = h->own_stream;
     if (hipStreamCreateWithFlags(&h->side_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&h->far_stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&h->ev_in, hipEventDisableTiming) != hipSuccess ||
@@ -187,6 +189,7 @@ int mqr_create(mqr_handle_t *handle, int device) {
         for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out})
             if (e) (void)hipEventDestroy(e);
         if (h->side_stream) (void)hipStreamDestroy(h->side_stream);
+        if (h->far_stream) (void)hipStreamDestroy(h->far_stream);
         (void)hipStreamDestroy(h->own_stream);
         delete h;
         return MQR_ERR_HIP;
@@ -199,6 +202,7 @@ int mqr_create(mqr_handle_t *handle, int device) {
         hipSuccess) {
         for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out}) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(h->side_stream);
+        (void)hipStreamDestroy(h->far_stream);
         (void)hipStreamDestroy(h->own_stream);
         delete h;
         return MQR_ERR_ALLOC;
@@ -209,6 +213,7 @@ int mqr_create(mqr_handle_t *handle, int device) {
         (void)hipHostFree(h->host_pinned);
         for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out}) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(h->side_stream);
+        (void)hipStreamDestroy(h->far_stream);
         (void)hipStreamDestroy(h->own_stream);
         delete h;
         return MQR_ERR_HIP;
@@ -221,6 +226,7 @@ int mqr_destroy(mqr_handle_t h) {
     MQR_GUARD(h);
     (void)hipStreamSynchronize(h->stream);
     (void)hipStreamSynchronize(h->side_stream);
+    (void)hipStreamSynchronize(h->far_stream);
     if (h->user_stream != h->stream) (void)hipStreamSynchronize(h->user_stream);
     if (h->ws.ptr) (void)hipFree(h->ws.ptr);
     if (h->ws_ir.ptr) (void)hipFree(h->ws_ir.ptr);
@@ -229,6 +235,7 @@ int mqr_destroy(mqr_handle_t h) {
     for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out})
         if (e) (void)hipEventDestroy(e);
     if (h->side_stream) (void)hipStreamDestroy(h->side_stream);
+    if (h->far_stream) (void)hipStreamDestroy(h->far_stream);
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     delete h;
     return MQR_SUCCESS;
@@ -249,7 +256,9 @@ int mqr_get_stream(mqr_handle_t h, void **stream) {
 
 int mqr_set_block_size(mqr_handle_t h, int nb) {
     if (!h) return -1;
-    if (nb < 16 || nb > kMaxNb || nb % 16 != 0) return -2;
+    // 16..64 in steps of 16: panels of nb; 128, 192, 256: outer blocks of
+    // 64-column panels (two-level blocking, T blocks nb wide)
+    if (nb < 16 || nb > kMaxOuterNb || nb % 16 != 0 || (nb > kMaxNb && nb % kMaxNb != 0)) return -2;
     h->nb = nb;
     return MQR_SUCCESS;
 }
@@ -265,6 +274,7 @@ int mqr_synchronize(mqr_handle_t h) {
     MQR_GUARD(h);
     MQR_HIP(hipStreamSynchronize(h->stream));
     MQR_HIP(hipStreamSynchronize(h->side_stream));
+    MQR_HIP(hipStreamSynchronize(h->far_stream));
     if (h->user_stream != h->stream) MQR_HIP(hipStreamSynchronize(h->user_stream));
     return MQR_SUCCESS;
 }

@@ -176,19 +176,61 @@ int apply_block(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double 
 
 }  // namespace
 
+namespace {
+int merged_t(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double *V, int64_t ldv,
+             const double *T, int64_t ldt, int64_t j0, double *G, double *Tb, double *tmp,
+             double *P, int64_t pcap);
+}  // namespace
+
 int form_y_f64(hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv, const double *Tp,
                int64_t ldt, bool trans, double *Y) {
     return form_y(st, h, b, Vp, ldv, Tp, ldt, trans, Y);
 }
 
-// Workspace (doubles) the factorisation of an m x n matrix uses: split-K
-// partials of the trailing update and of the next-block update, two Y
-// factors, and the tall-panel scratch.
+// Workspace (doubles) the factorisation of an m x n matrix uses with panels
+// of ib columns and outer blocks of ob: split-K partials of the near updates
+// (side stream), of the deferred and merged updates (far stream) and of the
+// next-block update, a ring of Y factors, the merged block reflector
+// (G, T, its Y), the next block's W partials and the tall-panel scratch.
 static int64_t w_parts_doubles(int64_t m) { return ((m + 63) / 64) * (int64_t)kMaxNb * kMaxNb; }
 
-static int64_t geqrf_ws_doubles(int64_t m, int64_t n, int nb) {
-    return (int64_t)16 * nb * (n + nb) + (int64_t)64 * nb * nb + (int64_t)2 * m * nb + 4096 +
-           w_parts_doubles(m) + 64 + (int64_t)(panel_cholqr_ws_bytes(m) / sizeof(double)) + 64;
+constexpr int kYRing = 8;  // Y factors kept alive: two outer blocks of four panels
+
+struct GeqrfWs {
+    double *P, *P2, *P3, *P4, *Wp, *G, *Tb, *tmp, *Yo;
+    double *Y[kYRing];
+    int64_t pcap, pcap2, pcap3, pcap4;
+    void *pws;
+    size_t doubles;
+};
+
+static GeqrfWs geqrf_carve(double *base, int64_t m, int64_t n, int ib, int ob) {
+    GeqrfWs w{};
+    double *p = base;
+    auto take = [&](int64_t cnt) {
+        double *q = p;
+        p = (double *)(((uintptr_t)(p + cnt) + 255) & ~(uintptr_t)255);
+        return q;
+    };
+    w.pcap = (int64_t)16 * ib * (n + ib);
+    w.pcap2 = (int64_t)64 * ib * ib;
+    w.P = take(w.pcap);
+    w.P2 = take(w.pcap2);
+    for (double *&y : w.Y) y = take(m * ib);
+    w.Wp = take(w_parts_doubles(m) + 1024);
+    if (ob > ib) {
+        w.pcap3 = std::max<int64_t>((int64_t)8 * ob * (n + ob), (int64_t)64 * ob * ob);
+        w.pcap4 = (int64_t)16 * ib * (ob + ib);
+        w.P3 = take(w.pcap3);
+        w.P4 = take(w.pcap4);
+        w.G = take((int64_t)ob * ob);
+        w.Tb = take((int64_t)ob * ob);
+        w.tmp = take((int64_t)ob * ob / 2 + ob);
+        w.Yo = take(m * ob);
+    }
+    w.pws = (void *)p;
+    w.doubles = (size_t)(p - base) + panel_cholqr_ws_bytes(m) / sizeof(double) + 64;
+    return w;
 }
 
 int panel_factor(mqr_handle_t hd, int64_t h, int b, double *V, int64_t lda, double *tau,
@@ -197,142 +239,274 @@ int panel_factor(mqr_handle_t hd, int64_t h, int b, double *V, int64_t lda, doub
     return panel_householder(h, b, V, lda, tau, T, ldt, hd->stream);
 }
 
-// Blocked factorisation with a look-ahead of one block: after panel j, the
-// handle's stream applies block j to the next block's columns and factors
-// the next panel, while the side stream applies block j to the remaining
-// columns.  Events order the two: the side stream starts block j's update
-// once panel j is done, and the handle's stream waits for it before touching
-// the next block again.  Per step the cost is max(rest update, next update +
-// panel) instead of their sum.
-// Blocked factorisation with a look-ahead of one block: after panel j, the
-// handle's stream applies block j to the next block's columns and factors
-// the next panel, while the side stream applies block j to the remaining
-// columns.  Events order the two: the side stream starts block j's update
-// once panel j (and its Y) is done, and the handle's stream waits for it
-// before touching the next block again.  Per step the cost is
-// max(rest update, next update + panel) instead of their sum.
 int geqrf_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, double *tau,
               double *T, int64_t ldt) {
     return geqrf_rhs_f64(hd, m, n, A, lda, tau, T, ldt, nullptr, 0, 0);
 }
 
-// The same, applying every block reflector also to the m x nrhs matrix B
-// (ldb), which ends as Q^T B.  B rides along on the side stream with the
-// trailing update, so Q^T B costs no separate pass over the reflectors.
+namespace {
+
+// Events of one factorisation, destroyed on every return path.
+struct FactorEvents {
+    hipEvent_t far_done[kYRing] = {};  // far stream, after panel q's deferred update
+    hipEvent_t ready = nullptr;        // far stream, end of its work
+    ~FactorEvents() {
+        for (hipEvent_t &x : far_done)
+            if (x) (void)hipEventDestroy(x);
+        if (ready) (void)hipEventDestroy(ready);
+    }
+    int create() {
+        for (hipEvent_t &x : far_done)
+            MQR_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+        MQR_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+        return MQR_SUCCESS;
+    }
+};
+
+}  // namespace
+
+// Blocked factorisation (QR/__Householder_Blocked.h:214-245) of A in place,
+// applying every block reflector also to the m x nrhs matrix B (ldb), which
+// ends as Q^T B.
+//
+// Panels of ib = min(nb, 64) columns, with a look-ahead of one panel: the
+// handle's stream applies panel j to the next panel's columns and factors
+// that panel (the critical chain) while other streams update the rest.
+//
+// Two-level blocking (nb = 64: outer blocks of 256 columns; nb = 128, 192,
+// 256: outer blocks of nb): for panel j of outer block J (columns
+// [c0, c1), the next block [c1, c2)):
+//   side stream  applies panel j at K = ib to the rest of block J,
+//   far stream   applies it to block J+1 (the deferred part), and after J's
+//                last panel applies block J as ONE block reflector (T merged
+//                from the panels' T blocks) to the far columns [c2, n) and to
+//                B, at K = ob instead of ob / ib passes at K = ib.
+// Far columns are thus read and written once per outer block.  The chain
+// waits for the far stream only where its next panel starts a new outer
+// block (that block's columns then need block J-1's merged update and block
+// J's deferred parts), one outer block after the merged update was queued.
+// With nb < 64 there is one level: the side stream applies each panel to
+// every remaining column and to B.
+// With nb > 64 the merged T of each outer block is what T returns (nb x nb
+// blocks in the dgeqrt layout); otherwise T holds the panels' blocks.
 int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, double *tau,
                   double *T, int64_t ldt, double *B, int64_t ldb, int64_t nrhs) {
     if (nrhs <= 0) B = nullptr;
     const int64_t k = std::min(m, n);
-    const int nb = hd->nb;
-    const int64_t pcap = (int64_t)16 * nb * (n + nb);
-    const int64_t pcap2 = (int64_t)64 * nb * nb;
-    MQR_CHECK(ws_reserve(hd, hd->ws, sizeof(double) * geqrf_ws_doubles(m, n, nb)));
-    double *P = (double *)hd->ws.ptr;
-    double *P2 = P + pcap;
-    double *Ybuf[2] = {P2 + pcap2, P2 + pcap2 + m * nb};
-    double *Wp = (double *)(((uintptr_t)(Ybuf[1] + m * nb) + 255) & ~(uintptr_t)255);
-    void *pws = (void *)(((uintptr_t)(Wp + w_parts_doubles(m) + 1024) + 255) & ~(uintptr_t)255);
-    hipStream_t s0 = hd->stream, s1 = hd->side_stream;
     if (k == 0) return MQR_SUCCESS;
-    // the side stream runs after everything queued on the handle's stream
+    const int nbt = hd->nb;                              // T block width
+    const int ib = std::min(nbt, kMaxNb);                // panel width
+    const int ob = nbt > kMaxNb ? nbt : (nbt == kMaxNb ? kDefaultOuter : ib);
+    const bool two = ob > ib;
+    GeqrfWs w = geqrf_carve(nullptr, m, n, ib, ob);
+    MQR_CHECK(ws_reserve(hd, hd->ws, sizeof(double) * w.doubles));
+    w = geqrf_carve((double *)hd->ws.ptr, m, n, ib, ob);
+    FactorEvents ev;
+    if (two) MQR_CHECK(ev.create());
+    hipStream_t s0 = hd->stream, s1 = hd->side_stream, s2 = hd->far_stream;
+    // the side and far streams run after everything queued on the handle's stream
     MQR_HIP(hipEventRecord(hd->ev_fork, s0));
     MQR_HIP(hipStreamWaitEvent(s1, hd->ev_fork, 0));
+    if (two) MQR_HIP(hipStreamWaitEvent(s2, hd->ev_fork, 0));
+    // outer block of column c: [c0, c1); the block after it ends at c2
+    auto c1_of = [&](int64_t c) { return two ? std::min(n, (c / ob + 1) * ob) : n; };
+    auto c2_of = [&](int64_t c) { return two ? std::min(n, (c / ob + 2) * ob) : n; };
     bool side_pending = false;  // the side stream's last update is not yet joined
+    int64_t merged_upto = 0;    // outer blocks [0, merged_upto) have had their merged update
+    int64_t s1_synced = 0;      // the side stream has waited for the far stream up to this block
+    // the far stream's merged update of outer block J (all its panels
+    // factored): T of the block, Y = V T^T, then [c2, n) and B.  Returns the
+    // block's column range end.
+    auto merge_block = [&](int64_t J) -> int {
+        const int64_t c0 = J * ob, bo = std::min<int64_t>(ob, k - c0);
+        const int64_t h = m - c0;
+        const int64_t c2 = std::min(n, c0 + 2 * ob);
+        const double *V = A + c0 + c0 * lda;
+        const bool far = c2 < n;
+        if (!far && !B && nbt <= kMaxNb) return MQR_SUCCESS;
+        const double *Tj = T + c0 * ldt;
+        int64_t ldtj = ldt;
+        if (bo > ib) {
+            MQR_CHECK(merged_t(hd, s2, h, (int)bo, V, lda, T, ldt, c0, w.G, w.Tb, w.tmp, w.P3,
+                               w.pcap3));
+            Tj = w.Tb;
+            ldtj = bo;
+        }
+        if (far || B) {
+            MQR_CHECK(form_y(s2, h, (int)bo, V, lda, Tj, ldtj, true, w.Yo));
+            if (far)
+                MQR_CHECK(apply_block(hd, s2, h, (int)bo, V, lda, w.Yo, A + c0 + c2 * lda, lda,
+                                      n - c2, w.P3, w.pcap3));
+            if (B)
+                MQR_CHECK(apply_block(hd, s2, h, (int)bo, V, lda, w.Yo, B + c0, ldb, nrhs, w.P3,
+                                      w.pcap3, K_APPLYQ, K_APPLYQ));
+        }
+        // nb > 64: T returns the merged nb x nb blocks
+        if (nbt > kMaxNb && bo > ib)
+            MQR_HIP(hipMemcpy2DAsync(T + c0 * ldt, sizeof(double) * ldt, w.Tb,
+                                     sizeof(double) * bo, sizeof(double) * bo, bo,
+                                     hipMemcpyDeviceToDevice, s2));
+        return MQR_SUCCESS;
+    };
     // a tall full-width panel runs the fused CholeskyQR sequence, which also
-    // forms Y = V T^T, takes the next-block update into its first launch and
-    // forms the following block's W = V^T C in its last
+    // forms Y = V T^T, takes the next-panel update into its first launch and
+    // forms the following panel's W = V^T C in its last
     auto fused = [&](int64_t hp, int bp, int64_t bcols) {
         return hp >= kCholqrMinRows && bp == bcols;
     };
     auto fused_at = [&](int64_t c) {  // the panel starting at column c
         if (c >= k || c >= n) return false;
-        return fused(m - c, (int)std::min<int64_t>(nb, k - c), std::min<int64_t>(nb, n - c));
+        return fused(m - c, (int)std::min<int64_t>(ib, k - c), std::min<int64_t>(ib, n - c));
     };
-    bool w_ready = false;  // Wp[0:4096] = V^T C of the current panel and the next block
-    int64_t b_next = 0;    // first panel column whose block is not yet applied to B
-    const int b0 = (int)std::min<int64_t>(nb, k);
-    if (fused(m, b0, b0)) {
-        MQR_CHECK(cholqr_gram1(s0, m, b0, 0, nullptr, 0, nullptr, 0, A, lda, 0, pws));
-        MQR_CHECK(cholqr_rest_a(s0, m, b0, A, lda, tau, T, ldt, pws));
+    bool w_ready = false;  // Wp[0:4096] = V^T C of the current panel and the next one
+    int64_t b_next = 0;    // first panel column whose block is not yet applied to B (one level)
+    const int b0 = (int)std::min<int64_t>(ib, k);
+    if (fused(m, b0, std::min<int64_t>(ib, n))) {
+        MQR_CHECK(cholqr_gram1(s0, m, b0, 0, nullptr, 0, nullptr, 0, A, lda, 0, w.pws));
+        MQR_CHECK(cholqr_rest_a(s0, m, b0, A, lda, tau, T, ldt, w.pws));
         w_ready = b0 < n && fused_at(b0);
-        MQR_CHECK(cholqr_finish(s0, m, b0, A, lda, T, ldt, b0 < n ? Ybuf[0] : nullptr,
-                                A + b0 * lda, lda, (int)std::min<int64_t>(nb, n - b0),
-                                w_ready ? Wp : nullptr, pws));
+        MQR_CHECK(cholqr_finish(s0, m, b0, A, lda, T, ldt, b0 < n ? w.Y[0] : nullptr,
+                                A + b0 * lda, lda, (int)std::min<int64_t>(ib, n - b0),
+                                w_ready ? w.Wp : nullptr, w.pws));
     } else {
-        MQR_CHECK(panel_factor(hd, m, b0, A, lda, tau, T, ldt, pws));
-        if (b0 < n) MQR_CHECK(form_y(s0, m, b0, A, lda, T, ldt, true, Ybuf[0]));
+        MQR_CHECK(panel_factor(hd, m, b0, A, lda, tau, T, ldt, w.pws));
+        if (b0 < n) MQR_CHECK(form_y(s0, m, b0, A, lda, T, ldt, true, w.Y[0]));
     }
-    for (int64_t j = 0, it = 0; j < k; j += nb, ++it) {
-        const int b = (int)std::min<int64_t>(nb, k - j);
+    for (int64_t j = 0, it = 0; j < k; j += ib, ++it) {
+        const int b = (int)std::min<int64_t>(ib, k - j);
         const int64_t h = m - j;
         const double *V = A + j + j * lda;
-        const double *Y = Ybuf[it & 1];
+        const double *Y = w.Y[it % kYRing];
         const int64_t next = j + b;
-        if (next >= n) break;
-        const int64_t bn = std::min<int64_t>(nb, n - next);
-        const int64_t rest0 = next + bn, nrest = n - rest0;
-        // block j-1's update of the remaining columns covers the next block
+        const int64_t c1 = c1_of(j), c2 = c2_of(j);
+        const bool last_of_block = next >= c1 || next >= k;
+        if (next >= n) {
+            if (two && last_of_block) {
+                MQR_HIP(hipEventRecord(hd->ev_fork, s0));
+                MQR_HIP(hipStreamWaitEvent(s2, hd->ev_fork, 0));
+                MQR_CHECK(merge_block(j / ob));
+                merged_upto = j / ob + 1;
+            }
+            break;
+        }
+        const int64_t bn = std::min<int64_t>(ib, n - next);
+        const int64_t rest0 = next + bn;
+        const int64_t near_end = std::max(rest0, c1);  // side stream: [rest0, near_end)
+        const int64_t def_end = std::max(near_end, c2); // far stream: [near_end, def_end)
+        // panel j-1's side-stream update covers the next panel's columns
         if (side_pending) {
             MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
             side_pending = false;
         }
-        if (nrest > 0 || B) {
+        // the far stream's deferred parts of the previous panels reach the
+        // next panel's columns when the next panel starts a new outer block
+        // (the previous block's merged update and this block's deferred
+        // parts), and when this panel is the first of its block (the
+        // previous block's last deferred part)
+        if (two && it > 0 && (j % ob == 0 || next >= c1))
+            MQR_HIP(hipStreamWaitEvent(s0, ev.far_done[(it - 1) % kYRing], 0));
+        const bool side_work = near_end > rest0 || (!two && B);
+        const bool def_work = two && def_end > near_end;
+        if (side_work || def_work || (two && last_of_block))
             MQR_HIP(hipEventRecord(hd->ev_fork, s0));
+        if (side_work) {
             MQR_HIP(hipStreamWaitEvent(s1, hd->ev_fork, 0));
-            if (nrest > 0)
-                MQR_CHECK(apply_block(hd, s1, h, b, V, lda, Y, A + j + rest0 * lda, lda, nrest, P,
-                                      pcap));
-            if (B) {
-                MQR_CHECK(apply_block(hd, s1, h, b, V, lda, Y, B + j, ldb, nrhs, P, pcap, K_APPLYQ,
-                                      K_APPLYQ));
+            // the first near update of an outer block: its columns had the
+            // previous block's deferred parts on the far stream (the last
+            // of them recorded for the previous block's last panel)
+            const int64_t blk = j / ob;
+            if (two && blk > 0 && s1_synced < blk) {
+                const int64_t last_prev = blk * (ob / ib) - 1;
+                MQR_HIP(hipStreamWaitEvent(s1, ev.far_done[last_prev % kYRing], 0));
+                s1_synced = blk;
+            }
+            if (near_end > rest0)
+                MQR_CHECK(apply_block(hd, s1, h, b, V, lda, Y, A + j + rest0 * lda, lda,
+                                      near_end - rest0, w.P, w.pcap, two ? K_NEAR : K_VTA,
+                                      two ? K_NEAR : K_UPDATE));
+            if (!two && B) {
+                MQR_CHECK(apply_block(hd, s1, h, b, V, lda, Y, B + j, ldb, nrhs, w.P, w.pcap,
+                                      K_APPLYQ, K_APPLYQ));
                 b_next = j + b;
             }
             MQR_HIP(hipEventRecord(hd->ev_join, s1));
             side_pending = true;
         }
-        const int bnext = next < k ? (int)std::min<int64_t>(nb, k - next) : 0;
+        if (two) {
+            if (def_work || last_of_block) MQR_HIP(hipStreamWaitEvent(s2, hd->ev_fork, 0));
+            if (def_work)
+                MQR_CHECK(apply_block(hd, s2, h, b, V, lda, Y, A + j + near_end * lda, lda,
+                                      def_end - near_end, w.P4, w.pcap4, K_NEAR, K_NEAR));
+            // recorded for every panel: the far stream's work through this
+            // panel's deferred part, before the merged update below
+            MQR_HIP(hipEventRecord(ev.far_done[it % kYRing], s2));
+            if (last_of_block) {
+                MQR_CHECK(merge_block(j / ob));
+                merged_upto = j / ob + 1;
+            }
+        }
+        const int bnext = next < k ? (int)std::min<int64_t>(ib, k - next) : 0;
         if (next < k && fused(m - next, bnext, bn)) {
             double *Cn = A + j + next * lda;
             if (!w_ready)
-                MQR_CHECK(block_w(hd, s0, h, b, V, lda, Cn, lda, bn, P2, pcap2, K_LOOKAHEAD));
-            MQR_CHECK(cholqr_gram1(s0, h, (int)bn, b, Y, h, w_ready ? Wp : P2,
-                                   w_ready ? (int64_t)kMaxNb : (int64_t)b, Cn, lda, b, pws));
+                MQR_CHECK(block_w(hd, s0, h, b, V, lda, Cn, lda, bn, w.P2, w.pcap2, K_LOOKAHEAD));
+            MQR_CHECK(cholqr_gram1(s0, h, (int)bn, b, Y, h, w_ready ? w.Wp : w.P2,
+                                   w_ready ? (int64_t)kMaxNb : (int64_t)b, Cn, lda, b, w.pws));
             double *Xn = Cn + b;
             MQR_CHECK(cholqr_rest_a(s0, m - next, bnext, Xn, lda, tau + next, T + next * ldt, ldt,
-                                    pws));
-            // W of the block after next needs this step's update of the
-            // remaining columns (side stream) first
+                                    w.pws));
+            // W of the panel after next needs this step's update of its
+            // columns first: the side stream's (same outer block) or the far
+            // stream's deferred part (next outer block)
             const int64_t next2 = next + bnext;
             w_ready = next2 < n && fused_at(next2);
-            if (w_ready && side_pending) {
-                MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
-                side_pending = false;
+            if (w_ready) {
+                if (side_pending) {
+                    MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
+                    side_pending = false;
+                }
+                if (two && next2 >= c1) MQR_HIP(hipStreamWaitEvent(s0, ev.far_done[it % kYRing], 0));
             }
             MQR_CHECK(cholqr_finish(s0, m - next, bnext, Xn, lda, T + next * ldt, ldt,
-                                    next2 < n ? Ybuf[(it + 1) & 1] : nullptr, A + next + next2 * lda,
-                                    lda, (int)std::min<int64_t>(nb, std::max<int64_t>(n - next2, 0)),
-                                    w_ready ? Wp : nullptr, pws));
+                                    next2 < n ? w.Y[(it + 1) % kYRing] : nullptr,
+                                    A + next + next2 * lda, lda,
+                                    (int)std::min<int64_t>(ib, std::max<int64_t>(n - next2, 0)),
+                                    w_ready ? w.Wp : nullptr, w.pws));
             continue;
         }
         w_ready = false;
-        MQR_CHECK(apply_block(hd, s0, h, b, V, lda, Y, A + j + next * lda, lda, bn, P2, pcap2,
+        MQR_CHECK(apply_block(hd, s0, h, b, V, lda, Y, A + j + next * lda, lda, bn, w.P2, w.pcap2,
                               K_LOOKAHEAD, K_LOOKAHEAD));
         if (next < k) {
             double *Vn = A + next + next * lda;
             MQR_CHECK(panel_factor(hd, m - next, bnext, Vn, lda, tau + next, T + next * ldt, ldt,
-                                   pws));
+                                   w.pws));
             if (next + bnext < n)
                 MQR_CHECK(form_y(s0, m - next, bnext, Vn, lda, T + next * ldt, ldt, true,
-                                 Ybuf[(it + 1) & 1]));
+                                 w.Y[(it + 1) % kYRing]));
         }
     }
     if (side_pending) MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
-    // blocks whose step ended the loop early (the last one when m >= n)
-    for (int64_t j = b_next; B && j < k; j += nb) {
-        const int b = (int)std::min<int64_t>(nb, k - j);
+    if (two) {
+        // outer blocks whose last panel ended the loop early (m >= n: the
+        // last panels have no columns to their right) still owe B and, with
+        // nb > 64, their merged T
+        const int64_t nblk = (k + ob - 1) / ob;
+        if (merged_upto < nblk) {
+            MQR_HIP(hipEventRecord(hd->ev_fork, s0));
+            MQR_HIP(hipStreamWaitEvent(s2, hd->ev_fork, 0));
+            for (int64_t J = merged_upto; J < nblk; ++J) MQR_CHECK(merge_block(J));
+        }
+        MQR_HIP(hipEventRecord(ev.ready, s2));
+        MQR_HIP(hipStreamWaitEvent(s0, ev.ready, 0));
+        return MQR_SUCCESS;
+    }
+    // one level: blocks whose step ended the loop early (the last one when m >= n)
+    for (int64_t j = b_next; B && j < k; j += ib) {
+        const int b = (int)std::min<int64_t>(ib, k - j);
         const double *V = A + j + j * lda;
-        MQR_CHECK(form_y(s0, m - j, b, V, lda, T + j * ldt, ldt, true, Ybuf[0]));
-        MQR_CHECK(apply_block(hd, s0, m - j, b, V, lda, Ybuf[0], B + j, ldb, nrhs, P, pcap,
+        MQR_CHECK(form_y(s0, m - j, b, V, lda, T + j * ldt, ldt, true, w.Y[0]));
+        MQR_CHECK(apply_block(hd, s0, m - j, b, V, lda, w.Y[0], B + j, ldb, nrhs, w.P, w.pcap,
                               K_APPLYQ, K_APPLYQ));
     }
     return MQR_SUCCESS;

@@ -104,10 +104,12 @@ int project_out(mqr_handle_t hd, int64_t m, int64_t j0, int b, const double *Q, 
 
 int qr_gramschmidt_f64(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t lda,
                        double *Q, int64_t ldq, double *R, int64_t ldr) {
-    const int nb = hd->nb;
+    // blocks of the panel width; T is stored with the handle's block size
+    // as leading dimension, the layout the Q formation reads
+    const int nb = std::min(hd->nb, kMaxNb), nbt = hd->nb;
     hipStream_t s = hd->stream;
     const int64_t pcap = std::max<int64_t>((int64_t)64 * nb * n, 1);
-    const size_t doubles = (size_t)(m * nb + 2 * n * nb + 2 * nb * nb + 2 * nb + (int64_t)nb * nb +
+    const size_t doubles = (size_t)(m * nb + 2 * n * nb + 2 * nb * nb + 2 * nb + (int64_t)nbt * nb +
                                     pcap) +
                            panel_cholqr_ws_bytes(m) / sizeof(double) + 8 * 64;
     MQR_CHECK(ws_reserve(hd, hd->ws_ir, sizeof(double) * doubles));
@@ -125,7 +127,7 @@ int qr_gramschmidt_f64(mqr_handle_t hd, int64_t m, int64_t n, const double *A, i
     w.Rh = take((int64_t)nb * nb);
     w.d = take(nb);
     w.tau = take(nb);
-    w.T = take((int64_t)nb * nb);
+    w.T = take((int64_t)nbt * nb);
     w.P = take(pcap);
     w.pcap = pcap;
     w.pws = (void *)p;

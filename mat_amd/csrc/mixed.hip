@@ -280,7 +280,9 @@ struct MixedWs {
 
 constexpr int kMaxSplits = 64;
 
-MixedWs carve(char *base, bool fp64, int64_t m, int64_t n, int nb) {
+// nb: panel width; nbt (>= nb): block width of the T storage (the handle's
+// block size, which the FP64 factorisation writes T in)
+MixedWs carve(char *base, bool fp64, int64_t m, int64_t n, int nb, int nbt) {
     MixedWs w;
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -310,7 +312,7 @@ MixedWs carve(char *base, bool fp64, int64_t m, int64_t n, int nb) {
     }
     w.X = (double *)take(sizeof(double) * n * n);
     w.tmp = (double *)take(sizeof(double) * ((n + 1) / 2) * n);
-    w.T = (double *)take(sizeof(double) * nb * n);
+    w.T = (double *)take(sizeof(double) * nbt * n);
     w.tau = (double *)take(sizeof(double) * n);
     w.r = (double *)take(sizeof(double) * m);
     w.z = (double *)take(sizeof(double) * n);
@@ -359,7 +361,7 @@ int apply_block_f32(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const flo
 int panel_fp32(mqr_handle_t hd, int64_t m, int64_t j, int b, MixedWs &w, double *V, float *V32,
                float *Y32) {
     hipStream_t s = hd->stream;
-    const int nb = hd->nb;
+    const int nb = std::min(hd->nb, kMaxNb);
     const int64_t h = m - j;
     {
         ProfScope ps(K_CAST, 0.0, 12.0 * h * b, s);
@@ -390,7 +392,7 @@ int panel_fp32(mqr_handle_t hd, int64_t m, int64_t j, int b, MixedWs &w, double 
 int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t lda,
                 const double *b, MixedWs &w, int bf16_terms) {
     hipStream_t s0 = hd->stream, s1 = hd->side_stream;
-    const int nb = hd->nb;
+    const int nb = std::min(hd->nb, kMaxNb);  // panel width; T internal with ld nb
     const int64_t k = std::min(m, n), nc = n + 1;
     const int64_t pcap = (int64_t)kMaxSplits * nb * nc, pcap2 = (int64_t)kMaxSplits * nb * nb;
     {
@@ -668,11 +670,11 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
     const double tol = (opts && opts->tol > 0.0) ? opts->tol : 1e-14;
     const bool fallback = opts ? opts->fallback != 0 : true;
     hipStream_t s = hd->stream;
-    const int nb = hd->nb;
+    const int nb = std::min(hd->nb, kMaxNb), nbt = hd->nb;
     const bool fp64 = prec == MQR_PREC_FP64;
-    MixedWs w = carve(nullptr, fp64, m, n, nb);
+    MixedWs w = carve(nullptr, fp64, m, n, nb, nbt);
     MQR_CHECK(ws_reserve(hd, hd->ws_ir, w.bytes + 4096));
-    w = carve((char *)hd->ws_ir.ptr, fp64, m, n, nb);
+    w = carve((char *)hd->ws_ir.ptr, fp64, m, n, nb, nbt);
 
     // ||A||_F^2 -> sc[0]
     {
@@ -690,7 +692,7 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
     if (fp64) {
         // b rides along as Q^T b
         MQR_HIP(hipMemcpyAsync(w.r, b, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
-        MQR_CHECK(geqrf_rhs_f64(hd, m, n, w.F, m, w.tau, w.T, nb, w.r, m, 1));
+        MQR_CHECK(geqrf_rhs_f64(hd, m, n, w.F, m, w.tau, w.T, nbt, w.r, m, 1));
     } else
         MQR_CHECK(factor_fp32(hd, m, n, A, lda, b, w,
                               prec == MQR_PREC_BF16X3 ? 3 : prec == MQR_PREC_BF16 ? 1 : 0));
@@ -749,13 +751,13 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
     if (!converged && fallback) {
         // plain FP64 solve: factor a copy of A, apply Q^T to b, back substitute
         fell_back = true;
-        MixedWs f = carve(nullptr, true, m, n, nb);
+        MixedWs f = carve(nullptr, true, m, n, nb, nbt);
         MQR_CHECK(ws_reserve(hd, hd->ws_ir, f.bytes + 4096));
-        f = carve((char *)hd->ws_ir.ptr, true, m, n, nb);
+        f = carve((char *)hd->ws_ir.ptr, true, m, n, nb, nbt);
         MQR_HIP(hipMemcpy2DAsync(f.F, sizeof(double) * m, A, sizeof(double) * lda,
                                  sizeof(double) * m, n, hipMemcpyDeviceToDevice, s));
         MQR_HIP(hipMemcpyAsync(f.r, b, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
-        MQR_CHECK(geqrf_rhs_f64(hd, m, n, f.F, m, f.tau, f.T, nb, f.r, m, 1));
+        MQR_CHECK(geqrf_rhs_f64(hd, m, n, f.F, m, f.tau, f.T, nbt, f.r, m, 1));
         MQR_CHECK(trsm_upper_f64(hd, false, n, 1, f.F, m, f.r, m));
         MQR_HIP(hipMemcpyAsync(x, f.r, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         hipLaunchKernelGGL(k_sq, dim3(grid_for(n)), dim3(256), 0, s, n, (const double *)x, f.xx);

@@ -52,7 +52,8 @@ enum KernelKind {
     K_OTHER = 9,
     K_LOOKAHEAD = 10,   // next-block update on the critical stream
     K_APPLYQ = 11,      // Q / Q^T applications (ormqr, orgqr)
-    K_COUNT = 12
+    K_NEAR = 12,        // one panel's update of the columns up to the next outer block's end
+    K_COUNT = 13
 };
 const char *kernel_kind_name(int k);
 
@@ -95,6 +96,9 @@ struct mqr_handle_st {
     // `stream` with ev_fork / ev_join; every call joins before returning
     hipStream_t side_stream = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // two-level blocking: an outer block's merged update of the far columns
+    // runs on far_stream (also default priority)
+    hipStream_t far_stream = nullptr;
     int nb = 64;                   // factorisation block size
     // block size each recent mqr_dgeqrf wrote its T blocks with (keyed by
     // the T pointer), so that a Q application after mqr_set_block_size is
@@ -253,7 +257,12 @@ int upper_inverse_levels(mqr_handle_t hd, hipStream_t s, int64_t n, const double
 // out[0:count] = sum over s < splits of P[s*stride + 0:count]; out = P when null.
 int reduce_partials(int64_t count, double *P, int splits, int64_t stride, hipStream_t s,
                     double *out = nullptr);
-// Largest block size the LDS-resident small kernels accept.
+// Largest panel width the LDS-resident small kernels accept.
 constexpr int kMaxNb = 64;
+// Largest block size mqr_set_block_size accepts: above kMaxNb it is the
+// outer block of two-level blocking (panels of kMaxNb, T blocks of nb).
+constexpr int kMaxOuterNb = 256;
+// Outer block of the default 64-column blocking (internal, T stays 64-wide).
+constexpr int kDefaultOuter = 256;
 
 }  // namespace mqr

@@ -10,7 +10,8 @@ thread_local Profiler *t_prof = nullptr;
 const char *kernel_kind_name(int k) {
     static const char *names[K_COUNT] = {"panel_gemm", "vta_gemm", "panel_small", "w2",
                                          "update_gemm", "trsm", "gemm", "resid",
-                                         "cast", "other", "lookahead", "apply_q"};
+                                         "cast", "other", "lookahead", "apply_q",
+                                         "near_update"};
     return (k >= 0 && k < K_COUNT) ? names[k] : "?";
 }
 
@@ -71,6 +72,7 @@ int mqr_profile_read(mqr_handle_t h, mqr_kernel_stat_t *stats, int max_stats, in
     if (!count) return -4;
     MQR_HIP(hipStreamSynchronize(h->stream));
     MQR_HIP(hipStreamSynchronize(h->side_stream));
+    MQR_HIP(hipStreamSynchronize(h->far_stream));
     mqr_kernel_stat_t acc[K_COUNT];
     std::memset(acc, 0, sizeof(acc));
     for (const auto &r : h->prof.recs) {

@@ -305,14 +305,17 @@ def test_tall_panel_fallback_on_dependent_columns(h, orc):
     assert abs(VR[70, 70]) < 1e-12 * np.linalg.norm(A) and abs(VR[200, 200]) < 1e-12 * np.linalg.norm(A)
 
 
-@pytest.mark.parametrize("nb", [16, 32, 48])
-def test_smaller_block_sizes_match_oracle(torch_mod, orc, nb):
-    """Block widths below 64 (mqr_set_block_size): the same R and Q as the
-    oracle, and each stored T block equal to the larft recurrence of its
-    reflectors (Q formation then applies the stored blocks one by one)."""
+@pytest.mark.parametrize("nb", [16, 32, 48, 128, 192, 256])
+def test_block_sizes_match_oracle(torch_mod, orc, nb):
+    """Every block width mqr_set_block_size takes: below 64 the panels are nb
+    wide; above, the panels are 64 wide and each nb-column block is applied
+    to the far columns as one block reflector, whose T (the larft recurrence
+    over all nb reflectors) is what the T blocks hold.  The same R and Q as
+    the oracle, and each stored T block equal to larft of its reflectors (Q
+    formation then applies the stored blocks)."""
     note(f"block size {nb}")
     with M.Handle(0, nb=nb) as hn:
-        for (m, n) in [(300, 200), (700, 150)]:
+        for (m, n) in [(300, 200), (700, 150), (900, 700)]:
             rng = np.random.default_rng(m + n + nb)
             A = rng.standard_normal((m, n))
             dA, tau, T = gpu_factor(hn, A)
