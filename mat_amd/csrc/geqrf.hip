@@ -187,6 +187,12 @@ int form_y_f64(hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv, 
     return form_y(st, h, b, Vp, ldv, Tp, ldt, trans, Y);
 }
 
+int merged_t_f64(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double *V, int64_t ldv,
+                 const double *T, int64_t ldt, int64_t j0, double *G, double *Tb, double *tmp,
+                 double *P, int64_t pcap) {
+    return merged_t(hd, st, h, b, V, ldv, T, ldt, j0, G, Tb, tmp, P, pcap);
+}
+
 // Workspace (doubles) the factorisation of an m x n matrix uses with panels
 // of ib columns and outer blocks of ob: split-K partials of the near updates
 // (side stream), of the deferred and merged updates (far stream) and of the

@@ -260,16 +260,26 @@ unsigned grid_for(int64_t count, int per = 256, int64_t cap = 8192) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cap, (count + per - 1) / per));
 }
 
+constexpr int kRing = 8;  // panels whose FP32 V and Y stay alive (two outer blocks)
+
 // Device workspace of one mixed solve (carved from the handle's IR scratch).
 struct MixedWs {
     float *Aw = nullptr;   // m x n FP32 working copy (FP32 mode)
     double *F = nullptr;   // m x n FP64 factorisation (FP64 mode)
-    double *Pb = nullptr;  // m x nb FP64 panel (FP32 mode)
     double *R = nullptr;   // R factor, ld ldr
     int64_t ldr = 0;
-    float *V32[2] = {nullptr, nullptr}, *Y32[2] = {nullptr, nullptr};
-    float *P32 = nullptr, *P32b = nullptr;
-    double *Pb2 = nullptr, *Yd = nullptr;
+    // FP32 mode: FP64 panels inside their outer block's buffer (m x ob, ld m,
+    // two buffers by block parity), FP32 copies of each panel's V and Y for
+    // the side and far streams (a ring), the merged reflector of an outer
+    // block (G, T, Y in FP64; V, Y in FP32) and split-K partials per stream
+    int ob = 0;
+    double *V64[2] = {nullptr, nullptr};
+    float *V32[kRing] = {}, *Y32[kRing] = {};
+    float *P32 = nullptr, *P32b = nullptr, *P32d = nullptr, *P32o = nullptr;
+    int64_t p32dcap = 0, p32ocap = 0, p64ocap = 0;
+    double *Yd = nullptr, *G = nullptr, *Tb = nullptr, *tmpo = nullptr, *Yo = nullptr,
+           *P64o = nullptr;
+    float *V32o = nullptr, *Y32o = nullptr;
     double *Wp = nullptr;  // partials of the next block's W (FP32 mode)
     double *X = nullptr, *T = nullptr, *tau = nullptr;
     double *r = nullptr, *z = nullptr, *y = nullptr, *part = nullptr, *sc = nullptr;
@@ -296,18 +306,38 @@ MixedWs carve(char *base, bool fp64, int64_t m, int64_t n, int nb, int nbt) {
         w.R = w.F;
         w.ldr = m;
     } else {
+        // outer blocks only when the handle's block size asks for them: at
+        // the default 64 the FP32 updates stay one-level (its K = 64 update is
+        // cheap enough in FP32 that the extra near / deferred launches and the
+        // FP64 merged T of two-level blocking cost more than they save:
+        // 16384 x 4096 mixed solve 19.9 ms one-level, 20.4 ms two-level)
+        const int ob = nbt > nb ? nbt : nb;
+        w.ob = ob;
         w.Aw = (float *)take(sizeof(float) * m * (n + 1));
-        w.Pb = (double *)take(sizeof(double) * m * nb);
-        w.Pb2 = (double *)take(sizeof(double) * m * nb);
+        for (double *&v : w.V64) v = (double *)take(sizeof(double) * m * ob);
         w.Yd = (double *)take(sizeof(double) * m * nb);
         w.R = (double *)take(sizeof(double) * n * n);
         w.ldr = n;
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < kRing; ++i) {
             w.V32[i] = (float *)take(sizeof(float) * m * nb);
             w.Y32[i] = (float *)take(sizeof(float) * m * nb);
         }
         w.P32 = (float *)take(sizeof(float) * kMaxSplits * nb * (n + 1));
         w.P32b = (float *)take(sizeof(float) * kMaxSplits * nb * nb);
+        if (ob > nb) {
+            w.p32dcap = (int64_t)kMaxSplits * nb * (ob + nb);
+            w.P32d = (float *)take(sizeof(float) * w.p32dcap);
+            w.p32ocap = (int64_t)8 * ob * (n + 1 + ob);
+            w.P32o = (float *)take(sizeof(float) * w.p32ocap);
+            w.p64ocap = (int64_t)64 * ob * ob;
+            w.P64o = (double *)take(sizeof(double) * w.p64ocap);
+            w.G = (double *)take(sizeof(double) * ob * ob);
+            w.Tb = (double *)take(sizeof(double) * ob * ob);
+            w.tmpo = (double *)take(sizeof(double) * (ob * ob / 2 + ob));
+            w.Yo = (double *)take(sizeof(double) * m * ob);
+            w.V32o = (float *)take(sizeof(float) * m * ob);
+            w.Y32o = (float *)take(sizeof(float) * m * ob);
+        }
         w.Wp = (double *)take(sizeof(double) * ((m + 63) / 64) * 64 * 64);
     }
     w.X = (double *)take(sizeof(double) * n * n);
@@ -386,13 +416,21 @@ int panel_fp32(mqr_handle_t hd, int64_t m, int64_t j, int b, MixedWs &w, double 
 }
 
 // Factorisation with FP32 trailing updates of [A | b]: R (upper, ld ldr) is
-// written, and Aw[0:n, n] ends as Q^T b (FP32).  Look-ahead of one block as
-// in the FP64 factorisation: the handle's stream updates the next block and
-// factors the next panel while the side stream updates the rest.
+// written, and Aw[0:n, n] ends as Q^T b (FP32).  The schedule of the FP64
+// factorisation: panels of nb (<= 64) columns with a look-ahead of one panel
+// on the handle's stream; with a handle block size of 128, 192 or 256,
+// two-level blocking over outer blocks of that width, the side stream
+// applying each panel to the rest of its outer block and the far stream to
+// the next outer block, and each outer block applied to the far columns
+// (b included) as one block reflector at K = the block width.  The panels are factored in FP64 straight into the
+// outer block's FP64 buffer (two buffers, by block parity), which the far
+// stream's merged T reads.
 int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t lda,
                 const double *b, MixedWs &w, int bf16_terms) {
-    hipStream_t s0 = hd->stream, s1 = hd->side_stream;
+    hipStream_t s0 = hd->stream, s1 = hd->side_stream, s2 = hd->far_stream;
     const int nb = std::min(hd->nb, kMaxNb);  // panel width; T internal with ld nb
+    const int ob = w.ob;
+    const bool two = ob > nb;
     const int64_t k = std::min(m, n), nc = n + 1;
     const int64_t pcap = (int64_t)kMaxSplits * nb * nc, pcap2 = (int64_t)kMaxSplits * nb * nb;
     {
@@ -403,15 +441,65 @@ int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t 
                            w.Aw + n * m, m);
         MQR_LAUNCH_CHECK();
     }
+    hipEvent_t far_done[kRing] = {}, ready = nullptr;
+    struct Destroy {
+        hipEvent_t *e;
+        int cnt;
+        hipEvent_t *r;
+        ~Destroy() {
+            for (int i = 0; i < cnt; ++i)
+                if (e[i]) (void)hipEventDestroy(e[i]);
+            if (*r) (void)hipEventDestroy(*r);
+        }
+    } destroy{far_done, kRing, &ready};
+    if (two) {
+        for (hipEvent_t &x : far_done) MQR_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+        MQR_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    }
     MQR_HIP(hipEventRecord(hd->ev_fork, s0));
     MQR_HIP(hipStreamWaitEvent(s1, hd->ev_fork, 0));
-    double *Vb[2] = {w.Pb, w.Pb2};
+    if (two) MQR_HIP(hipStreamWaitEvent(s2, hd->ev_fork, 0));
+    // the FP64 copy of the panel starting at column c (ld m), inside its
+    // outer block's buffer
+    auto vpanel = [&](int64_t c) {
+        const int64_t c0 = (c / ob) * ob;
+        return w.V64[(c / ob) & 1] + (c - c0) + (c - c0) * m;
+    };
+    auto c1_of = [&](int64_t c) { return two ? std::min(nc, (c / ob + 1) * ob) : nc; };
+    auto c2_of = [&](int64_t c) { return two ? std::min(nc, (c / ob + 2) * ob) : nc; };
     bool side_pending = false;
+    int64_t s1_synced = 0;
+    // far stream: outer block J as one block reflector on [c2, nc)
+    auto merge_block = [&](int64_t J) -> int {
+        const int64_t c0 = J * ob, bo = std::min<int64_t>(ob, k - c0), h = m - c0;
+        const int64_t c2 = std::min(nc, c0 + 2 * ob);
+        if (c2 >= nc) return MQR_SUCCESS;
+        const double *V = w.V64[J & 1];
+        const double *Tj = w.T + c0 * nb;
+        int64_t ldtj = nb;
+        if (bo > nb) {
+            MQR_CHECK(merged_t_f64(hd, s2, h, (int)bo, V, m, w.T, nb, c0, w.G, w.Tb, w.tmpo,
+                                   w.P64o, w.p64ocap));
+            Tj = w.Tb;
+            ldtj = bo;
+        }
+        MQR_CHECK(form_y_f64(s2, h, (int)bo, V, m, Tj, ldtj, true, w.Yo));
+        {
+            ProfScope ps(K_CAST, 0.0, 24.0 * h * bo, s2);
+            hipLaunchKernelGGL(k_cast_v, dim3(grid_for(h * bo)), dim3(256), 0, s2, h, (int)bo, V, m,
+                               w.V32o);
+            hipLaunchKernelGGL(k_cast_d2s, dim3(grid_for(h * bo)), dim3(256), 0, s2, h, bo,
+                               (const double *)w.Yo, h, w.Y32o, h);
+            MQR_LAUNCH_CHECK();
+        }
+        return apply_block_f32(hd, s2, h, (int)bo, w.V32o, w.Y32o, w.Aw + c0 + c2 * m, m, nc - c2,
+                               w.P32o, w.p32ocap, K_VTA, K_UPDATE, bf16_terms);
+    };
     // Tall full-width panels take the fused sequence of the FP64 path: the
-    // first launch updates the next block (FP32 storage, FP64 arithmetic),
+    // first launch updates the next panel (FP32 storage, FP64 arithmetic),
     // writes the panel's FP64 copy and its Gram matrix; the last forms V2,
     // Y = V T^T, their FP32 copies, the panel's R block and the following
-    // block's W.
+    // panel's W.
     auto fused = [&](int64_t hp, int bp, int64_t bcols) {
         return hp >= kCholqrMinRows && bp == bcols;
     };
@@ -419,69 +507,102 @@ int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t 
         if (c >= k || c >= nc) return false;
         return fused(m - c, (int)std::min<int64_t>(nb, k - c), std::min<int64_t>(nb, nc - c));
     };
-    bool w_ready = false;  // w.Wp[0:4096] = V^T C of the current panel and the next block
+    bool w_ready = false;  // w.Wp[0:4096] = V^T C of the current panel and the next one
     const int b0 = (int)std::min<int64_t>(nb, k);
     if (fused_at(0)) {
         MQR_CHECK(cholqr_gram1(s0, m, b0, 0, nullptr, 0, nullptr, 0, w.Aw, m, 0, w.pws, true,
-                               Vb[0], m));
-        MQR_CHECK(cholqr_rest_a(s0, m, b0, Vb[0], m, w.tau, w.T, nb, w.pws));
+                               vpanel(0), m));
+        MQR_CHECK(cholqr_rest_a(s0, m, b0, vpanel(0), m, w.tau, w.T, nb, w.pws));
         w_ready = fused_at(b0);
-        MQR_CHECK(cholqr_finish(s0, m, b0, Vb[0], m, w.T, nb, w.Yd, w.Aw + b0 * m, m,
+        MQR_CHECK(cholqr_finish(s0, m, b0, vpanel(0), m, w.T, nb, w.Yd, w.Aw + b0 * m, m,
                                 (int)std::min<int64_t>(nb, nc - b0), w_ready ? w.Wp : nullptr,
                                 w.pws, true, w.V32[0], w.Y32[0], w.R, w.ldr));
     } else {
-        MQR_CHECK(panel_fp32(hd, m, 0, b0, w, Vb[0], w.V32[0], w.Y32[0]));
+        MQR_CHECK(panel_fp32(hd, m, 0, b0, w, vpanel(0), w.V32[0], w.Y32[0]));
     }
     for (int64_t j = 0, it = 0; j < k; j += nb, ++it) {
         const int bj = (int)std::min<int64_t>(nb, k - j);
         const int64_t h = m - j;
-        const float *V32 = w.V32[it & 1], *Y32 = w.Y32[it & 1];
+        const float *V32 = w.V32[it % kRing], *Y32 = w.Y32[it % kRing];
         const int64_t next = j + bj;
         const int64_t bn = std::min<int64_t>(nb, nc - next);
-        const int64_t rest0 = next + bn, nrest = nc - rest0;
+        const int64_t rest0 = next + bn;
+        const int64_t c1 = c1_of(j), c2 = c2_of(j);
+        const bool last_of_block = next >= c1 || next >= k;
+        const int64_t near_end = std::max(rest0, c1), def_end = std::max(near_end, c2);
         if (side_pending) {
             MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
             side_pending = false;
         }
-        if (nrest > 0) {
+        // the far stream's deferred parts reach the next panel's columns when
+        // it starts a new outer block or this panel is the first of its block
+        if (two && it > 0 && (j % ob == 0 || next >= c1))
+            MQR_HIP(hipStreamWaitEvent(s0, far_done[(it - 1) % kRing], 0));
+        const bool side_work = near_end > rest0;
+        const bool def_work = two && def_end > near_end;
+        if (side_work || def_work || (two && last_of_block))
             MQR_HIP(hipEventRecord(hd->ev_fork, s0));
+        if (side_work) {
             MQR_HIP(hipStreamWaitEvent(s1, hd->ev_fork, 0));
-            MQR_CHECK(apply_block_f32(hd, s1, h, bj, V32, Y32, w.Aw + j + rest0 * m, m, nrest,
-                                      w.P32, pcap, K_VTA, K_UPDATE, bf16_terms));
+            const int64_t blk = j / ob;
+            if (two && blk > 0 && s1_synced < blk) {
+                MQR_HIP(hipStreamWaitEvent(s1, far_done[(blk * (ob / nb) - 1) % kRing], 0));
+                s1_synced = blk;
+            }
+            MQR_CHECK(apply_block_f32(hd, s1, h, bj, V32, Y32, w.Aw + j + rest0 * m, m,
+                                      near_end - rest0, w.P32, pcap, two ? K_NEAR : K_VTA,
+                                      two ? K_NEAR : K_UPDATE, bf16_terms));
             MQR_HIP(hipEventRecord(hd->ev_join, s1));
             side_pending = true;
         }
+        if (two) {
+            if (def_work || last_of_block) MQR_HIP(hipStreamWaitEvent(s2, hd->ev_fork, 0));
+            if (def_work)
+                MQR_CHECK(apply_block_f32(hd, s2, h, bj, V32, Y32, w.Aw + j + near_end * m, m,
+                                          def_end - near_end, w.P32d, w.p32dcap, K_NEAR, K_NEAR,
+                                          bf16_terms));
+            MQR_HIP(hipEventRecord(far_done[it % kRing], s2));
+            if (last_of_block) MQR_CHECK(merge_block(j / ob));
+        }
         const int bnext = next < k ? (int)std::min<int64_t>(nb, k - next) : 0;
         if (w_ready && next < k && fused(m - next, bnext, bn)) {
-            double *Vn = Vb[(it + 1) & 1];
+            double *Vn = vpanel(next);
             MQR_CHECK(cholqr_gram1(s0, h, (int)bn, bj, w.Yd, h, w.Wp, 64, w.Aw + j + next * m, m,
                                    bj, w.pws, true, Vn, m));
             MQR_CHECK(cholqr_rest_a(s0, m - next, bnext, Vn, m, w.tau + next, w.T + next * nb, nb,
                                     w.pws));
-            // W of the block after next needs this step's update of the
-            // remaining columns (side stream) first
+            // W of the panel after next needs this step's update of its
+            // columns first (side stream, or the far stream's deferred part)
             const int64_t next2 = next + bnext;
             w_ready = fused_at(next2);
-            if (w_ready && side_pending) {
-                MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
-                side_pending = false;
+            if (w_ready) {
+                if (side_pending) {
+                    MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
+                    side_pending = false;
+                }
+                if (two && next2 >= c1) MQR_HIP(hipStreamWaitEvent(s0, far_done[it % kRing], 0));
             }
             MQR_CHECK(cholqr_finish(s0, m - next, bnext, Vn, m, w.T + next * nb, nb, w.Yd,
                                     w.Aw + next + std::min(next2, nc - 1) * m, m,
                                     (int)std::min<int64_t>(nb, std::max<int64_t>(nc - next2, 0)),
-                                    w_ready ? w.Wp : nullptr, w.pws, true, w.V32[(it + 1) & 1],
-                                    w.Y32[(it + 1) & 1], w.R + next + next * w.ldr, w.ldr));
+                                    w_ready ? w.Wp : nullptr, w.pws, true,
+                                    w.V32[(it + 1) % kRing], w.Y32[(it + 1) % kRing],
+                                    w.R + next + next * w.ldr, w.ldr));
             continue;
         }
         w_ready = false;
         MQR_CHECK(apply_block_f32(hd, s0, h, bj, V32, Y32, w.Aw + j + next * m, m, bn, w.P32b,
                                   pcap2, K_LOOKAHEAD, K_LOOKAHEAD, bf16_terms));
         if (next < k) {
-            MQR_CHECK(panel_fp32(hd, m, next, bnext, w, Vb[(it + 1) & 1], w.V32[(it + 1) & 1],
-                                 w.Y32[(it + 1) & 1]));
+            MQR_CHECK(panel_fp32(hd, m, next, bnext, w, vpanel(next), w.V32[(it + 1) % kRing],
+                                 w.Y32[(it + 1) % kRing]));
         }
     }
     if (side_pending) MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
+    if (two) {
+        MQR_HIP(hipEventRecord(ready, s2));
+        MQR_HIP(hipStreamWaitEvent(s0, ready, 0));
+    }
     {
         ProfScope ps(K_CAST, 0.0, 12.0 * n * n, s0);
         hipLaunchKernelGGL(k_copy_rupper,

@@ -248,6 +248,13 @@ int panel_factor(mqr_handle_t hd, int64_t h, int b, double *V, int64_t lda, doub
 // the block reflector I - V T V^T = I - (V T) V^T.
 int form_y_f64(hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv, const double *Tp,
                int64_t ldt, bool trans, double *Y);
+// Tb (b x b, ld b): the compact-WY factor of the stored 64-column panels
+// j0, j0 + 64, ... (b columns in all, T blocks T(0:64, j0 + c), ld ldt) taken
+// as one block reflector, from G = V^T V (split-K partials in P, pcap
+// doubles) by recursive doubling; G: b x b, tmp: b/2 x b + b doubles.
+int merged_t_f64(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double *V, int64_t ldv,
+                 const double *T, int64_t ldt, int64_t j0, double *G, double *Tb, double *tmp,
+                 double *P, int64_t pcap);
 // The off-diagonal blocks of X = M^{-1} (ld n) for an upper n x n M whose
 // strictly upper 64-blocks are read from R (ld ldr), given the inverses of
 // M's diagonal 64-blocks in X's diagonal blocks (X zero below them), by

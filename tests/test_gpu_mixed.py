@@ -268,3 +268,23 @@ def test_mixed_nonfinite_input_falls_back(h):
     h.synchronize()
     assert info.converged == 0 and info.fell_back == 1
     assert not np.all(np.isfinite(x.cpu().numpy()))
+
+
+@pytest.mark.parametrize("nb,prec", [(256, "fp32"), (128, "fp32"), (256, "fp64")])
+def test_mixed_outer_blocks(torch_mod, orc, nb, prec):
+    """With a block size above 64 the solves use two-level blocking (64-column
+    FP64 panels, each nb-column block applied to the far columns as one
+    block reflector, in FP32 for the mixed solve): the same FP64-level
+    solution, backward error <= 1e-14."""
+    m, n = 3000, 900
+    A, b = ls_problem(m, n, 1e4, seed=nb + len(prec))
+    note(f"outer blocks nb={nb} {prec} {m}x{n}")
+    with M.Handle(0, nb=nb) as hn:
+        x, info = M.gels_mixed(hn, prec, M.colmajor(A), M.colmajor(b[:, None])[:, 0].contiguous())
+        hn.synchronize()
+        x = x.cpu().numpy()
+    be = ls_backward_error(A, b, x)
+    note(f"  iterations={info.iterations} converged={info.converged} kw={be['kw']:.2e}")
+    assert info.fell_back == 0 and be["kw"] <= 1e-14
+    xo = orc.gels(A, b)
+    np.testing.assert_allclose(x, xo, rtol=0, atol=1e-8 * np.abs(xo).max())
