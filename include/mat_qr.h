@@ -232,6 +232,8 @@ typedef struct {
     int max_iter;    /* refinement steps allowed (default 30) */
     double tol;      /* stop when the backward-error estimate <= tol (default 1e-14) */
     int fallback;    /* 1: on stagnation refactor in plain FP64 and solve (default 1) */
+    int krylov_max_iter; /* preconditioned CGLS steps allowed once the stationary
+                            refinement stalls (0: default 400; < 0: none) */
 } mqr_ir_opts_t;
 
 typedef struct {
@@ -241,15 +243,20 @@ typedef struct {
     double backward_error;/* ||R^{-T} A^T r|| / (||A||_F ||x||) at exit: bounds the
                              Karlson-Walden normwise backward error from above */
     double residual_norm; /* ||b - A x|| at exit */
+    int krylov_iterations;/* preconditioned CGLS steps taken (included in iterations) */
 } mqr_ir_info_t;
 
 /* x = argmin ||A x - b|| for one right-hand side, m >= n.  A (m x n, FP64,
  * device) and b (m, device) are not modified; x (n, device) is written.
  * The factorisation runs its trailing updates in `prec` (panels, T factors
- * and R stay FP64); refinement (corrected semi-normal equations with FP64
- * residuals r = b - A x from the caller's A) converges to the FP64
- * least-squares solution while the estimate keeps falling; on stagnation the
- * plain FP64 solve runs instead when opts->fallback is set (info->fell_back).
+ * and R stay FP64).  Refinement with FP64 residuals r = b - A x from the
+ * caller's A: first corrected semi-normal equations (x += R^{-1} R^{-T} A^T r,
+ * converging while u_prec * cond(A) < 1); if that stalls, conjugate
+ * gradients on the least-squares problem preconditioned by R (CGLS on
+ * A R^{-1}, whose condition number is about 1 + u_prec * cond(A)), which
+ * carries split-BF16 factors to FP64 accuracy at cond(A) = 1e6; if that
+ * stalls too, the plain FP64 solve runs instead when opts->fallback is set
+ * (info->fell_back).  opts may be NULL (defaults).
  * Blocks until the solve is done (the stopping test is read on the host).
  * No reference counterpart. */
 int mqr_gels_mixed(mqr_handle_t handle, mqr_precision_t prec, int64_t m, int64_t n,

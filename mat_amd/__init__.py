@@ -93,12 +93,14 @@ SIGNATURES = {
 
 
 class IROpts(C.Structure):
-    _fields_ = [("max_iter", C.c_int), ("tol", C.c_double), ("fallback", C.c_int)]
+    _fields_ = [("max_iter", C.c_int), ("tol", C.c_double), ("fallback", C.c_int),
+                ("krylov_max_iter", C.c_int)]
 
 
 class IRInfo(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("fell_back", C.c_int),
-                ("backward_error", C.c_double), ("residual_norm", C.c_double)]
+                ("backward_error", C.c_double), ("residual_norm", C.c_double),
+                ("krylov_iterations", C.c_int)]
 
 
 class KernelStat(C.Structure):
@@ -335,12 +337,12 @@ def gels(h: Handle, A, B):
 
 
 def gels_mixed(h: Handle, prec: str, A, b, max_iter: int = 30, tol: float = 0.0,
-               fallback: bool = True):
+               fallback: bool = True, krylov_max_iter: int = 0):
     """Mixed-precision least squares with refinement; returns (x, IRInfo)."""
     import torch
     m, n = A.shape
     x = torch.zeros(n, device=A.device, dtype=torch.float64)
-    opts = IROpts(max_iter, tol, 1 if fallback else 0)
+    opts = IROpts(max_iter, tol, 1 if fallback else 0, krylov_max_iter)
     info = IRInfo()
     check(lib().mqr_gels_mixed(h.h, PREC[prec], m, n, _ptr(A), ld(A), _ptr(b), _ptr(x),
                                C.byref(opts), C.byref(info)), "mqr_gels_mixed")

@@ -223,6 +223,46 @@ __global__ void k_fill(int64_t n, double *__restrict__ x, double v) {
     if (i < n) x[i] = v;
 }
 
+// out[i] = sum_p part[p * len + i]
+__global__ void k_sum_parts(int64_t len, int nparts, const double *__restrict__ part,
+                            double *__restrict__ out) {
+    const int64_t i = gidx();
+    if (i >= len) return;
+    double s = 0.0;
+    for (int p = 0; p < nparts; ++p) s += part[(int64_t)p * len + i];
+    out[i] = s;
+}
+
+// Conjugate-gradient step with alpha = sc[g] / sc[d] read on the device:
+// x += alpha t (xx = x^2) for i < n, r -= alpha q for i < m.
+__global__ void k_cg_step(int64_t n, int64_t m, const double *__restrict__ sc, int g, int d,
+                          const double *__restrict__ t, double *__restrict__ x,
+                          double *__restrict__ xx, const double *__restrict__ q,
+                          double *__restrict__ r) {
+    const int64_t i = gidx();
+    const double alpha = sc[d] > 0.0 ? sc[g] / sc[d] : 0.0;
+    if (i < n) {
+        const double v = fma(alpha, t[i], x[i]);
+        x[i] = v;
+        xx[i] = v * v;
+    }
+    if (i < m) r[i] = fma(-alpha, q[i], r[i]);
+}
+
+// p = s + (sc[gn] / sc[go]) p
+__global__ void k_cg_dir(int64_t n, const double *__restrict__ sc, int gn, int go,
+                         const double *__restrict__ s, double *__restrict__ p) {
+    const int64_t i = gidx();
+    if (i >= n) return;
+    const double beta = sc[go] > 0.0 ? sc[gn] / sc[go] : 0.0;
+    p[i] = fma(beta, p[i], s[i]);
+}
+
+__global__ void k_copy_d(int64_t n, const double *__restrict__ a, double *__restrict__ b) {
+    const int64_t i = gidx();
+    if (i < n) b[i] = a[i];
+}
+
 // X (ld ldx) = inverse of each 64 x 64 diagonal block of upper R (ld ldr)
 __global__ __launch_bounds__(small::NT) void k_inv_diag_blocks(int64_t n,
                                                                const double *__restrict__ R,
@@ -283,6 +323,7 @@ struct MixedWs {
     double *Wp = nullptr;  // partials of the next block's W (FP32 mode)
     double *X = nullptr, *T = nullptr, *tau = nullptr;
     double *r = nullptr, *z = nullptr, *y = nullptr, *part = nullptr, *sc = nullptr;
+    double *t = nullptr, *q = nullptr, *p = nullptr, *xbest = nullptr;  // CGLS
     double *xx = nullptr, *zz = nullptr, *tmp = nullptr;
     void *pws = nullptr;
     size_t bytes = 0;
@@ -353,6 +394,10 @@ MixedWs carve(char *base, bool fp64, int64_t m, int64_t n, int nb, int nbt) {
     w.sc = (double *)take(sizeof(double) * 64);
     w.xx = (double *)take(sizeof(double) * n);
     w.zz = (double *)take(sizeof(double) * n);
+    w.t = (double *)take(sizeof(double) * n);
+    w.q = (double *)take(sizeof(double) * m);
+    w.p = (double *)take(sizeof(double) * n);
+    w.xbest = (double *)take(sizeof(double) * n);
     if (!fp64) w.pws = take(panel_cholqr_ws_bytes(m));
     w.bytes = off;
     return w;
@@ -780,6 +825,99 @@ int estimate(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t lda
     return MQR_SUCCESS;
 }
 
+// Preconditioned CGLS: conjugate gradients on min ||A X u - r||, X = R^{-1},
+// x += X u, from the current x, every product in FP64 with the caller's A.
+// R^{-1} makes A X nearly orthogonal (condition about 1 + u_prec cond(A)),
+// so the steps needed grow with that condition, not with cond(A)^2.  Every
+// kCgCheck steps the true residual r = b - A x replaces the recurrence's
+// (keeping the search direction) and the stopping test runs on the host;
+// the other steps make no host round trip.  *steps: steps taken.
+constexpr int kCgCheck = 10;
+int cgls(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t lda, const double *b,
+         double *x, double nA, double tol, int max_steps, MixedWs &w, bool trace, int *steps,
+         bool *converged, double *omega, double *rnorm, double *ynorm) {
+    hipStream_t s = hd->stream;
+    const int64_t nch = (n + GV_COLS - 1) / GV_COLS, nxch = (n + GV_XCOLS - 1) / GV_XCOLS;
+    const unsigned gx = (unsigned)((n + GV_ROWS - 1) / GV_ROWS);
+    const unsigned ga = (unsigned)((m + GV_ROWS - 1) / GV_ROWS);
+    *steps = 0;
+    *converged = false;
+    MQR_CHECK(estimate(hd, m, n, A, lda, b, x, nA, w, omega, rnorm, ynorm));
+    if (*omega <= tol) {
+        *converged = true;
+        return MQR_SUCCESS;
+    }
+    double best = *omega;
+    int stall = 0;
+    // p = y = X^T A^T r, gamma = ||y||^2 (slots 10 / 11 alternate)
+    hipLaunchKernelGGL(k_copy_d, dim3(grid_for(n)), dim3(256), 0, s, n, (const double *)w.y, w.p);
+    hipLaunchKernelGGL(k_copy_d, dim3(1), dim3(64), 0, s, (int64_t)1, (const double *)(w.sc + 5),
+                       w.sc + 10);
+    MQR_LAUNCH_CHECK();
+    int go = 10;
+    const int total = ((std::max(max_steps, 1) + kCgCheck - 1) / kCgCheck) * kCgCheck;
+    for (int i = 1; i <= total; ++i) {
+        {
+            ProfScope ps(K_RESID, 2.0 * n * n + 2.0 * m * n, 8.0 * (n * n / 2.0 + m * n), s);
+            // t = X p (upper), q = A t, sc[12] = ||q||^2
+            hipLaunchKernelGGL(k_gemv_n, dim3(gx, (unsigned)nxch), dim3(256), 0, s, n, n,
+                               (const double *)w.X, n, (const double *)w.p, w.part, GV_XCOLS, 1);
+            hipLaunchKernelGGL(k_sum_parts, dim3(grid_for(n, 256, 1 << 22)), dim3(256), 0, s, n,
+                               (int)nxch, (const double *)w.part, w.t);
+            hipLaunchKernelGGL(k_gemv_n, dim3(ga, (unsigned)nch), dim3(256), 0, s, m, n, A, lda,
+                               (const double *)w.t, w.part, GV_COLS, 0);
+            hipLaunchKernelGGL(k_sum_parts, dim3(grid_for(m, 256, 1 << 22)), dim3(256), 0, s, m,
+                               (int)nch, (const double *)w.part, w.q);
+            MQR_LAUNCH_CHECK();
+            MQR_CHECK(sumsq_vec(m, w.q, w.part, w.sc, 12, s));
+            // x += alpha t, r -= alpha q, alpha = gamma / ||q||^2
+            hipLaunchKernelGGL(k_cg_step, dim3(grid_for(std::max(m, n), 256, 1 << 22)), dim3(256),
+                               0, s, n, m, (const double *)w.sc, go, 12, (const double *)w.t, x,
+                               w.xx, (const double *)w.q, w.r);
+            MQR_LAUNCH_CHECK();
+        }
+        const int gn = go == 10 ? 11 : 10;
+        *steps = i;
+        if (i % kCgCheck == 0) {
+            // true residual, its preconditioned normal residual and the test
+            MQR_CHECK(estimate(hd, m, n, A, lda, b, x, nA, w, omega, rnorm, ynorm));
+            if (trace)
+                std::fprintf(stderr, "[mqr ir] cgls step %d omega %.3e |r| %.9e\n", i, *omega,
+                             *rnorm);
+            if (*omega <= tol) {
+                *converged = true;
+                break;
+            }
+            if (!std::isfinite(*omega)) break;
+            if (*omega < 0.5 * best) {
+                best = *omega;
+                stall = 0;
+            } else if (++stall >= 3) {
+                break;
+            }
+            hipLaunchKernelGGL(k_copy_d, dim3(1), dim3(64), 0, s, (int64_t)1,
+                               (const double *)(w.sc + 5), w.sc + gn);
+            MQR_LAUNCH_CHECK();
+        } else {
+            ProfScope ps(K_RESID, 2.0 * m * n + 1.0 * n * n, 8.0 * (m * n + n * n / 2.0), s);
+            // y = X^T A^T r, sc[gn] = ||y||^2
+            hipLaunchKernelGGL(k_gemv_t, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, m, n, A,
+                               lda, (const double *)w.r, w.z, (double *)nullptr, 0);
+            hipLaunchKernelGGL(k_gemv_t, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, n,
+                               (const double *)w.X, n, (const double *)w.z, w.y, w.zz, 1);
+            hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, s, (int)n, (const double *)w.zz,
+                               w.sc, gn);
+            MQR_LAUNCH_CHECK();
+        }
+        // p = y + (gamma_new / gamma_old) p
+        hipLaunchKernelGGL(k_cg_dir, dim3(grid_for(n)), dim3(256), 0, s, n, (const double *)w.sc,
+                           gn, go, (const double *)w.y, w.p);
+        MQR_LAUNCH_CHECK();
+        go = gn;
+    }
+    return MQR_SUCCESS;
+}
+
 }  // namespace
 
 int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, const double *A,
@@ -790,6 +928,7 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
     const int max_iter = (opts && opts->max_iter > 0) ? opts->max_iter : 30;
     const double tol = (opts && opts->tol > 0.0) ? opts->tol : 1e-14;
     const bool fallback = opts ? opts->fallback != 0 : true;
+    const int krylov = (opts && opts->krylov_max_iter != 0) ? opts->krylov_max_iter : 400;
     hipStream_t s = hd->stream;
     const int nb = std::min(hd->nb, kMaxNb), nbt = hd->nb;
     const bool fp64 = prec == MQR_PREC_FP64;
@@ -844,8 +983,8 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
     // Progress is judged on ||y|| = ||R^{-T} A^T r||, which falls by the
     // contraction factor every step; omega also depends on ||x||, which is
     // far off in the first steps when kappa is large.
-    double omega = INFINITY, best = INFINITY, rnorm = 0.0, ynorm = 0.0;
-    int it = 0, stall = 0;
+    double omega = INFINITY, best = INFINITY, rnorm = 0.0, ynorm = 0.0, best_omega = INFINITY;
+    int it = 0, stall = 0, kit = 0;
     bool converged = false;
     for (;;) {
         MQR_CHECK(estimate(hd, m, n, A, lda, b, x, nA, w, &omega, &rnorm, &ynorm));
@@ -855,6 +994,13 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
         if (omega <= tol) {
             converged = true;
             break;
+        }
+        // the best iterate so far is where the Krylov stage starts
+        if (omega < best_omega) {
+            best_omega = omega;
+            hipLaunchKernelGGL(k_copy_d, dim3(grid_for(n)), dim3(256), 0, s, n, (const double *)x,
+                               w.xbest);
+            MQR_LAUNCH_CHECK();
         }
         // x = 0 at the first test: omega is infinite unless A^T b = 0
         if (!std::isfinite(ynorm) || (it > 0 && !std::isfinite(omega))) break;
@@ -867,6 +1013,17 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
         if (it >= max_iter) break;
         MQR_CHECK(correction(hd, n, x, w));
         ++it;
+    }
+    // the stationary iteration stalled (u_prec cond(A) too close to 1 or
+    // above): conjugate gradients preconditioned by the same R
+    if (!converged && !fp64 && krylov > 0 && std::isfinite(best_omega)) {
+        hipLaunchKernelGGL(k_copy_d, dim3(grid_for(n)), dim3(256), 0, s, n,
+                           (const double *)w.xbest, x);
+        hipLaunchKernelGGL(k_sq, dim3(grid_for(n)), dim3(256), 0, s, n, (const double *)x, w.xx);
+        MQR_LAUNCH_CHECK();
+        MQR_CHECK(cgls(hd, m, n, A, lda, b, x, nA, tol, krylov, w, trace, &kit, &converged,
+                       &omega, &rnorm, &ynorm));
+        it += kit;
     }
     bool fell_back = false;
     if (!converged && fallback) {
@@ -894,6 +1051,7 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
         info->fell_back = fell_back ? 1 : 0;
         info->backward_error = omega;
         info->residual_norm = rnorm;
+        info->krylov_iterations = kit;
     }
     return (converged || fell_back) ? MQR_SUCCESS : MQR_ERR_NOT_CONVERGED;
 }

@@ -288,3 +288,28 @@ def test_mixed_outer_blocks(torch_mod, orc, nb, prec):
     assert info.fell_back == 0 and be["kw"] <= 1e-14
     xo = orc.gels(A, b)
     np.testing.assert_allclose(x, xo, rtol=0, atol=1e-8 * np.abs(xo).max())
+
+
+@pytest.mark.parametrize("prec,cond", [("bf16x3", 1e6), ("bf16x3", 1e5)])
+def test_mixed_krylov_stage_converges(h, orc, prec, cond):
+    """Split-BF16 updates at kappa = 1e6: u * kappa > 1, so the stationary
+    refinement stalls; conjugate gradients preconditioned by the split-BF16 R
+    (condition of A R^{-1} about 1 + u kappa) carry the solution to FP64
+    backward error without the FP64 fallback."""
+    m, n = 2000, 500
+    A, b = ls_problem(m, n, cond, seed=31, resid=1e-10)
+    x, info = run_mixed(h, A, b, prec=prec)
+    be = ls_backward_error(A, b, x)
+    note(f"  {prec} cond={cond:g}: iterations={info.iterations} krylov={info.krylov_iterations} "
+         f"converged={info.converged} fell_back={info.fell_back} kw={be['kw']:.2e}")
+    assert info.converged == 1 and info.fell_back == 0
+    assert be["kw"] <= 1e-14
+    xo = orc.gels(A, b)
+    np.testing.assert_allclose(x, xo, rtol=0, atol=1e-7 * np.abs(xo).max())
+
+
+def test_mixed_krylov_stage_can_be_switched_off(h):
+    A, b = ls_problem(2000, 500, 1e6, seed=31, resid=1e-10)
+    x, info = run_mixed(h, A, b, prec="bf16x3", krylov_max_iter=-1)
+    assert info.krylov_iterations == 0 and info.fell_back == 1
+    assert ls_backward_error(A, b, x)["kw"] <= 1e-14
