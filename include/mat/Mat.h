@@ -410,6 +410,91 @@ inline void QR(MAT<t1> &A, MAT<t2> &Q, MAT<t2> &R) {
 }
 
 // ---------------------------------------------------------------------------
+// LU, PLU and the direct solve on the GPU (LU/__LU.h, LU/__PLU.h,
+// LU/__Solve_direct.h; LU_init, Mat.h:128-131).  L is m x m (unit lower: the
+// multipliers in its first min(m, n) columns, the identity beyond), U is
+// m x n (zero below the diagonal), so A = L U, or row pos[i] of A = row i of
+// L U for PLU.  Where the reference stops at an exactly zero pivot, the GPU
+// factorisation continues LAPACK-style; the 1-based column of the first such
+// pivot is returned (0: none).
+// ---------------------------------------------------------------------------
+template <typename t1, typename t2>
+void LU_init(MAT<t1> &A, MAT<t2> &L, MAT<t2> &U) {
+    L.init(A.rows, A.rows);
+    U.init(A.rows, A.cols);
+}
+
+namespace mat {
+namespace detail {
+template <typename t1, typename t2>
+int lu_split(const MAT<t1> &A, int pivot, int *pos, MAT<t2> &L, MAT<t2> &U) {
+    const int m = A.rows, n = A.cols, s = std::min(m, n);
+    std::vector<double> abuf, f((size_t)m * n);
+    const double *a = as_f64(A, abuf);
+    std::vector<int32_t> p((size_t)std::max(m, 1));
+    int32_t info = 0;
+    check(mqr_lu_host(handle(), pivot, m, n, a, f.data(), p.data(), &info), "mqr_lu_host");
+    L.init(m, m);
+    U.init(m, n);
+    for (int c = 0; c < m; ++c)
+        for (int r = 0; r < m; ++r)
+            L(r, c) = (t2)(r == c ? 1.0 : (c < s && r > c) ? f[(size_t)c * m + r] : 0.0);
+    for (int c = 0; c < n; ++c)
+        for (int r = 0; r < m; ++r) U(r, c) = (t2)(r <= c ? f[(size_t)c * m + r] : 0.0);
+    if (pos)
+        for (int i = 0; i < m; ++i) pos[i] = p[(size_t)i];
+    return info;
+}
+}  // namespace detail
+}  // namespace mat
+
+// LU/__LU.h:86-103, no pivoting
+template <typename t1, typename t2>
+int LU(MAT<t1> &A, MAT<t2> &L, MAT<t2> &U) {
+    return mat::detail::lu_split(A, 0, nullptr, L, U);
+}
+
+// LU/__PLU.h:153-187, partial pivoting: pos (A.rows entries) receives the
+// row permutation
+template <typename t1, typename t2>
+int PLU(MAT<t1> &A, int *pos, MAT<t2> &L, MAT<t2> &U) {
+    return mat::detail::lu_split(A, 1, pos, L, U);
+}
+
+// LU/__PLU.h:189-195: P with P(pos[i], i) = 1, so that A = P L U
+template <typename t>
+void PLU_getP(int *pos, MAT<t> &P) {
+    P.fill(t(0));
+    for (int i = 0; i < P.rows; ++i) P(pos[i], i) = t(1);
+}
+
+// LU/__LU.h:105-130: L and U from a combined L \ U matrix
+template <typename t1, typename t2>
+void LU_getLU(MAT<t1> &F, MAT<t2> &L, MAT<t2> &U) {
+    const int s = std::min(F.rows, F.cols);
+    for (int c = 0; c < L.cols; ++c)
+        for (int r = 0; r < L.rows; ++r)
+            L(r, c) = (t2)(r == c ? 1 : (c < s && r > c) ? F(r, c) : 0);
+    for (int c = 0; c < U.cols; ++c)
+        for (int r = 0; r < U.rows; ++r)
+            U(r, c) = (t2)((r <= c || c >= s) && r < F.rows && c < F.cols ? F(r, c) : 0);
+}
+
+// LU/__Solve_direct.h:175-220: X = A^{-1} B by PLU on the GPU
+template <typename T>
+MAT<T> Solve_direct(MAT<T> A, MAT<T> &B) {
+    MAT<T> X(B.rows, B.cols);
+    if (A.rows != A.cols || A.rows != B.rows)
+        throw mat::Error(-1, "Solve_direct: A must be square with A.rows == B.rows");
+    std::vector<double> abuf, bbuf, x((size_t)B.rows * B.cols);
+    const double *a = mat::detail::as_f64(A, abuf), *b = mat::detail::as_f64(B, bbuf);
+    mat::check(mqr_solve_direct_host(mat::handle(), A.rows, B.cols, a, b, x.data()),
+               "mqr_solve_direct_host");
+    mat::detail::from_f64(X, x);
+    return X;
+}
+
+// ---------------------------------------------------------------------------
 // Least squares x = argmin ||A x - b|| (m >= n), no reference counterpart.
 // prec: MQR_PREC_FP64 (plain FP64 QR solve) or MQR_PREC_FP32 (FP32 trailing
 // updates + FP64 refinement to the FP64 solution).  x is resized to n x 1.

@@ -227,6 +227,25 @@ int mqr_dgeqr_givens(mqr_handle_t handle, int64_t m, int64_t n, const double *A,
 int mqr_dgeqr_gramschmidt(mqr_handle_t handle, int64_t m, int64_t n, const double *A, int64_t lda,
                           double *Q, int64_t ldq, double *R, int64_t ldr);
 
+/* ---- LU and the direct solve (LU/__LU.h, LU/__PLU.h, LU/__Solve_direct.h) */
+/* Blocked LU of A (m x n, lda) in place: L unit lower (strictly below the
+ * diagonal), U on and above.  ipiv (device int32[min(m, n)]) selects partial
+ * pivoting, the first row of largest magnitude as in PLU (LU/__PLU.h:18-50):
+ * row i was exchanged with row ipiv[i] (0-based, in order); NULL factors
+ * without pivoting as LU (LU/__LU.h:2-16).  info (device int32, may be
+ * NULL): 0, or the 1-based column of the first exactly zero pivot, where the
+ * reference stops (`break`); here the factorisation goes on LAPACK-style. */
+int mqr_dgetrf(mqr_handle_t handle, int64_t m, int64_t n, double *A, int64_t lda, int32_t *ipiv,
+               int32_t *info);
+/* B (n x nrhs, ldb) <- A^{-1} B from mqr_dgetrf's factors of the n x n A
+ * (ipiv NULL when it factored without pivoting). */
+int mqr_dgetrs(mqr_handle_t handle, int64_t n, int64_t nrhs, const double *LU, int64_t ldlu,
+               const int32_t *ipiv, double *B, int64_t ldb);
+/* Solve_direct (LU/__Solve_direct.h:175-220): A (n x n) is overwritten by
+ * its PLU factors, ipiv (device, n) by the pivots, B (n x nrhs) by X. */
+int mqr_dgesv(mqr_handle_t handle, int64_t n, int64_t nrhs, double *A, int64_t lda, int32_t *ipiv,
+              double *B, int64_t ldb, int32_t *info);
+
 /* ---- mixed precision least squares with iterative refinement ------------ */
 typedef struct {
     int max_iter;    /* refinement steps allowed (default 30) */
@@ -334,6 +353,17 @@ int mqr_qr_householder_host(mqr_handle_t handle, int64_t m, int64_t n, const dou
 typedef enum { MQR_QR_HOUSEHOLDER = 0, MQR_QR_GIVENS = 1, MQR_QR_GRAMSCHMIDT = 2 } mqr_qr_type_t;
 int mqr_qr_host(mqr_handle_t handle, mqr_qr_type_t type, int64_t m, int64_t n, const double *A,
                 double *Q, double *R);
+/* LU(A, L, U) / PLU(A, pos, L, U) on host memory: A m x n (lda = m) read,
+ * F (m x n) receives L \ U (L unit lower strictly below the diagonal, U on
+ * and above), pos (m) the row permutation (row i of L U is row pos[i] of A;
+ * the identity when pivot = 0), info (host, may be NULL) as mqr_dgetrf's.
+ * Blocks. */
+int mqr_lu_host(mqr_handle_t handle, int pivot, int64_t m, int64_t n, const double *A, double *F,
+                int32_t *pos, int32_t *info);
+/* X = A^{-1} B (Solve_direct) on host memory: A n x n, B and X n x nrhs.
+ * Blocks. */
+int mqr_solve_direct_host(mqr_handle_t handle, int64_t n, int64_t nrhs, const double *A,
+                          const double *B, double *X);
 /* Host least squares: A m x n (lda = m), b (m), x (n) host memory; prec as
  * above (MQR_PREC_FP64 = plain solve, no refinement).  Blocks. */
 int mqr_gels_host(mqr_handle_t handle, mqr_precision_t prec, int64_t m, int64_t n,

@@ -87,6 +87,11 @@ SIGNATURES = {
     "mqr_dgeqr_givens": (_i32, [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64]),
     "mqr_dgeqr_gramschmidt": (_i32, [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64]),
     "mqr_qr_host": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp]),
+    "mqr_dgetrf": (_i32, [_vp, _i64, _i64, _vp, _i64, _vp, _vp]),
+    "mqr_dgetrs": (_i32, [_vp, _i64, _i64, _vp, _i64, _vp, _vp, _i64]),
+    "mqr_dgesv": (_i32, [_vp, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
+    "mqr_lu_host": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp]),
+    "mqr_solve_direct_host": (_i32, [_vp, _i64, _i64, _vp, _vp, _vp]),
     "mqr_qr_householder_host": (_i32, [_vp, _i64, _i64, _vp, _vp, _vp]),
     "mqr_gels_host": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp]),
 }
@@ -649,3 +654,58 @@ def gemm_host(A, B):
              B.ctypes.data_as(C.c_void_p), Cm.ctypes.data_as(C.c_void_p)),
           "mqr_sgemm_host" if f32 else "mqr_dgemm_host")
     return Cm
+
+
+# ---------------------------------------------------------------------------
+# LU, PLU and the direct solve
+# ---------------------------------------------------------------------------
+
+def getrf(h: Handle, A, pivot: bool = True):
+    """LU of a column-major float64 device matrix in place (L \\ U); returns
+    (ipiv device int32 tensor or None, info device int32 tensor)."""
+    import torch
+    m, n = A.shape
+    k = min(m, n)
+    ipiv = torch.zeros(max(k, 1), device=A.device, dtype=torch.int32) if pivot else None
+    info = torch.zeros(1, device=A.device, dtype=torch.int32)
+    check(lib().mqr_dgetrf(h.h, m, n, _ptr(A), ld(A), _ptr(ipiv), _ptr(info)), "mqr_dgetrf")
+    return ipiv, info
+
+
+def getrs(h: Handle, LU, ipiv, B):
+    n = LU.shape[0]
+    check(lib().mqr_dgetrs(h.h, n, B.shape[1], _ptr(LU), ld(LU), _ptr(ipiv), _ptr(B), ld(B)),
+          "mqr_dgetrs")
+    return B
+
+
+def LU(A, pivot: bool = True, handle: Optional[Handle] = None):
+    """LU(A, L, U) / PLU(A, pos, L, U) of a host float64 matrix on the GPU:
+    returns (F, pos, info) with F = L \\ U (m x n, Fortran order)."""
+    import numpy as np
+    h = handle or default_handle()
+    A = np.asfortranarray(A, dtype=np.float64)
+    m, n = A.shape
+    F = np.zeros((m, n), order="F")
+    pos = np.zeros(max(m, 1), dtype=np.int32)
+    info = C.c_int32(0)
+    check(lib().mqr_lu_host(h.h, 1 if pivot else 0, m, n, A.ctypes.data_as(C.c_void_p),
+                            F.ctypes.data_as(C.c_void_p), pos.ctypes.data_as(C.c_void_p),
+                            C.byref(info)), "mqr_lu_host")
+    return F, pos[:m], info.value
+
+
+def Solve_direct(A, B, handle: Optional[Handle] = None):
+    """X = A^{-1} B (the reference's Solve_direct) for host float64 arrays."""
+    import numpy as np
+    h = handle or default_handle()
+    A = np.asfortranarray(A, dtype=np.float64)
+    B = np.asfortranarray(B, dtype=np.float64)
+    if B.ndim == 1:
+        B = B[:, None]
+    n, nrhs = B.shape
+    X = np.zeros((n, nrhs), order="F")
+    check(lib().mqr_solve_direct_host(h.h, n, nrhs, A.ctypes.data_as(C.c_void_p),
+                                      B.ctypes.data_as(C.c_void_p), X.ctypes.data_as(C.c_void_p)),
+          "mqr_solve_direct_host")
+    return X

@@ -36,6 +36,10 @@ int qr_givens_f64(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_
                   int64_t ldq, double *R, int64_t ldr);
 int qr_gramschmidt_f64(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t lda,
                        double *Q, int64_t ldq, double *R, int64_t ldr);
+int getrf_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda, int32_t *ipiv,
+              int32_t *info, int32_t *scratch_piv, int32_t *scratch_info);
+int getrs_f64(mqr_handle_t hd, int64_t n, int64_t nrhs, const double *LU, int64_t ldlu,
+              const int32_t *ipiv, double *B, int64_t ldb);
 int batched_noalloc_init(int device);
 int geqrf_batched_noalloc(mqr_handle_t hd, bool f64, int64_t batch, const int32_t *m,
                           const int32_t *n, const int64_t *lda, void *const *A, void *const *tau,
@@ -231,6 +235,7 @@ int mqr_destroy(mqr_handle_t h) {
     if (h->ws.ptr) (void)hipFree(h->ws.ptr);
     if (h->ws_ir.ptr) (void)hipFree(h->ws_ir.ptr);
     if (h->ws_vec.ptr) (void)hipFree(h->ws_vec.ptr);
+    if (h->ws_lu.ptr) (void)hipFree(h->ws_lu.ptr);
     if (h->host_pinned) (void)hipHostFree(h->host_pinned);
     for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out})
         if (e) (void)hipEventDestroy(e);
@@ -822,6 +827,163 @@ int mqr_qr_host(mqr_handle_t h, mqr_qr_type_t type, int64_t m, int64_t n, const 
         st = MQR_ERR_HIP;
     (void)hipStreamSynchronize(h->stream);
     (void)hipFree(d);
+    return st;
+}
+
+// pivot and info scratch of an LU call (the caller's arrays when given)
+static int lu_scratch(mqr_handle_t h, int64_t k, int32_t **piv, int32_t **info) {
+    MQR_CHECK(ws_reserve(h, h->ws_lu, sizeof(int32_t) * (size_t)(k + 64)));
+    *info = (int32_t *)h->ws_lu.ptr;
+    *piv = *info + 16;
+    return MQR_SUCCESS;
+}
+
+int mqr_dgetrf(mqr_handle_t h, int64_t m, int64_t n, double *A, int64_t lda, int32_t *ipiv,
+               int32_t *info) {
+    if (!h) return -1;
+    if (m < 0 || m > INT32_MAX) return -2;
+    if (n < 0 || n > INT32_MAX) return -3;
+    if (m * n > 0 && !A) return -4;
+    if (lda < std::max<int64_t>(1, m)) return -5;
+    if (std::min(m, n) == 0) return MQR_SUCCESS;
+    MQR_GUARD(h);
+    MQR_BRIDGE(h);
+    int32_t *sp = nullptr, *si = nullptr;
+    MQR_CHECK(lu_scratch(h, std::min(m, n), &sp, &si));
+    return getrf_f64(h, m, n, A, lda, ipiv, info, sp, si);
+}
+
+int mqr_dgetrs(mqr_handle_t h, int64_t n, int64_t nrhs, const double *LU, int64_t ldlu,
+               const int32_t *ipiv, double *B, int64_t ldb) {
+    if (!h) return -1;
+    if (n < 0 || n > INT32_MAX) return -2;
+    if (nrhs < 0 || nrhs > INT32_MAX) return -3;
+    if (n > 0 && !LU) return -4;
+    if (ldlu < std::max<int64_t>(1, n)) return -5;
+    if (n * nrhs > 0 && !B) return -7;
+    if (ldb < std::max<int64_t>(1, n)) return -8;
+    if (n * nrhs == 0) return MQR_SUCCESS;
+    MQR_GUARD(h);
+    MQR_BRIDGE(h);
+    return getrs_f64(h, n, nrhs, LU, ldlu, ipiv, B, ldb);
+}
+
+int mqr_dgesv(mqr_handle_t h, int64_t n, int64_t nrhs, double *A, int64_t lda, int32_t *ipiv,
+              double *B, int64_t ldb, int32_t *info) {
+    if (!h) return -1;
+    if (n < 0 || n > INT32_MAX) return -2;
+    if (nrhs < 0 || nrhs > INT32_MAX) return -3;
+    if (n > 0 && !A) return -4;
+    if (lda < std::max<int64_t>(1, n)) return -5;
+    if (n > 0 && !ipiv) return -6;
+    if (n * nrhs > 0 && !B) return -7;
+    if (ldb < std::max<int64_t>(1, n)) return -8;
+    if (n == 0) return MQR_SUCCESS;
+    MQR_GUARD(h);
+    MQR_BRIDGE(h);
+    int32_t *sp = nullptr, *si = nullptr;
+    MQR_CHECK(lu_scratch(h, n, &sp, &si));
+    MQR_CHECK(getrf_f64(h, n, n, A, lda, ipiv, info, sp, si));
+    if (nrhs == 0) return MQR_SUCCESS;
+    return getrs_f64(h, n, nrhs, A, lda, ipiv, B, ldb);
+}
+
+int mqr_lu_host(mqr_handle_t h, int pivot, int64_t m, int64_t n, const double *A, double *F,
+                int32_t *pos, int32_t *info) {
+    if (!h) return -1;
+    if (pivot != 0 && pivot != 1) return -2;
+    if (m < 0 || m > INT32_MAX) return -3;
+    if (n < 0 || n > INT32_MAX) return -4;
+    if (m * n > 0 && !A) return -5;
+    if (m * n > 0 && !F) return -6;
+    if (m > 0 && !pos) return -7;
+    for (int64_t i = 0; i < m; ++i) pos[i] = (int32_t)i;
+    if (info) *info = 0;
+    const int64_t k = std::min(m, n);
+    if (k == 0) {
+        if (m * n > 0) std::memcpy(F, A, sizeof(double) * (size_t)(m * n));
+        return MQR_SUCCESS;
+    }
+    MQR_GUARD(h);
+    MQR_BRIDGE(h);
+    double *dA = nullptr;
+    int32_t *dp = nullptr;
+    const size_t na = (size_t)(m * n);
+    if (hipMalloc(&dA, sizeof(double) * na) != hipSuccess ||
+        hipMalloc(&dp, sizeof(int32_t) * (size_t)(k + 16)) != hipSuccess) {
+        (void)hipGetLastError();
+        if (dA) (void)hipFree(dA);
+        return MQR_ERR_ALLOC;
+    }
+    int st = MQR_SUCCESS;
+    std::vector<int32_t> ipiv((size_t)k);
+    int32_t hinfo = 0;
+    if (hipMemcpyAsync(dA, A, sizeof(double) * na, hipMemcpyHostToDevice, h->stream) != hipSuccess)
+        st = MQR_ERR_HIP;
+    if (st == MQR_SUCCESS) {
+        int32_t *sp = nullptr, *si = nullptr;
+        st = lu_scratch(h, k, &sp, &si);
+        if (st == MQR_SUCCESS) st = getrf_f64(h, m, n, dA, m, pivot ? dp : nullptr, dp + k, sp, si);
+    }
+    if (st == MQR_SUCCESS &&
+        (hipMemcpyAsync(F, dA, sizeof(double) * na, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+         (pivot && hipMemcpyAsync(ipiv.data(), dp, sizeof(int32_t) * (size_t)k,
+                                  hipMemcpyDeviceToHost, h->stream) != hipSuccess) ||
+         hipMemcpyAsync(&hinfo, dp + k, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream) !=
+             hipSuccess ||
+         hipStreamSynchronize(h->stream) != hipSuccess))
+        st = MQR_ERR_HIP;
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(dA);
+    (void)hipFree(dp);
+    if (st != MQR_SUCCESS) return st;
+    // the sequence of swaps -> the row permutation pos (row i of L U is row
+    // pos[i] of A), as the reference keeps it (LU/__PLU.h:37-40)
+    if (pivot)
+        for (int64_t i = 0; i < k; ++i) std::swap(pos[i], pos[ipiv[i]]);
+    if (info) *info = hinfo;
+    return MQR_SUCCESS;
+}
+
+int mqr_solve_direct_host(mqr_handle_t h, int64_t n, int64_t nrhs, const double *A,
+                          const double *B, double *X) {
+    if (!h) return -1;
+    if (n < 0 || n > INT32_MAX) return -2;
+    if (nrhs < 0 || nrhs > INT32_MAX) return -3;
+    if (n > 0 && !A) return -4;
+    if (n * nrhs > 0 && !B) return -5;
+    if (n * nrhs > 0 && !X) return -6;
+    if (n * nrhs == 0) return MQR_SUCCESS;
+    MQR_GUARD(h);
+    MQR_BRIDGE(h);
+    double *d = nullptr;
+    int32_t *dp = nullptr;
+    const size_t na = (size_t)(n * n), nb = (size_t)(n * nrhs);
+    if (hipMalloc(&d, sizeof(double) * (na + nb)) != hipSuccess ||
+        hipMalloc(&dp, sizeof(int32_t) * (size_t)(n + 16)) != hipSuccess) {
+        (void)hipGetLastError();
+        if (d) (void)hipFree(d);
+        return MQR_ERR_ALLOC;
+    }
+    int st = MQR_SUCCESS;
+    if (hipMemcpyAsync(d, A, sizeof(double) * na, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipMemcpyAsync(d + na, B, sizeof(double) * nb, hipMemcpyHostToDevice, h->stream) !=
+            hipSuccess)
+        st = MQR_ERR_HIP;
+    if (st == MQR_SUCCESS) {
+        int32_t *sp = nullptr, *si = nullptr;
+        st = lu_scratch(h, n, &sp, &si);
+        if (st == MQR_SUCCESS) st = getrf_f64(h, n, n, d, n, dp, dp + n, sp, si);
+        if (st == MQR_SUCCESS) st = getrs_f64(h, n, nrhs, d, n, dp, d + na, n);
+    }
+    if (st == MQR_SUCCESS &&
+        (hipMemcpyAsync(X, d + na, sizeof(double) * nb, hipMemcpyDeviceToHost, h->stream) !=
+             hipSuccess ||
+         hipStreamSynchronize(h->stream) != hipSuccess))
+        st = MQR_ERR_HIP;
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
+    (void)hipFree(dp);
     return st;
 }
 

@@ -112,6 +112,7 @@ struct mqr_handle_st {
     mqr::Workspace ws;             // general scratch
     mqr::Workspace ws_ir;          // mixed-precision / refinement scratch
     mqr::Workspace ws_vec;         // vector scratch of the triangular solve
+    mqr::Workspace ws_lu;          // pivot / info scratch of the LU path
     double *host_pinned = nullptr; // small pinned buffer for device->host scalars
     mqr::Profiler prof;
 };

@@ -701,3 +701,76 @@ void orc_qr_gramschmidt(int m, int n, const double *A, double *Q, double *R) {
         for (int r = 0; r < m; ++r) qj[r] = qj[r] * inv;
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* LU / PLU / Solve_direct: LU/__LU.h:2-16, LU/__PLU.h:18-50,           */
+/* LU/__Solve_direct.h:142-220 (the unblocked forms; the blocked ones   */
+/* reorder the same updates)                                            */
+/* ------------------------------------------------------------------ */
+/* In place on the m x n matrix F (ld m): L (unit, strictly below the
+ * diagonal) and U (on and above).  Column r first takes the updates of the
+ * earlier columns (Temp[i+1:] -= L[i+1:, i] Temp[i], fused like __mixed),
+ * then, with pivoting, the first row of largest |F(i, r)|, i >= r, is swapped
+ * into row r across the whole row (L's columns < r and U's >= r) and
+ * pos[] follows it; the column below the pivot is scaled by 1 / pivot.
+ * Without pivoting a zero pivot ends the factorisation (the source's
+ * `break`); with pivoting an all-zero column does.  Columns beyond the
+ * square part of a wide matrix take every update (the blocked source's
+ * L^{-1} A12; its unblocked loop stops after the first of them).  Returns
+ * the number of columns factored. */
+int orc_lu_factor(int m, int n, double *F, int pivot, int *pos) {
+    const int s = imin(m, n);
+    for (int i = 0; i < m; ++i) pos[i] = i;
+    for (int r = 0; r < n; ++r) {
+        double *col = F + (size_t)r * m;
+        for (int i = 0; i < imin(r, s); ++i)
+            axpy_into(col + i + 1, col + i + 1, F + (size_t)i * m + i + 1, -col[i], m - i - 1);
+        /* columns right of the square part (m < n) only take the updates,
+         * as the blocked source's triangular solve gives them */
+        if (r >= s) continue;
+        if (pivot) {
+            double maxv = 0.0;
+            for (int i = r; i < m; ++i) maxv = fabs(col[i]) > maxv ? fabs(col[i]) : maxv;
+            if (maxv == 0.0) return r;
+            int j = r;
+            for (int i = r; i < m; ++i)
+                if (fabs(col[i]) == maxv) { j = i; break; }
+            if (j != r) {
+                for (int c = 0; c < n; ++c) {
+                    double *p = F + (size_t)c * m;
+                    const double t = p[r]; p[r] = p[j]; p[j] = t;
+                }
+                const int t = pos[r]; pos[r] = pos[j]; pos[j] = t;
+            }
+        } else if (col[r] == 0.0) {
+            return r;
+        }
+        const double inv = 1.0 / col[r];
+        for (int i = r + 1; i < m; ++i) col[i] = col[i] * inv;
+    }
+    return s;
+}
+
+/* Solve_direct(A, B) (LU/__Solve_direct.h:142-220): A n x n (ld n), B and X
+ * n x nrhs (ld n): PLU of a copy of A, b = B with rows permuted by pos,
+ * forward substitution with the unit L, back substitution with U. */
+void orc_solve_direct(int n, int nrhs, const double *A, const double *B, double *X) {
+    double *F = (double *)malloc(sizeof(double) * (size_t)n * n);
+    int *pos = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    memcpy(F, A, sizeof(double) * (size_t)n * n);
+    orc_lu_factor(n, n, F, 1, pos);
+    for (int j = 0; j < nrhs; ++j) {
+        double *x = X + (size_t)j * n;
+        const double *b = B + (size_t)j * n;
+        for (int i = 0; i < n; ++i) x[i] = b[pos[i]];
+        for (int k = 0; k + 1 < n; ++k)
+            axpy_into(x + k + 1, x + k + 1, F + (size_t)k * n + k + 1, -x[k], n - k - 1);
+        for (int r = n - 1; r > 0; --r) {
+            x[r] = x[r] / F[(size_t)r * n + r];
+            axpy_into(x, x, F + (size_t)r * n, -x[r], r);
+        }
+        if (n > 0) x[0] = x[0] / F[0];
+    }
+    free(F);
+    free(pos);
+}

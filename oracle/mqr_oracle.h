@@ -93,6 +93,16 @@ void orc_qr_givens(int m, int n, const double *A, double *Q, double *R, int fix_
  * and R m x n (upper n x n triangle, zero elsewhere) written. */
 void orc_qr_gramschmidt(int m, int n, const double *A, double *Q, double *R);
 
+/* ---- LU / PLU / Solve_direct (LU/__LU.h, LU/__PLU.h, LU/__Solve_direct.h) ---- */
+/* In place on F (m x n, ld m): L unit lower (strictly below the diagonal),
+ * U on and above; pivot = 1 for partial pivoting (first row of largest
+ * magnitude), pos[m] the row permutation (row i of L U is row pos[i] of A).
+ * Returns the number of columns factored (min(m, n) unless a zero pivot /
+ * zero column stopped it). */
+int orc_lu_factor(int m, int n, double *F, int pivot, int *pos);
+/* X = A^{-1} B by PLU, A n x n, B and X n x nrhs (ld n). */
+void orc_solve_direct(int n, int nrhs, const double *A, const double *B, double *X);
+
 /* ---- least squares (no reference counterpart) ---- */
 /* x = argmin ||A x - b|| for m >= n via the blocked factorisation, Q^T b
  * through the compact-WY blocks, and back substitution with R

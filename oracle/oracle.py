@@ -66,6 +66,9 @@ def _bind(lib: C.CDLL) -> C.CDLL:
     lib.orc_gels.argtypes = [C.c_int, C.c_int, vp, vp, vp]
     lib.orc_qr_givens.argtypes = [C.c_int, C.c_int, vp, vp, vp, C.c_int]
     lib.orc_qr_gramschmidt.argtypes = [C.c_int, C.c_int, vp, vp, vp]
+    lib.orc_lu_factor.argtypes = [C.c_int, C.c_int, vp, C.c_int, vp]
+    lib.orc_lu_factor.restype = C.c_int
+    lib.orc_solve_direct.argtypes = [C.c_int, C.c_int, vp, vp, vp]
     lib.orc_trsv_upper.argtypes = [C.c_int, vp, i64, vp]
     return lib
 
@@ -232,6 +235,24 @@ class Oracle:
         R = np.zeros((m, n), order="F")
         self.lib.orc_qr_gramschmidt(m, n, _p(A), _p(Q), _p(R))
         return Q, R
+
+    def lu(self, A, pivot: bool = True):
+        """LU / PLU in place on a copy: returns (F, pos, factored) with L
+        unit lower below F's diagonal and U on and above, row i of L U being
+        row pos[i] of A."""
+        F = _f(A).copy(order="F")
+        m, n = F.shape
+        pos = np.zeros(max(m, 1), dtype=np.int32)
+        done = self.lib.orc_lu_factor(m, n, _p(F), 1 if pivot else 0, pos.ctypes.data)
+        return F, pos[:m], int(done)
+
+    def solve_direct(self, A, B) -> np.ndarray:
+        A, B = _f(A), _f(B)
+        n = A.shape[0]
+        nrhs = B.shape[1]
+        X = np.zeros((n, nrhs), order="F")
+        self.lib.orc_solve_direct(n, nrhs, _p(A), _p(B), _p(X))
+        return X
 
     def gels(self, A, b) -> np.ndarray:
         A = _f(A)

@@ -285,6 +285,54 @@ int main() {
         std::remove(fr.c_str());
         expect(ok && threw, "MAT file save / load / size round trips", 0.0, 0.0);
     }
+    {
+        // LU, PLU, PLU_getP, LU_getLU and Solve_direct (LU/)
+        MAT<double> A = randn<double>(150, 150, 13), L, U, P, Lc, Uc;
+        std::vector<int> pos(150);
+        LU_init(A, L, U);
+        const int info = PLU(A, pos.data(), L, U);
+        P.init(150, 150);
+        PLU_getP(pos.data(), P);
+        MAT<double> PLU_ = P * (L * U);
+        double num = 0, den = 0, up = 0;
+        for (int c = 0; c < 150; ++c)
+            for (int r = 0; r < 150; ++r) {
+                num += (PLU_(r, c) - A(r, c)) * (PLU_(r, c) - A(r, c));
+                den += A(r, c) * A(r, c);
+                if (r > c) up = std::max(up, std::fabs(U(r, c)));
+                if (r < c) up = std::max(up, std::fabs(L(r, c)));
+            }
+        expect(info == 0 && up == 0.0, "PLU: L unit lower, U upper", up, 0.0);
+        expect(std::sqrt(num / den) < 64 * 150 * u, "PLU: ||P L U - A|| / ||A||",
+               std::sqrt(num / den), 64 * 150 * u);
+        MAT<double> D = randn<double>(80, 80, 14);
+        for (int i = 0; i < 80; ++i) D(i, i) += 30.0;
+        LU(D, L, U);
+        MAT<double> F(80, 80);
+        for (int c = 0; c < 80; ++c)
+            for (int r = 0; r < 80; ++r) F(r, c) = r > c ? L(r, c) : U(r, c);
+        Lc.init(80, 80);
+        Uc.init(80, 80);
+        LU_getLU(F, Lc, Uc);
+        MAT<double> LUd = Lc * Uc;
+        num = den = 0;
+        for (int i = 0; i < 80 * 80; ++i) {
+            num += (LUd.data[i] - D.data[i]) * (LUd.data[i] - D.data[i]);
+            den += D.data[i] * D.data[i];
+        }
+        expect(std::sqrt(num / den) < 64 * 80 * u, "LU (no pivoting) + LU_getLU: ||L U - A||",
+               std::sqrt(num / den), 64 * 80 * u);
+        MAT<double> B = randn<double>(150, 3, 15);
+        MAT<double> X = Solve_direct(A, B);
+        MAT<double> AX = A * X;
+        num = den = 0;
+        for (int i = 0; i < 150 * 3; ++i) {
+            num += (AX.data[i] - B.data[i]) * (AX.data[i] - B.data[i]);
+            den += B.data[i] * B.data[i];
+        }
+        expect(std::sqrt(num / den) < 1e-10, "Solve_direct: ||A X - B|| / ||B||",
+               std::sqrt(num / den), 1e-10);
+    }
     std::printf("%s\n", failures ? "FAILED" : "ALL PASSED");
     return failures ? 1 : 0;
 }

@@ -474,3 +474,56 @@ def test_planted_mistakes_in_givens_and_gs_are_caught():
     Rg_bad = Rg.copy()
     Rg_bad[1, 4] = 0.0                                  # dropped projection coefficient
     assert np.linalg.norm(Qg @ Rg_bad[:7] - A) > 1e-3
+
+
+# ---------------------------------------------------------------------------
+# LU / PLU / Solve_direct (LU/__LU.h, LU/__PLU.h, LU/__Solve_direct.h)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("m,n", [(1, 1), (5, 5), (40, 40), (90, 33), (33, 90), (300, 300)])
+def test_plu_matches_lapack_getrf(orc, m, n):
+    """Partial pivoting picks the first row of largest magnitude, as LAPACK's
+    dgetrf (idamax) does: the same permutation, and L, U equal to rounding."""
+    rng = np.random.default_rng(m * 5 + n)
+    A = rng.standard_normal((m, n))
+    F, pos, done = orc.lu(A)
+    assert done == min(m, n)
+    lu, piv = sla.lu_factor(A) if m == n else (None, None)
+    # LAPACK's sequential swaps -> the permutation vector
+    import scipy.linalg.lapack as la
+    Fl, ipiv, info = la.dgetrf(A)
+    perm = np.arange(m)
+    for i, p in enumerate(ipiv):
+        perm[[i, p]] = perm[[p, i]]
+    assert np.array_equal(pos, perm)
+    np.testing.assert_allclose(F, Fl, rtol=0, atol=64 * max(m, n) * EPS * np.abs(A).max() * 10)
+    k = min(m, n)
+    L = np.tril(F[:, :k], -1) + np.eye(m, k)
+    U = np.triu(F[:k])
+    assert np.linalg.norm(L @ U - A[pos]) < 16 * max(m, n) * EPS * np.linalg.norm(A)
+
+
+def test_lu_without_pivoting_and_zero_pivot(orc):
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((50, 50)) + 60 * np.eye(50)   # diagonally dominant
+    F, pos, done = orc.lu(A, pivot=False)
+    assert done == 50 and np.array_equal(pos, np.arange(50))
+    L = np.tril(F, -1) + np.eye(50)
+    assert np.linalg.norm(L @ np.triu(F) - A) < 1e-13 * np.linalg.norm(A)
+    Z = A.copy()
+    Z[7, 7] = 0.0
+    Z[7, :7] = 0.0      # a zero pivot at column 7 without pivoting
+    _, _, done = orc.lu(Z, pivot=False)
+    assert done == 7
+
+
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (7, 3), (100, 1), (250, 6)])
+def test_solve_direct_matches_scipy(orc, n, nrhs):
+    rng = np.random.default_rng(n + nrhs)
+    A = rng.standard_normal((n, n))
+    B = rng.standard_normal((n, nrhs))
+    X = orc.solve_direct(A, B)
+    ref = sla.solve(A, B)
+    np.testing.assert_allclose(X, ref, rtol=0, atol=1e-10 * np.abs(ref).max() * n)
+    # a planted mistake (transposed system) is caught
+    assert not np.allclose(orc.solve_direct(A.T, B), ref, atol=1e-6) or n == 1
