@@ -97,12 +97,13 @@ def backward_errors(P, x):
 # their own stream inside the timed region; no other kernel is, so the
 # stream's dispatch gaps stay as they are.
 ROOF_KIND = "update_gemm"
-# HBM bytes moved per algorithmic byte by the FP64 trailing update, from two
-# rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by a 537 MB device copy
-# in the same run (profiles/trailing_update_gemm_ls16384x4096.md).
-TRAFFIC_RATIO_FP64_UPDATE = 1.226
-# The same for the FP32 trailing update of the mixed solve.
-TRAFFIC_RATIO_FP32_UPDATE = 1.187
+# HBM bytes moved per algorithmic byte by the FP64 trailing update (the
+# K = 256 update of each 256-column outer block), from two rocprofv3 --pmc
+# runs (FETCH_SIZE, WRITE_SIZE) scaled by a 537 MB device copy in the same
+# run (profiles/mfma_hbm_counters_ls16384x4096.md).
+TRAFFIC_RATIO_FP64_UPDATE = 1.639
+# The same for the FP32 trailing update of the mixed solve (K = 64).
+TRAFFIC_RATIO_FP32_UPDATE = 1.125
 
 
 def run_plain(M, h, P, steps: int, warmup: int):
@@ -167,8 +168,10 @@ def roofline_from(timed: dict, prof: dict, prec: str) -> dict:
     region (algorithmic bytes and flops per launch / mean HIP-event
     duration), plus the per-kind breakdown of one separately profiled solve.
     The bound is the one the tile's algorithmic intensity selects: HBM when
-    flops per byte lie below the ridge point peak_flops / 8 TB/s (at K = 64
-    they do: 8 flop/B in FP64 against 9.8, 16 in FP32 against 19.7)."""
+    flops per byte lie below the ridge point peak_flops / 8 TB/s.  The FP32
+    update at K = 64 does (16 flop/B against 19.7); the FP64 update at
+    K = 256 does not (30 flop/B against 9.8) and is priced against the FP64
+    MFMA peak."""
     p = timed.get(ROOF_KIND)
     if not p:
         return None

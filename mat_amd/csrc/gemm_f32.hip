@@ -42,30 +42,42 @@ __device__ __forceinline__ void split4(const float (&x)[4], v4s &hi, v4s &lo) {
 }
 
 // N consecutive floats of one column to / from memory: one vector access
-// when the address is aligned to the vector, else scalar ones
-template <int N>
+// when the address is aligned to the vector, else scalar ones.  STREAM marks
+// the accesses non-temporal: the output matrix C is read and written once
+// per launch, and keeping it out of L2 leaves L2 to the operand Y, which
+// every column of tiles reads again.
+template <int N, bool STREAM = false>
 __device__ __forceinline__ void load_run(float (&v)[N], const float *src) {
     typedef float vt __attribute__((ext_vector_type(N)));
     if ((reinterpret_cast<uintptr_t>(src) & (N * sizeof(float) - 1)) == 0) {
-        const vt x = *reinterpret_cast<const vt *>(src);
+        const vt x = STREAM ? __builtin_nontemporal_load(reinterpret_cast<const vt *>(src))
+                            : *reinterpret_cast<const vt *>(src);
 #pragma unroll
         for (int i = 0; i < N; ++i) v[i] = x[i];
     } else {
 #pragma unroll
-        for (int i = 0; i < N; ++i) v[i] = src[i];
+        for (int i = 0; i < N; ++i) v[i] = STREAM ? __builtin_nontemporal_load(src + i) : src[i];
     }
 }
-template <int N>
+template <int N, bool STREAM = false>
 __device__ __forceinline__ void store_run(float *dst, const float (&v)[N]) {
     typedef float vt __attribute__((ext_vector_type(N)));
     if ((reinterpret_cast<uintptr_t>(dst) & (N * sizeof(float) - 1)) == 0) {
         vt x;
 #pragma unroll
         for (int i = 0; i < N; ++i) x[i] = v[i];
-        *reinterpret_cast<vt *>(dst) = x;
+        if (STREAM)
+            __builtin_nontemporal_store(x, reinterpret_cast<vt *>(dst));
+        else
+            *reinterpret_cast<vt *>(dst) = x;
     } else {
 #pragma unroll
-        for (int i = 0; i < N; ++i) dst[i] = v[i];
+        for (int i = 0; i < N; ++i) {
+            if (STREAM)
+                __builtin_nontemporal_store(v[i], dst + i);
+            else
+                dst[i] = v[i];
+        }
     }
 }
 
@@ -153,12 +165,18 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_f32(GemmF32 gin, int kchunk) {
     constexpr int NV = TILE * BK / NT, BW = TILE / 32, HALF = TILE / 2;
     __shared__ float As[2][BK][TILE + 4];
     __shared__ float Bs[2][BK][TILE + 4];
-    // blockIdx.z = batch index * splits + split index
+    // tile z = batch index * splits + split index
     GemmF32 g = gin;
-    int split = blockIdx.z;
+    // W = V^T C style products (transposed A, few row tiles) take the
+    // XCD-aware order, so that the row tiles of one column slice of C share
+    // an L2; the others keep the dispatch order, in which each XCD holds the
+    // same rows of op(A) for every column of tiles.
+    int tx = blockIdx.x, ty = blockIdx.y, tz = blockIdx.z;
+    if constexpr (TA) xcd_tile(tx, ty, tz);
+    int split = tz;
     if (g.batch > 1) {
-        const int bz = blockIdx.z / g.splits;
-        split = blockIdx.z - bz * g.splits;
+        const int bz = tz / g.splits;
+        split = tz - bz * g.splits;
         g.A += bz * g.strideA;
         g.B += bz * g.strideB;
         g.C += bz * g.strideC;
@@ -166,7 +184,7 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_f32(GemmF32 gin, int kchunk) {
     }
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
-    const int m0 = blockIdx.x * TILE, n0 = blockIdx.y * TILE;
+    const int m0 = tx * TILE, n0 = ty * TILE;
     const int kb = split * kchunk;
     const int ke = min(g.K, kb + kchunk);
 
@@ -192,7 +210,7 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_f32(GemmF32 gin, int kchunk) {
                 const float *src = Cb + (int64_t)(m0 + arow) + (int64_t)cn * ldb_c;
                 float v[BW];
                 if (interior) {
-                    load_run<BW>(v, src);
+                    load_run<BW, true>(v, src);
                 } else {
 #pragma unroll
                     for (int i = 0; i < BW; ++i) {
@@ -296,7 +314,7 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_f32(GemmF32 gin, int kchunk) {
             for (int i = 0; i < BW; ++i) v[i] = acc[i][j][r];
             float *dst = Cout + (int64_t)(m0 + arow) + (int64_t)gn * g.ldc;
             if (interior) {
-                store_run<BW>(dst, v);
+                store_run<BW, true>(dst, v);
             } else if (gn < g.N) {
 #pragma unroll
                 for (int i = 0; i < BW; ++i)

@@ -46,13 +46,15 @@ __device__ __forceinline__ void epilogue(const GemmF64 &g, v4d (&acc)[BI][BJ], i
             for (int r = 0; r < 4; ++r) {
                 const int gn = colbase + j * 16 + (lane >> 4) + 4 * r;
                 if (interior || (gm < g.M && gn < g.N))
-                    Cout[(int64_t)gm + (int64_t)gn * g.ldc] = acc[i][j][r];
+                    __builtin_nontemporal_store(acc[i][j][r], Cout + (int64_t)gm + (int64_t)gn * g.ldc);
             }
     }
 }
 
 // acc = beta C for the tile (clamped, always in-bounds loads), issued before
-// the K loop so that the loads of C overlap the first slices.
+// the K loop so that the loads of C overlap the first slices.  C is read and
+// written non-temporally (here and in the epilogue): each launch touches it
+// once, and L2 stays with the operand every column of tiles reads again.
 template <int BI, int BJ>
 __device__ __forceinline__ void preload_c(const GemmF64 &g, v4d (&acc)[BI][BJ], int rowbase,
                                           int colbase) {
@@ -67,8 +69,9 @@ __device__ __forceinline__ void preload_c(const GemmF64 &g, v4d (&acc)[BI][BJ], 
             for (int r = 0; r < 4; ++r) {
                 const int gn = colbase + j * 16 + (lane >> 4) + 4 * r;
                 const int cn = gn < g.N ? gn : g.N - 1;
-                acc[i][j][r] = g.beta * (g.Cin ? g.Cin[(int64_t)cm + (int64_t)cn * g.ldcin]
-                                               : g.C[(int64_t)cm + (int64_t)cn * g.ldc]);
+                acc[i][j][r] =
+                    g.beta * __builtin_nontemporal_load(g.Cin ? g.Cin + (int64_t)cm + (int64_t)cn * g.ldcin
+                                                              : g.C + (int64_t)cm + (int64_t)cn * g.ldc);
             }
     }
 }
@@ -179,12 +182,18 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_f64(GemmF64 gin, int kchunk) {
     __shared__ double As[2][BK][TILE + 2];
     __shared__ double Bs[2][BK][TILE + 2];
     if (gin.skip_if && *gin.skip_if) return;
-    // blockIdx.z = batch index * splits + split index
+    // tile z = batch index * splits + split index
     GemmF64 g = gin;
-    int split = blockIdx.z;
+    // W = V^T C style products (transposed A, few row tiles) take the
+    // XCD-aware order, so that the row tiles of one column slice of C share
+    // an L2; the others keep the dispatch order, in which each XCD holds the
+    // same rows of op(A) for every column of tiles.
+    int tx = blockIdx.x, ty = blockIdx.y, tz = blockIdx.z;
+    if constexpr (TA) xcd_tile(tx, ty, tz);
+    int split = tz;
     if (g.batch > 1) {
-        const int bz = blockIdx.z / g.splits;
-        split = blockIdx.z - bz * g.splits;
+        const int bz = tz / g.splits;
+        split = tz - bz * g.splits;
         g.A += bz * g.strideA;
         g.B += bz * g.strideB;
         g.C += bz * g.strideC;
@@ -192,7 +201,7 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_f64(GemmF64 gin, int kchunk) {
     }
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
-    const int m0 = blockIdx.x * TILE, n0 = blockIdx.y * TILE;
+    const int m0 = tx * TILE, n0 = ty * TILE;
     int kb = split * kchunk;
     int ke = min(g.K, kb + kchunk);
     // triangular operands: rows >= m0 of an upper op(A) are zero for k < m0,

@@ -122,6 +122,25 @@ namespace mqr {
 // Ensure ws holds at least `bytes`; may synchronise the stream.
 int ws_reserve(mqr_handle_t h, Workspace &ws, size_t bytes);
 
+// Tile coordinates of the calling workgroup in an XCD-aware order.  The
+// dispatcher deals consecutive workgroup ids round-robin to the eight XCDs
+// (id % 8), each with its own L2.  Renumbering so that XCD x runs the
+// contiguous range [x per, (x + 1) per) of the x-fastest tile order puts
+// neighbouring tiles, which share operand slices (the row tiles of
+// W = V^T C read the same slice of C), on one XCD and close in time.  Ids
+// past 8 per keep their number.  A bijection of the grid: every tile is
+// still computed exactly once, with the same arithmetic.
+__device__ __forceinline__ void xcd_tile(int &bx, int &by, int &bz) {
+    const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+    const unsigned per = gx * gy * gz / 8;
+    unsigned id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    if (id < per * 8) id = (id % 8) * per + id / 8;
+    bx = (int)(id % gx);
+    const unsigned t = id / gx;
+    by = (int)(t % gy);
+    bz = (int)(t / gy);
+}
+
 // ---- FP64 MFMA GEMM -------------------------------------------------------
 // C[M x N] = alpha * op(A) * op(B) + beta * C, column-major.
 // Mask modes apply to the *stored* operand before op():
