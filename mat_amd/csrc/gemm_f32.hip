@@ -11,7 +11,8 @@
 // them to BF16 when the MFMA operands are formed: x = hi + lo with
 // hi = bf16(x), lo = bf16(x - hi); MODE 1 multiplies hi * hi, MODE 3 adds
 // hi * lo + lo * hi (the lo * lo term, ~2^-16 relative, is dropped), all on
-// v_mfma_f32_16x16x16_bf16 with FP32 accumulation.
+// gfx950's v_mfma_f32_16x16x32_bf16 (32-deep K slices in LDS) with FP32
+// accumulation.
 #include <algorithm>
 #include <climits>
 
@@ -21,7 +22,8 @@ namespace mqr {
 namespace {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
-typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 constexpr int NT = 256, BK = 16;
 
 // float -> bf16 bits, round to nearest even (finite inputs)
@@ -31,14 +33,17 @@ __device__ __forceinline__ unsigned bf16_bits(float x) {
     return u >> 16;
 }
 
-// hi/lo BF16 operands of four FP32 values
-__device__ __forceinline__ void split4(const float (&x)[4], v4s &hi, v4s &lo) {
+// hi/lo BF16 operands of eight FP32 values
+__device__ __forceinline__ void split8(const float (&x)[8], v8bf &hi, v8bf &lo) {
+    v8s h8, l8;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 8; ++q) {
         const unsigned h = bf16_bits(x[q]);
-        hi[q] = (short)h;
-        lo[q] = (short)bf16_bits(x[q] - __uint_as_float(h << 16));
+        h8[q] = (short)h;
+        l8[q] = (short)bf16_bits(x[q] - __uint_as_float(h << 16));
     }
+    hi = __builtin_bit_cast(v8bf, h8);
+    lo = __builtin_bit_cast(v8bf, l8);
 }
 
 // N consecutive floats of one column to / from memory: one vector access
@@ -81,29 +86,29 @@ __device__ __forceinline__ void store_run(float *dst, const float (&v)[N]) {
     }
 }
 
-template <int TILE, bool KMAJOR>
+template <int TILE, bool KMAJOR, int BKT = BK>
 __device__ __forceinline__ void slot(int tid, int q, int &r, int &k) {
-    constexpr int NV = TILE * BK / NT;
+    constexpr int NV = TILE * BKT / NT;
     if (!KMAJOR) {
         constexpr int RT = TILE / 2;
         r = (tid % RT) * 2 + (q & 1);
         k = (tid / RT) + (NT / RT) * (q >> 1);
     } else {
-        constexpr int KT = BK / NV;
+        constexpr int KT = BKT / NV;
         r = tid / KT;
         k = (tid % KT) * NV + q;
     }
 }
 
-template <int TILE, bool KMAJOR>
+template <int TILE, bool KMAJOR, int BKT = BK>
 __device__ __forceinline__ void fetch(const float *__restrict__ X, int64_t ld, int r0, int k0,
                                       int R, int kend, int tid, float *v) {
-    constexpr int NV = TILE * BK / NT;
+    constexpr int NV = TILE * BKT / NT;
     if (KMAJOR) {
         // a thread's NV values are consecutive k of one row: one vector load
         // when the row and the whole run are in range
         int r, k;
-        slot<TILE, true>(tid, 0, r, k);
+        slot<TILE, true, BKT>(tid, 0, r, k);
         r += r0;
         k += k0;
         if (r < R && k + NV <= kend) {
@@ -120,14 +125,14 @@ __device__ __forceinline__ void fetch(const float *__restrict__ X, int64_t ld, i
 #pragma unroll
         for (int q = 0; q < NV; q += 2) {
             int r, k;
-            slot<TILE, false>(tid, q, r, k);
+            slot<TILE, false, BKT>(tid, q, r, k);
             fast = fast && (r0 + r + 1 < R) && (k0 + k < kend);
         }
         if (fast) {
 #pragma unroll
             for (int q = 0; q < NV; q += 2) {
                 int r, k;
-                slot<TILE, false>(tid, q, r, k);
+                slot<TILE, false, BKT>(tid, q, r, k);
                 float t[2];
                 load_run<2>(t, X + (int64_t)(r0 + r) + (int64_t)(k0 + k) * ld);
                 v[q] = t[0];
@@ -139,7 +144,7 @@ __device__ __forceinline__ void fetch(const float *__restrict__ X, int64_t ld, i
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
         int r, k;
-        slot<TILE, KMAJOR>(tid, q, r, k);
+        slot<TILE, KMAJOR, BKT>(tid, q, r, k);
         r += r0;
         k += k0;
         const int rr = r < R ? r : R - 1, kc = k < kend ? k : kend - 1;
@@ -147,14 +152,14 @@ __device__ __forceinline__ void fetch(const float *__restrict__ X, int64_t ld, i
     }
 }
 
-template <int TILE, bool KMAJOR>
+template <int TILE, bool KMAJOR, int BKT = BK>
 __device__ __forceinline__ void stash(float (*S)[TILE + 4], int tid, const float *v, int r0, int k0,
                                       int R, int kend, float scale) {
-    constexpr int NV = TILE * BK / NT;
+    constexpr int NV = TILE * BKT / NT;
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
         int r, k;
-        slot<TILE, KMAJOR>(tid, q, r, k);
+        slot<TILE, KMAJOR, BKT>(tid, q, r, k);
         const bool in = (r0 + r < R) && (k0 + k < kend);
         S[k][r] = scale * __int_as_float(__float_as_int(v[q]) & (in ? ~0 : 0));
     }
@@ -162,9 +167,12 @@ __device__ __forceinline__ void stash(float (*S)[TILE + 4], int tid, const float
 
 template <int TILE, bool TA, bool TB, int MODE>
 __global__ __launch_bounds__(NT, 2) void k_gemm_f32(GemmF32 gin, int kchunk) {
-    constexpr int NV = TILE * BK / NT, BW = TILE / 32, HALF = TILE / 2;
-    __shared__ float As[2][BK][TILE + 4];
-    __shared__ float Bs[2][BK][TILE + 4];
+    // K slice depth: 16 for the FP32 MFMA (four K = 4 steps), 32 for the
+    // BF16 modes (one v_mfma_f32_16x16x32_bf16 per tile and term)
+    constexpr int BKM = MODE == 0 ? BK : 2 * BK;
+    constexpr int NV = TILE * BKM / NT, BW = TILE / 32, HALF = TILE / 2;
+    __shared__ float As[2][BKM][TILE + 4];
+    __shared__ float Bs[2][BKM][TILE + 4];
     // tile z = batch index * splits + split index
     GemmF32 g = gin;
     // W = V^T C style products (transposed A, few row tiles) take the
@@ -230,17 +238,17 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_f32(GemmF32 gin, int kchunk) {
 
     if (kb < ke) {
         float va[NV], vb[NV];
-        fetch<TILE, TA>(g.A, g.lda, m0, kb, g.M, ke, tid, va);
-        fetch<TILE, !TB>(g.B, g.ldb, n0, kb, g.N, ke, tid, vb);
-        stash<TILE, TA>(As[0], tid, va, m0, kb, g.M, ke, g.alpha);
-        stash<TILE, !TB>(Bs[0], tid, vb, n0, kb, g.N, ke, 1.f);
+        fetch<TILE, TA, BKM>(g.A, g.lda, m0, kb, g.M, ke, tid, va);
+        fetch<TILE, !TB, BKM>(g.B, g.ldb, n0, kb, g.N, ke, tid, vb);
+        stash<TILE, TA, BKM>(As[0], tid, va, m0, kb, g.M, ke, g.alpha);
+        stash<TILE, !TB, BKM>(Bs[0], tid, vb, n0, kb, g.N, ke, 1.f);
         __syncthreads();
         int buf = 0;
-        for (int k0 = kb; k0 < ke; k0 += BK) {
-            const bool more = k0 + BK < ke;
+        for (int k0 = kb; k0 < ke; k0 += BKM) {
+            const bool more = k0 + BKM < ke;
             if (more) {
-                fetch<TILE, TA>(g.A, g.lda, m0, k0 + BK, g.M, ke, tid, va);
-                fetch<TILE, !TB>(g.B, g.ldb, n0, k0 + BK, g.N, ke, tid, vb);
+                fetch<TILE, TA, BKM>(g.A, g.lda, m0, k0 + BKM, g.M, ke, tid, va);
+                fetch<TILE, !TB, BKM>(g.B, g.ldb, n0, k0 + BKM, g.N, ke, tid, vb);
             }
             if constexpr (MODE == 0) {
 #pragma unroll
@@ -259,42 +267,43 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_f32(GemmF32 gin, int kchunk) {
                                 __builtin_amdgcn_mfma_f32_16x16x4f32(b[j], a[i], acc[i][j], 0, 0, 0);
                 }
             } else {
-                // one 16-deep MFMA per tile and term: lane l supplies
-                // k = 4 (l >> 4) + q, q = 0..3, of row / column l & 15
-                v4s ah[BW], al[BW], bh[BW], bl[BW];
+                // one 32-deep MFMA per tile and term: lane l supplies
+                // k = 8 (l >> 4) + q, q = 0..7, of row / column l & 15 (A and
+                // B pair the same k in the same lane and slot)
+                v8bf ah[BW], al[BW], bh[BW], bl[BW];
 #pragma unroll
                 for (int i = 0; i < BW; ++i) {
-                    float x[4];
+                    float x[8];
 #pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        x[q] = As[buf][4 * (lane >> 4) + q][arow + i];
-                    split4(x, ah[i], al[i]);
+                    for (int q = 0; q < 8; ++q)
+                        x[q] = As[buf][8 * (lane >> 4) + q][arow + i];
+                    split8(x, ah[i], al[i]);
                 }
 #pragma unroll
                 for (int j = 0; j < BW; ++j) {
-                    float x[4];
+                    float x[8];
 #pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        x[q] = Bs[buf][4 * (lane >> 4) + q][wn * HALF + j * 16 + (lane & 15)];
-                    split4(x, bh[j], bl[j]);
+                    for (int q = 0; q < 8; ++q)
+                        x[q] = Bs[buf][8 * (lane >> 4) + q][wn * HALF + j * 16 + (lane & 15)];
+                    split8(x, bh[j], bl[j]);
                 }
 #pragma unroll
                 for (int i = 0; i < BW; ++i)
 #pragma unroll
                     for (int j = 0; j < BW; ++j) {
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bh[j], ah[i],
-                                                                              acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i],
+                                                                            acc[i][j], 0, 0, 0);
                         if constexpr (MODE == 3) {
-                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
                                 bh[j], al[i], acc[i][j], 0, 0, 0);
-                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
                                 bl[j], ah[i], acc[i][j], 0, 0, 0);
                         }
                     }
             }
             if (more) {
-                stash<TILE, TA>(As[buf ^ 1], tid, va, m0, k0 + BK, g.M, ke, g.alpha);
-                stash<TILE, !TB>(Bs[buf ^ 1], tid, vb, n0, k0 + BK, g.N, ke, 1.f);
+                stash<TILE, TA, BKM>(As[buf ^ 1], tid, va, m0, k0 + BKM, g.M, ke, g.alpha);
+                stash<TILE, !TB, BKM>(Bs[buf ^ 1], tid, vb, n0, k0 + BKM, g.N, ke, 1.f);
             }
             __syncthreads();
             buf ^= 1;
@@ -326,8 +335,10 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_f32(GemmF32 gin, int kchunk) {
 template <bool TA, bool TB, int MODE>
 void launch_mode(const GemmF32 &g, hipStream_t s) {
     const int tile = (g.M >= 128 && g.N >= 128) ? 128 : 64;
+    // split-K chunks start on whole slices of the mode's depth
+    const int bkm = MODE == 0 ? BK : 2 * BK;
     int kchunk = (g.K + g.splits - 1) / g.splits;
-    kchunk = ((kchunk + BK - 1) / BK) * BK;
+    kchunk = ((kchunk + bkm - 1) / bkm) * bkm;
     dim3 grid((g.M + tile - 1) / tile, (g.N + tile - 1) / tile, g.splits * std::max(g.batch, 1));
     if (tile == 128)
         hipLaunchKernelGGL((k_gemm_f32<128, TA, TB, MODE>), grid, dim3(NT), 0, s, g, kchunk);

@@ -210,8 +210,75 @@ static void pack_UT(const double *T, int k, double *UTp) {
 /* accumulators, updated in k order with one fused multiply-add (add)   */
 /* or multiply-subtract (nmul) per element and step, then stored.       */
 /* ------------------------------------------------------------------ */
+#if defined(ORC_SIMD)
+/* The same micro-kernel with the accumulators in vector registers (the
+ * ORC_SIMD builds, used as the timed CPU baseline): C in MR/W vectors per
+ * column, per k step one vector load of the packed A column piece and one
+ * broadcast of each packed B value, one fused multiply-add (or negated
+ * multiply-add) per element and step in k order, as in the scalar loop.
+ * The rounding is therefore identical element by element; partial edge
+ * tiles take the scalar loop. */
+#include <immintrin.h>
+#if defined(ORC_ISA_AVX512)
+#define VW 8
+typedef __m512d vd;
+#define V_LOAD(p) _mm512_loadu_pd(p)
+#define V_STORE(p, v) _mm512_storeu_pd((p), (v))
+#define V_BCAST(x) _mm512_set1_pd(x)
+#define V_FMADD(a, b, c) _mm512_fmadd_pd((a), (b), (c))
+#define V_FNMADD(a, b, c) _mm512_fnmadd_pd((a), (b), (c))
+#else
+#define VW 4
+typedef __m256d vd;
+#define V_LOAD(p) _mm256_loadu_pd(p)
+#define V_STORE(p, v) _mm256_storeu_pd((p), (v))
+#define V_BCAST(x) _mm256_set1_pd(x)
+#define V_FMADD(a, b, c) _mm256_fmadd_pd((a), (b), (c))
+#define V_FNMADD(a, b, c) _mm256_fnmadd_pd((a), (b), (c))
+#endif
+#define MV (MR / VW)
+
+static void micro_kernel_simd(const double *Ap, const double *Bp, double *C, int64_t ldC, int k,
+                              int add) {
+    vd acc[NR][MV];
+    for (int j = 0; j < NR; ++j)
+        for (int v = 0; v < MV; ++v) acc[j][v] = V_LOAD(C + (size_t)j * ldC + v * VW);
+    if (add) {
+        for (int p = 0; p < k; ++p) {
+            const double *a = Ap + (size_t)p * MR;
+            const double *b = Bp + (size_t)p * NR;
+            vd av[MV];
+            for (int v = 0; v < MV; ++v) av[v] = V_LOAD(a + v * VW);
+            for (int j = 0; j < NR; ++j) {
+                const vd bj = V_BCAST(b[j]);
+                for (int v = 0; v < MV; ++v) acc[j][v] = V_FMADD(av[v], bj, acc[j][v]);
+            }
+        }
+    } else {
+        for (int p = 0; p < k; ++p) {
+            const double *a = Ap + (size_t)p * MR;
+            const double *b = Bp + (size_t)p * NR;
+            vd av[MV];
+            for (int v = 0; v < MV; ++v) av[v] = V_LOAD(a + v * VW);
+            for (int j = 0; j < NR; ++j) {
+                const vd bj = V_BCAST(b[j]);
+                for (int v = 0; v < MV; ++v) acc[j][v] = V_FNMADD(av[v], bj, acc[j][v]);
+            }
+        }
+    }
+    for (int j = 0; j < NR; ++j)
+        for (int v = 0; v < MV; ++v) V_STORE(C + (size_t)j * ldC + v * VW, acc[j][v]);
+}
+#endif
+
 void orc_micro_kernel(const double *Ap, const double *Bp, double *C, int64_t ldC,
                       int k, int mm, int nn, int add) {
+#if defined(ORC_SIMD)
+    if (mm == MR && nn == NR) {
+        micro_kernel_simd(Ap, Bp, C, ldC, k, add);
+        return;
+    }
+#endif
     double acc[NR][MR];
     const int full = (mm == MR && nn == NR);
     for (int j = 0; j < NR; ++j)

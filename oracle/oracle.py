@@ -73,11 +73,19 @@ def _bind(lib: C.CDLL) -> C.CDLL:
     return lib
 
 
+def resolve(variant: str) -> str:
+    """``"auto"`` -> the host's widest ISA; ``"auto_simd"`` likewise with the
+    register-accumulator micro-kernel."""
+    isa = "avx512" if cpu_has_avx512() else "avx2"
+    return {"auto": isa, "auto_simd": isa + "_simd"}.get(variant, variant)
+
+
 def load(variant: str = "avx2") -> C.CDLL:
     """Load one oracle build: ``"avx2"`` (BS 96, 8x6 tile) or ``"avx512"``
-    (BS 48, 16x12 tile), or ``"auto"`` (AVX-512 when the host has it)."""
-    if variant == "auto":
-        variant = "avx512" if cpu_has_avx512() else "avx2"
+    (BS 48, 16x12 tile), ``"avx2_simd"`` / ``"avx512_simd"`` (the same with
+    the micro-kernel's accumulators in vector registers, bit-identical), or
+    ``"auto"`` / ``"auto_simd"`` (AVX-512 when the host has it)."""
+    variant = resolve(variant)
     if variant not in _libs:
         path = os.path.join(_LIB_DIR, f"liborc_{variant}.so")
         if not os.path.exists(path):
@@ -98,7 +106,7 @@ class Oracle:
     """One oracle build with numpy-level helpers."""
 
     def __init__(self, variant: str = "avx2"):
-        self.variant = variant if variant != "auto" else ("avx512" if cpu_has_avx512() else "avx2")
+        self.variant = resolve(variant)
         self.lib = load(self.variant)
         cfg = (C.c_int * 6)()
         self.lib.orc_config(C.cast(cfg, C.c_void_p))

@@ -263,6 +263,29 @@ def test_variants_agree():
     np.testing.assert_allclose(Qa, Qb, atol=1e-12)
 
 
+@pytest.mark.parametrize("isa", ["avx2", "avx512"])
+def test_simd_builds_are_bit_identical(isa):
+    """The register-accumulator micro-kernel (the timed CPU baseline) rounds
+    exactly like the scalar restatement: same fused multiply-adds in the same
+    k order per element.  Edge tiles (300 = 18 MR + 12 rows for 16-row tiles,
+    211 columns) take the scalar path inside the same call."""
+    if isa == "avx512":
+        from oracle.oracle import cpu_has_avx512
+        if not cpu_has_avx512():
+            pytest.skip("host has no AVX-512")
+    a, b = Oracle(isa), Oracle(isa + "_simd")
+    rng = np.random.default_rng(21)
+    A = rng.standard_normal((300, 170))
+    B = rng.standard_normal((170, 211))
+    assert np.array_equal(a.gemm(A, B), b.gemm(A, B))
+    M = np.asfortranarray(rng.standard_normal((700, 300)))
+    y = rng.standard_normal(700)
+    assert np.array_equal(a.gels(M, y), b.gels(M, y))
+    Qa, Ra, _ = a.qr(M[:, :250])
+    Qb, Rb, _ = b.qr(M[:, :250])
+    assert np.array_equal(Ra, Rb) and np.array_equal(Qa, Qb)
+
+
 def test_c1_invariants_and_lapack_r(golden):
     """The CPU configuration: 1024 x 1024 iid N(0,1), blocked branch."""
     o = Oracle("avx2")
