@@ -21,6 +21,7 @@ LIB = os.path.join(HERE, "lib", "libmatqr.so")
 CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_mat_api.cpp")
 CPP_TEST_BIN = os.path.join(HERE, "lib", "test_mat_api")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM_LIB = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(HIPCC))), "lib")
 ARCH = os.environ.get("MQR_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [
@@ -57,7 +58,9 @@ def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    # roctx (phase ranges for rocprofv3 --marker-trace) from the ROCm install
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
+           f"-L{ROCM_LIB}", "-lrocprofiler-sdk-roctx", f"-Wl,-rpath,{ROCM_LIB}"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

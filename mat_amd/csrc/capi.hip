@@ -188,9 +188,10 @@ int mqr_create(mqr_handle_t *handle, int device) {
         hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&h->ev_in, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&h->ev_out, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&h->ev_out, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_host, hipEventDisableTiming) != hipSuccess) {
         (void)hipGetLastError();
-        for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out})
+        for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out, h->ev_host})
             if (e) (void)hipEventDestroy(e);
         if (h->side_stream) (void)hipStreamDestroy(h->side_stream);
         if (h->far_stream) (void)hipStreamDestroy(h->far_stream);
@@ -204,7 +205,7 @@ int mqr_create(mqr_handle_t *handle, int device) {
         h->cu_count = cus;
     if (hipHostMalloc((void **)&h->host_pinned, 64 * sizeof(double), hipHostMallocDefault) !=
         hipSuccess) {
-        for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out}) (void)hipEventDestroy(e);
+        for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out, h->ev_host}) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(h->side_stream);
         (void)hipStreamDestroy(h->far_stream);
         (void)hipStreamDestroy(h->own_stream);
@@ -215,7 +216,7 @@ int mqr_create(mqr_handle_t *handle, int device) {
     if (batched_noalloc_init(device) != MQR_SUCCESS) {
         (void)hipGetLastError();
         (void)hipHostFree(h->host_pinned);
-        for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out}) (void)hipEventDestroy(e);
+        for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out, h->ev_host}) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(h->side_stream);
         (void)hipStreamDestroy(h->far_stream);
         (void)hipStreamDestroy(h->own_stream);
@@ -237,7 +238,7 @@ int mqr_destroy(mqr_handle_t h) {
     if (h->ws_vec.ptr) (void)hipFree(h->ws_vec.ptr);
     if (h->ws_lu.ptr) (void)hipFree(h->ws_lu.ptr);
     if (h->host_pinned) (void)hipHostFree(h->host_pinned);
-    for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out})
+    for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out, h->ev_host})
         if (e) (void)hipEventDestroy(e);
     if (h->side_stream) (void)hipStreamDestroy(h->side_stream);
     if (h->far_stream) (void)hipStreamDestroy(h->far_stream);
@@ -328,6 +329,7 @@ int mqr_memcpy_dtoh(mqr_handle_t h, void *dst, const void *src, size_t bytes) {
 int mqr_dgemm(mqr_handle_t h, char transa, char transb, int64_t m, int64_t n, int64_t k,
               double alpha, const double *A, int64_t lda, const double *B, int64_t ldb,
               double beta, double *C, int64_t ldc) {
+    mqr::Range rg_("mqr_dgemm");
     if (!h) return -1;
     const bool ta = transa == 'T' || transa == 't', tb = transb == 'T' || transb == 't';
     if (!ta && transa != 'N' && transa != 'n') return -2;
@@ -385,12 +387,14 @@ static int sgemm_common(mqr_handle_t h, int terms, char transa, char transb, int
 int mqr_sgemm(mqr_handle_t h, char transa, char transb, int64_t m, int64_t n, int64_t k,
               float alpha, const float *A, int64_t lda, const float *B, int64_t ldb, float beta,
               float *C, int64_t ldc) {
+    mqr::Range rg_("mqr_sgemm");
     return sgemm_common(h, 0, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 int mqr_sgemm_bf16(mqr_handle_t h, int terms, char transa, char transb, int64_t m, int64_t n,
                    int64_t k, float alpha, const float *A, int64_t lda, const float *B,
                    int64_t ldb, float beta, float *C, int64_t ldc) {
+    mqr::Range rg_("mqr_sgemm_bf16");
     if (!h) return -1;
     if (terms != 1 && terms != 3) return -2;
     const int st = sgemm_common(h, terms, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C,
@@ -474,16 +478,19 @@ extern "C" {
 
 int mqr_dgemm_host(mqr_handle_t h, int64_t m, int64_t n, int64_t k, const double *A,
                    const double *B, double *C) {
+    mqr::Range rg_("mqr_dgemm_host");
     return gemm_host<double>(h, m, n, k, A, B, C);
 }
 
 int mqr_sgemm_host(mqr_handle_t h, int64_t m, int64_t n, int64_t k, const float *A,
                    const float *B, float *C) {
+    mqr::Range rg_("mqr_sgemm_host");
     return gemm_host<float>(h, m, n, k, A, B, C);
 }
 
 int mqr_dgeqrf(mqr_handle_t h, int64_t m, int64_t n, double *A, int64_t lda, double *tau,
                double *T, int64_t ldt) {
+    mqr::Range rg_("mqr_dgeqrf");
     if (!h) return -1;
     if (m < 0) return -2;
     if (n < 0) return -3;
@@ -503,6 +510,7 @@ int mqr_dgeqrf(mqr_handle_t h, int64_t m, int64_t n, double *A, int64_t lda, dou
 
 int mqr_dorgqr(mqr_handle_t h, int64_t m, int64_t nq, int64_t k, const double *VR, int64_t ldvr,
                const double *T, int64_t ldt, double *Q, int64_t ldq) {
+    mqr::Range rg_("mqr_dorgqr");
     if (!h) return -1;
     if (m < 0) return -2;
     if (nq < 0 || nq > m) return -3;
@@ -522,6 +530,7 @@ int mqr_dorgqr(mqr_handle_t h, int64_t m, int64_t nq, int64_t k, const double *V
 
 int mqr_dormqr(mqr_handle_t h, char trans, int64_t m, int64_t nrhs, int64_t k, const double *VR,
                int64_t ldvr, const double *T, int64_t ldt, double *C, int64_t ldc) {
+    mqr::Range rg_("mqr_dormqr");
     if (!h) return -1;
     const bool tr = trans == 'T' || trans == 't';
     if (!tr && trans != 'N' && trans != 'n') return -2;
@@ -543,6 +552,7 @@ int mqr_dormqr(mqr_handle_t h, char trans, int64_t m, int64_t nrhs, int64_t k, c
 
 int mqr_dtrsm_upper(mqr_handle_t h, char trans, int64_t n, int64_t nrhs, const double *R,
                     int64_t ldr, double *B, int64_t ldb) {
+    mqr::Range rg_("mqr_dtrsm_upper");
     if (!h) return -1;
     const bool tr = trans == 'T' || trans == 't';
     if (!tr && trans != 'N' && trans != 'n') return -2;
@@ -560,6 +570,7 @@ int mqr_dtrsm_upper(mqr_handle_t h, char trans, int64_t n, int64_t nrhs, const d
 
 int mqr_dgels(mqr_handle_t h, int64_t m, int64_t n, int64_t nrhs, double *A, int64_t lda,
               double *B, int64_t ldb) {
+    mqr::Range rg_("mqr_dgels");
     if (!h) return -1;
     if (m < 0) return -2;
     if (n < 0 || n > m) return -3;
@@ -584,6 +595,7 @@ int mqr_dgels(mqr_handle_t h, int64_t m, int64_t n, int64_t nrhs, double *A, int
 int mqr_gels_mixed(mqr_handle_t h, mqr_precision_t prec, int64_t m, int64_t n, const double *A,
                    int64_t lda, const double *b, double *x, const mqr_ir_opts_t *opts,
                    mqr_ir_info_t *info) {
+    mqr::Range rg_("mqr_gels_mixed");
     if (!h) return -1;
     if (prec < MQR_PREC_FP64 || prec > MQR_PREC_BF16) return -2;
     if (m < 0) return -3;
@@ -621,6 +633,7 @@ static int check_batched(mqr_handle_t h, mqr_dtype_t dtype, int64_t batch, const
 int mqr_geqrf_batched_uneven(mqr_handle_t h, mqr_dtype_t dtype, int64_t batch, const int32_t *m,
                              const int32_t *n, const int64_t *lda, void *const *A,
                              void *const *tau, int tile) {
+    mqr::Range rg_("mqr_geqrf_batched_uneven");
     const int a = check_batched(h, dtype, batch, m, n, lda, A, tau, tile);
     if (a) return a;
     if (batch == 0) return MQR_SUCCESS;
@@ -632,6 +645,7 @@ int mqr_geqrf_batched_uneven(mqr_handle_t h, mqr_dtype_t dtype, int64_t batch, c
 int mqr_geqrf_batched_uneven_noalloc(mqr_handle_t h, mqr_dtype_t dtype, int64_t batch,
                                      const int32_t *m, const int32_t *n, const int64_t *lda,
                                      void *const *A, void *const *tau, int tile) {
+    mqr::Range rg_("mqr_geqrf_batched_uneven_noalloc");
     if (!h) return -1;
     if (batch > MQR_NOALLOC_MAX_BATCH) return MQR_ERR_TOO_LARGE;
     const int a = check_batched(h, dtype, batch, m, n, lda, A, tau, tile);
@@ -645,6 +659,7 @@ int mqr_geqrf_batched_uneven_noalloc(mqr_handle_t h, mqr_dtype_t dtype, int64_t 
 int mqr_batched_plan_create(mqr_handle_t h, mqr_dtype_t dtype, int64_t batch, const int32_t *m,
                             const int32_t *n, const int64_t *lda, void *const *A,
                             void *const *tau, int tile, mqr_batched_plan_t *plan) {
+    mqr::Range rg_("mqr_batched_plan_create");
     const int a = check_batched(h, dtype, batch, m, n, lda, A, tau, tile);
     if (a) return a;
     if (!plan) return -10;
@@ -654,6 +669,7 @@ int mqr_batched_plan_create(mqr_handle_t h, mqr_dtype_t dtype, int64_t batch, co
 }
 
 int mqr_batched_plan_run(mqr_handle_t h, mqr_batched_plan_t plan) {
+    mqr::Range rg_("mqr_batched_plan_run");
     if (!plan) return -2;
     MQR_GUARD(h);
     if (batched_plan_device(plan) != h->device) return -2;
@@ -673,6 +689,7 @@ int mqr_qr_backward(mqr_handle_t h, mqr_dtype_t dtype, int64_t batch, int64_t m,
                     int64_t strideR, const void *dQ, int64_t lddq, int64_t stridedQ,
                     const void *dR, int64_t lddr, int64_t stridedR, void *dA, int64_t ldda,
                     int64_t stridedA) {
+    mqr::Range rg_("mqr_qr_backward");
     if (!h) return -1;
     if (dtype != MQR_F64 && dtype != MQR_F32) return -2;
     if (batch < 0 || batch > 65535) return -3;
@@ -701,6 +718,7 @@ int mqr_qr_backward(mqr_handle_t h, mqr_dtype_t dtype, int64_t batch, int64_t m,
 
 int mqr_qr_householder_host(mqr_handle_t h, int64_t m, int64_t n, const double *A, double *Q,
                             double *R) {
+    mqr::Range rg_("mqr_qr_householder_host");
     if (!h) return -1;
     if (m < 0) return -2;
     if (n < 0) return -3;
@@ -745,6 +763,7 @@ int mqr_qr_householder_host(mqr_handle_t h, int64_t m, int64_t n, const double *
 
 int mqr_dgeqr_givens(mqr_handle_t h, int64_t m, int64_t n, const double *A, int64_t lda,
                      double *Q, int64_t ldq, double *R, int64_t ldr) {
+    mqr::Range rg_("mqr_dgeqr_givens");
     if (!h) return -1;
     if (m < 0 || m > INT32_MAX) return -2;
     if (n < 0 || n > INT32_MAX) return -3;
@@ -763,6 +782,7 @@ int mqr_dgeqr_givens(mqr_handle_t h, int64_t m, int64_t n, const double *A, int6
 
 int mqr_dgeqr_gramschmidt(mqr_handle_t h, int64_t m, int64_t n, const double *A, int64_t lda,
                           double *Q, int64_t ldq, double *R, int64_t ldr) {
+    mqr::Range rg_("mqr_dgeqr_gramschmidt");
     if (!h) return -1;
     if (m < 0 || m > INT32_MAX) return -2;
     if (n < 0 || n > m) return -3;
@@ -781,6 +801,7 @@ int mqr_dgeqr_gramschmidt(mqr_handle_t h, int64_t m, int64_t n, const double *A,
 
 int mqr_qr_host(mqr_handle_t h, mqr_qr_type_t type, int64_t m, int64_t n, const double *A,
                 double *Q, double *R) {
+    mqr::Range rg_("mqr_qr_host");
     if (!h) return -1;
     if (type == MQR_QR_HOUSEHOLDER) {
         const int st = mqr_qr_householder_host(h, m, n, A, Q, R);
@@ -840,6 +861,7 @@ static int lu_scratch(mqr_handle_t h, int64_t k, int32_t **piv, int32_t **info) 
 
 int mqr_dgetrf(mqr_handle_t h, int64_t m, int64_t n, double *A, int64_t lda, int32_t *ipiv,
                int32_t *info) {
+    mqr::Range rg_("mqr_dgetrf");
     if (!h) return -1;
     if (m < 0 || m > INT32_MAX) return -2;
     if (n < 0 || n > INT32_MAX) return -3;
@@ -855,6 +877,7 @@ int mqr_dgetrf(mqr_handle_t h, int64_t m, int64_t n, double *A, int64_t lda, int
 
 int mqr_dgetrs(mqr_handle_t h, int64_t n, int64_t nrhs, const double *LU, int64_t ldlu,
                const int32_t *ipiv, double *B, int64_t ldb) {
+    mqr::Range rg_("mqr_dgetrs");
     if (!h) return -1;
     if (n < 0 || n > INT32_MAX) return -2;
     if (nrhs < 0 || nrhs > INT32_MAX) return -3;
@@ -870,6 +893,7 @@ int mqr_dgetrs(mqr_handle_t h, int64_t n, int64_t nrhs, const double *LU, int64_
 
 int mqr_dgesv(mqr_handle_t h, int64_t n, int64_t nrhs, double *A, int64_t lda, int32_t *ipiv,
               double *B, int64_t ldb, int32_t *info) {
+    mqr::Range rg_("mqr_dgesv");
     if (!h) return -1;
     if (n < 0 || n > INT32_MAX) return -2;
     if (nrhs < 0 || nrhs > INT32_MAX) return -3;
@@ -890,6 +914,7 @@ int mqr_dgesv(mqr_handle_t h, int64_t n, int64_t nrhs, double *A, int64_t lda, i
 
 int mqr_lu_host(mqr_handle_t h, int pivot, int64_t m, int64_t n, const double *A, double *F,
                 int32_t *pos, int32_t *info) {
+    mqr::Range rg_("mqr_lu_host");
     if (!h) return -1;
     if (pivot != 0 && pivot != 1) return -2;
     if (m < 0 || m > INT32_MAX) return -3;
@@ -947,6 +972,7 @@ int mqr_lu_host(mqr_handle_t h, int pivot, int64_t m, int64_t n, const double *A
 
 int mqr_solve_direct_host(mqr_handle_t h, int64_t n, int64_t nrhs, const double *A,
                           const double *B, double *X) {
+    mqr::Range rg_("mqr_solve_direct_host");
     if (!h) return -1;
     if (n < 0 || n > INT32_MAX) return -2;
     if (nrhs < 0 || nrhs > INT32_MAX) return -3;
@@ -989,6 +1015,7 @@ int mqr_solve_direct_host(mqr_handle_t h, int64_t n, int64_t nrhs, const double 
 
 int mqr_gels_host(mqr_handle_t h, mqr_precision_t prec, int64_t m, int64_t n, const double *A,
                   const double *b, double *x, mqr_ir_info_t *info) {
+    mqr::Range rg_("mqr_gels_host");
     if (!h) return -1;
     if (prec < MQR_PREC_FP64 || prec > MQR_PREC_BF16) return -2;
     if (m < 0) return -3;

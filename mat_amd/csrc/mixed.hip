@@ -324,6 +324,7 @@ struct MixedWs {
     double *X = nullptr, *T = nullptr, *tau = nullptr;
     double *r = nullptr, *z = nullptr, *y = nullptr, *part = nullptr, *sc = nullptr;
     double *t = nullptr, *q = nullptr, *p = nullptr, *xbest = nullptr;  // CGLS
+    double *xalt = nullptr;  // the refinement's next iterate (two buffers)
     double *xx = nullptr, *zz = nullptr, *tmp = nullptr;
     void *pws = nullptr;
     size_t bytes = 0;
@@ -398,6 +399,7 @@ MixedWs carve(char *base, bool fp64, int64_t m, int64_t n, int nb, int nbt) {
     w.q = (double *)take(sizeof(double) * m);
     w.p = (double *)take(sizeof(double) * n);
     w.xbest = (double *)take(sizeof(double) * n);
+    w.xalt = (double *)take(sizeof(double) * n);
     if (!fp64) w.pws = take(panel_cholqr_ws_bytes(m));
     w.bytes = off;
     return w;
@@ -739,14 +741,6 @@ int upper_inverse_levels(mqr_handle_t hd, hipStream_t s, int64_t n, const double
 
 namespace {
 
-int read_scalars(mqr_handle_t hd, const double *dev, int cnt, double *host) {
-    MQR_HIP(hipMemcpyAsync(hd->host_pinned, dev, sizeof(double) * cnt, hipMemcpyDeviceToHost,
-                           hd->stream));
-    MQR_HIP(hipStreamSynchronize(hd->stream));
-    for (int i = 0; i < cnt; ++i) host[i] = hd->host_pinned[i];
-    return MQR_SUCCESS;
-}
-
 // sc[slot] = sum of squares of v[0:len]
 int sumsq_vec(int64_t len, const double *v, double *partial, double *sc, int slot,
               hipStream_t s) {
@@ -808,21 +802,43 @@ int correction(mqr_handle_t hd, int64_t n, double *x, MixedWs &w) {
 // (A^T A + phi^2 I)^{-1/2} <= (A^T A)^{-1/2} = R^{-1} R^{-T} in the Loewner
 // order, omega bounds the Karlson-Walden estimate of the normwise least-squares
 // backward error from above (up to the accuracy of R).
-int estimate(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t lda, const double *b,
-             const double *x, double nA, MixedWs &w, double *omega, double *rnorm,
-             double *ynorm) {
+// Enqueues the estimate's kernels and the copy of sc[0:6] (||A||_F^2 and the
+// three sums) to the handle's pinned buffer, then records ev_host; the host
+// may enqueue more work before estimate_finish waits for that event only.
+int estimate_enqueue(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t lda,
+                     const double *b, const double *x, MixedWs &w) {
     MQR_CHECK(residual_pass(hd, m, n, A, lda, b, x, w));
     MQR_CHECK(precondition(hd, n, w));
-    double hs[3];
-    MQR_CHECK(read_scalars(hd, w.sc + 3, 3, hs));
-    *rnorm = std::sqrt(hs[0]);
-    const double nx = std::sqrt(hs[1]), ny = std::sqrt(hs[2]);
+    MQR_HIP(hipMemcpyAsync(hd->host_pinned, w.sc, sizeof(double) * 6, hipMemcpyDeviceToHost,
+                           hd->stream));
+    MQR_HIP(hipEventRecord(hd->ev_host, hd->stream));
+    return MQR_SUCCESS;
+}
+
+// Waits for the last estimate_enqueue and forms omega, ||r||, ||y||.  With
+// nA_read, ||A||_F comes from sc[0] of this workspace (and is returned there)
+// instead of from nA.
+int estimate_finish(mqr_handle_t hd, double nA, double *omega, double *rnorm, double *ynorm,
+                    double *nA_read = nullptr) {
+    MQR_HIP(hipEventSynchronize(hd->ev_host));
+    const double *hs = hd->host_pinned;
+    if (nA_read) *nA_read = std::sqrt(hs[0]);
+    const double na = nA_read ? *nA_read : nA;
+    *rnorm = std::sqrt(hs[3]);
+    const double nx = std::sqrt(hs[4]), ny = std::sqrt(hs[5]);
     *ynorm = ny;
     if (ny == 0.0)
         *omega = 0.0;
     else
-        *omega = (nx > 0.0 && nA > 0.0) ? ny / (nA * nx) : INFINITY;
+        *omega = (nx > 0.0 && na > 0.0) ? ny / (na * nx) : INFINITY;
     return MQR_SUCCESS;
+}
+
+int estimate(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t lda, const double *b,
+             const double *x, double nA, MixedWs &w, double *omega, double *rnorm,
+             double *ynorm) {
+    MQR_CHECK(estimate_enqueue(hd, m, n, A, lda, b, x, w));
+    return estimate_finish(hd, nA, omega, rnorm, ynorm);
 }
 
 // Preconditioned CGLS: conjugate gradients on min ||A X u - r||, X = R^{-1},
@@ -949,14 +965,20 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
         MQR_HIP(hipMemcpy2DAsync(w.F, sizeof(double) * m, A, sizeof(double) * lda,
                                  sizeof(double) * m, n, hipMemcpyDeviceToDevice, s));
     }
-    if (fp64) {
-        // b rides along as Q^T b
-        MQR_HIP(hipMemcpyAsync(w.r, b, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
-        MQR_CHECK(geqrf_rhs_f64(hd, m, n, w.F, m, w.tau, w.T, nbt, w.r, m, 1));
-    } else
-        MQR_CHECK(factor_fp32(hd, m, n, A, lda, b, w,
-                              prec == MQR_PREC_BF16X3 ? 3 : prec == MQR_PREC_BF16 ? 1 : 0));
-    MQR_CHECK(tri_inverse(hd, n, w.R, w.ldr, w.X, w.tmp));
+    {
+        Range rg("gels_mixed: factorisation");
+        if (fp64) {
+            // b rides along as Q^T b
+            MQR_HIP(hipMemcpyAsync(w.r, b, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
+            MQR_CHECK(geqrf_rhs_f64(hd, m, n, w.F, m, w.tau, w.T, nbt, w.r, m, 1));
+        } else
+            MQR_CHECK(factor_fp32(hd, m, n, A, lda, b, w,
+                                  prec == MQR_PREC_BF16X3 ? 3 : prec == MQR_PREC_BF16 ? 1 : 0));
+    }
+    {
+        Range rg("gels_mixed: R inverse");
+        MQR_CHECK(tri_inverse(hd, n, w.R, w.ldr, w.X, w.tmp));
+    }
 
     // starting point: the QR solution x0 = R^{-1} (Q^T b)[0:n] of the
     // factorisation (the FP32 path carried b through its updates as column n)
@@ -971,9 +993,7 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
         MQR_LAUNCH_CHECK();
     }
     MQR_CHECK(correction(hd, n, x, w));
-    double hA;
-    MQR_CHECK(read_scalars(hd, w.sc, 1, &hA));
-    const double nA = std::sqrt(hA);
+    double nA = 0.0;  // ||A||_F, read with the first estimate
 
     // MQR_IR_TRACE=1 prints the stopping-test quantities of every step
     static const bool trace = [] {
@@ -986,8 +1006,25 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
     double omega = INFINITY, best = INFINITY, rnorm = 0.0, ynorm = 0.0, best_omega = INFINITY;
     int it = 0, stall = 0, kit = 0;
     bool converged = false;
+    Range rg_refine("gels_mixed: refinement");
+    // Each step enqueues the estimate of the current iterate xc and, before
+    // the host waits for the estimate's scalars, the next iterate
+    // xn = xc + X y (into the other buffer), so that the GPU keeps working
+    // through the host's stopping test.  When the test stops at xc, xn is
+    // simply not used.  The kernels that form xn run after the estimate's in
+    // stream order; xn's squares overwrite xx only after the estimate has
+    // summed xc's.
+    double *xc = x, *xn = w.xalt;
     for (;;) {
-        MQR_CHECK(estimate(hd, m, n, A, lda, b, x, nA, w, &omega, &rnorm, &ynorm));
+        MQR_CHECK(estimate_enqueue(hd, m, n, A, lda, b, xc, w));
+        const bool ahead = it < max_iter;
+        if (ahead) {
+            hipLaunchKernelGGL(k_copy_d, dim3(grid_for(n)), dim3(256), 0, s, n,
+                               (const double *)xc, xn);
+            MQR_LAUNCH_CHECK();
+            MQR_CHECK(correction(hd, n, xn, w));
+        }
+        MQR_CHECK(estimate_finish(hd, nA, &omega, &rnorm, &ynorm, it == 0 ? &nA : nullptr));
         if (trace)
             std::fprintf(stderr, "[mqr ir] step %d omega %.3e |y| %.3e |r| %.9e\n", it, omega,
                          ynorm, rnorm);
@@ -998,8 +1035,8 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
         // the best iterate so far is where the Krylov stage starts
         if (omega < best_omega) {
             best_omega = omega;
-            hipLaunchKernelGGL(k_copy_d, dim3(grid_for(n)), dim3(256), 0, s, n, (const double *)x,
-                               w.xbest);
+            hipLaunchKernelGGL(k_copy_d, dim3(grid_for(n)), dim3(256), 0, s, n,
+                               (const double *)xc, w.xbest);
             MQR_LAUNCH_CHECK();
         }
         // x = 0 at the first test: omega is infinite unless A^T b = 0
@@ -1010,9 +1047,15 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
         } else if (++stall >= 3 || ynorm > 1e3 * best) {
             break;
         }
-        if (it >= max_iter) break;
-        MQR_CHECK(correction(hd, n, x, w));
+        if (!ahead) break;
+        std::swap(xc, xn);
         ++it;
+    }
+    // the result lives in the caller's x
+    if (xc != x) {
+        hipLaunchKernelGGL(k_copy_d, dim3(grid_for(n)), dim3(256), 0, s, n, (const double *)xc,
+                           x);
+        MQR_LAUNCH_CHECK();
     }
     // the stationary iteration stalled (u_prec cond(A) too close to 1 or
     // above): conjugate gradients preconditioned by the same R
@@ -1021,6 +1064,7 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
                            (const double *)w.xbest, x);
         hipLaunchKernelGGL(k_sq, dim3(grid_for(n)), dim3(256), 0, s, n, (const double *)x, w.xx);
         MQR_LAUNCH_CHECK();
+        Range rg("gels_mixed: preconditioned CGLS");
         MQR_CHECK(cgls(hd, m, n, A, lda, b, x, nA, tol, krylov, w, trace, &kit, &converged,
                        &omega, &rnorm, &ynorm));
         it += kit;
@@ -1028,6 +1072,7 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
     bool fell_back = false;
     if (!converged && fallback) {
         // plain FP64 solve: factor a copy of A, apply Q^T to b, back substitute
+        Range rg("gels_mixed: FP64 fallback");
         fell_back = true;
         MixedWs f = carve(nullptr, true, m, n, nb, nbt);
         MQR_CHECK(ws_reserve(hd, hd->ws_ir, f.bytes + 4096));

@@ -1,6 +1,7 @@
 // Internal declarations shared by the HIP translation units of libmatqr.
 #pragma once
 
+#include <rocprofiler-sdk-roctx/roctx.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -82,6 +83,15 @@ struct ProfScope {
     ~ProfScope();
 };
 
+// A named roctx range over the host-side span of an entry point or phase
+// (`rocprofv3 --marker-trace` shows it beside the kernels it enqueued).
+struct Range {
+    explicit Range(const char *name) { roctxRangePushA(name); }
+    ~Range() { roctxRangePop(); }
+    Range(const Range &) = delete;
+    Range &operator=(const Range &) = delete;
+};
+
 }  // namespace mqr
 
 struct mqr_handle_st {
@@ -96,6 +106,9 @@ struct mqr_handle_st {
     // `stream` with ev_fork / ev_join; every call joins before returning
     hipStream_t side_stream = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // the refinement's host reads wait on ev_host (recorded after the copy of
+    // the scalars) while later work is already queued behind it
+    hipEvent_t ev_host = nullptr;
     // two-level blocking: an outer block's merged update of the far columns
     // runs on far_stream (also default priority)
     hipStream_t far_stream = nullptr;
