@@ -38,6 +38,7 @@ namespace mqr {
 namespace {
 
 constexpr int PB = kMaxNb;  // padded panel width (small::N)
+constexpr int kFbCounters = 128;  // barrier counters of the breakdown fallback
 
 // Second pass and reconstruction.  Inputs: reduced Gram G2 of Z = X R1^{-1}
 // (b x b, ld b), the top b rows of Z (ld ldz), R1 (padded, ld PB).  Outputs:
@@ -69,7 +70,9 @@ __global__ __launch_bounds__(small::NT) void k_recon(int64_t h, int b,
     // the fallback's LDS, carved from the reconstruction's buffers
     const hh::PanelLds fb{reinterpret_cast<double(*)[hh::CH]>(&S2[0][0]), &S2[8][0], S0,
                           reinterpret_cast<small::RowBuf *>(&S1[0][0]), &S3[0][0]};
-    if (*flag) {
+    const int f0 = *flag;
+    if (f0 == 2) return;  // factored by k_apply_gram's workgroups
+    if (f0) {
         hh::panel_hh_body(h, b, A, lda, tau, T, ldt, fb);
         return;
     }
@@ -265,9 +268,12 @@ __global__ __launch_bounds__(small::NT) void k_upd_gram(int64_t hc, int kin, int
                                                         const double *__restrict__ W, int64_t ldw,
                                                         TC *__restrict__ C, int64_t ldc,
                                                         int64_t g0, double *__restrict__ Gp,
-                                                        double *__restrict__ Xo, int64_t ldxo) {
+                                                        double *__restrict__ Xo, int64_t ldxo,
+                                                        int *__restrict__ bar) {
     __shared__ small::Lds Ys[RC], Ws[PB];
     const int q = small::wave_id();
+    // the next k_apply_gram's fallback barriers start from zero
+    if (blockIdx.x == 0 && threadIdx.x < kFbCounters) bar[threadIdx.x] = 0;
     sb::v4d g[3] = {sb::zero4(), sb::zero4(), sb::zero4()};
     if (kin > 0) {
         for (int e = threadIdx.x; e < PB * PB; e += small::NT) {
@@ -329,6 +335,227 @@ __global__ __launch_bounds__(small::NT) void k_upd_gram(int64_t hc, int kin, int
     gram_store(g, Gp + (size_t)blockIdx.x * PB * PB);
 }
 
+// ---- breakdown fallback on all of k_apply_gram's workgroups --------------
+//
+// When the first Cholesky fails, every workgroup of k_apply_gram knows it
+// (they all factored the same G1), and together they factor the panel by
+// Householder reflections column by column, as the reference does
+// (QR/__Householder_Blocked.h:62-84), instead of leaving it to one
+// workgroup.  Each workgroup keeps its row chunks (at most two of 64 rows)
+// in LDS; per column it publishes its partial sums (||x||^2 below the
+// diagonal and x^T a_c for the later columns) and, from the chunk holding
+// the diagonal row, the top row, then passes a grid barrier, sums the
+// partials of all workgroups in a fixed order (so every workgroup derives
+// the same tau and coefficients), and updates its rows.  T then comes from
+// G = V^T V: T^{-1} = strict_upper(G) + diag(1 / tau).
+//
+// The barriers are bounded: a workgroup that waits longer than 20 ms (its
+// peers could not all become resident beside the other streams' work)
+// cancels the barrier, and every workgroup leaves.  The panel X is only
+// written after the last barrier, so a cancelled attempt leaves it as it
+// was, with the flag at 1 for the single-workgroup fallback in k_recon;
+// a completed one sets the flag to 2.
+constexpr int kBarCancelled = INT_MIN / 2;
+constexpr uint64_t kBarTimeoutTicks = 2000000;  // 20 ms of the 100 MHz real-time counter
+constexpr int kFbMaxChunks = 2;                  // row chunks per workgroup held in LDS
+
+// Grid barrier on counter *cnt (zero before first use) for nwg workgroups;
+// false when it was cancelled.  Every thread of the workgroup calls it.
+__device__ bool grid_sync(int *cnt, int nwg, int *ok) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        int v = atomicAdd(cnt, 1) + 1;
+        bool res = v > 0;
+        if (res && v < nwg) {
+            const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+            for (;;) {
+                v = __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (v >= nwg) break;
+                if (v < 0) {
+                    res = false;
+                    break;
+                }
+                if (__builtin_amdgcn_s_memrealtime() - t0 > kBarTimeoutTicks) {
+                    // cancel unless the last workgroup arrived meanwhile
+                    if (atomicCAS(cnt, v, kBarCancelled) == v) {
+                        res = false;
+                        break;
+                    }
+                    continue;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __threadfence();
+        *ok = res ? 1 : 0;
+    }
+    __syncthreads();
+    return *ok != 0;
+}
+
+// g += V^T V over one LDS chunk of rows r0.. (h rows in all): V(i, c) =
+// S[i - r0][c] below the diagonal, 1 on it, 0 above and past row h.
+__device__ __forceinline__ void gram_acc_v(const small::Lds *S, int64_t r0, int64_t h, int b,
+                                           sb::v4d (&g)[3]) {
+    const int q = small::wave_id(), l = small::lane_id(), i = l & 15, kq = l >> 4;
+    int ti[3], tj[3];
+#pragma unroll
+    for (int u = 0; u < 3; ++u) upper_tile(q + 4 * u < 10 ? q + 4 * u : 9, ti[u], tj[u]);
+    const bool third = q + 8 < 10;
+    auto v = [&](int kk, int c) -> double {
+        const int64_t gr = r0 + kk;
+        if (gr >= h || c >= b) return 0.0;
+        return gr > c ? S[kk][c] : (gr == c ? 1.0 : 0.0);
+    };
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int kk = 4 * k + kq;
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            if (u == 2 && !third) continue;
+            const double av = v(kk, 16 * ti[u] + i), bv = v(kk, 16 * tj[u] + i);
+            g[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, av, g[u], 0, 0, 0);
+        }
+    }
+}
+
+struct FbLds {
+    double dsum[PB], tops[PB], cfs[PB], tauv[PB];
+    double quarter[4][PB];
+    int ok;
+};
+
+// The grid fallback (see above).  S[0..nch) hold this workgroup's chunks;
+// Gp (partials), G (64 x 64, ld PB) and bar come from the panel workspace.
+__device__ void fallback_grid(int64_t h, int b, double *__restrict__ X, int64_t ldx,
+                              double *__restrict__ Z, double *__restrict__ Gp,
+                              double *__restrict__ G,
+                              double *__restrict__ tau, double *__restrict__ T, int64_t ldt,
+                              int *__restrict__ flag, int *__restrict__ bar, small::Lds *S0,
+                              small::Lds *S1, FbLds &L) {
+    const int tid = threadIdx.x, lr = tid & 63, cg = tid >> 6, lane = small::lane_id();
+    const int nwg = gridDim.x, wg = blockIdx.x;
+    small::Lds *S[kFbMaxChunks] = {S0, S1};
+    int64_t r0[kFbMaxChunks];
+    int nch = 0;
+    for (int k = 0; k < kFbMaxChunks; ++k) {
+        r0[k] = ((int64_t)wg + (int64_t)k * nwg) * RC;
+        if (r0[k] < h) nch = k + 1;
+    }
+    __syncthreads();  // the caller's last reads of the LDS buffers are done
+    for (int k = 0; k < nch; ++k) chunk_load(S[k], X, ldx, r0[k], h, b);
+    // per-column partials and top rows in Z (unused by the fallback; it
+    // holds h x PB >= 2 nwg PB + 2 PB doubles), the Gram partials in Gp
+    double *part = Z;                          // [2][nwg][PB]
+    double *top = Z + (size_t)2 * nwg * PB;   // [2][PB]
+    int nbar = 0;
+    // partial sums of column r (rows below r) into slot r & 1, and the top
+    // row from the chunk holding row r
+    auto publish = [&](int r) {
+        __syncthreads();
+        for (int ch = 0; ch < nch; ++ch)
+            if (r0[ch] + lr == r)
+                for (int c = r + cg; c < b; c += 4) top[(r & 1) * PB + c] = S[ch][lr][c];
+        // column c = cg + 4k of wave cg: sum over the rows below r
+        for (int c = cg + 4 * ((r - cg + 3) >> 2 > 0 ? (r - cg + 3) >> 2 : 0); c < b; c += 4) {
+            double a = 0.0;
+            for (int ch = 0; ch < nch; ++ch) {
+                const int64_t i = r0[ch] + lr;
+                if (i > r && i < h) a = fma(S[ch][lr][r], S[ch][lr][c], a);
+            }
+            const double sum = small::wave_sum(a);  // c is uniform in the wave
+            if (lane == 0) part[((size_t)(r & 1) * nwg + wg) * PB + c] = sum;
+        }
+    };
+    publish(0);
+    for (int r = 0; r < b; ++r) {
+        if (!grid_sync(bar + nbar++, nwg, &L.ok)) return;
+        // column sums in a fixed order (each wave a quarter of the
+        // workgroups, then the four quarters), the top row
+        {
+            const int c = lr, w0q = (cg * nwg) >> 2, w1q = ((cg + 1) * nwg) >> 2;
+            double sum = 0.0;
+            if (c >= r && c < b)
+                for (int w = w0q; w < w1q; ++w) sum += part[((size_t)(r & 1) * nwg + w) * PB + c];
+            L.quarter[cg][c] = sum;
+        }
+        __syncthreads();
+        if (tid < PB) {
+            const int c = tid;
+            L.dsum[c] = (L.quarter[0][c] + L.quarter[1][c]) + (L.quarter[2][c] + L.quarter[3][c]);
+            L.tops[c] = (c >= r && c < b) ? top[(r & 1) * PB + c] : 0.0;
+        }
+        __syncthreads();
+        const double alpha = L.tops[r], nrm = sqrt(alpha * alpha + L.dsum[r]);
+        const bool zero = nrm == 0.0;
+        const double cr = alpha < 0.0 ? -nrm : nrm;
+        const double w0 = alpha + cr;
+        const double hr = zero ? 0.0 : 1.0 / w0, t = zero ? 0.0 : w0 / cr;
+        if (tid < PB) {
+            const int c = tid;
+            L.cfs[c] = (c > r && c < b) ? -t * (L.tops[c] + hr * L.dsum[c]) : 0.0;
+            if (c == r) L.tauv[r] = t;
+        }
+        __syncthreads();
+        if (!zero) {
+            // v_i = x_i hr below the diagonal, 1 on it; a_c += cf_c v
+            for (int ch = 0; ch < nch; ++ch) {
+                const int64_t i = r0[ch] + lr;
+                if (i < r || i >= h) continue;
+                const double x = S[ch][lr][r];
+                const double v = i == r ? 1.0 : x * hr;
+                for (int c = r + 1 + ((cg - (r + 1)) & 3); c < b; c += 4)
+                    S[ch][lr][c] = fma(L.cfs[c], v, S[ch][lr][c]);
+            }
+            __syncthreads();  // column r read by every wave above
+            if (cg == (r & 3))
+                for (int ch = 0; ch < nch; ++ch) {
+                    const int64_t i = r0[ch] + lr;
+                    if (i == r) S[ch][lr][r] = -cr;
+                    else if (i > r && i < h) S[ch][lr][r] *= hr;
+                }
+        }
+        if (r + 1 < b) publish(r + 1);
+    }
+    // G = V^T V: partials, then the 4096 sums split over the workgroups
+    __syncthreads();
+    sb::v4d g[3] = {sb::zero4(), sb::zero4(), sb::zero4()};
+    for (int ch = 0; ch < nch; ++ch) gram_acc_v(S[ch], r0[ch], h, b, g);
+    gram_store(g, Gp + (size_t)wg * PB * PB);
+    if (!grid_sync(bar + nbar++, nwg, &L.ok)) return;
+    for (int e = wg * small::NT + tid; e < PB * PB; e += nwg * small::NT) {
+        double sum = 0.0;
+        for (int w = 0; w < nwg; ++w) sum += Gp[(size_t)w * PB * PB + e];
+        G[e] = sum;
+    }
+    if (!grid_sync(bar + nbar++, nwg, &L.ok)) return;
+    // no barrier can be cancelled any more: write the factored rows back
+    for (int ch = 0; ch < nch; ++ch)
+        for (int e = tid; e < RC * PB; e += small::NT) {
+            const int rr = e & (RC - 1), c = e >> 6;
+            if (r0[ch] + rr < h && c < b) X[r0[ch] + rr + (int64_t)c * ldx] = S[ch][rr][c];
+        }
+    if (wg != 0) return;
+    // T (b x b) by T(i, c) = tau_i (delta_ic - sum_{i < l <= c} G(i, l) T(l, c)),
+    // one column per thread, in LDS (S0 is free: its rows are written back)
+    __syncthreads();
+    for (int e = tid; e < PB * PB; e += small::NT) S0[e & (PB - 1)][e >> 6] = G[e];
+    __syncthreads();
+    small::Lds *Tl = S1;  // S1 is free as well
+    if (tid < b) {
+        const int c = tid;
+        for (int i = c; i >= 0; --i) {
+            double sum = 0.0;
+            for (int l = i + 1; l <= c; ++l) sum = fma(S0[i][l], Tl[l][c], sum);
+            Tl[i][c] = L.tauv[i] * ((i == c ? 1.0 : 0.0) - sum);
+        }
+        for (int i = 0; i < b; ++i) T[i + (int64_t)c * ldt] = i <= c ? Tl[i][c] : 0.0;
+        tau[c] = L.tauv[c];
+    }
+    if (tid == 0) *flag = 2;
+}
+
 // First-pass Cholesky and second-pass Gram.  Every workgroup factors the
 // reduced Gram matrix G1 (b x b, ld PB) = R1^T R1 and inverts R1 itself:
 // the same operations on the same data, so all workgroups hold the same R1
@@ -342,15 +569,19 @@ __global__ __launch_bounds__(small::NT) void k_upd_gram(int64_t hc, int kin, int
 // value raises the flag; every workgroup then stops, and the rest of the
 // sequence leaves the panel to the column-by-column fallback.
 __global__ __launch_bounds__(small::NT) void k_apply_gram(int64_t h, int b,
-                                                          const double *__restrict__ G1,
-                                                          const double *__restrict__ X,
+                                                          double *__restrict__ G1,
+                                                          double *__restrict__ X,
                                                           int64_t ldx, double *__restrict__ R1,
                                                           double *__restrict__ Z, int64_t ldz,
                                                           double *__restrict__ Gp,
-                                                          int *__restrict__ flag) {
+                                                          int *__restrict__ flag,
+                                                          double *__restrict__ tau,
+                                                          double *__restrict__ T, int64_t ldt,
+                                                          int *__restrict__ bar) {
     __shared__ small::Lds Xs[RC], Ms[PB];
     __shared__ double thr[PB];
     __shared__ int fail;
+    __shared__ FbLds fbl;
     const int tid = threadIdx.x, q = small::wave_id();
     if (tid == 0) fail = 0;
     small::lds_load(Xs, G1, PB, b);
@@ -363,7 +594,12 @@ __global__ __launch_bounds__(small::NT) void k_apply_gram(int64_t h, int b,
         small::lds_store(Xs, R1, PB, PB, PB);
         if (tid == 0) *flag = fail;
     }
-    if (fail) return;  // uniform: written before chol_inv's last barrier
+    if (fail) {  // uniform: written before chol_inv's last barrier
+        // every workgroup has read G1 and (workgroup 0) stored R1 above
+        if ((int64_t)gridDim.x * kFbMaxChunks * RC >= h)
+            fallback_grid(h, b, X, ldx, Z, Gp, G1, tau, T, ldt, flag, bar, Xs, Ms, fbl);
+        return;
+    }
     sb::v4d g[3] = {sb::zero4(), sb::zero4(), sb::zero4()};
     for (int64_t r0 = (int64_t)blockIdx.x * RC; r0 < h; r0 += (int64_t)gridDim.x * RC) {
         __syncthreads();
@@ -518,6 +754,7 @@ __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
 struct CqrWs {
     double *Z, *R1, *G1, *Mi, *Gp;
     int *flag;
+    int *bar;  // the fallback's barrier counters
 };
 
 CqrWs carve(void *ws, int64_t h) {
@@ -528,6 +765,7 @@ CqrWs carve(void *ws, int64_t h) {
     w.Mi = w.G1 + PB * PB;
     w.Gp = w.Mi + PB * PB;
     w.flag = (int *)(w.Gp + (size_t)kMaxGramParts * PB * PB);
+    w.bar = w.flag + 64;
     return w;
 }
 
@@ -537,9 +775,11 @@ int gram_parts(int64_t rows) {
 
 }  // namespace
 
-// Workspace: Z (h x PB), R1, G1, Minv (PB x PB), the Gram partials, flag.
+// Workspace: Z (h x PB), R1, G1, Minv (PB x PB), the Gram partials, flag
+// and the fallback's barrier counters.
 size_t panel_cholqr_ws_bytes(int64_t h) {
-    return sizeof(double) * ((size_t)h * PB + 3 * PB * PB + (size_t)kMaxGramParts * PB * PB) + 256;
+    return sizeof(double) * ((size_t)h * PB + 3 * PB * PB + (size_t)kMaxGramParts * PB * PB) +
+           256 + sizeof(int) * kFbCounters;
 }
 
 // First pass: C (hc x bn) -= Y W when kin > 0, and the Gram matrix G1 of
@@ -557,10 +797,10 @@ int cholqr_gram1(hipStream_t s, int64_t hc, int bn, int kin, const double *Y, in
                      8.0 * hh * (kin + (kin > 0 ? 2.0 : 1.0) * bn), s);
         if (c_f32)
             hipLaunchKernelGGL(k_upd_gram<float>, dim3(parts), dim3(small::NT), 0, s, hc, kin, bn,
-                               Y, ldy, W, ldw, (float *)C, ldc, g0, w.Gp, Xo, ldxo);
+                               Y, ldy, W, ldw, (float *)C, ldc, g0, w.Gp, Xo, ldxo, w.bar);
         else
             hipLaunchKernelGGL(k_upd_gram<double>, dim3(parts), dim3(small::NT), 0, s, hc, kin, bn,
-                               Y, ldy, W, ldw, (double *)C, ldc, g0, w.Gp, Xo, ldxo);
+                               Y, ldy, W, ldw, (double *)C, ldc, g0, w.Gp, Xo, ldxo, w.bar);
         MQR_LAUNCH_CHECK();
     }
     // G1 goes to its own buffer: the next launch overwrites the partials
@@ -577,9 +817,8 @@ int cholqr_rest_a(hipStream_t s, int64_t h, int b, double *X, int64_t ld, double
     {
         ProfScope ps(K_PANEL, hh * bb * bb + hh * bb * bb + 2.0 * bb * bb * bb / 3.0,
                      16.0 * hh * bb + 8.0 * bb * bb * 2.0, s);
-        hipLaunchKernelGGL(k_apply_gram, dim3(parts), dim3(small::NT), 0, s, h, b,
-                           (const double *)w.G1, (const double *)X, ld, w.R1, w.Z, h, w.Gp,
-                           w.flag);
+        hipLaunchKernelGGL(k_apply_gram, dim3(parts), dim3(small::NT), 0, s, h, b, w.G1, X, ld,
+                           w.R1, w.Z, h, w.Gp, w.flag, tau, T, ldt, w.bar);
         MQR_LAUNCH_CHECK();
     }
     MQR_CHECK(reduce_partials((int64_t)PB * PB, w.Gp, parts, (int64_t)PB * PB, s));

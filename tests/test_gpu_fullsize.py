@@ -24,6 +24,7 @@ against the oracle measures 7 u ||a_j|| iid and 22 u ||a_j|| graded at
 32768 x 1024); the tests ask for 512 u ||a_j||.
 """
 import os
+import time
 from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
@@ -201,6 +202,50 @@ def test_tall_32768x1024_matches_oracle(h, orc, torch_mod, kind):
     dT = np.abs(tau[:lead] - tau_o[:lead]).max()
     note(f"tall {kind}: leading V {dV:.2e} tau {dT:.2e}")
     assert dV <= 512 * EPS and dT <= 512 * EPS
+
+
+@pytest.mark.parametrize("m", [16384, 20000])
+def test_breakdown_panels_at_full_height(h, torch_mod, m):
+    """Every 64-column panel holds duplicated and exactly zero columns, so
+    the CholeskyQR panel breaks down each time and the panel is factored
+    column by column (QR/__Householder_Blocked.h:62-84) by all workgroups of
+    the row-chunk kernel; at 20000 rows each of them holds two 64-row chunks.
+    On the GPU: ||QR - A||, ||Q^T Q - I||; the reference's convention for a
+    zero column (tau = 0, the column left as it is); and a time that only
+    the many-workgroup factorisation reaches (one workgroup per panel took
+    about 46 ms per 16384-row panel, 0.74 s for these 16 panels)."""
+    torch = torch_mod
+    n = 1024
+    note(f"breakdown panels {m}x{n}")
+    g = torch.Generator(device="cuda").manual_seed(m)
+    A = torch.randn(m, n, generator=g, device="cuda", dtype=torch.float64)
+    A[:, 1::8] = A[:, 0::8]   # duplicated columns in every panel
+    A[:, 5::64] = 0.0         # one zero column per panel
+    A0 = M.colmajor(A)
+    del A
+    dA = M.empty_colmajor(m, n)
+    best = float("inf")
+    for _ in range(2):
+        dA.copy_(A0)
+        h.synchronize()
+        t0 = time.perf_counter()
+        M.geqrf(h, dA)
+        h.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    del dA
+    note(f"breakdown panels {m}x{n}: factorisation {best * 1e3:.1f} ms")
+    assert best < 0.3
+    VR, tau, res, orth = factor_and_check(torch, h, A0)
+    note(f"breakdown panels: |QR-A|/|A| {res:.2e} |QtQ-I| {orth:.2e}")
+    assert res <= 8 * m * EPS and orth <= 8 * m * EPS
+    zc = np.arange(5, n, 64)
+    assert np.all(tau[zc] == 0.0)
+    for j in zc:
+        assert np.all(VR[j:, j] == 0.0)
+    # a duplicated column leaves a pivot at rounding level
+    d = np.abs(np.diag(VR[:n]))
+    colnorm = np.linalg.norm(host(A0), axis=0)
+    assert np.all(d[1::8] <= 1e-12 * colnorm[1::8])
 
 
 def test_square_8192_invariants_and_lapack_r(h, torch_mod):
