@@ -470,6 +470,48 @@ def batched_rates(M, h, count: int = 10000, seed: int = 4321) -> list:
     return out
 
 
+def breakdown_rates(M, h, m: int = 16384, n: int = 4096, steps: int = 3, seed: int = 5) -> list:
+    """mqr_dgeqrf of m x n inputs whose panels defeat CholeskyQR: iid
+    N(0,1) as the reference point; every odd column a copy of the even one
+    before it (each panel breaks down and is factored column by column by
+    the row-chunk kernel's workgroups); columns scaled by logspace(0, -14)
+    (ill-conditioned, yet each panel stays well conditioned after the
+    earlier ones are removed).  Events on the handle's stream; the best of
+    `steps` runs after one warm-up, with ||QR - A|| / ||A|| of the last."""
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    base = torch.randn(m, n, generator=g, device="cuda", dtype=torch.float64)
+    dup = base.clone()
+    dup[:, 1::2] = dup[:, 0::2]
+    scaled = base * torch.logspace(0, -14, n, device="cuda", dtype=torch.float64)[None, :]
+    out = []
+    for name, A in (("iid", base), ("duplicated_columns", dup), ("columns_scaled_1e-14", scaled)):
+        A0 = M.colmajor(A)
+        Aw = M.empty_colmajor(m, n)
+        best = float("inf")
+        for rep in range(steps + 1):
+            Aw.copy_(A0)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            tau, T = M.geqrf(h, Aw)
+            e1.record()
+            torch.cuda.synchronize()
+            if rep > 0:
+                best = min(best, e0.elapsed_time(e1))
+        Q = M.orgqr(h, Aw, T, n, nq=n)
+        h.synchronize()
+        res = float(torch.linalg.norm(Q @ torch.triu(Aw[:n, :]) - A0) / torch.linalg.norm(A0))
+        rec = {"input": name, "m": m, "n": n, "factor_ms": round(best, 3),
+               "tflops": round(qr_flops(m, n) / (best * 1e-3) / 1e12, 3),
+               "rel_residual": res}
+        print(f"[bench] breakdown {name}: {best:.3f} ms |QR-A|/|A| {res:.2e}", file=sys.stderr,
+              flush=True)
+        out.append(rec)
+        del A0, Aw, Q, tau, T
+    return out
+
+
 def adjoint_rates(M, h, batch: int = 256, m: int = 1024, n: int = 256, seed: int = 99) -> list:
     """QR adjoint (mqr_qr_backward) on a batch of thin QRs of iid N(0,1)
     matrices, FP32 and FP64, loss sum(Wq o Q) + sum(Wr o R) so that dQ = Wq,
@@ -516,9 +558,14 @@ def main() -> int:
     ap.add_argument("--prec", default="fp32", choices=["fp32", "bf16x3", "bf16", "none"],
                     help="mixed-precision trailing updates ('none': plain FP64 only)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernel", choices=["gemm", "batched", "adjoint", "qr"], default=None,
-                    help="time one kernel family instead of the solve")
+    ap.add_argument("--kernel", choices=["gemm", "batched", "adjoint", "qr", "breakdown"],
+                    default=None, help="time one kernel family instead of the solve")
     args = ap.parse_args()
+    if args.kernel == "breakdown":
+        import mat_amd as M
+        h = M.Handle(0)
+        print(json.dumps({"qr_breakdown": breakdown_rates(M, h)}))
+        return 0
     if args.kernel == "adjoint":
         import mat_amd as M
         h = M.Handle(0)

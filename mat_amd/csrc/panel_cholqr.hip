@@ -435,7 +435,11 @@ __device__ void fallback_grid(int64_t h, int b, double *__restrict__ X, int64_t 
                               int *__restrict__ flag, int *__restrict__ bar, small::Lds *S0,
                               small::Lds *S1, FbLds &L) {
     const int tid = threadIdx.x, lr = tid & 63, cg = tid >> 6, lane = small::lane_id();
-    const int nwg = gridDim.x, wg = blockIdx.x;
+    // as few workgroups as hold every chunk in LDS (each one reads all
+    // workgroups' partials per column, so their number counts twice)
+    const int chunks = (int)((h + RC - 1) / RC);
+    const int nwg = (chunks + kFbMaxChunks - 1) / kFbMaxChunks, wg = blockIdx.x;
+    if (wg >= nwg) return;  // uniform; before any barrier
     small::Lds *S[kFbMaxChunks] = {S0, S1};
     int64_t r0[kFbMaxChunks];
     int nch = 0;
