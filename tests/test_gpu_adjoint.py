@@ -49,7 +49,7 @@ def to_dev(torch, x, dt):
 
 @pytest.mark.parametrize("dtype,batch,m,n", [("f64", 3, 60, 20), ("f64", 2, 300, 130),
                                              ("f64", 1, 64, 64), ("f32", 4, 200, 70),
-                                             ("f32", 2, 1024, 256)])
+                                             ("f32", 2, 1024, 256), ("f64", 2, 1024, 256)])
 def test_adjoint_matches_analytic(h, torch_mod, dtype, batch, m, n):
     torch = torch_mod
     dt = torch.float64 if dtype == "f64" else torch.float32
@@ -98,6 +98,46 @@ def test_adjoint_gradcheck_finite_differences(torch_mod):
     A = M.colmajor_batched(A).requires_grad_(True)
     assert torch.autograd.gradcheck(lambda X: M.qr_autograd(X), (A,), eps=1e-6, atol=1e-6,
                                     rtol=1e-5)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_adjoint_full_c5_batch(h, torch_mod, dtype):
+    """The whole C5 workload, 256 strided matrices of 1024 x 256, in one
+    call; sampled matrices (first, last and two inside) against the analytic
+    formula, so that a wrong batch stride or a matrix left out shows."""
+    torch = torch_mod
+    dt = torch.float64 if dtype == "f64" else torch.float32
+    batch, m, n = 256, 1024, 256
+    note(f"adjoint full C5 batch {dtype}")
+    g = torch.Generator(device="cuda").manual_seed(77)
+    A = torch.randn(batch, m, n, generator=g, device="cuda", dtype=torch.float64)
+    Q, R = torch.linalg.qr(A.cpu(), mode="reduced")
+    dQ = torch.randn(batch, m, n, generator=g, device="cuda", dtype=torch.float64).cpu()
+    dR = torch.triu(torch.randn(batch, n, n, generator=g, device="cuda", dtype=torch.float64)).cpu()
+    dev = [M.colmajor_batched(x.to(device="cuda", dtype=dt)) for x in (Q, R, dQ, dR)]
+    dA = M.qr_backward(h, *dev)
+    torch.cuda.synchronize()
+    u = np.finfo(np.float64 if dtype == "f64" else np.float32).eps
+    for i in (0, 97, 200, batch - 1):
+        got = dA[i].cpu().double().numpy()
+        Qi, Ri, dQi, dRi = (x[i].to(dt).double().numpy() for x in (Q, R, dQ, dR))
+        ref = qr_adjoint(Qi, Ri, dQi, dRi)
+        err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        kappa = np.linalg.cond(Ri)
+        note(f"  matrix {i}: rel err {err:.2e} (kappa(R) {kappa:.1f})")
+        assert err <= 50 * n * kappa * u
+
+
+def test_adjoint_gradcheck_larger_fast_mode(torch_mod):
+    """gradcheck's fast mode (random projections of the Jacobian) at
+    2 x (256 x 64), FP64: the blocked solve's 64-column diagonal blocks and
+    the off-diagonal GEMMs all take part."""
+    torch = torch_mod
+    g = torch.Generator(device="cuda").manual_seed(4)
+    A = torch.randn(2, 256, 64, device="cuda", dtype=torch.float64, generator=g)
+    A = M.colmajor_batched(A).requires_grad_(True)
+    assert torch.autograd.gradcheck(lambda X: M.qr_autograd(X), (A,), eps=1e-6, atol=1e-6,
+                                    rtol=1e-5, fast_mode=True)
 
 
 def test_adjoint_matches_torch_linalg_qr(torch_mod):
