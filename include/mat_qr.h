@@ -320,6 +320,23 @@ int mqr_batched_plan_create(mqr_handle_t handle, mqr_dtype_t dtype, int64_t batc
                             const int32_t *m, const int32_t *n, const int64_t *lda,
                             void *const *A, void *const *tau, int tile,
                             mqr_batched_plan_t *plan);
+/* No reference counterpart: the batched launch's schedule, the space the
+ * autotuner (mat_amd/autotune_batched.py) searches.  nb: panel width (0: the
+ * tuned width per size bin, else 8 or 16); per_cu: resident workgroups per
+ * CU (-1: tuned, 0: as many as the kernel's registers and LDS allow, else
+ * at most that many); order: the queue (-1: tuned, 0: decreasing flop
+ * count, 1: as given, 2: largest and smallest alternating).  The factors do
+ * not depend on the schedule; only which workgroup takes which matrix, and
+ * when, does.  A bad schedule is argument -9. */
+typedef struct {
+    int nb;
+    int per_cu;
+    int order;
+} mqr_batched_schedule_t;
+int mqr_batched_plan_create_sched(mqr_handle_t handle, mqr_dtype_t dtype, int64_t batch,
+                                  const int32_t *m, const int32_t *n, const int64_t *lda,
+                                  void *const *A, void *const *tau,
+                                  const mqr_batched_schedule_t *sched, mqr_batched_plan_t *plan);
 int mqr_batched_plan_run(mqr_handle_t handle, mqr_batched_plan_t plan);
 int mqr_batched_plan_destroy(mqr_batched_plan_t plan);
 

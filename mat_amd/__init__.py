@@ -72,6 +72,8 @@ SIGNATURES = {
     "mqr_geqrf_batched_uneven": (_i32, [_vp, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _i32]),
     "mqr_geqrf_batched_uneven_noalloc": (_i32, [_vp, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _i32]),
     "mqr_batched_plan_create": (_i32, [_vp, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _i32, _vp]),
+    "mqr_batched_plan_create_sched": (_i32, [_vp, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp,
+                                             _vp]),
     "mqr_batched_plan_run": (_i32, [_vp, _vp]),
     "mqr_batched_plan_destroy": (_i32, [_vp]),
     "mqr_qr_backward": (_i32, [_vp, _i32, _i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64,
@@ -416,7 +418,10 @@ class BatchedPlan:
     """The batched factorisation of fixed device matrices captured in a
     hipGraph; run() replays it on the handle's stream."""
 
-    def __init__(self, h: Handle, mats, tile: int = -1):
+    def __init__(self, h: Handle, mats, tile: int = -1, per_cu: Optional[int] = None,
+                 order: Optional[int] = None):
+        """tile: -1 tuned, 0 panel width 8, 1 width 16.  per_cu / order: the
+        launch's schedule (mqr_batched_schedule_t); None takes the tuned one."""
         import torch
         self.h = h
         self.mats = mats
@@ -424,11 +429,17 @@ class BatchedPlan:
                      for t in mats]
         self.plan = C.c_void_p()
         dt, b, (m, n, lda, A, T) = _batch_args(mats, self.taus, h.device)
-        check(lib().mqr_batched_plan_create(h.h, dt, b, m.ctypes.data_as(C.c_void_p),
-                                            n.ctypes.data_as(C.c_void_p),
-                                            lda.ctypes.data_as(C.c_void_p),
-                                            C.cast(A, C.c_void_p), C.cast(T, C.c_void_p), tile,
-                                            C.byref(self.plan)), "mqr_batched_plan_create")
+        args = (m.ctypes.data_as(C.c_void_p), n.ctypes.data_as(C.c_void_p),
+                lda.ctypes.data_as(C.c_void_p), C.cast(A, C.c_void_p), C.cast(T, C.c_void_p))
+        if per_cu is None and order is None:
+            check(lib().mqr_batched_plan_create(h.h, dt, b, *args, tile, C.byref(self.plan)),
+                  "mqr_batched_plan_create")
+        else:
+            sched = (C.c_int * 3)({-1: 0, 0: 8, 1: 16}[tile],
+                                  -1 if per_cu is None else per_cu, -1 if order is None else order)
+            check(lib().mqr_batched_plan_create_sched(h.h, dt, b, *args, C.cast(sched, C.c_void_p),
+                                                      C.byref(self.plan)),
+                  "mqr_batched_plan_create_sched")
 
     def run(self):
         check(lib().mqr_batched_plan_run(self.h.h, self.plan), "mqr_batched_plan_run")

@@ -497,6 +497,14 @@ int batched_nb(bool f64, int m, int tile) {
     return kTunedNb[f64 ? 1 : 0][bin];
 }
 
+// The launch's schedule: resident workgroups per CU (0: as many as the
+// kernel's resources allow) and the order of the queue (0: decreasing flop
+// count, 1: as given, 2: alternately the largest and the smallest left).
+struct BatchedSched {
+    int per_cu = -1;  // -1: the tuned value
+    int order = -1;   // -1: the tuned value
+};
+
 struct BatchedPlan {
     bool f64 = true;
     int batch = 0;
@@ -510,14 +518,27 @@ struct BatchedPlan {
 
 static int batched_setup(mqr_handle_t hd, bool f64, int64_t batch, const int32_t *m,
                          const int32_t *n, const int64_t *lda, void *const *A, void *const *tau,
-                         int tile, BatchedPlan &p) {
+                         int tile, BatchedPlan &p, BatchedSched sched = BatchedSched()) {
+    const int qorder = sched.order >= 0 ? sched.order : kTunedOrder[f64 ? 1 : 0];
+    int req_cu = sched.per_cu >= 0 ? sched.per_cu : kTunedPerCu[f64 ? 1 : 0];
     std::vector<int> order(batch);
     std::iota(order.begin(), order.end(), 0);
     auto cost = [&](int i) {
         const double mm = m[i], nn = std::min(m[i], n[i]);
         return 2.0 * mm * nn * n[i] - 2.0 * nn * nn * nn / 3.0 + mm;
     };
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+    if (qorder != 1)
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return cost(a) > cost(b); });
+    if (qorder == 2) {  // largest, smallest, next largest, next smallest, ...
+        std::vector<int> alt;
+        alt.reserve(batch);
+        for (int64_t lo = 0, hi = batch - 1; lo <= hi; ++lo, --hi) {
+            alt.push_back(order[lo]);
+            if (lo != hi) alt.push_back(order[hi]);
+        }
+        order.swap(alt);
+    }
     std::vector<BatchDesc> h(batch);
     for (int64_t e = 0; e < batch; ++e) {
         const int i = order[e];
@@ -543,6 +564,7 @@ static int batched_setup(mqr_handle_t hd, bool f64, int64_t batch, const int32_t
     int per_cu = 0;
     MQR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BNT, p.smem));
     per_cu = std::max(per_cu, 1);
+    if (req_cu > 0) per_cu = std::min(per_cu, req_cu);
     p.grid = (unsigned)std::min<int64_t>(batch, (int64_t)per_cu * hd->cu_count);
     return MQR_SUCCESS;
 }
@@ -591,11 +613,14 @@ namespace mqr {
 
 int batched_plan_create(mqr_handle_t hd, bool f64, int64_t batch, const int32_t *m,
                         const int32_t *n, const int64_t *lda, void *const *A, void *const *tau,
-                        int tile, mqr_batched_plan_st **out) {
+                        int tile, mqr_batched_plan_st **out, int per_cu, int order) {
     mqr_batched_plan_st *pl = new (std::nothrow) mqr_batched_plan_st();
     if (!pl) return MQR_ERR_ALLOC;
     pl->device = hd->device;
-    int st = batched_setup(hd, f64, batch, m, n, lda, A, tau, tile, pl->p);
+    BatchedSched sched;
+    sched.per_cu = per_cu;
+    sched.order = order;
+    int st = batched_setup(hd, f64, batch, m, n, lda, A, tau, tile, pl->p, sched);
     if (st == MQR_SUCCESS && batch > 0) {
         // capture the single persistent launch; replays need no memset node
         // because the kernel resets its own counters
@@ -693,6 +718,9 @@ int batched_noalloc_init(int device) {
                                                          BNT, 0));
     NoAllocDevice &d = g_scratch_ptr[device];
     d.ptr = (unsigned char *)p;
+    // residency as tuned for the allocating path (0: the occupancy limit)
+    if (kTunedPerCu[1] > 0) pc64 = std::min(pc64, kTunedPerCu[1]);
+    if (kTunedPerCu[0] > 0) pc32 = std::min(pc32, kTunedPerCu[0]);
     d.grid64 = (unsigned)(std::max(pc64, 1) * std::max(cus, 1));
     d.grid32 = (unsigned)(std::max(pc32, 1) * std::max(cus, 1));
     g_scratch_ready[device] = true;

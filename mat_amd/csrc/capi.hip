@@ -22,7 +22,7 @@ int geqrf_batched(mqr_handle_t hd, bool f64, int64_t batch, const int32_t *m, co
                   const int64_t *lda, void *const *A, void *const *tau, int tile);
 int batched_plan_create(mqr_handle_t hd, bool f64, int64_t batch, const int32_t *m,
                         const int32_t *n, const int64_t *lda, void *const *A, void *const *tau,
-                        int tile, mqr_batched_plan_st **out);
+                        int tile, mqr_batched_plan_st **out, int per_cu, int order);
 int batched_plan_run(mqr_handle_t hd, mqr_batched_plan_st *pl);
 void batched_plan_destroy(mqr_batched_plan_st *pl);
 int batched_plan_device(const mqr_batched_plan_st *pl);
@@ -665,7 +665,25 @@ int mqr_batched_plan_create(mqr_handle_t h, mqr_dtype_t dtype, int64_t batch, co
     if (!plan) return -10;
     *plan = nullptr;
     MQR_GUARD(h);
-    return batched_plan_create(h, dtype == MQR_F64, batch, m, n, lda, A, tau, tile, plan);
+    return batched_plan_create(h, dtype == MQR_F64, batch, m, n, lda, A, tau, tile, plan, -1, -1);
+}
+
+int mqr_batched_plan_create_sched(mqr_handle_t h, mqr_dtype_t dtype, int64_t batch,
+                                  const int32_t *m, const int32_t *n, const int64_t *lda,
+                                  void *const *A, void *const *tau,
+                                  const mqr_batched_schedule_t *sched, mqr_batched_plan_t *plan) {
+    mqr::Range rg_("mqr_batched_plan_create_sched");
+    if (!sched) return -9;
+    if (sched->nb != 0 && sched->nb != 8 && sched->nb != 16) return -9;
+    if (sched->per_cu < -1 || sched->order < -1 || sched->order > 2) return -9;
+    const int tile = sched->nb == 8 ? 0 : sched->nb == 16 ? 1 : -1;
+    const int a = check_batched(h, dtype, batch, m, n, lda, A, tau, tile);
+    if (a) return a;
+    if (!plan) return -10;
+    *plan = nullptr;
+    MQR_GUARD(h);
+    return batched_plan_create(h, dtype == MQR_F64, batch, m, n, lda, A, tau, tile, plan,
+                               sched->per_cu, sched->order);
 }
 
 int mqr_batched_plan_run(mqr_handle_t h, mqr_batched_plan_t plan) {

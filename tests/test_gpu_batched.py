@@ -151,6 +151,30 @@ def test_batched_plan_replays(h, orc, torch_mod):
         assert np.array_equal(a, r.cpu().numpy())
 
 
+@pytest.mark.parametrize("per_cu,order", [(1, 0), (1, 1), (0, 2), (2, 1)])
+def test_batched_schedules_give_the_same_factors(h, torch_mod, per_cu, order):
+    """Every schedule the autotuner can pick (residency, queue order, panel
+    width 16) factors each matrix with the same arithmetic: bit-identical to
+    the default plan, in FP64 and FP32, over more matrices than the grid has
+    workgroups."""
+    torch = torch_mod
+    note(f"batched schedule per_cu={per_cu} order={order}")
+    for dt in (torch.float64, torch.float32):
+        shapes, mats = uneven_batch(700, seed=31, mmax=300)
+        ref = [M.colmajor(A).to(dt) for A in mats]
+        plan0 = M.BatchedPlan(h, ref, tile=1)
+        plan0.run()
+        h.synchronize()
+        got = [M.colmajor(A).to(dt) for A in mats]
+        plan = M.BatchedPlan(h, got, tile=1, per_cu=per_cu, order=order)
+        plan.run()
+        h.synchronize()
+        for a, b, ta, tb in zip(got, ref, plan.taus, plan0.taus):
+            assert torch.equal(a, b) and torch.equal(ta, tb)
+        plan.close()
+        plan0.close()
+
+
 def test_batched_argument_errors(h, torch_mod):
     import ctypes as C
     lib = M.lib()
