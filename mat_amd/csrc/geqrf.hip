@@ -72,44 +72,47 @@ __global__ __launch_bounds__(256) void k_trsm_diag(int nbk, int64_t nrhs, const 
 }
 
 // One TRB-row block i0 of the back substitution R x = c (one right-hand
-// side).  Every workgroup loads the bs x bs diagonal block and c[i0:i0+bs]
-// (already reduced by the later blocks) and solves it with one wave (lane i
-// holds x_i; the pivot value is broadcast by v_readlane); workgroup 0 stores
-// x to xout, and workgroup w then subtracts R[r, i0:i0+bs] x from
-// c[r] for its 256 rows r < i0.
-__global__ __launch_bounds__(256) void k_trsv_block(int64_t i0, int bs, const double *__restrict__ R,
-                                                    int64_t ldr, double *__restrict__ c,
-                                                    double *__restrict__ xout) {
-    __shared__ double Rs[TRB][TRB + 1];
-    __shared__ double xs[TRB];
-    const int tid = threadIdx.x;
-    for (int e = tid; e < bs * bs; e += 256) {
-        const int r = e % bs, cc = e / bs;
-        Rs[r][cc] = R[i0 + r + (i0 + cc) * ldr];
-    }
+// side), with the diagonal blocks' inverses precomputed (Xd: the 64 x 64
+// inverses side by side, ld 64).  Every workgroup forms x = Xd_q c[i0:i0+bs]
+// (c already reduced by the later blocks) as a parallel product: four threads
+// per row, each a quarter of the columns, summed in a fixed order, so no
+// sequential elimination remains on the chain.  Workgroup 0 stores x to xout;
+// workgroup w then subtracts R[r, i0:i0+bs] x from c[r] for its 64 rows
+// r < i0.
+__global__ __launch_bounds__(256) void k_trsv_inv_block(int64_t i0, int bs,
+                                                        const double *__restrict__ R, int64_t ldr,
+                                                        const double *__restrict__ Xd,
+                                                        double *__restrict__ c,
+                                                        double *__restrict__ xout) {
+    __shared__ double cs[TRB], xs[TRB], part[4][TRB];
+    const int tid = threadIdx.x, row = tid & (TRB - 1), qt = tid >> 6;
+    if (tid < TRB) cs[tid] = tid < bs ? c[i0 + tid] : 0.0;
     __syncthreads();
-    if (tid < 64) {
-        double xi = tid < bs ? c[i0 + tid] : 0.0;
-        for (int j = bs - 1; j >= 0; --j) {
-            // x_j is final in lane j once the later columns are eliminated
-            const double vj = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(xi), j),
-                                               __builtin_amdgcn_readlane(__double2loint(xi), j)) /
-                              Rs[j][j];
-            if (tid == j) xi = vj;
-            if (tid < j) xi = fma(-Rs[tid][j], vj, xi);
-        }
-        if (tid < bs) {
-            xs[tid] = xi;
-            if (blockIdx.x == 0) xout[i0 + tid] = xi;
-        }
-    }
-    __syncthreads();
-    const int64_t r = (int64_t)blockIdx.x * 256 + tid;
-    if (r < i0) {
+    {
+        const double *Xq = Xd + i0 * TRB;  // block q's inverse, ld TRB
         double acc = 0.0;
-        for (int j = 0; j < bs; ++j) acc = fma(R[r + (i0 + j) * ldr], xs[j], acc);
-        c[r] -= acc;
+        if (row < bs)
+            for (int k = qt * 16; k < qt * 16 + 16; ++k)
+                if (k >= row && k < bs) acc = fma(Xq[row + (int64_t)k * TRB], cs[k], acc);
+        part[qt][row] = acc;
     }
+    __syncthreads();
+    if (tid < TRB) {
+        const double xi = (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
+        xs[tid] = xi;
+        if (blockIdx.x == 0 && tid < bs) xout[i0 + tid] = xi;
+    }
+    __syncthreads();
+    // rows [64 b, 64 b + 64) above the block: four threads per row, each a
+    // quarter of the block's columns, partial sums added in a fixed order
+    const int64_t r = (int64_t)blockIdx.x * TRB + row;
+    double acc = 0.0;
+    if (r < i0)
+        for (int j = qt * 16; j < qt * 16 + 16; ++j)
+            if (j < bs) acc = fma(R[r + (i0 + j) * ldr], xs[j], acc);
+    part[qt][row] = acc;
+    __syncthreads();
+    if (tid < TRB && r < i0) c[r] -= (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
 }
 
 __global__ void k_copy_vec(int64_t n, const double *__restrict__ x, double *__restrict__ y) {
@@ -694,17 +697,25 @@ int orgqr_f64(mqr_handle_t hd, int64_t m, int64_t nq, int64_t k, const double *V
 // vector (other workgroups of the launch still read c there) and is copied
 // into c at the end.
 int trsv_upper_f64(mqr_handle_t hd, int64_t n, const double *R, int64_t ldr, double *c) {
-    MQR_CHECK(ws_reserve(hd, hd->ws_vec, sizeof(double) * (size_t)(n + 64)));
+    // x (n + 64), then the diagonal blocks' inverses (64 x n, 256-byte aligned)
+    const size_t xoff = ((size_t)(n + 64) * sizeof(double) + 255) & ~(size_t)255;
+    MQR_CHECK(ws_reserve(hd, hd->ws_vec, xoff + sizeof(double) * (size_t)TRB * (size_t)n));
     double *x = (double *)hd->ws_vec.ptr;
+    double *Xd = (double *)((char *)hd->ws_vec.ptr + xoff);
     const int64_t nblk = (n + TRB - 1) / TRB;
+    {
+        // all diagonal blocks inverted at once, in parallel
+        ProfScope ps(K_TRSM, (double)n * TRB * TRB / 3.0, 16.0 * (double)n * TRB, hd->stream);
+        MQR_CHECK(inv_diag_blocks_f64(hd->stream, n, R, ldr, Xd));
+    }
     for (int64_t q = nblk - 1; q >= 0; --q) {
         const int64_t i0 = q * TRB;
         const int bs = (int)std::min<int64_t>(TRB, n - i0);
-        const unsigned grid = (unsigned)std::max<int64_t>(1, (i0 + 255) / 256);
+        const unsigned grid64 = (unsigned)std::max<int64_t>(1, (i0 + TRB - 1) / TRB);
         ProfScope ps(K_TRSM, (double)bs * bs + 2.0 * i0 * bs, 8.0 * ((double)bs * bs + i0 * bs),
                      hd->stream);
-        hipLaunchKernelGGL(k_trsv_block, dim3(grid), dim3(256), 0, hd->stream, i0, bs, R, ldr, c,
-                           x);
+        hipLaunchKernelGGL(k_trsv_inv_block, dim3(grid64), dim3(256), 0, hd->stream, i0, bs, R, ldr,
+                           (const double *)Xd, c, x);
         MQR_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL(k_copy_vec, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, hd->stream, n,

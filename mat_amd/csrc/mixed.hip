@@ -263,11 +263,13 @@ __global__ void k_copy_d(int64_t n, const double *__restrict__ a, double *__rest
     if (i < n) b[i] = a[i];
 }
 
-// X (ld ldx) = inverse of each 64 x 64 diagonal block of upper R (ld ldr)
+// X (ld ldx) = inverse of each 64 x 64 diagonal block of upper R (ld ldr);
+// block i0 goes to X + i0 * xrow + i0 * ldx (xrow 1: the diagonal of an n x n
+// X; xrow 0: side by side in a 64 x n array)
 __global__ __launch_bounds__(small::NT) void k_inv_diag_blocks(int64_t n,
                                                                const double *__restrict__ R,
                                                                int64_t ldr, double *__restrict__ X,
-                                                               int64_t ldx) {
+                                                               int64_t ldx, int64_t xrow) {
     __shared__ small::Lds S[small::N];
     __shared__ small::RowBuf rb;
     __shared__ double rd[small::N];
@@ -284,7 +286,7 @@ __global__ __launch_bounds__(small::NT) void k_inv_diag_blocks(int64_t n,
     __syncthreads();
     small::to_lds(x, S);
     __syncthreads();
-    small::lds_store(S, X + i0 + i0 * ldx, ldx, b, b);
+    small::lds_store(S, X + i0 * xrow + i0 * ldx, ldx, b, b);
 }
 
 // R (ldr) b x b = upper triangle of the factored panel P (ld), zero below
@@ -672,13 +674,23 @@ int tri_inverse(mqr_handle_t hd, int64_t n, const double *R, int64_t ldr, double
         hipLaunchKernelGGL(k_fill, dim3(grid_for(n * n, 256, 1 << 22)), dim3(256), 0, s, n * n, X,
                            0.0);
         hipLaunchKernelGGL(k_inv_diag_blocks, dim3((unsigned)((n + 63) / 64)), dim3(small::NT), 0, s, n, R,
-                           ldr, X, n);
+                           ldr, X, n, (int64_t)1);
         MQR_LAUNCH_CHECK();
     }
     return upper_inverse_levels(hd, s, n, R, ldr, X, tmp);
 }
 
 }  // namespace
+
+// The inverses of the 64 x 64 diagonal blocks of the upper n x n R (ld ldr),
+// side by side in Xd (64 x n, ld 64): block q at columns [64 q, 64 q + 64).
+int inv_diag_blocks_f64(hipStream_t s, int64_t n, const double *R, int64_t ldr, double *Xd) {
+    if (n <= 0) return MQR_SUCCESS;
+    hipLaunchKernelGGL(k_inv_diag_blocks, dim3((unsigned)((n + 63) / 64)), dim3(small::NT), 0, s, n,
+                       R, ldr, Xd, (int64_t)64, (int64_t)0);
+    MQR_LAUNCH_CHECK();
+    return MQR_SUCCESS;
+}
 
 // The off-diagonal blocks of X = M^{-1} (ld n) for the upper n x n M whose
 // strictly upper 64-blocks are read from R (ld ldr), given the inverses of

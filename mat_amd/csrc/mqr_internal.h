@@ -285,6 +285,7 @@ int form_y_f64(hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv, 
 // j0, j0 + 64, ... (b columns in all, T blocks T(0:64, j0 + c), ld ldt) taken
 // as one block reflector, from G = V^T V (split-K partials in P, pcap
 // doubles) by recursive doubling; G: b x b, tmp: b/2 x b + b doubles.
+int inv_diag_blocks_f64(hipStream_t s, int64_t n, const double *R, int64_t ldr, double *Xd);
 int merged_t_f64(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double *V, int64_t ldv,
                  const double *T, int64_t ldt, int64_t j0, double *G, double *Tb, double *tmp,
                  double *P, int64_t pcap);
