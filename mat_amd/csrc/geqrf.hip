@@ -226,7 +226,7 @@ static GeqrfWs geqrf_carve(double *base, int64_t m, int64_t n, int ib, int ob) {
     w.P = take(w.pcap);
     w.P2 = take(w.pcap2);
     for (double *&y : w.Y) y = take(m * ib);
-    w.Wp = take(w_parts_doubles(m) + 1024);
+    w.Wp = take(2 * w_parts_doubles(m) + 4096 + 1024);  // [W | Gold] partials, C's top rows
     if (ob > ib) {
         w.pcap3 = std::max<int64_t>((int64_t)8 * ob * (n + ob), (int64_t)64 * ob * ob);
         w.pcap4 = (int64_t)16 * ib * (ob + ib);
@@ -372,16 +372,26 @@ int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda,
         if (c >= k || c >= n) return false;
         return fused(m - c, (int)std::min<int64_t>(ib, k - c), std::min<int64_t>(ib, n - c));
     };
+    // the panel at column c, after a panel of bprev columns, takes its first
+    // Gram matrix from the previous panel's [W | Gold] (cholqr_fused_a)
+    auto gold_at = [&](int64_t c, int64_t bprev) {
+        return ib == kMaxNb && bprev == kMaxNb && fused_at(c) && k - c >= kMaxNb &&
+               n - c >= kMaxNb && m - c <= kCholqrFusedMaxRows;
+    };
     bool w_ready = false;  // Wp[0:4096] = V^T C of the current panel and the next one
+    double *const ctop = w.Wp + 2 * w_parts_doubles(m);  // Cn's top rows (gold)
+    bool g_ready = false;  // Wp[4096:8192] = that C's Gram matrix before the update
     int64_t b_next = 0;    // first panel column whose block is not yet applied to B (one level)
     const int b0 = (int)std::min<int64_t>(ib, k);
     if (fused(m, b0, std::min<int64_t>(ib, n))) {
         MQR_CHECK(cholqr_gram1(s0, m, b0, 0, nullptr, 0, nullptr, 0, A, lda, 0, w.pws));
         MQR_CHECK(cholqr_rest_a(s0, m, b0, A, lda, tau, T, ldt, w.pws));
         w_ready = b0 < n && fused_at(b0);
+        g_ready = w_ready && gold_at(b0, b0);
         MQR_CHECK(cholqr_finish(s0, m, b0, A, lda, T, ldt, b0 < n ? w.Y[0] : nullptr,
                                 A + b0 * lda, lda, (int)std::min<int64_t>(ib, n - b0),
-                                w_ready ? w.Wp : nullptr, w.pws));
+                                w_ready ? w.Wp : nullptr, w.pws, false, nullptr, nullptr, nullptr,
+                                0, g_ready ? ctop : nullptr));
     } else {
         MQR_CHECK(panel_factor(hd, m, b0, A, lda, tau, T, ldt, w.pws));
         if (b0 < n) MQR_CHECK(form_y(s0, m, b0, A, lda, T, ldt, true, w.Y[0]));
@@ -462,18 +472,26 @@ int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda,
         const int bnext = next < k ? (int)std::min<int64_t>(ib, k - next) : 0;
         if (next < k && fused(m - next, bnext, bn)) {
             double *Cn = A + j + next * lda;
-            if (!w_ready)
-                MQR_CHECK(block_w(hd, s0, h, b, V, lda, Cn, lda, bn, w.P2, w.pcap2, K_LOOKAHEAD));
-            MQR_CHECK(cholqr_gram1(s0, h, (int)bn, b, Y, h, w_ready ? w.Wp : w.P2,
-                                   w_ready ? (int64_t)kMaxNb : (int64_t)b, Cn, lda, b, w.pws));
             double *Xn = Cn + b;
-            MQR_CHECK(cholqr_rest_a(s0, m - next, bnext, Xn, lda, tau + next, T + next * ldt, ldt,
-                                    w.pws));
+            if (g_ready) {
+                MQR_CHECK(cholqr_fused_a(s0, m - next, Y, h, w.Wp, ctop, Cn, lda, false, Xn,
+                                         lda, tau + next, T + next * ldt, ldt, w.pws));
+            } else {
+                if (!w_ready)
+                    MQR_CHECK(block_w(hd, s0, h, b, V, lda, Cn, lda, bn, w.P2, w.pcap2,
+                                      K_LOOKAHEAD));
+                MQR_CHECK(cholqr_gram1(s0, h, (int)bn, b, Y, h, w_ready ? w.Wp : w.P2,
+                                       w_ready ? (int64_t)kMaxNb : (int64_t)b, Cn, lda, b,
+                                       w.pws));
+                MQR_CHECK(cholqr_rest_a(s0, m - next, bnext, Xn, lda, tau + next,
+                                        T + next * ldt, ldt, w.pws));
+            }
             // W of the panel after next needs this step's update of its
             // columns first: the side stream's (same outer block) or the far
             // stream's deferred part (next outer block)
             const int64_t next2 = next + bnext;
             w_ready = next2 < n && fused_at(next2);
+            g_ready = w_ready && gold_at(next2, bnext);
             if (w_ready) {
                 if (side_pending) {
                     MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
@@ -485,10 +503,12 @@ int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda,
                                     next2 < n ? w.Y[(it + 1) % kYRing] : nullptr,
                                     A + next + next2 * lda, lda,
                                     (int)std::min<int64_t>(ib, std::max<int64_t>(n - next2, 0)),
-                                    w_ready ? w.Wp : nullptr, w.pws));
+                                    w_ready ? w.Wp : nullptr, w.pws, false, nullptr, nullptr,
+                                    nullptr, 0, g_ready ? ctop : nullptr));
             continue;
         }
         w_ready = false;
+        g_ready = false;
         MQR_CHECK(apply_block(hd, s0, h, b, V, lda, Y, A + j + next * lda, lda, bn, w.P2, w.pcap2,
                               K_LOOKAHEAD, K_LOOKAHEAD));
         if (next < k) {

@@ -382,7 +382,8 @@ MixedWs carve(char *base, bool fp64, int64_t m, int64_t n, int nb, int nbt) {
             w.V32o = (float *)take(sizeof(float) * m * ob);
             w.Y32o = (float *)take(sizeof(float) * m * ob);
         }
-        w.Wp = (double *)take(sizeof(double) * ((m + 63) / 64) * 64 * 64);
+        // [W | Gold] partials, then C's top rows
+        w.Wp = (double *)take(sizeof(double) * (2 * ((m + 63) / 64) + 1) * 64 * 64);
     }
     w.X = (double *)take(sizeof(double) * n * n);
     w.tmp = (double *)take(sizeof(double) * ((n + 1) / 2) * n);
@@ -556,16 +557,26 @@ int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t 
         if (c >= k || c >= nc) return false;
         return fused(m - c, (int)std::min<int64_t>(nb, k - c), std::min<int64_t>(nb, nc - c));
     };
+    // the panel at column c, after a panel of bprev columns, takes its first
+    // Gram matrix from the previous panel's [W | Gold] (cholqr_fused_a)
+    auto gold_at = [&](int64_t c, int64_t bprev) {
+        return nb == kMaxNb && bprev == kMaxNb && fused_at(c) && k - c >= kMaxNb &&
+               nc - c >= kMaxNb && m - c <= kCholqrFusedMaxRows;
+    };
     bool w_ready = false;  // w.Wp[0:4096] = V^T C of the current panel and the next one
+    double *const ctop = w.Wp + 2 * ((m + 63) / 64) * 64 * 64;  // Cn's top rows (gold)
+    bool g_ready = false;  // w.Wp[4096:8192] = that C's Gram matrix before the update
     const int b0 = (int)std::min<int64_t>(nb, k);
     if (fused_at(0)) {
         MQR_CHECK(cholqr_gram1(s0, m, b0, 0, nullptr, 0, nullptr, 0, w.Aw, m, 0, w.pws, true,
                                vpanel(0), m));
         MQR_CHECK(cholqr_rest_a(s0, m, b0, vpanel(0), m, w.tau, w.T, nb, w.pws));
         w_ready = fused_at(b0);
+        g_ready = w_ready && gold_at(b0, b0);
         MQR_CHECK(cholqr_finish(s0, m, b0, vpanel(0), m, w.T, nb, w.Yd, w.Aw + b0 * m, m,
                                 (int)std::min<int64_t>(nb, nc - b0), w_ready ? w.Wp : nullptr,
-                                w.pws, true, w.V32[0], w.Y32[0], w.R, w.ldr));
+                                w.pws, true, w.V32[0], w.Y32[0], w.R, w.ldr,
+                                g_ready ? ctop : nullptr));
     } else {
         MQR_CHECK(panel_fp32(hd, m, 0, b0, w, vpanel(0), w.V32[0], w.Y32[0]));
     }
@@ -616,14 +627,21 @@ int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t 
         const int bnext = next < k ? (int)std::min<int64_t>(nb, k - next) : 0;
         if (w_ready && next < k && fused(m - next, bnext, bn)) {
             double *Vn = vpanel(next);
-            MQR_CHECK(cholqr_gram1(s0, h, (int)bn, bj, w.Yd, h, w.Wp, 64, w.Aw + j + next * m, m,
-                                   bj, w.pws, true, Vn, m));
-            MQR_CHECK(cholqr_rest_a(s0, m - next, bnext, Vn, m, w.tau + next, w.T + next * nb, nb,
-                                    w.pws));
+            if (g_ready) {
+                MQR_CHECK(cholqr_fused_a(s0, m - next, w.Yd, h, w.Wp, ctop, w.Aw + j + next * m,
+                                         m, true, Vn, m, w.tau + next, w.T + next * nb, nb,
+                                         w.pws));
+            } else {
+                MQR_CHECK(cholqr_gram1(s0, h, (int)bn, bj, w.Yd, h, w.Wp, 64,
+                                       w.Aw + j + next * m, m, bj, w.pws, true, Vn, m));
+                MQR_CHECK(cholqr_rest_a(s0, m - next, bnext, Vn, m, w.tau + next,
+                                        w.T + next * nb, nb, w.pws));
+            }
             // W of the panel after next needs this step's update of its
             // columns first (side stream, or the far stream's deferred part)
             const int64_t next2 = next + bnext;
             w_ready = fused_at(next2);
+            g_ready = w_ready && gold_at(next2, bnext);
             if (w_ready) {
                 if (side_pending) {
                     MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
@@ -636,10 +654,11 @@ int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t 
                                     (int)std::min<int64_t>(nb, std::max<int64_t>(nc - next2, 0)),
                                     w_ready ? w.Wp : nullptr, w.pws, true,
                                     w.V32[(it + 1) % kRing], w.Y32[(it + 1) % kRing],
-                                    w.R + next + next * w.ldr, w.ldr));
+                                    w.R + next + next * w.ldr, w.ldr, g_ready ? ctop : nullptr));
             continue;
         }
         w_ready = false;
+        g_ready = false;
         MQR_CHECK(apply_block_f32(hd, s0, h, bj, V32, Y32, w.Aw + j + next * m, m, bn, w.P32b,
                                   pcap2, K_LOOKAHEAD, K_LOOKAHEAD, bf16_terms));
         if (next < k) {

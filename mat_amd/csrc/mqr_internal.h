@@ -258,10 +258,23 @@ int cholqr_rest_a(hipStream_t s, int64_t h, int b, double *X, int64_t ld, double
 // Cn is FP64, or FP32 when cn_f32; with Yf, V32 and Y32 (ld h) receive FP32
 // copies of V (unit-lower V1 explicit) and Yf, and Rout (ldr) the panel's
 // b x b R block (zero below the diagonal).
+// With ctop (and Wp), Wp[4096 : 8192] also receives Cn^T Cn, the Gram of
+// the next block's columns before this panel's update (Wp then holds
+// 2 x ceil(h / 64) x 4096 doubles), and ctop (64 x 64, ld 64, outside Wp)
+// Cn's top 64 rows, for cholqr_fused_a.
 int cholqr_finish(hipStream_t s, int64_t h, int b, double *X, int64_t ld, const double *T,
                   int64_t ldt, double *Yf, const void *Cn, int64_t ldcn, int bn, double *Wp,
                   void *ws, bool cn_f32 = false, float *V32 = nullptr, float *Y32 = nullptr,
-                  double *Rout = nullptr, int64_t ldr = 0);
+                  double *Rout = nullptr, int64_t ldr = 0, double *ctop = nullptr);
+// cholqr_gram1 + cholqr_rest_a for a 64-column panel of h <= 16384 rows after
+// a 64-column panel, when that panel's cholqr_finish formed [W | Gold] (WG):
+// the update and the first pass in one launch, the first Gram matrix from
+// Gold - R12^T R12 (C: the panel's columns from the previous panel's top
+// row, hc = h + 64 rows; X = the panel, C + 64 for FP64 C, its FP64 copy
+// for FP32 C).
+int cholqr_fused_a(hipStream_t s, int64_t h, const double *Y, int64_t ldy, const double *WG,
+                   const double *Ctop, void *C, int64_t ldc, bool c_f32, double *X, int64_t ldx,
+                   double *tau, double *T, int64_t ldt, void *ws);
 int cholqr_rest(hipStream_t s, int64_t h, int b, double *X, int64_t ld, double *tau, double *T,
                 int64_t ldt, double *Yf, void *ws);
 // Blocked FP64 QR of A (m x n, lda) in place with tau and the T blocks (ldt);
@@ -274,6 +287,9 @@ int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda,
 // rows (two barriers and a global round trip per 8-column round), so it is
 // kept for panels under one 64-row chunk and for breakdown.
 constexpr int64_t kCholqrMinRows = 64;
+// Tallest panel of cholqr_fused_a (one 64-row chunk per workgroup, at most
+// 256 workgroups).
+constexpr int64_t kCholqrFusedMaxRows = 256 * 64;
 // Either of the above by height.
 int panel_factor(mqr_handle_t hd, int64_t h, int b, double *V, int64_t lda, double *tau,
                  double *T, int64_t ldt, void *ws);

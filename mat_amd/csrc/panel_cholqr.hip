@@ -51,6 +51,13 @@ constexpr int kFbCounters = 128;  // barrier counters of the breakdown fallback
 // columns are Q S satisfy V1 = L, V2 = Q2 U^{-1} and T = -U S L^{-T}
 // (top block: Q1 S - I = -V1 T V1^T).
 //
+// Two 64 x 64 matrices of LDS (67 KB), so the kernel starts on a CU that
+// still runs one FP64 update tile (66.5 KB) instead of waiting for an empty
+// one.  Operands that only one wave reads as its tile row stay in that
+// wave's registers (Z's top rows, R2) or, for R2^{-1} between its inverse
+// and Minv, in the 64 x 64 global scratch `stash` (each lane re-reads the
+// values it wrote).
+//
 // Breakdown: when the first pass raised the flag, or the second Cholesky
 // fails here, the untouched h x b panel A is factored column by column in
 // this same workgroup (the reference's panel loop), with the flag left up
@@ -63,13 +70,17 @@ __global__ __launch_bounds__(small::NT) void k_recon(int64_t h, int b,
                                                      double *__restrict__ tau,
                                                      double *__restrict__ T, int64_t ldt,
                                                      double *__restrict__ Minv,
+                                                     double *__restrict__ stash,
                                                      int *__restrict__ flag) {
-    __shared__ small::Lds S0[PB], S1[PB], S2[PB], S3[PB];
+    __shared__ small::Lds P[PB], Q[PB];
     __shared__ double sg[PB];
     __shared__ int fail;
-    // the fallback's LDS, carved from the reconstruction's buffers
-    const hh::PanelLds fb{reinterpret_cast<double(*)[hh::CH]>(&S2[0][0]), &S2[8][0], S0,
-                          reinterpret_cast<small::RowBuf *>(&S1[0][0]), &S3[0][0]};
+    // the fallback's LDS: P, and its small buffers carved from Q
+    double *const qa = &Q[0][0];
+    const hh::PanelLds fb{reinterpret_cast<double(*)[hh::CH]>(qa), qa + 64, P,
+                          reinterpret_cast<small::RowBuf *>(qa + 128), qa + 128 + 2 * 4 * PB};
+    static_assert(128 + 2 * 4 * PB + PB <= PB * (PB + 1), "fallback LDS exceeds Q");
+    static_assert(sizeof(small::RowBuf) == 2 * 4 * PB * sizeof(double), "RowBuf layout");
     const int f0 = *flag;
     if (f0 == 2) return;  // factored by k_apply_gram's workgroups
     if (f0) {
@@ -77,65 +88,113 @@ __global__ __launch_bounds__(small::NT) void k_recon(int64_t h, int b,
         return;
     }
     const int tid = threadIdx.x, q = small::wave_id();
+    const int l = small::lane_id(), li = l & 15, kq = l >> 4, arow = 16 * q + li;
     if (tid == 0) fail = 0;
-    small::lds_load(S0, Z, ldz, b);
-    small::lds_load(S1, G, ldg, b);
+    small::lds_load(P, G, ldg, b);
     __syncthreads();
-    // R2 = chol(G2) -> S1, R2^{-1} -> S2
-    sb::chol_inv(S1, S2, b, [](int) { return 1e-2; }, &fail);
+    // R2 = chol(G2) -> P, R2^{-1} -> Q
+    sb::chol_inv(P, Q, b, [](int) { return 1e-2; }, &fail);
     if (fail) {  // uniform: read after the barrier that ends chol_inv
         if (tid == 0) *flag = 1;
         __syncthreads();  // the LDS buffers are reused by the fallback
         hh::panel_hh_body(h, b, A, lda, tau, T, ldt, fb);
         return;
     }
-    // Q1 = Z_top R2^{-1} -> S3 (wave q: tile row q)
-    for (int jb = 0; jb < 4; ++jb) {
-        sb::v4d acc = sb::zero4();
-        for (int k = 0; k <= jb; ++k) acc = sb::tmma<false, false>(acc, S0, q, k, S2, k, jb);
-        sb::tstore(S3, q, jb, acc);
+    // tile row q of R2 as MFMA operands; R2^{-1}'s to the stash
+    double r2[4][4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int k4 = 0; k4 < 4; ++k4) {
+            const int c = 16 * k + 4 * k4 + kq;
+            r2[k][k4] = P[arow][c];
+            stash[arow + (int64_t)c * PB] = Q[arow][c];
+        }
+    // Q1 = Z_top R2^{-1} -> tile row q of P (Z_top padded with I like G2)
+    {
+        double z[4][4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int k4 = 0; k4 < 4; ++k4) {
+                const int c = 16 * k + 4 * k4 + kq;
+                z[k][k4] = (arow < b && c < b) ? Z[arow + (int64_t)c * ldz] : (arow == c ? 1.0 : 0.0);
+            }
+        sb::v4d acc[4];
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb) {
+            acc[jb] = sb::zero4();
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (k <= jb) acc[jb] = sb::tmma_reg(acc[jb], z[k], Q, k, jb);
+        }
+        // this wave's reads of its tile row of P are done before it writes it
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb) sb::tstore(P, q, jb, acc[jb]);
     }
-    __syncthreads();
+    __syncthreads();  // R2^{-1} no longer read
     // RR = R2 R1, kept in registers (tile row q)
-    small::lds_load(S0, R1, PB, PB);
+    small::lds_load(Q, R1, PB, PB);
     __syncthreads();
     sb::v4d rr[4];
+#pragma unroll
     for (int jb = 0; jb < 4; ++jb) {
         rr[jb] = sb::zero4();
-        for (int k = q; k <= jb; ++k) rr[jb] = sb::tmma<false, false>(rr[jb], S1, q, k, S0, k, jb);
-    }
-    __syncthreads();
-    // LU of Q1 - S -> S3; U^{-1} -> S0, L^{-1} -> S1
-    sb::lu_sign_inv(S3, S0, S1, sg);
-    // Minv = R2^{-1} U^{-1} straight to global
-    for (int jb = 0; jb < 4; ++jb) {
-        sb::v4d acc = sb::zero4();
-        for (int k = q; k <= jb; ++k) acc = sb::tmma<false, false>(acc, S2, q, k, S0, k, jb);
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-            Minv[16 * q + sb::trow(r) + (int64_t)(16 * jb + sb::tcol()) * PB] = acc[r];
+        for (int k = 0; k < 4; ++k)
+            if (k >= q && k <= jb) rr[jb] = sb::tmma_reg(rr[jb], r2[k], Q, k, jb);
     }
-    __syncthreads();  // S2 no longer read
-    // U S -> S2
-    for (int e = tid; e < PB * PB; e += small::NT) {
-        const int r = e & (PB - 1), cc = e >> 6;
-        S2[r][cc] = r <= cc ? S3[r][cc] * sg[cc] : 0.0;
+    __syncthreads();  // R1 no longer read
+    // LU of Q1 - S -> P; U^{-1} -> Q
+    sb::lu_sign_invu(P, Q, sg);
+    // Minv = R2^{-1} U^{-1} straight to global, R2^{-1} back from the stash
+    {
+        double ri[4][4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int k4 = 0; k4 < 4; ++k4)
+                ri[k][k4] = stash[arow + (int64_t)(16 * k + 4 * k4 + kq) * PB];
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb) {
+            sb::v4d acc = sb::zero4();
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (k >= q && k <= jb) acc = sb::tmma_reg(acc, ri[k], Q, k, jb);
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                Minv[16 * q + sb::trow(r) + (int64_t)(16 * jb + sb::tcol()) * PB] = acc[r];
+        }
     }
-    __syncthreads();
+    __syncthreads();  // U^{-1} no longer read
+    // L^{-1} -> Q
+    sb::inv_unit_lower(P, Q);
     // T = -(U S) L^{-T} (upper), R = S (R2 R1) with V1 strictly below
+#pragma unroll
     for (int jb = 0; jb < 4; ++jb) {
         sb::v4d acc = sb::zero4();
-        for (int k = q; k <= jb; ++k) acc = sb::tmma<false, true>(acc, S2, q, k, S1, k, jb, -1.0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (k < q || k > jb) continue;
+#pragma unroll
+            for (int k4 = 0; k4 < 4; ++k4) {
+                const int c = 16 * k + 4 * k4 + kq;
+                const double us = arow <= c ? P[arow][c] * sg[c] : 0.0;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-1.0 * us, Q[16 * jb + li][c], acc, 0,
+                                                           0, 0);
+            }
+        }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int rr_ = 16 * q + sb::trow(r), cc = 16 * jb + sb::tcol();
             if (rr_ < b && cc < b) {
                 T[rr_ + (int64_t)cc * ldt] = rr_ <= cc ? acc[r] : 0.0;
-                A[rr_ + (int64_t)cc * lda] = rr_ <= cc ? sg[rr_] * rr[jb][r] : S3[rr_][cc];
+                A[rr_ + (int64_t)cc * lda] = rr_ <= cc ? sg[rr_] * rr[jb][r] : P[rr_][cc];
             }
         }
     }
-    for (int i = tid; i < b; i += small::NT) tau[i] = -sg[i] * S3[i][i];
+    for (int i = tid; i < b; i += small::NT) tau[i] = -sg[i] * P[i][i];
 }
 
 // ---- row-chunk kernels of the tall-panel sequence ----------------------
@@ -428,7 +487,7 @@ struct FbLds {
 
 // The grid fallback (see above).  S[0..nch) hold this workgroup's chunks;
 // Gp (partials), G (64 x 64, ld PB) and bar come from the panel workspace.
-__device__ void fallback_grid(int64_t h, int b, double *__restrict__ X, int64_t ldx,
+__device__ __forceinline__ void fallback_grid(int64_t h, int b, double *__restrict__ X, int64_t ldx,
                               double *__restrict__ Z, double *__restrict__ Gp,
                               double *__restrict__ G,
                               double *__restrict__ tau, double *__restrict__ T, int64_t ldt,
@@ -629,11 +688,192 @@ __global__ __launch_bounds__(small::NT) void k_apply_gram(int64_t h, int b,
     gram_store(g, Gp + (size_t)blockIdx.x * PB * PB);
 }
 
+// First pass of a full panel (b = 64) fused with the previous panel's update
+// of its columns, one 64-row chunk per workgroup (h <= 256 x 64).
+//
+// The previous panel's reflectors (Y, and W = V^T C of this panel's columns
+// C, in the previous panel's frame of hc = h + 64 rows) change C to
+// X = C - Y W.  They are orthogonal on those rows, so the Gram matrix of X's
+// rows below the top 64 (this panel, the first CholeskyQR pass's G1) is
+// Gold - Xt^T Xt, with Gold = C^T C over the frame (formed beside W by
+// k_v2_formy) and Xt = X's top 64 rows (this panel's R12 block above the
+// diagonal).  Every workgroup forms Xt and G1 itself (the same operations on
+// the same data) and factors G1 as k_apply_gram does, so the panel's own
+// Gram pass and its reduction are not needed.  The identity loses accuracy
+// where most of a column lies in the previous panel's span: a pivot
+// d_i <= 1e-10 Gold_ii (or 1e-12 G1_ii, k_apply_gram's test) counts as a
+// breakdown.  The first pass only preconditions X; the second (k_recon,
+// from Z's own Gram) sets the factor's accuracy.
+//
+// C's top rows are read from Ctop (their copy, ld 64), since workgroup 0
+// overwrites them with Xt.
+// Writes: Xt to C's top rows (TC), X to the panel X (FP64, ldx; for an FP64
+// C, X = C + 64), R1, Z and the Gram partials of Z, the flag.  Breakdown:
+// all workgroups store X, meet at a grid barrier and take the grid
+// fallback of k_apply_gram; a cancelled barrier leaves the flag at 1 for
+// k_recon.
+template <typename TC>
+__global__ __launch_bounds__(small::NT) void k_upd_apply(int64_t h, const double *__restrict__ Y,
+                                                         int64_t ldy,
+                                                         const double *__restrict__ WG,
+                                                         const double *__restrict__ Ctop,
+                                                         TC *__restrict__ C, int64_t ldc,
+                                                         double *__restrict__ X, int64_t ldx,
+                                                         double *__restrict__ G1,
+                                                         double *__restrict__ R1,
+                                                         double *__restrict__ Z, int64_t ldz,
+                                                         double *__restrict__ Gp,
+                                                         int *__restrict__ flag,
+                                                         double *__restrict__ tau,
+                                                         double *__restrict__ T, int64_t ldt,
+                                                         int *__restrict__ bar) {
+    __shared__ small::Lds As[RC], Bs[PB];
+    __shared__ double thr[PB];
+    __shared__ int fail;
+    __shared__ FbLds fbl;
+    constexpr int b = PB;
+    const int tid = threadIdx.x, q = small::wave_id();
+    const int l = small::lane_id(), li = l & 15, kq = l >> 4;
+    const int64_t hc = h + PB, r0 = (int64_t)blockIdx.x * RC;
+    const double *Gold = WG + PB * PB;
+    if (tid == 0) fail = 0;
+    // acc -= Y[rows of As] W: the update of one 64-row slice (row-per-lane)
+    auto update = [&](sb::v4d (&acc)[4]) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int kk = 4 * k + kq;
+            const double av = -As[16 * q + li][kk];
+#pragma unroll
+            for (int jb = 0; jb < 4; ++jb)
+                acc[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(Bs[kk][16 * jb + li], av, acc[jb],
+                                                               0, 0, 0);
+        }
+    };
+    // Xt = C[0:64] - Y[0:64] W, C's top rows from their copy
+    for (int e = tid; e < PB * PB; e += small::NT) Bs[e & (PB - 1)][e >> 6] = WG[e];
+    chunk_load(As, Y, ldy, 0, hc, PB);
+    sb::v4d xt[4], xc[4];
+#pragma unroll
+    for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) xt[jb][r] = Ctop[16 * q + lrow() + (16 * jb + lcol(r)) * PB];
+    __syncthreads();
+    update(xt);
+    if (blockIdx.x == 0) {
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                C[16 * q + lrow() + (int64_t)(16 * jb + lcol(r)) * ldc] = (TC)xt[jb][r];
+    }
+    // this workgroup's chunk: X = C - Y W on rows 64 + r0 .. of the frame
+    __syncthreads();
+    chunk_load(As, Y, ldy, PB + r0, hc, PB);
+    {
+        const int64_t row = PB + r0 + 16 * q + lrow();
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                xc[jb][r] = row < hc ? (double)C[row + (int64_t)(16 * jb + lcol(r)) * ldc] : 0.0;
+    }
+    __syncthreads();
+    update(xc);
+    {
+        const int64_t row = r0 + 16 * q + lrow();
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (row < h) X[row + (int64_t)(16 * jb + lcol(r)) * ldx] = xc[jb][r];
+    }
+    // G1 = Gold - Xt^T Xt -> As (upper tiles; the strictly lower ones zero)
+    __syncthreads();
+#pragma unroll
+    for (int jb = 0; jb < 4; ++jb) tstoreT(As, q, jb, xt[jb]);
+    __syncthreads();
+    {
+        sb::v4d g[3] = {sb::zero4(), sb::zero4(), sb::zero4()};
+        gram_acc(As, g);
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            const int t = q + 4 * u;
+            if (t < 10) {
+                int i, j;
+                upper_tile(t, i, j);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int gr = 16 * i + lrow(), gc = 16 * j + lcol(r);
+                    As[gr][gc] = Gold[gr + gc * PB] - g[u][r];
+                }
+            }
+            if (t < 6) {
+                int i, j;
+                lower_tile(t, i, j);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) As[16 * i + lrow()][16 * j + lcol(r)] = 0.0;
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < PB) thr[tid] = fmax(1e-12 * As[tid][tid], 1e-10 * Gold[tid * (PB + 1)]);
+    __syncthreads();
+    // R1 -> As, R1^{-1} -> Bs; ends with a barrier
+    sb::chol_inv(As, Bs, b, [&](int i) { return thr[i]; }, &fail);
+    if (blockIdx.x == 0) {
+        small::lds_store(As, R1, PB, PB, PB);
+        if (tid == 0) *flag = fail;
+    }
+    if (fail) {  // uniform: written before chol_inv's last barrier
+        // every workgroup's rows of X are stored before the fallback reads them
+        if (!grid_sync(bar + kFbCounters - 1, (int)gridDim.x, &fbl.ok)) return;
+        fallback_grid(h, b, X, ldx, Z, Gp, G1, tau, T, ldt, flag, bar, As, Bs, fbl);
+        return;
+    }
+    // Z = X R1^{-1} for the chunk, and its Gram partial.  X comes back from
+    // global memory (each lane reads what it stored; rows past h are zero)
+    // rather than staying in registers through the Cholesky.
+    const int64_t row = r0 + 16 * q + lrow();
+#pragma unroll
+    for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            xc[jb][r] = row < h ? X[row + (int64_t)(16 * jb + lcol(r)) * ldx] : 0.0;
+    __syncthreads();  // workgroup 0's reads of R1 are done
+#pragma unroll
+    for (int jb = 0; jb < 4; ++jb) tstoreT(As, q, jb, xc[jb]);
+    __syncthreads();
+    sb::v4d acc[4];
+    rowtile_upper(acc, As, Bs, q);
+#pragma unroll
+    for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            if (row < h) Z[row + (int64_t)(16 * jb + lcol(r)) * ldz] = acc[jb][r];
+    __syncthreads();
+#pragma unroll
+    for (int jb = 0; jb < 4; ++jb) tstoreT(As, q, jb, row < h ? acc[jb] : sb::zero4());
+    __syncthreads();
+    sb::v4d g[3] = {sb::zero4(), sb::zero4(), sb::zero4()};
+    gram_acc(As, g);
+    gram_store(g, Gp + (size_t)blockIdx.x * PB * PB);
+}
+
 // One 64-row chunk per workgroup: V(b:h) = Z(b:h) Minv into the panel A
 // (unless *flag: then the fallback has written V), and, when Yf, the block
 // reflector's left factor Yf (ld h) = V T^T with V = [unit-lower V1; V2].
 // With Wp, also the chunk's partial of W = V^T Cn (the next block's bn
 // columns at the panel's rows, ldcn) into Wp[blockIdx.x] (64 x 64, ld 64).
+// With ctop, the partial holds [W | Cn^T Cn] (stride 2 x 64 x 64): the Gram
+// of the next block's columns before this panel's update, from which
+// k_upd_apply forms the next panel's first Gram matrix; workgroup 0 also
+// copies Cn's top 64 rows to ctop (64 x 64, ld 64), so that k_upd_apply's
+// workgroups read them there while its workgroup 0 overwrites them in Cn.
+// zbar (when given) is zeroed by workgroup 0: the barrier counters of the
+// next panel's first pass.
+// Two 64 x 64 matrices of LDS (67 KB: the chunk, then Minv, T and the Cn
+// chunk in turn), so the kernel fits beside one FP64 update tile on a CU.
 template <typename TC>
 __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
                                                         const double *__restrict__ Z,
@@ -647,8 +887,12 @@ __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
                                                         const int *__restrict__ flag,
                                                         float *__restrict__ V32,
                                                         float *__restrict__ Y32,
-                                                        double *__restrict__ Rout, int64_t ldr) {
-    __shared__ small::Lds Vs[RC], Ms[PB], Ts[PB];
+                                                        double *__restrict__ Rout, int64_t ldr,
+                                                        double *__restrict__ ctop,
+                                                        int *__restrict__ zbar) {
+    const bool gold = ctop != nullptr;
+    __shared__ small::Lds Vs[RC], Ms[PB];
+    if (zbar && blockIdx.x == 0 && threadIdx.x < kFbCounters) zbar[threadIdx.x] = 0;
     const int q = small::wave_id();
     const bool fb = *flag != 0;
     const int64_t r0 = (int64_t)blockIdx.x * RC;
@@ -688,11 +932,21 @@ __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
             }
         }
     }
+    // T -> Ms (Minv is no longer read)
     for (int e = threadIdx.x; e < PB * PB; e += small::NT) {
         const int r = e & (PB - 1), c = e >> 6;
-        Ts[r][c] = (r < b && c < b && r <= c) ? T[r + (int64_t)c * ldt] : 0.0;
+        Ms[r][c] = (r < b && c < b && r <= c) ? T[r + (int64_t)c * ldt] : 0.0;
     }
-    if (Wp) chunk_load(Ms, Cn, ldcn, r0, h, bn);  // Minv is no longer read
+    // the Cn chunk into registers now, into LDS once T is no longer read
+    constexpr int kCn = RC * PB / small::NT;
+    double cn[kCn];
+    if (Wp) {
+#pragma unroll
+        for (int it = 0; it < kCn; ++it) {
+            const int e = threadIdx.x + it * small::NT, r = e & (RC - 1), c = e >> 6;
+            cn[it] = (r0 + r < h && c < bn) ? (double)Cn[r0 + r + (int64_t)c * ldcn] : 0.0;
+        }
+    }
     __syncthreads();
     if (V32) {
 #pragma unroll
@@ -717,7 +971,7 @@ __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
 #pragma unroll
             for (int jb = 0; jb < 4; ++jb)
                 if (jb <= (k >> 2))
-                    acc[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ts[16 * jb + i][kk], av, acc[jb],
+                    acc[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ms[16 * jb + i][kk], av, acc[jb],
                                                                    0, 0, 0);
         }
     }
@@ -733,6 +987,14 @@ __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
         }
     }
     if (!Wp) return;
+    __syncthreads();  // T no longer read
+#pragma unroll
+    for (int it = 0; it < kCn; ++it) {
+        const int e = threadIdx.x + it * small::NT;
+        Ms[e & (RC - 1)][e >> 6] = cn[it];
+        if (gold && blockIdx.x == 0) ctop[e] = cn[it];  // e = r + 64 c
+    }
+    __syncthreads();
     // W partial: tile row q of V^T Cn over the chunk's 64 rows
     {
         const int l = small::lane_id(), i = l & 15, kq = l >> 4;
@@ -748,11 +1010,16 @@ __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
                                                                0, 0);
         }
     }
-    double *Wb = Wp + (size_t)blockIdx.x * PB * PB;
+    double *Wb = Wp + (size_t)blockIdx.x * (gold ? 2 : 1) * PB * PB;
 #pragma unroll
     for (int jb = 0; jb < 4; ++jb)
 #pragma unroll
         for (int r = 0; r < 4; ++r) Wb[16 * q + lrow() + (16 * jb + lcol(r)) * PB] = acc[jb][r];
+    if (gold) {
+        sb::v4d g[3] = {sb::zero4(), sb::zero4(), sb::zero4()};
+        gram_acc(Ms, g);
+        gram_store(g, Wb + PB * PB);
+    }
 }
 
 struct CqrWs {
@@ -761,15 +1028,21 @@ struct CqrWs {
     int *bar;  // the fallback's barrier counters
 };
 
+// The flag and the barrier counters sit at the front, at the same address
+// for every panel height, so one panel's last launch can clear the next
+// panel's counters.
+constexpr size_t kCqrHead = 1024;  // bytes: flag (64 ints), counters (kFbCounters ints)
+static_assert(sizeof(int) * (64 + kFbCounters) <= kCqrHead, "workspace head");
+
 CqrWs carve(void *ws, int64_t h) {
     CqrWs w;
-    w.Z = (double *)ws;
+    w.flag = (int *)ws;
+    w.bar = w.flag + 64;
+    w.Z = (double *)((char *)ws + kCqrHead);
     w.R1 = w.Z + (size_t)h * PB;
     w.G1 = w.R1 + PB * PB;
     w.Mi = w.G1 + PB * PB;
     w.Gp = w.Mi + PB * PB;
-    w.flag = (int *)(w.Gp + (size_t)kMaxGramParts * PB * PB);
-    w.bar = w.flag + 64;
     return w;
 }
 
@@ -782,8 +1055,8 @@ int gram_parts(int64_t rows) {
 // Workspace: Z (h x PB), R1, G1, Minv (PB x PB), the Gram partials, flag
 // and the fallback's barrier counters.
 size_t panel_cholqr_ws_bytes(int64_t h) {
-    return sizeof(double) * ((size_t)h * PB + 3 * PB * PB + (size_t)kMaxGramParts * PB * PB) +
-           256 + sizeof(int) * kFbCounters;
+    return kCqrHead +
+           sizeof(double) * ((size_t)h * PB + 3 * PB * PB + (size_t)kMaxGramParts * PB * PB);
 }
 
 // First pass: C (hc x bn) -= Y W when kin > 0, and the Gram matrix G1 of
@@ -830,7 +1103,7 @@ int cholqr_rest_a(hipStream_t s, int64_t h, int b, double *X, int64_t ld, double
         ProfScope ps(K_PANEL_SMALL, 8.0 * bb * bb * bb / 3.0, 8.0 * bb * bb * 6.0, s);
         hipLaunchKernelGGL(k_recon, dim3(1), dim3(small::NT), 0, s, h, b, (const double *)w.Gp,
                            (int64_t)PB, (const double *)w.Z, h, (const double *)w.R1, X, ld, tau,
-                           T, ldt, w.Mi, w.flag);
+                           T, ldt, w.Mi, w.G1, w.flag);
         MQR_LAUNCH_CHECK();
     }
     return MQR_SUCCESS;
@@ -838,11 +1111,16 @@ int cholqr_rest_a(hipStream_t s, int64_t h, int b, double *X, int64_t ld, double
 
 // V2 of the panel; with Yf, Yf (h x b, ld h) = V T^T; with Wp, also
 // W = V^T Cn (b x bn, ld 64) for the next block's columns Cn (ldcn) into
-// Wp[0 : 64 x 64] (Wp holds ceil(h / 64) partials).
+// Wp[0 : 64 x 64] (Wp holds ceil(h / 64) partials), and with ctop also
+// Cn^T Cn into Wp[64 x 64 : 2 x 64 x 64] (twice the partials) and Cn's top
+// 64 rows into ctop (ld 64).
 int cholqr_finish(hipStream_t s, int64_t h, int b, double *X, int64_t ld, const double *T,
                   int64_t ldt, double *Yf, const void *Cn, int64_t ldcn, int bn, double *Wp,
-                  void *ws, bool cn_f32, float *V32, float *Y32, double *Rout, int64_t ldr) {
+                  void *ws, bool cn_f32, float *V32, float *Y32, double *Rout, int64_t ldr,
+                  double *ctop) {
     const CqrWs w = carve(ws, h);
+    if (!Wp) ctop = nullptr;
+    const bool gold = ctop != nullptr;
     const double hh = (double)h, bb = b;
     const unsigned chunks = (unsigned)((h + RC - 1) / RC);
     if (!Yf) Wp = nullptr, V32 = nullptr, Y32 = nullptr;
@@ -853,15 +1131,53 @@ int cholqr_finish(hipStream_t s, int64_t h, int b, double *X, int64_t ld, const 
             hipLaunchKernelGGL(k_v2_formy<float>, dim3(chunks), dim3(small::NT), 0, s, h, b,
                                (const double *)w.Z, (const double *)w.Mi, X, ld, T, ldt, Yf,
                                (const float *)Cn, ldcn, bn, Wp, (const int *)w.flag, V32, Y32,
-                               Rout, ldr);
+                               Rout, ldr, ctop, gold ? w.bar : nullptr);
         else
             hipLaunchKernelGGL(k_v2_formy<double>, dim3(chunks), dim3(small::NT), 0, s, h, b,
                                (const double *)w.Z, (const double *)w.Mi, X, ld, T, ldt, Yf,
                                (const double *)Cn, ldcn, bn, Wp, (const int *)w.flag, V32, Y32,
-                               Rout, ldr);
+                               Rout, ldr, ctop, gold ? w.bar : nullptr);
         MQR_LAUNCH_CHECK();
     }
-    if (Wp) MQR_CHECK(reduce_partials((int64_t)PB * PB, Wp, (int)chunks, (int64_t)PB * PB, s));
+    const int64_t part = (int64_t)(gold ? 2 : 1) * PB * PB;
+    if (Wp) MQR_CHECK(reduce_partials(part, Wp, (int)chunks, part, s));
+    return MQR_SUCCESS;
+}
+
+// The panel of h = hc - 64 rows at X after the previous 64-column panel's
+// update (Y: hc x 64, ld ldy; WG = [W | Gold] from cholqr_finish with gold)
+// of its columns C (hc x 64, ldc; FP32 when c_f32, then X is the panel's
+// FP64 copy, else X = C + 64): update, first pass, second Gram and
+// reconstruction in k_upd_apply + reduction + k_recon; cholqr_finish
+// follows as after cholqr_rest_a.  Needs h <= 256 x 64.
+int cholqr_fused_a(hipStream_t s, int64_t h, const double *Y, int64_t ldy, const double *WG,
+                   const double *Ctop, void *C, int64_t ldc, bool c_f32, double *X, int64_t ldx,
+                   double *tau, double *T, int64_t ldt, void *ws) {
+    const CqrWs w = carve(ws, h);
+    const int parts = (int)((h + RC - 1) / RC);
+    if (h <= 0 || parts > kMaxGramParts) return MQR_ERR_INTERNAL;
+    const double hh = (double)h, bb = PB;
+    {
+        ProfScope ps(K_PANEL, 2.0 * (hh + bb) * bb * bb + 3.0 * hh * bb * bb,
+                     8.0 * (hh + bb) * bb * (c_f32 ? 2.5 : 3.0) + 16.0 * hh * bb, s);
+        if (c_f32)
+            hipLaunchKernelGGL(k_upd_apply<float>, dim3(parts), dim3(small::NT), 0, s, h, Y, ldy,
+                               WG, Ctop, (float *)C, ldc, X, ldx, w.G1, w.R1, w.Z, h, w.Gp, w.flag, tau,
+                               T, ldt, w.bar);
+        else
+            hipLaunchKernelGGL(k_upd_apply<double>, dim3(parts), dim3(small::NT), 0, s, h, Y, ldy,
+                               WG, Ctop, (double *)C, ldc, X, ldx, w.G1, w.R1, w.Z, h, w.Gp, w.flag, tau,
+                               T, ldt, w.bar);
+        MQR_LAUNCH_CHECK();
+    }
+    MQR_CHECK(reduce_partials((int64_t)PB * PB, w.Gp, parts, (int64_t)PB * PB, s));
+    {
+        ProfScope ps(K_PANEL_SMALL, 8.0 * bb * bb * bb / 3.0, 8.0 * bb * bb * 6.0, s);
+        hipLaunchKernelGGL(k_recon, dim3(1), dim3(small::NT), 0, s, h, (int)PB,
+                           (const double *)w.Gp, (int64_t)PB, (const double *)w.Z, h,
+                           (const double *)w.R1, X, ldx, tau, T, ldt, w.Mi, w.G1, w.flag);
+        MQR_LAUNCH_CHECK();
+    }
     return MQR_SUCCESS;
 }
 

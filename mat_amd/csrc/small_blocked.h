@@ -44,6 +44,19 @@ __device__ __forceinline__ v4d tmma(v4d acc, const Lds *A, int ia, int ka, const
     return acc;
 }
 
+// acc += A[tile ia, k] * B[tile k, jb] with A's tile held by the calling
+// lane as the MFMA operand: a[k4] = A[16 ia + (l & 15)][16 k + 4 k4 + (l >> 4)]
+// (the values tmma<false, ...> reads from LDS).
+__device__ __forceinline__ v4d tmma_reg(v4d acc, const double (&a)[4], const Lds *B, int kb,
+                                        int jb) {
+    const int l = small::lane_id(), i = l & 15, kq = l >> 4;
+#pragma unroll
+    for (int k4 = 0; k4 < 4; ++k4)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[k4], B[16 * kb + 4 * k4 + kq][16 * jb + i], acc,
+                                                   0, 0, 0);
+    return acc;
+}
+
 __device__ __forceinline__ int trow(int r) { return (small::lane_id() >> 4) + 4 * r; }
 __device__ __forceinline__ int tcol() { return small::lane_id() & 15; }
 
@@ -209,10 +222,10 @@ __device__ __forceinline__ void diag_inv_u(const Lds *S, Lds *XU, int kb, const 
 }
 
 // One wave: L^{-1} (unit lower, ones on the diagonal) of the diagonal block
-// kb of S (after diag_lu) into the block of XL, column j by forward
+// kb of S (after diag_lu) into tile (kb, jt) of XL, column j by forward
 // substitution in lane j.
-__device__ __forceinline__ void diag_inv_l(const Lds *S, Lds *XL, int kb) {
-    const int j = small::lane_id() & 15, o = 16 * kb;
+__device__ __forceinline__ void diag_inv_l(const Lds *S, Lds *XL, int kb, int jt) {
+    const int j = small::lane_id() & 15, o = 16 * kb, ot = 16 * jt;
     double x[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) x[i] = (i == j) ? 1.0 : 0.0;
@@ -223,7 +236,7 @@ __device__ __forceinline__ void diag_inv_l(const Lds *S, Lds *XL, int kb) {
     }
     if (small::lane_id() < 16) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) XL[o + i][o + j] = (i >= j) ? x[i] : 0.0;
+        for (int i = 0; i < 16; ++i) XL[o + i][ot + j] = (i >= j) ? x[i] : 0.0;
     }
 }
 
@@ -344,8 +357,12 @@ __device__ __forceinline__ void chol_inv(Lds *S, Lds *X, int b, Thr thr, int *fa
 
 // LU without pivoting of S - diag(sg), the signs chosen during the
 // elimination (sg[i] = -sign of the current diagonal, sign(0) = +): L (unit,
-// strictly below) and U in place; XU = U^{-1}, XL = L^{-1} (unit lower).
-__device__ __forceinline__ void lu_sign_inv(Lds *S, Lds *XU, Lds *XL, double *sg) {
+// strictly below) and U in place; X = U^{-1}.  Two matrices of LDS: the
+// inverse of L's diagonal block kb, used only in its own block step, is
+// kept in X's tile (kb, kb + 1) until the doubling overwrites it (block 3
+// needs none).  L^{-1} follows from diag_inv_l on every diagonal block and
+// inv_lower_dbl.
+__device__ __forceinline__ void lu_sign_invu(Lds *S, Lds *X, double *sg) {
     const int q = small::wave_id();
 #pragma unroll 1
     for (int kb = 0; kb < 4; ++kb) {
@@ -354,18 +371,18 @@ __device__ __forceinline__ void lu_sign_inv(Lds *S, Lds *XU, Lds *XL, double *sg
         double ip[16];
         if (q == 0) diag_lu(S, kb, sg, ip);
         __syncthreads();
-        if (q == 0) diag_inv_u(S, XU, kb, ip);
-        else if (q == 1) diag_inv_l(S, XL, kb);
+        if (q == 0) diag_inv_u(S, X, kb, ip);
+        else if (q == 1 && kb < 3) diag_inv_l(S, X, kb, kb + 1);
         __syncthreads();
         // L tiles S[ib][kb] <- S[ib][kb] U_kk^{-1}; U tiles S[kb][jb] <- L_kk^{-1} S[kb][jb]
         const int nt = 3 - kb;
         for (int t = q; t < 2 * nt; t += 4) {
             if (t < nt) {
                 const int ib = kb + 1 + t;
-                tstore(S, ib, kb, tmma<false, false>(zero4(), S, ib, kb, XU, kb, kb));
+                tstore(S, ib, kb, tmma<false, false>(zero4(), S, ib, kb, X, kb, kb));
             } else {
                 const int jb = kb + 1 + (t - nt);
-                tstore(S, kb, jb, tmma<false, false>(zero4(), XL, kb, kb, S, kb, jb));
+                tstore(S, kb, jb, tmma<false, false>(zero4(), X, kb, kb + 1, S, kb, jb));
             }
         }
         __syncthreads();
@@ -376,8 +393,17 @@ __device__ __forceinline__ void lu_sign_inv(Lds *S, Lds *XU, Lds *XL, double *sg
         }
         __syncthreads();
     }
-    inv_upper_dbl(S, XU);
-    inv_lower_dbl(S, XL);
+    inv_upper_dbl(S, X);
+    __syncthreads();
+}
+
+// X = L^{-1} for the unit lower L (strictly below the diagonal of S, after
+// lu_sign_invu); X's previous contents are overwritten.
+__device__ __forceinline__ void inv_unit_lower(const Lds *S, Lds *X) {
+    const int q = small::wave_id();
+    diag_inv_l(S, X, q, q);
+    __syncthreads();
+    inv_lower_dbl(S, X);
     __syncthreads();
 }
 
