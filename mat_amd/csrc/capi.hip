@@ -187,11 +187,12 @@ int mqr_create(mqr_handle_t *handle, int device) {
         hipStreamCreateWithFlags(&h->far_stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_side, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&h->ev_in, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&h->ev_out, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&h->ev_host, hipEventDisableTiming) != hipSuccess) {
         (void)hipGetLastError();
-        for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out, h->ev_host})
+        for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_side, h->ev_in, h->ev_out, h->ev_host})
             if (e) (void)hipEventDestroy(e);
         if (h->side_stream) (void)hipStreamDestroy(h->side_stream);
         if (h->far_stream) (void)hipStreamDestroy(h->far_stream);
@@ -205,7 +206,7 @@ int mqr_create(mqr_handle_t *handle, int device) {
         h->cu_count = cus;
     if (hipHostMalloc((void **)&h->host_pinned, 64 * sizeof(double), hipHostMallocDefault) !=
         hipSuccess) {
-        for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out, h->ev_host}) (void)hipEventDestroy(e);
+        for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_side, h->ev_in, h->ev_out, h->ev_host}) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(h->side_stream);
         (void)hipStreamDestroy(h->far_stream);
         (void)hipStreamDestroy(h->own_stream);
@@ -216,7 +217,7 @@ int mqr_create(mqr_handle_t *handle, int device) {
     if (batched_noalloc_init(device) != MQR_SUCCESS) {
         (void)hipGetLastError();
         (void)hipHostFree(h->host_pinned);
-        for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out, h->ev_host}) (void)hipEventDestroy(e);
+        for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_side, h->ev_in, h->ev_out, h->ev_host}) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(h->side_stream);
         (void)hipStreamDestroy(h->far_stream);
         (void)hipStreamDestroy(h->own_stream);
@@ -238,7 +239,7 @@ int mqr_destroy(mqr_handle_t h) {
     if (h->ws_vec.ptr) (void)hipFree(h->ws_vec.ptr);
     if (h->ws_lu.ptr) (void)hipFree(h->ws_lu.ptr);
     if (h->host_pinned) (void)hipHostFree(h->host_pinned);
-    for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_in, h->ev_out, h->ev_host})
+    for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_side, h->ev_in, h->ev_out, h->ev_host})
         if (e) (void)hipEventDestroy(e);
     if (h->side_stream) (void)hipStreamDestroy(h->side_stream);
     if (h->far_stream) (void)hipStreamDestroy(h->far_stream);

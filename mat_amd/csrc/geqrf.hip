@@ -326,6 +326,7 @@ int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda,
     auto c1_of = [&](int64_t c) { return two ? std::min(n, (c / ob + 1) * ob) : n; };
     auto c2_of = [&](int64_t c) { return two ? std::min(n, (c / ob + 2) * ob) : n; };
     bool side_pending = false;  // the side stream's last update is not yet joined
+    bool side_any = false;      // side-stream work was queued (joined at the end by ev_side)
     int64_t merged_upto = 0;    // outer blocks [0, merged_upto) have had their merged update
     int64_t s1_synced = 0;      // the side stream has waited for the far stream up to this block
     // the far stream's merged update of outer block J (all its panels
@@ -444,17 +445,29 @@ int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda,
                 MQR_HIP(hipStreamWaitEvent(s1, ev.far_done[last_prev % kYRing], 0));
                 s1_synced = blk;
             }
+            // the panel after next (its first ib columns) first: the handle's
+            // stream waits for those (ev_join) and not for the rest (ev_side)
+            const int64_t first = std::min<int64_t>(ib, near_end - rest0);
+            const bool split = near_end - rest0 > first;
             if (near_end > rest0)
                 MQR_CHECK(apply_block(hd, s1, h, b, V, lda, Y, A + j + rest0 * lda, lda,
-                                      near_end - rest0, w.P, w.pcap, two ? K_NEAR : K_VTA,
-                                      two ? K_NEAR : K_UPDATE));
+                                      split ? first : near_end - rest0, w.P, w.pcap,
+                                      two ? K_NEAR : K_VTA, two ? K_NEAR : K_UPDATE));
+            if (split) {
+                MQR_HIP(hipEventRecord(hd->ev_join, s1));
+                MQR_CHECK(apply_block(hd, s1, h, b, V, lda, Y, A + j + (rest0 + first) * lda,
+                                      lda, near_end - rest0 - first, w.P, w.pcap,
+                                      two ? K_NEAR : K_VTA, two ? K_NEAR : K_UPDATE));
+            }
             if (!two && B) {
                 MQR_CHECK(apply_block(hd, s1, h, b, V, lda, Y, B + j, ldb, nrhs, w.P, w.pcap,
                                       K_APPLYQ, K_APPLYQ));
                 b_next = j + b;
             }
-            MQR_HIP(hipEventRecord(hd->ev_join, s1));
+            if (!split) MQR_HIP(hipEventRecord(hd->ev_join, s1));
+            MQR_HIP(hipEventRecord(hd->ev_side, s1));
             side_pending = true;
+            side_any = true;
         }
         if (two) {
             if (def_work || last_of_block) MQR_HIP(hipStreamWaitEvent(s2, hd->ev_fork, 0));
@@ -520,7 +533,7 @@ int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda,
                                  w.Y[(it + 1) % kYRing]));
         }
     }
-    if (side_pending) MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
+    if (side_any) MQR_HIP(hipStreamWaitEvent(s0, hd->ev_side, 0));
     if (two) {
         // outer blocks whose last panel ended the loop early (m >= n: the
         // last panels have no columns to their right) still owe B and, with

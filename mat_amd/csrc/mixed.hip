@@ -518,6 +518,7 @@ int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t 
     auto c1_of = [&](int64_t c) { return two ? std::min(nc, (c / ob + 1) * ob) : nc; };
     auto c2_of = [&](int64_t c) { return two ? std::min(nc, (c / ob + 2) * ob) : nc; };
     bool side_pending = false;
+    bool side_any = false;  // side-stream work was queued (joined at the end by ev_side)
     int64_t s1_synced = 0;
     // far stream: outer block J as one block reflector on [c2, nc)
     auto merge_block = [&](int64_t J) -> int {
@@ -609,11 +610,22 @@ int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t 
                 MQR_HIP(hipStreamWaitEvent(s1, far_done[(blk * (ob / nb) - 1) % kRing], 0));
                 s1_synced = blk;
             }
+            // the panel after next (its first nb columns) first: the handle's
+            // stream waits for those (ev_join) and not for the rest (ev_side)
+            const int64_t first = std::min<int64_t>(nb, near_end - rest0);
+            const bool split = near_end - rest0 > first;
             MQR_CHECK(apply_block_f32(hd, s1, h, bj, V32, Y32, w.Aw + j + rest0 * m, m,
-                                      near_end - rest0, w.P32, pcap, two ? K_NEAR : K_VTA,
-                                      two ? K_NEAR : K_UPDATE, bf16_terms));
+                                      split ? first : near_end - rest0, w.P32, pcap,
+                                      two ? K_NEAR : K_VTA, two ? K_NEAR : K_UPDATE, bf16_terms));
             MQR_HIP(hipEventRecord(hd->ev_join, s1));
+            if (split)
+                MQR_CHECK(apply_block_f32(hd, s1, h, bj, V32, Y32, w.Aw + j + (rest0 + first) * m,
+                                          m, near_end - rest0 - first, w.P32, pcap,
+                                          two ? K_NEAR : K_VTA, two ? K_NEAR : K_UPDATE,
+                                          bf16_terms));
+            MQR_HIP(hipEventRecord(hd->ev_side, s1));
             side_pending = true;
+            side_any = true;
         }
         if (two) {
             if (def_work || last_of_block) MQR_HIP(hipStreamWaitEvent(s2, hd->ev_fork, 0));
@@ -666,7 +678,7 @@ int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t 
                                  w.Y32[(it + 1) % kRing]));
         }
     }
-    if (side_pending) MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
+    if (side_any) MQR_HIP(hipStreamWaitEvent(s0, hd->ev_side, 0));
     if (two) {
         MQR_HIP(hipEventRecord(ready, s2));
         MQR_HIP(hipStreamWaitEvent(s0, ready, 0));

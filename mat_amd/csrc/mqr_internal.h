@@ -103,9 +103,11 @@ struct mqr_handle_st {
     hipStream_t user_stream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     // look-ahead: trailing updates run on side_stream, ordered against
-    // `stream` with ev_fork / ev_join; every call joins before returning
+    // `stream` with ev_fork / ev_join; every call joins before returning.
+    // A side update that starts with the columns the handle's stream reads
+    // next records ev_join after those and ev_side at its end.
     hipStream_t side_stream = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_side = nullptr;
     // the refinement's host reads wait on ev_host (recorded after the copy of
     // the scalars) while later work is already queued behind it
     hipEvent_t ev_host = nullptr;
