@@ -256,10 +256,10 @@ __device__ __forceinline__ void tstoreT(small::Lds *S, int ia, int jb, sb::v4d v
 template <typename TM>
 __device__ __forceinline__ void chunk_load(small::Lds *S, const TM *__restrict__ M, int64_t ld,
                                            int64_t r0, int64_t rows, int cols) {
-    for (int e = threadIdx.x; e < RC * PB; e += small::NT) {
-        const int r = e & (RC - 1), c = e >> 6;
-        S[r][c] = (r0 + r < rows && c < cols) ? (double)M[r0 + r + (int64_t)c * ld] : 0.0;
-    }
+    static_assert(RC == small::N && PB == small::N, "chunk shape");
+    small::lds_fill(S, [&](int r, int c) {
+        return (r0 + r < rows && c < cols) ? (double)M[r0 + r + (int64_t)c * ld] : 0.0;
+    });
 }
 
 // upper tile u (0..9) in row order: (0,0) (0,1) (0,2) (0,3) (1,1) ... (3,3)
@@ -335,10 +335,9 @@ __global__ __launch_bounds__(small::NT) void k_upd_gram(int64_t hc, int kin, int
     if (blockIdx.x == 0 && threadIdx.x < kFbCounters) bar[threadIdx.x] = 0;
     sb::v4d g[3] = {sb::zero4(), sb::zero4(), sb::zero4()};
     if (kin > 0) {
-        for (int e = threadIdx.x; e < PB * PB; e += small::NT) {
-            const int r = e & (PB - 1), c = e >> 6;
-            Ws[r][c] = (r < kin && c < bn) ? W[r + (int64_t)c * ldw] : 0.0;
-        }
+        small::lds_fill(Ws, [&](int r, int c) {
+            return (r < kin && c < bn) ? W[r + (int64_t)c * ldw] : 0.0;
+        });
     }
     for (int64_t r0 = (int64_t)blockIdx.x * RC; r0 < hc; r0 += (int64_t)gridDim.x * RC) {
         __syncthreads();  // the previous chunk's Gram reads are done
@@ -750,7 +749,7 @@ __global__ __launch_bounds__(small::NT) void k_upd_apply(int64_t h, const double
         }
     };
     // Xt = C[0:64] - Y[0:64] W, C's top rows from their copy
-    for (int e = tid; e < PB * PB; e += small::NT) Bs[e & (PB - 1)][e >> 6] = WG[e];
+    small::lds_fill(Bs, [&](int r, int c) { return WG[r + PB * c]; });
     chunk_load(As, Y, ldy, 0, hc, PB);
     sb::v4d xt[4], xc[4];
 #pragma unroll
@@ -922,21 +921,29 @@ __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
     __syncthreads();
     // rows above b: the unit-lower V1 (R sits above its diagonal in A)
     if (r0 < b) {
-        for (int e = threadIdx.x; e < RC * PB; e += small::NT) {
-            const int r = e & (RC - 1), c = e >> 6;
+        constexpr int IT = RC * PB / small::NT;
+        double av[IT];  // every load issued before the first use
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int e = threadIdx.x + it * small::NT, r = e & (RC - 1), c = e >> 6;
+            const int64_t gr = r0 + r;
+            av[it] = (gr < b && c < b) ? A[gr + (int64_t)c * lda] : 0.0;
+        }
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int e = threadIdx.x + it * small::NT, r = e & (RC - 1), c = e >> 6;
             const int64_t gr = r0 + r;
             if (gr < b) {
-                const double a = c < b ? A[gr + (int64_t)c * lda] : 0.0;
+                const double a = av[it];
                 Vs[r][c] = c < b ? (gr > c ? a : (gr == c ? 1.0 : 0.0)) : 0.0;
                 if (Rout && c < b) Rout[gr + (int64_t)c * ldr] = gr <= c ? a : 0.0;
             }
         }
     }
     // T -> Ms (Minv is no longer read)
-    for (int e = threadIdx.x; e < PB * PB; e += small::NT) {
-        const int r = e & (PB - 1), c = e >> 6;
-        Ms[r][c] = (r < b && c < b && r <= c) ? T[r + (int64_t)c * ldt] : 0.0;
-    }
+    small::lds_fill(Ms, [&](int r, int c) {
+        return (r < b && c < b && r <= c) ? T[r + (int64_t)c * ldt] : 0.0;
+    });
     // the Cn chunk into registers now, into LDS once T is no longer read
     constexpr int kCn = RC * PB / small::NT;
     double cn[kCn];

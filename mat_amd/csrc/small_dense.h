@@ -63,19 +63,45 @@ __device__ __forceinline__ double recip(double d) {
 }
 
 // S = M(0:b, 0:b) (column-major, ld), identity outside; coalesced reads.
-__device__ __forceinline__ void lds_load(Lds *S, const double *__restrict__ M, int64_t ld, int b) {
-    for (int e = threadIdx.x; e < N * N; e += NT) {
-        const int r = e & (N - 1), c = e >> 6;
-        S[r][c] = (r < b && c < b) ? M[r + (int64_t)c * ld] : (r == c ? 1.0 : 0.0);
+// S[r][c] = f(r, c) over the N x N matrix.  Every thread issues its sixteen
+// loads before its first LDS store, so the global latency is paid once, not
+// once per element.
+template <typename F>
+__device__ __forceinline__ void lds_fill(Lds *S, F f) {
+    constexpr int IT = N * N / NT;
+    double v[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int e = threadIdx.x + it * NT;
+        v[it] = f(e & (N - 1), e >> 6);
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int e = threadIdx.x + it * NT;
+        S[e & (N - 1)][e >> 6] = v[it];
     }
 }
 
-// M(0:rows, 0:cols) = S; coalesced writes.
+__device__ __forceinline__ void lds_load(Lds *S, const double *__restrict__ M, int64_t ld, int b) {
+    lds_fill(S, [&](int r, int c) {
+        return (r < b && c < b) ? M[r + (int64_t)c * ld] : (r == c ? 1.0 : 0.0);
+    });
+}
+
+// M(0:rows, 0:cols) = S; coalesced writes, the LDS reads issued first.
 __device__ __forceinline__ void lds_store(Lds *S, double *__restrict__ M, int64_t ld, int rows,
                                           int cols) {
-    for (int e = threadIdx.x; e < N * N; e += NT) {
-        const int r = e & (N - 1), c = e >> 6;
-        if (r < rows && c < cols) M[r + (int64_t)c * ld] = S[r][c];
+    constexpr int IT = N * N / NT;
+    double v[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int e = threadIdx.x + it * NT;
+        v[it] = S[e & (N - 1)][e >> 6];
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int e = threadIdx.x + it * NT, r = e & (N - 1), c = e >> 6;
+        if (r < rows && c < cols) M[r + (int64_t)c * ld] = v[it];
     }
 }
 
