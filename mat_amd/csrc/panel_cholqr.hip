@@ -252,14 +252,24 @@ __device__ __forceinline__ void tstoreT(small::Lds *S, int ia, int jb, sb::v4d v
     for (int r = 0; r < 4; ++r) S[16 * ia + lrow()][16 * jb + lcol(r)] = v[r];
 }
 
+// v = the calling thread's values of M[r0 + r, c] (r0 + r < rows and
+// c < cols, else 0) in small::tile_fetch's order
+template <typename TM>
+__device__ __forceinline__ void chunk_fetch(double (&v)[small::TV], const TM *__restrict__ M,
+                                            int64_t ld, int64_t r0, int64_t rows, int cols) {
+    static_assert(RC == small::N && PB == small::N, "chunk shape");
+    small::tile_fetch(v, [&](int r, int c) {
+        return (r0 + r < rows && c < cols) ? (double)M[r0 + r + (int64_t)c * ld] : 0.0;
+    });
+}
+
 // S[r][c] = M[r0 + r, c] for r0 + r < rows and c < cols, else 0
 template <typename TM>
 __device__ __forceinline__ void chunk_load(small::Lds *S, const TM *__restrict__ M, int64_t ld,
                                            int64_t r0, int64_t rows, int cols) {
-    static_assert(RC == small::N && PB == small::N, "chunk shape");
-    small::lds_fill(S, [&](int r, int c) {
-        return (r0 + r < rows && c < cols) ? (double)M[r0 + r + (int64_t)c * ld] : 0.0;
-    });
+    double v[small::TV];
+    chunk_fetch(v, M, ld, r0, rows, cols);
+    small::tile_stash(S, v);
 }
 
 // upper tile u (0..9) in row order: (0,0) (0,1) (0,2) (0,3) (1,1) ... (3,3)
@@ -748,14 +758,32 @@ __global__ __launch_bounds__(small::NT) void k_upd_apply(int64_t h, const double
                                                                0, 0, 0);
         }
     };
-    // Xt = C[0:64] - Y[0:64] W, C's top rows from their copy
-    small::lds_fill(Bs, [&](int r, int c) { return WG[r + PB * c]; });
-    chunk_load(As, Y, ldy, 0, hc, PB);
+    // Xt = C[0:64] - Y[0:64] W, C's top rows from their copy; this
+    // workgroup's chunk of Y and C is fetched meanwhile
+    constexpr int TV = small::TV;
     sb::v4d xt[4], xc[4];
+    double yc[TV];
+    {
+        double wv[TV], yv[TV];
+        small::tile_fetch(wv, [&](int r, int c) { return WG[r + PB * c]; });
+        chunk_fetch(yv, Y, ldy, 0, hc, PB);
 #pragma unroll
-    for (int jb = 0; jb < 4; ++jb)
+        for (int jb = 0; jb < 4; ++jb)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) xt[jb][r] = Ctop[16 * q + lrow() + (16 * jb + lcol(r)) * PB];
+            for (int r = 0; r < 4; ++r)
+                xt[jb][r] = Ctop[16 * q + lrow() + (16 * jb + lcol(r)) * PB];
+        small::tile_stash(Bs, wv);
+        small::tile_stash(As, yv);
+    }
+    chunk_fetch(yc, Y, ldy, PB + r0, hc, PB);
+    {
+        const int64_t row = PB + r0 + 16 * q + lrow();
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                xc[jb][r] = row < hc ? (double)C[row + (int64_t)(16 * jb + lcol(r)) * ldc] : 0.0;
+    }
     __syncthreads();
     update(xt);
     if (blockIdx.x == 0) {
@@ -767,15 +795,7 @@ __global__ __launch_bounds__(small::NT) void k_upd_apply(int64_t h, const double
     }
     // this workgroup's chunk: X = C - Y W on rows 64 + r0 .. of the frame
     __syncthreads();
-    chunk_load(As, Y, ldy, PB + r0, hc, PB);
-    {
-        const int64_t row = PB + r0 + 16 * q + lrow();
-#pragma unroll
-        for (int jb = 0; jb < 4; ++jb)
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                xc[jb][r] = row < hc ? (double)C[row + (int64_t)(16 * jb + lcol(r)) * ldc] : 0.0;
-    }
+    small::tile_stash(As, yc);
     __syncthreads();
     update(xc);
     {
@@ -792,8 +812,22 @@ __global__ __launch_bounds__(small::NT) void k_upd_apply(int64_t h, const double
     for (int jb = 0; jb < 4; ++jb) tstoreT(As, q, jb, xt[jb]);
     __syncthreads();
     {
+        // Gold's tiles (and its diagonal for the pivot thresholds) are
+        // fetched before the Gram products
+        double go[3][4];
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            const int t = q + 4 * u;
+            int i = 0, j = 0;
+            if (t < 10) upper_tile(t, i, j);
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                go[u][r] = t < 10 ? Gold[16 * i + lrow() + (16 * j + lcol(r)) * PB] : 0.0;
+        }
+        const double gdiag = tid < PB ? Gold[tid * (PB + 1)] : 0.0;
         sb::v4d g[3] = {sb::zero4(), sb::zero4(), sb::zero4()};
         gram_acc(As, g);
+        if (tid < PB) thr[tid] = 1e-10 * gdiag;
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < 3; ++u) {
@@ -804,7 +838,7 @@ __global__ __launch_bounds__(small::NT) void k_upd_apply(int64_t h, const double
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int gr = 16 * i + lrow(), gc = 16 * j + lcol(r);
-                    As[gr][gc] = Gold[gr + gc * PB] - g[u][r];
+                    As[gr][gc] = go[u][r] - g[u][r];
                 }
             }
             if (t < 6) {
@@ -816,7 +850,7 @@ __global__ __launch_bounds__(small::NT) void k_upd_apply(int64_t h, const double
         }
     }
     __syncthreads();
-    if (tid < PB) thr[tid] = fmax(1e-12 * As[tid][tid], 1e-10 * Gold[tid * (PB + 1)]);
+    if (tid < PB) thr[tid] = fmax(1e-12 * As[tid][tid], thr[tid]);
     __syncthreads();
     // R1 -> As, R1^{-1} -> Bs; ends with a barrier
     sb::chol_inv(As, Bs, b, [&](int i) { return thr[i]; }, &fail);
@@ -896,9 +930,27 @@ __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
     const bool fb = *flag != 0;
     const int64_t r0 = (int64_t)blockIdx.x * RC;
     const int64_t row = r0 + 16 * q + lrow();
+    // loads that depend on nothing this kernel writes are issued early: T,
+    // the Cn chunk and V1's rows go out while V2 is formed
+    constexpr int TV = small::TV;
+    double tv[TV], cn[TV], v1v[TV];
+    auto fetch_rest = [&]() {
+        small::tile_fetch(tv, [&](int r, int c) {
+            return (r < b && c < b && r <= c) ? T[r + (int64_t)c * ldt] : 0.0;
+        });
+        if (Wp) chunk_fetch(cn, Cn, ldcn, r0, h, bn);
+        // rows above b: the unit-lower V1 (R sits above its diagonal in A)
+        if (r0 < b) chunk_fetch(v1v, A, lda, r0, (int64_t)b, b);
+    };
     if (!fb) {
-        small::lds_load(Ms, Minv, PB, PB);
-        chunk_load(Vs, Z, h, r0, h, b);
+        {
+            double mv[TV], zv[TV];
+            small::tile_fetch(mv, [&](int r, int c) { return Minv[r + (int64_t)c * PB]; });
+            chunk_fetch(zv, Z, h, r0, h, b);
+            small::tile_stash(Ms, mv);
+            small::tile_stash(Vs, zv);
+        }
+        if (Yf) fetch_rest();
         __syncthreads();
         sb::v4d acc[4];
         rowtile_upper(acc, Vs, Ms, q);
@@ -916,44 +968,24 @@ __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
         for (int jb = 0; jb < 4; ++jb) tstoreT(Vs, q, jb, acc[jb]);
     } else {
         if (!Yf) return;
+        fetch_rest();
         chunk_load(Vs, A, lda, r0, h, b);
     }
     __syncthreads();
-    // rows above b: the unit-lower V1 (R sits above its diagonal in A)
     if (r0 < b) {
-        constexpr int IT = RC * PB / small::NT;
-        double av[IT];  // every load issued before the first use
 #pragma unroll
-        for (int it = 0; it < IT; ++it) {
-            const int e = threadIdx.x + it * small::NT, r = e & (RC - 1), c = e >> 6;
-            const int64_t gr = r0 + r;
-            av[it] = (gr < b && c < b) ? A[gr + (int64_t)c * lda] : 0.0;
-        }
-#pragma unroll
-        for (int it = 0; it < IT; ++it) {
+        for (int it = 0; it < TV; ++it) {
             const int e = threadIdx.x + it * small::NT, r = e & (RC - 1), c = e >> 6;
             const int64_t gr = r0 + r;
             if (gr < b) {
-                const double a = av[it];
+                const double a = v1v[it];
                 Vs[r][c] = c < b ? (gr > c ? a : (gr == c ? 1.0 : 0.0)) : 0.0;
                 if (Rout && c < b) Rout[gr + (int64_t)c * ldr] = gr <= c ? a : 0.0;
             }
         }
     }
     // T -> Ms (Minv is no longer read)
-    small::lds_fill(Ms, [&](int r, int c) {
-        return (r < b && c < b && r <= c) ? T[r + (int64_t)c * ldt] : 0.0;
-    });
-    // the Cn chunk into registers now, into LDS once T is no longer read
-    constexpr int kCn = RC * PB / small::NT;
-    double cn[kCn];
-    if (Wp) {
-#pragma unroll
-        for (int it = 0; it < kCn; ++it) {
-            const int e = threadIdx.x + it * small::NT, r = e & (RC - 1), c = e >> 6;
-            cn[it] = (r0 + r < h && c < bn) ? (double)Cn[r0 + r + (int64_t)c * ldcn] : 0.0;
-        }
-    }
+    small::tile_stash(Ms, tv);
     __syncthreads();
     if (V32) {
 #pragma unroll
@@ -996,7 +1028,7 @@ __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
     if (!Wp) return;
     __syncthreads();  // T no longer read
 #pragma unroll
-    for (int it = 0; it < kCn; ++it) {
+    for (int it = 0; it < TV; ++it) {
         const int e = threadIdx.x + it * small::NT;
         Ms[e & (RC - 1)][e >> 6] = cn[it];
         if (gold && blockIdx.x == 0) ctop[e] = cn[it];  // e = r + 64 c

@@ -66,20 +66,31 @@ __device__ __forceinline__ double recip(double d) {
 // S[r][c] = f(r, c) over the N x N matrix.  Every thread issues its sixteen
 // loads before its first LDS store, so the global latency is paid once, not
 // once per element.
+// A matrix in flight: the sixteen values of the calling thread, element
+// e = threadIdx.x + NT it at row e & (N - 1), column e >> 6.  tile_fetch
+// issues the loads, tile_stash writes them to LDS; kernels fetch what they
+// need early and stash it once the LDS buffer is free.
+constexpr int TV = N * N / NT;
 template <typename F>
-__device__ __forceinline__ void lds_fill(Lds *S, F f) {
-    constexpr int IT = N * N / NT;
-    double v[IT];
+__device__ __forceinline__ void tile_fetch(double (&v)[TV], F f) {
 #pragma unroll
-    for (int it = 0; it < IT; ++it) {
+    for (int it = 0; it < TV; ++it) {
         const int e = threadIdx.x + it * NT;
         v[it] = f(e & (N - 1), e >> 6);
     }
+}
+__device__ __forceinline__ void tile_stash(Lds *S, const double (&v)[TV]) {
 #pragma unroll
-    for (int it = 0; it < IT; ++it) {
+    for (int it = 0; it < TV; ++it) {
         const int e = threadIdx.x + it * NT;
         S[e & (N - 1)][e >> 6] = v[it];
     }
+}
+template <typename F>
+__device__ __forceinline__ void lds_fill(Lds *S, F f) {
+    double v[TV];
+    tile_fetch(v, f);
+    tile_stash(S, v);
 }
 
 __device__ __forceinline__ void lds_load(Lds *S, const double *__restrict__ M, int64_t ld, int b) {
