@@ -253,14 +253,12 @@ __device__ __forceinline__ void tstoreT(small::Lds *S, int ia, int jb, sb::v4d v
 }
 
 // v = the calling thread's values of M[r0 + r, c] (r0 + r < rows and
-// c < cols, else 0) in small::tile_fetch's order
+// c < cols, else 0) in the pair layout (small::pair_fetch)
 template <typename TM>
 __device__ __forceinline__ void chunk_fetch(double (&v)[small::TV], const TM *__restrict__ M,
                                             int64_t ld, int64_t r0, int64_t rows, int cols) {
     static_assert(RC == small::N && PB == small::N, "chunk shape");
-    small::tile_fetch(v, [&](int r, int c) {
-        return (r0 + r < rows && c < cols) ? (double)M[r0 + r + (int64_t)c * ld] : 0.0;
-    });
+    small::pair_fetch(v, M + r0, ld, rows - r0, cols);
 }
 
 // S[r][c] = M[r0 + r, c] for r0 + r < rows and c < cols, else 0
@@ -269,7 +267,7 @@ __device__ __forceinline__ void chunk_load(small::Lds *S, const TM *__restrict__
                                            int64_t r0, int64_t rows, int cols) {
     double v[small::TV];
     chunk_fetch(v, M, ld, r0, rows, cols);
-    small::tile_stash(S, v);
+    small::pair_stash(S, v);
 }
 
 // upper tile u (0..9) in row order: (0,0) (0,1) (0,2) (0,3) (1,1) ... (3,3)
@@ -345,9 +343,9 @@ __global__ __launch_bounds__(small::NT) void k_upd_gram(int64_t hc, int kin, int
     if (blockIdx.x == 0 && threadIdx.x < kFbCounters) bar[threadIdx.x] = 0;
     sb::v4d g[3] = {sb::zero4(), sb::zero4(), sb::zero4()};
     if (kin > 0) {
-        small::lds_fill(Ws, [&](int r, int c) {
-            return (r < kin && c < bn) ? W[r + (int64_t)c * ldw] : 0.0;
-        });
+        double wv[small::TV];
+        small::pair_fetch(wv, W, ldw, kin, bn);
+        small::pair_stash(Ws, wv);
     }
     for (int64_t r0 = (int64_t)blockIdx.x * RC; r0 < hc; r0 += (int64_t)gridDim.x * RC) {
         __syncthreads();  // the previous chunk's Gram reads are done
@@ -765,15 +763,15 @@ __global__ __launch_bounds__(small::NT) void k_upd_apply(int64_t h, const double
     double yc[TV];
     {
         double wv[TV], yv[TV];
-        small::tile_fetch(wv, [&](int r, int c) { return WG[r + PB * c]; });
+        small::pair_fetch(wv, WG, PB, PB, PB);
         chunk_fetch(yv, Y, ldy, 0, hc, PB);
 #pragma unroll
         for (int jb = 0; jb < 4; ++jb)
 #pragma unroll
             for (int r = 0; r < 4; ++r)
                 xt[jb][r] = Ctop[16 * q + lrow() + (16 * jb + lcol(r)) * PB];
-        small::tile_stash(Bs, wv);
-        small::tile_stash(As, yv);
+        small::pair_stash(Bs, wv);
+        small::pair_stash(As, yv);
     }
     chunk_fetch(yc, Y, ldy, PB + r0, hc, PB);
     {
@@ -795,7 +793,7 @@ __global__ __launch_bounds__(small::NT) void k_upd_apply(int64_t h, const double
     }
     // this workgroup's chunk: X = C - Y W on rows 64 + r0 .. of the frame
     __syncthreads();
-    small::tile_stash(As, yc);
+    small::pair_stash(As, yc);
     __syncthreads();
     update(xc);
     {
@@ -935,9 +933,7 @@ __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
     constexpr int TV = small::TV;
     double tv[TV], cn[TV], v1v[TV];
     auto fetch_rest = [&]() {
-        small::tile_fetch(tv, [&](int r, int c) {
-            return (r < b && c < b && r <= c) ? T[r + (int64_t)c * ldt] : 0.0;
-        });
+        small::pair_fetch(tv, T, ldt, b, b);  // upper part kept when stashed
         if (Wp) chunk_fetch(cn, Cn, ldcn, r0, h, bn);
         // rows above b: the unit-lower V1 (R sits above its diagonal in A)
         if (r0 < b) chunk_fetch(v1v, A, lda, r0, (int64_t)b, b);
@@ -945,10 +941,10 @@ __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
     if (!fb) {
         {
             double mv[TV], zv[TV];
-            small::tile_fetch(mv, [&](int r, int c) { return Minv[r + (int64_t)c * PB]; });
+            small::pair_fetch(mv, Minv, PB, PB, PB);
             chunk_fetch(zv, Z, h, r0, h, b);
-            small::tile_stash(Ms, mv);
-            small::tile_stash(Vs, zv);
+            small::pair_stash(Ms, mv);
+            small::pair_stash(Vs, zv);
         }
         if (Yf) fetch_rest();
         __syncthreads();
@@ -974,18 +970,26 @@ __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
     __syncthreads();
     if (r0 < b) {
 #pragma unroll
-        for (int it = 0; it < TV; ++it) {
-            const int e = threadIdx.x + it * small::NT, r = e & (RC - 1), c = e >> 6;
-            const int64_t gr = r0 + r;
-            if (gr < b) {
-                const double a = v1v[it];
-                Vs[r][c] = c < b ? (gr > c ? a : (gr == c ? 1.0 : 0.0)) : 0.0;
-                if (Rout && c < b) Rout[gr + (int64_t)c * ldr] = gr <= c ? a : 0.0;
+        for (int it = 0; it < TV / 2; ++it)
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int r = small::prow() + u, c = small::pcol(it);
+                const int64_t gr = r0 + r;
+                if (gr < b) {
+                    const double a = v1v[2 * it + u];
+                    Vs[r][c] = c < b ? (gr > c ? a : (gr == c ? 1.0 : 0.0)) : 0.0;
+                    if (Rout && c < b) Rout[gr + (int64_t)c * ldr] = gr <= c ? a : 0.0;
+                }
             }
-        }
     }
-    // T -> Ms (Minv is no longer read)
-    small::tile_stash(Ms, tv);
+    // T's upper part -> Ms (Minv is no longer read)
+#pragma unroll
+    for (int it = 0; it < TV / 2; ++it)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int r = small::prow() + u, c = small::pcol(it);
+            Ms[r][c] = r <= c ? tv[2 * it + u] : 0.0;
+        }
     __syncthreads();
     if (V32) {
 #pragma unroll
@@ -1028,11 +1032,13 @@ __global__ __launch_bounds__(small::NT) void k_v2_formy(int64_t h, int b,
     if (!Wp) return;
     __syncthreads();  // T no longer read
 #pragma unroll
-    for (int it = 0; it < TV; ++it) {
-        const int e = threadIdx.x + it * small::NT;
-        Ms[e & (RC - 1)][e >> 6] = cn[it];
-        if (gold && blockIdx.x == 0) ctop[e] = cn[it];  // e = r + 64 c
-    }
+    for (int it = 0; it < TV / 2; ++it)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int r = small::prow() + u, c = small::pcol(it);
+            Ms[r][c] = cn[2 * it + u];
+            if (gold && blockIdx.x == 0) ctop[r + PB * c] = cn[2 * it + u];
+        }
     __syncthreads();
     // W partial: tile row q of V^T Cn over the chunk's 64 rows
     {

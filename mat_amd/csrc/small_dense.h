@@ -93,10 +93,63 @@ __device__ __forceinline__ void lds_fill(Lds *S, F f) {
     tile_stash(S, v);
 }
 
+// Pair layout of a matrix in flight: thread t holds rows 2 (t & 31) + u,
+// u = 0, 1, of column (t >> 5) + 8 it as v[2 it + u], so that the two rows
+// are one 8- or 16-byte load; eight loads per thread instead of sixteen
+// keep a wave's loads of several matrices within its counter of
+// outstanding loads.
+__device__ __forceinline__ int prow() { return 2 * (int)(threadIdx.x & 31); }
+__device__ __forceinline__ int pcol(int it) { return (int)(threadIdx.x >> 5) + 8 * it; }
+
+// v = the calling thread's values of M[r, c] (leading dimension ld) in the
+// pair layout; rows r >= rows and columns c >= cols read as zero.  The
+// full block with an even leading dimension and an aligned base takes the
+// vector loads (the condition is uniform over the workgroup).
+template <typename TM>
+__device__ __forceinline__ void pair_fetch(double (&v)[TV], const TM *__restrict__ M, int64_t ld,
+                                           int64_t rows, int cols) {
+    typedef TM v2 __attribute__((ext_vector_type(2)));
+    const int r = prow();
+    const bool vec = rows >= N && cols >= N && (ld & 1) == 0 &&
+                     ((uintptr_t)M & (2 * sizeof(TM) - 1)) == 0;
+    if (vec) {
+#pragma unroll
+        for (int it = 0; it < TV / 2; ++it) {
+            const v2 x = *(const v2 *)(M + r + (int64_t)pcol(it) * ld);
+            v[2 * it] = (double)x[0];
+            v[2 * it + 1] = (double)x[1];
+        }
+    } else {
+#pragma unroll
+        for (int it = 0; it < TV / 2; ++it)
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int c = pcol(it);
+                v[2 * it + u] =
+                    (r + u < rows && c < cols) ? (double)M[r + u + (int64_t)c * ld] : 0.0;
+            }
+    }
+}
+
+__device__ __forceinline__ void pair_stash(Lds *S, const double (&v)[TV]) {
+#pragma unroll
+    for (int it = 0; it < TV / 2; ++it)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) S[prow() + u][pcol(it)] = v[2 * it + u];
+}
+
+// S = M (b x b, ld) padded with the identity to N x N
 __device__ __forceinline__ void lds_load(Lds *S, const double *__restrict__ M, int64_t ld, int b) {
-    lds_fill(S, [&](int r, int c) {
-        return (r < b && c < b) ? M[r + (int64_t)c * ld] : (r == c ? 1.0 : 0.0);
-    });
+    double v[TV];
+    pair_fetch(v, M, ld, b, b);
+#pragma unroll
+    for (int it = 0; it < TV / 2; ++it)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int r = prow() + u, c = pcol(it);
+            if (r == c && r >= b) v[2 * it + u] = 1.0;
+        }
+    pair_stash(S, v);
 }
 
 // M(0:rows, 0:cols) = S; coalesced writes, the LDS reads issued first.
