@@ -249,7 +249,8 @@ int mqr_dgesv(mqr_handle_t handle, int64_t n, int64_t nrhs, double *A, int64_t l
 /* ---- mixed precision least squares with iterative refinement ------------ */
 typedef struct {
     int max_iter;    /* refinement steps allowed (default 30) */
-    double tol;      /* stop when the backward-error estimate <= tol (default 1e-14) */
+    double tol;      /* stop when the backward-error estimate <= tol (default 1e-14);
+                        the preconditioned CGLS stage stops at tol / 16 */
     int fallback;    /* 1: on stagnation refactor in plain FP64 and solve (default 1) */
     int krylov_max_iter; /* preconditioned CGLS steps allowed once the stationary
                             refinement stalls (0: default 400; < 0: none) */

@@ -306,6 +306,15 @@ int gemm_pick_splits(int M, int N, int K, int target_blocks) {
     return s < 1 ? 1 : s;
 }
 
+int w_pick_splits(int b, int64_t nc, int64_t h, int cus) {
+    const int t = (b >= BIG && nc >= BIG) ? BIG : SMALL;
+    const int64_t tiles = ((b + t - 1) / t) * ((nc + t - 1) / t);
+    int64_t sp = (4 * (int64_t)cus + tiles - 1) / std::max<int64_t>(tiles, 1);
+    sp = std::min<int64_t>(sp, (h + 127) / 128);
+    sp = std::min<int64_t>(sp, 256);
+    return (int)std::max<int64_t>(sp, 1);
+}
+
 int gemm_f64(const GemmF64 &g, hipStream_t s) {
     if (g.M <= 0 || g.N <= 0) return MQR_SUCCESS;
     if (g.K <= 0) return MQR_ERR_INTERNAL;  // callers never ask for an empty sum

@@ -139,12 +139,17 @@ int form_y(hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv, cons
 // QR/__Householder_Blocked.h:143-212.  P (pcap doubles) receives the split-K
 // partials of W = V^T C.
 // W = V^T C (b x nc, ld b) into P[0 : b nc] on stream st; P (pcap doubles)
-// holds the split-K partials.
+// holds the split-K partials.  fill: split K for four workgroups per CU
+// (w_pick_splits; the factorisation's narrow updates), else as the other
+// GEMMs (Q formation keeps the same splits, and so the same sums, for a thin
+// and a full Q).
 int block_w(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv,
-            const double *C, int64_t ldc, int64_t nc, double *P, int64_t pcap, int kind_w) {
+            const double *C, int64_t ldc, int64_t nc, double *P, int64_t pcap, int kind_w,
+            bool fill = false) {
     // W itself (splits = 1) must fit the partial buffer; callers chunk C
     if ((int64_t)b * nc > pcap) return MQR_ERR_INTERNAL;
-    int splits = gemm_pick_splits(b, (int)nc, (int)h, target_blocks(hd));
+    int splits = fill ? w_pick_splits(b, nc, h, hd->cu_count)
+                      : gemm_pick_splits(b, (int)nc, (int)h, target_blocks(hd));
     while (splits > 1 && (int64_t)splits * b * nc > pcap) --splits;
     GemmF64 g;
     g.M = b; g.N = (int)nc; g.K = (int)h;
@@ -158,7 +163,7 @@ int block_w(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double *Vp,
 
 int apply_block(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv,
                 const double *Y, double *C, int64_t ldc, int64_t nc, double *P, int64_t pcap,
-                int kind_w = K_VTA, int kind_u = K_UPDATE) {
+                int kind_w = K_VTA, int kind_u = K_UPDATE, bool fill = false) {
     if (nc <= 0) return MQR_SUCCESS;
     // column chunks whose W (b x chunk) fits the partial buffer P, so that a
     // right-hand side wider than the matrix (least squares with many
@@ -167,7 +172,7 @@ int apply_block(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double 
     for (int64_t c0 = 0; c0 < nc; c0 += chunk) {
         const int64_t ncc = std::min<int64_t>(chunk, nc - c0);
         double *Cc = C + c0 * ldc;
-        MQR_CHECK(block_w(hd, st, h, b, Vp, ldv, Cc, ldc, ncc, P, pcap, kind_w));
+        MQR_CHECK(block_w(hd, st, h, b, Vp, ldv, Cc, ldc, ncc, P, pcap, kind_w, fill));
         GemmF64 u;
         u.M = (int)h; u.N = (int)ncc; u.K = b;
         u.A = Y; u.lda = h; u.B = P; u.ldb = b;
@@ -229,7 +234,7 @@ static GeqrfWs geqrf_carve(double *base, int64_t m, int64_t n, int ib, int ob) {
     w.Wp = take(2 * w_parts_doubles(m) + 4096 + 1024);  // [W | Gold] partials, C's top rows
     if (ob > ib) {
         w.pcap3 = std::max<int64_t>((int64_t)8 * ob * (n + ob), (int64_t)64 * ob * ob);
-        w.pcap4 = (int64_t)16 * ib * (ob + ib);
+        w.pcap4 = (int64_t)64 * ib * (ob + ib);
         w.P3 = take(w.pcap3);
         w.P4 = take(w.pcap4);
         w.G = take((int64_t)ob * ob);
@@ -452,12 +457,12 @@ int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda,
             if (near_end > rest0)
                 MQR_CHECK(apply_block(hd, s1, h, b, V, lda, Y, A + j + rest0 * lda, lda,
                                       split ? first : near_end - rest0, w.P, w.pcap,
-                                      two ? K_NEAR : K_VTA, two ? K_NEAR : K_UPDATE));
+                                      two ? K_NEAR : K_VTA, two ? K_NEAR : K_UPDATE, true));
             if (split) {
                 MQR_HIP(hipEventRecord(hd->ev_join, s1));
                 MQR_CHECK(apply_block(hd, s1, h, b, V, lda, Y, A + j + (rest0 + first) * lda,
                                       lda, near_end - rest0 - first, w.P, w.pcap,
-                                      two ? K_NEAR : K_VTA, two ? K_NEAR : K_UPDATE));
+                                      two ? K_NEAR : K_VTA, two ? K_NEAR : K_UPDATE, true));
             }
             if (!two && B) {
                 MQR_CHECK(apply_block(hd, s1, h, b, V, lda, Y, B + j, ldb, nrhs, w.P, w.pcap,
@@ -473,7 +478,7 @@ int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda,
             if (def_work || last_of_block) MQR_HIP(hipStreamWaitEvent(s2, hd->ev_fork, 0));
             if (def_work)
                 MQR_CHECK(apply_block(hd, s2, h, b, V, lda, Y, A + j + near_end * lda, lda,
-                                      def_end - near_end, w.P4, w.pcap4, K_NEAR, K_NEAR));
+                                      def_end - near_end, w.P4, w.pcap4, K_NEAR, K_NEAR, true));
             // recorded for every panel: the far stream's work through this
             // panel's deferred part, before the merged update below
             MQR_HIP(hipEventRecord(ev.far_done[it % kYRing], s2));

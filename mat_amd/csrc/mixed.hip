@@ -414,7 +414,7 @@ int apply_block_f32(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const flo
                     const float *Y32, float *C, int64_t ldc, int64_t nc, float *P, int64_t pcap,
                     int kind_w, int kind_u, int bf16_terms) {
     if (nc <= 0) return MQR_SUCCESS;
-    int sp = std::min(kMaxSplits, gemm_pick_splits(b, (int)nc, (int)h, 4 * hd->cu_count));
+    int sp = w_pick_splits(b, nc, h, hd->cu_count);
     while (sp > 1 && (int64_t)sp * b * nc > pcap) --sp;
     GemmF32 f;
     f.M = b; f.N = (int)nc; f.K = (int)h; f.ta = true;
@@ -1101,14 +1101,18 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
         MQR_LAUNCH_CHECK();
     }
     // the stationary iteration stalled (u_prec cond(A) too close to 1 or
-    // above): conjugate gradients preconditioned by the same R
+    // above): conjugate gradients preconditioned by the same R.  Its
+    // stopping test is tightened 16-fold: omega takes R for the exact
+    // factor, and a low-precision R with u_prec cond(A) above one can make
+    // ||R^{-T} A^T r|| understate the backward error by up to about
+    // 1 + u_prec cond(A) (16 covers split BF16 at cond(A) = 1e6).
     if (!converged && !fp64 && krylov > 0 && std::isfinite(best_omega)) {
         hipLaunchKernelGGL(k_copy_d, dim3(grid_for(n)), dim3(256), 0, s, n,
                            (const double *)w.xbest, x);
         hipLaunchKernelGGL(k_sq, dim3(grid_for(n)), dim3(256), 0, s, n, (const double *)x, w.xx);
         MQR_LAUNCH_CHECK();
         Range rg("gels_mixed: preconditioned CGLS");
-        MQR_CHECK(cgls(hd, m, n, A, lda, b, x, nA, tol, krylov, w, trace, &kit, &converged,
+        MQR_CHECK(cgls(hd, m, n, A, lda, b, x, nA, tol / 16.0, krylov, w, trace, &kit, &converged,
                        &omega, &rnorm, &ynorm));
         it += kit;
     }
