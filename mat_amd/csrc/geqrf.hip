@@ -262,15 +262,21 @@ namespace {
 
 // Events of one factorisation, destroyed on every return path.
 struct FactorEvents {
-    hipEvent_t far_done[kYRing] = {};  // far stream, after panel q's deferred update
+    hipEvent_t far_done[kYRing] = {};  // side stream, after panel q's deferred update
+    hipEvent_t merged_a[2] = {};       // far stream, outer block J's merged update of the
+                                       // next-but-one block (slot J & 1)
     hipEvent_t ready = nullptr;        // far stream, end of its work
     ~FactorEvents() {
         for (hipEvent_t &x : far_done)
+            if (x) (void)hipEventDestroy(x);
+        for (hipEvent_t &x : merged_a)
             if (x) (void)hipEventDestroy(x);
         if (ready) (void)hipEventDestroy(ready);
     }
     int create() {
         for (hipEvent_t &x : far_done)
+            MQR_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+        for (hipEvent_t &x : merged_a)
             MQR_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
         MQR_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
         return MQR_SUCCESS;
@@ -333,17 +339,22 @@ int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda,
     bool side_pending = false;  // the side stream's last update is not yet joined
     bool side_any = false;      // side-stream work was queued (joined at the end by ev_side)
     int64_t merged_upto = 0;    // outer blocks [0, merged_upto) have had their merged update
-    int64_t s1_synced = 0;      // the side stream has waited for the far stream up to this block
+    int64_t s1_synced = 0;      // the side stream has waited for a merged update up to this block
     // the far stream's merged update of outer block J (all its panels
-    // factored): T of the block, Y = V T^T, then [c2, n) and B.  Returns the
-    // block's column range end.
+    // factored): T of the block, Y = V T^T, then [c2, n) and B.  The
+    // next-but-one block's columns [c2, c2 + ob) go first, and
+    // ev.merged_a[J & 1] marks them: the deferred parts of block J + 1's
+    // panels (side stream) wait for those columns only, not for the rest.
     auto merge_block = [&](int64_t J) -> int {
         const int64_t c0 = J * ob, bo = std::min<int64_t>(ob, k - c0);
         const int64_t h = m - c0;
         const int64_t c2 = std::min(n, c0 + 2 * ob);
         const double *V = A + c0 + c0 * lda;
         const bool far = c2 < n;
-        if (!far && !B && nbt <= kMaxNb) return MQR_SUCCESS;
+        if (!far && !B && nbt <= kMaxNb) {
+            MQR_HIP(hipEventRecord(ev.merged_a[J & 1], s2));
+            return MQR_SUCCESS;
+        }
         const double *Tj = T + c0 * ldt;
         int64_t ldtj = ldt;
         if (bo > ib) {
@@ -354,13 +365,23 @@ int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda,
         }
         if (far || B) {
             MQR_CHECK(form_y(s2, h, (int)bo, V, lda, Tj, ldtj, true, w.Yo));
-            if (far)
+            if (far) {
+                const int64_t na = std::min<int64_t>(ob, n - c2);
                 MQR_CHECK(apply_block(hd, s2, h, (int)bo, V, lda, w.Yo, A + c0 + c2 * lda, lda,
-                                      n - c2, w.P3, w.pcap3));
+                                      na, w.P3, w.pcap3));
+                MQR_HIP(hipEventRecord(ev.merged_a[J & 1], s2));
+                if (n - c2 > na)
+                    MQR_CHECK(apply_block(hd, s2, h, (int)bo, V, lda, w.Yo,
+                                          A + c0 + (c2 + na) * lda, lda, n - c2 - na, w.P3,
+                                          w.pcap3));
+            } else {
+                MQR_HIP(hipEventRecord(ev.merged_a[J & 1], s2));
+            }
             if (B)
                 MQR_CHECK(apply_block(hd, s2, h, (int)bo, V, lda, w.Yo, B + c0, ldb, nrhs, w.P3,
                                       w.pcap3, K_APPLYQ, K_APPLYQ));
         }
+        if (!(far || B)) MQR_HIP(hipEventRecord(ev.merged_a[J & 1], s2));
         // nb > 64: T returns the merged nb x nb blocks
         if (nbt > kMaxNb && bo > ib)
             MQR_HIP(hipMemcpy2DAsync(T + c0 * ldt, sizeof(double) * ldt, w.Tb,
@@ -428,28 +449,19 @@ int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda,
             MQR_HIP(hipStreamWaitEvent(s0, hd->ev_join, 0));
             side_pending = false;
         }
-        // the far stream's deferred parts of the previous panels reach the
+        // the side stream's deferred parts of the previous panels reach the
         // next panel's columns when the next panel starts a new outer block
-        // (the previous block's merged update and this block's deferred
-        // parts), and when this panel is the first of its block (the
-        // previous block's last deferred part)
+        // (this block's deferred parts, after the previous block's merged
+        // update of these columns), and when this panel is the first of its
+        // block (the previous block's last deferred part)
         if (two && it > 0 && (j % ob == 0 || next >= c1))
             MQR_HIP(hipStreamWaitEvent(s0, ev.far_done[(it - 1) % kYRing], 0));
         const bool side_work = near_end > rest0 || (!two && B);
         const bool def_work = two && def_end > near_end;
         if (side_work || def_work || (two && last_of_block))
             MQR_HIP(hipEventRecord(hd->ev_fork, s0));
-        if (side_work) {
+        if (side_work || def_work) {
             MQR_HIP(hipStreamWaitEvent(s1, hd->ev_fork, 0));
-            // the first near update of an outer block: its columns had the
-            // previous block's deferred parts on the far stream (the last
-            // of them recorded for the previous block's last panel)
-            const int64_t blk = j / ob;
-            if (two && blk > 0 && s1_synced < blk) {
-                const int64_t last_prev = blk * (ob / ib) - 1;
-                MQR_HIP(hipStreamWaitEvent(s1, ev.far_done[last_prev % kYRing], 0));
-                s1_synced = blk;
-            }
             // the panel after next (its first ib columns) first: the handle's
             // stream waits for those (ev_join) and not for the rest (ev_side)
             const int64_t first = std::min<int64_t>(ib, near_end - rest0);
@@ -470,19 +482,28 @@ int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda,
                 b_next = j + b;
             }
             if (!split) MQR_HIP(hipEventRecord(hd->ev_join, s1));
+            // the deferred part: the next outer block's columns, which had
+            // the previous block's merged update first (its first part,
+            // ev.merged_a; the block before has no merged update)
+            if (def_work) {
+                const int64_t blk = j / ob;
+                if (blk > 0 && s1_synced < blk) {
+                    MQR_HIP(hipStreamWaitEvent(s1, ev.merged_a[(blk - 1) & 1], 0));
+                    s1_synced = blk;
+                }
+                MQR_CHECK(apply_block(hd, s1, h, b, V, lda, Y, A + j + near_end * lda, lda,
+                                      def_end - near_end, w.P4, w.pcap4, K_NEAR, K_NEAR, true));
+            }
             MQR_HIP(hipEventRecord(hd->ev_side, s1));
             side_pending = true;
             side_any = true;
         }
         if (two) {
-            if (def_work || last_of_block) MQR_HIP(hipStreamWaitEvent(s2, hd->ev_fork, 0));
-            if (def_work)
-                MQR_CHECK(apply_block(hd, s2, h, b, V, lda, Y, A + j + near_end * lda, lda,
-                                      def_end - near_end, w.P4, w.pcap4, K_NEAR, K_NEAR, true));
-            // recorded for every panel: the far stream's work through this
-            // panel's deferred part, before the merged update below
-            MQR_HIP(hipEventRecord(ev.far_done[it % kYRing], s2));
+            // recorded for every panel: the side stream's work through this
+            // panel's deferred part
+            MQR_HIP(hipEventRecord(ev.far_done[it % kYRing], s1));
             if (last_of_block) {
+                MQR_HIP(hipStreamWaitEvent(s2, hd->ev_fork, 0));
                 MQR_CHECK(merge_block(j / ob));
                 merged_upto = j / ob + 1;
             }
