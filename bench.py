@@ -95,13 +95,15 @@ def backward_errors(P, x):
 # (C -= Y W), which carries half of the factorisation's flops and, with
 # W = V^T C, all of its MFMA work.  Its launches are timed with HIP events on
 # their own stream inside the timed region; no other kernel is, so the
-# stream's dispatch gaps stay as they are.
+# stream's dispatch gaps stay as they are.  The bulk launches count: the
+# narrow ones that go first (the next panel's 64 columns, or the next outer
+# block's 256 in the FP64 far update) are timed as look-ahead.
 ROOF_KIND = "update_gemm"
 # HBM bytes moved per algorithmic byte by the FP64 trailing update (the
 # K = 256 update of each 256-column outer block), from two rocprofv3 --pmc
 # runs (FETCH_SIZE, WRITE_SIZE) scaled by a 537 MB device copy in the same
 # run (profiles/mfma_hbm_counters_ls16384x4096.md).
-TRAFFIC_RATIO_FP64_UPDATE = 1.639
+TRAFFIC_RATIO_FP64_UPDATE = 1.575
 # The same for the FP32 trailing update of the mixed solve (K = 64).
 TRAFFIC_RATIO_FP32_UPDATE = 1.125
 

@@ -368,7 +368,7 @@ int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda,
             if (far) {
                 const int64_t na = std::min<int64_t>(ob, n - c2);
                 MQR_CHECK(apply_block(hd, s2, h, (int)bo, V, lda, w.Yo, A + c0 + c2 * lda, lda,
-                                      na, w.P3, w.pcap3));
+                                      na, w.P3, w.pcap3, K_LOOKAHEAD, K_LOOKAHEAD));
                 MQR_HIP(hipEventRecord(ev.merged_a[J & 1], s2));
                 if (n - c2 > na)
                     MQR_CHECK(apply_block(hd, s2, h, (int)bo, V, lda, w.Yo,
@@ -466,10 +466,12 @@ int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda,
             // stream waits for those (ev_join) and not for the rest (ev_side)
             const int64_t first = std::min<int64_t>(ib, near_end - rest0);
             const bool split = near_end - rest0 > first;
+            // (profiled as look-ahead: the bulk update is the roofline kernel)
             if (near_end > rest0)
                 MQR_CHECK(apply_block(hd, s1, h, b, V, lda, Y, A + j + rest0 * lda, lda,
                                       split ? first : near_end - rest0, w.P, w.pcap,
-                                      two ? K_NEAR : K_VTA, two ? K_NEAR : K_UPDATE, true));
+                                      two ? K_NEAR : K_LOOKAHEAD, two ? K_NEAR : K_LOOKAHEAD,
+                                      true));
             if (split) {
                 MQR_HIP(hipEventRecord(hd->ev_join, s1));
                 MQR_CHECK(apply_block(hd, s1, h, b, V, lda, Y, A + j + (rest0 + first) * lda,

@@ -614,9 +614,11 @@ int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t 
             // stream waits for those (ev_join) and not for the rest (ev_side)
             const int64_t first = std::min<int64_t>(nb, near_end - rest0);
             const bool split = near_end - rest0 > first;
+            // (profiled as look-ahead: the bulk update is the roofline kernel)
             MQR_CHECK(apply_block_f32(hd, s1, h, bj, V32, Y32, w.Aw + j + rest0 * m, m,
                                       split ? first : near_end - rest0, w.P32, pcap,
-                                      two ? K_NEAR : K_VTA, two ? K_NEAR : K_UPDATE, bf16_terms));
+                                      two ? K_NEAR : K_LOOKAHEAD, two ? K_NEAR : K_LOOKAHEAD,
+                                      bf16_terms));
             MQR_HIP(hipEventRecord(hd->ev_join, s1));
             if (split)
                 MQR_CHECK(apply_block_f32(hd, s1, h, bj, V32, Y32, w.Aw + j + (rest0 + first) * m,
