@@ -644,7 +644,7 @@ int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t 
             if (g_ready) {
                 MQR_CHECK(cholqr_fused_a(s0, m - next, w.Yd, h, w.Wp, ctop, w.Aw + j + next * m,
                                          m, true, Vn, m, w.tau + next, w.T + next * nb, nb,
-                                         w.pws));
+                                         w.pws, true));
             } else {
                 MQR_CHECK(cholqr_gram1(s0, h, (int)bn, bj, w.Yd, h, w.Wp, 64,
                                        w.Aw + j + next * m, m, bj, w.pws, true, Vn, m));

@@ -278,9 +278,13 @@ int cholqr_finish(hipStream_t s, int64_t h, int b, double *X, int64_t ld, const 
 // Gold - R12^T R12 (C: the panel's columns from the previous panel's top
 // row, hc = h + 64 rows; X = the panel, C + 64 for FP64 C, its FP64 copy
 // for FP32 C).
+// one_pass: the caller tolerates an orthonormal factor accurate to about
+// 1e-9 (the mixed-precision factorisation, whose FP32 updates perturb it by
+// more); the second CholeskyQR pass is then skipped, on the device, for
+// panels whose first factor predicts that accuracy.
 int cholqr_fused_a(hipStream_t s, int64_t h, const double *Y, int64_t ldy, const double *WG,
                    const double *Ctop, void *C, int64_t ldc, bool c_f32, double *X, int64_t ldx,
-                   double *tau, double *T, int64_t ldt, void *ws);
+                   double *tau, double *T, int64_t ldt, void *ws, bool one_pass = false);
 int cholqr_rest(hipStream_t s, int64_t h, int b, double *X, int64_t ld, double *tau, double *T,
                 int64_t ldt, double *Yf, void *ws);
 // Blocked FP64 QR of A (m x n, lda) in place with tau and the T blocks (ldt);
@@ -318,8 +322,9 @@ int merged_t_f64(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double
 int upper_inverse_levels(mqr_handle_t hd, hipStream_t s, int64_t n, const double *R, int64_t ldr,
                          double *X, double *tmp);
 // out[0:count] = sum over s < splits of P[s*stride + 0:count]; out = P when null.
+// With skip, the launch does nothing when *skip is nonzero on the device.
 int reduce_partials(int64_t count, double *P, int splits, int64_t stride, hipStream_t s,
-                    double *out = nullptr);
+                    double *out = nullptr, const int *skip = nullptr);
 // Largest panel width the LDS-resident small kernels accept.
 constexpr int kMaxNb = 64;
 // Largest block size mqr_set_block_size accepts: above kMaxNb it is the

@@ -41,8 +41,9 @@ int panel_householder(int64_t h, int b, double *P, int64_t ld, double *tau, doub
 // and many partials (a 64 x 64 Gram matrix from 128 row chunks).
 __global__ __launch_bounds__(256) void k_reduce_partials(int64_t count, const double *P,
                                                          int splits, int64_t stride, int G,
-                                                         double *out) {
+                                                         double *out, const int *skip) {
     __shared__ double part[256];
+    if (skip && *skip) return;  // uniform: the producer found the sum not needed
     const int E = 256 / G;
     const int t = threadIdx.x, g = t / E;
     const int64_t e = (int64_t)blockIdx.x * E + (t % E);
@@ -65,7 +66,7 @@ __global__ __launch_bounds__(256) void k_reduce_partials(int64_t count, const do
 }
 
 int reduce_partials(int64_t count, double *P, int splits, int64_t stride, hipStream_t s,
-                    double *out) {
+                    double *out, const int *skip) {
     if (!out) out = P;
     if (count <= 0 || (splits <= 1 && out == P)) return MQR_SUCCESS;
     ProfScope ps(K_OTHER, (double)count * (splits - 1), 8.0 * count * (splits + 1), s);
@@ -74,7 +75,7 @@ int reduce_partials(int64_t count, double *P, int splits, int64_t stride, hipStr
     while (G < 16 && G * 2 <= splits && (count * G) / 256 < 512) G *= 2;
     const int E = 256 / G;
     hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)((count + E - 1) / E)), dim3(256), 0, s,
-                       count, (const double *)P, splits, stride, G, out);
+                       count, (const double *)P, splits, stride, G, out, skip);
     MQR_LAUNCH_CHECK();
     return MQR_SUCCESS;
 }

@@ -23,6 +23,11 @@
 // instead (panel_hh.h), which also keeps the reference's tau = 0 for zero
 // columns.
 //
+// The mixed-precision factorisation, whose FP32 trailing updates perturb the
+// factors by about 6e-8 anyway, lets the first pass decide on the device to
+// skip the second when the first Cholesky factor predicts Z orthonormal to
+// 1e-9 (k_upd_apply); the FP64 factorisation always takes both passes.
+//
 // The b x b factorisations run register-resident in one workgroup
 // (small_dense.h); every barrier is executed by all of its waves.  The
 // compact-WY factor T comes out of the reconstruction as well.
@@ -62,6 +67,10 @@ constexpr int kFbCounters = 128;  // barrier counters of the breakdown fallback
 // fails here, the untouched h x b panel A is factored column by column in
 // this same workgroup (the reference's panel loop), with the flag left up
 // for the V2 kernel; no separate fallback launch.
+//
+// With one_pass and *one_pass set (by k_upd_apply, when its first factor
+// predicts Z orthonormal to 1e-9), Z itself is the orthonormal factor:
+// R2 = I, so Q1 = Z_top, R = S R1 and Minv = U^{-1}, and G is not read.
 __global__ __launch_bounds__(small::NT) void k_recon(int64_t h, int b,
                                                      const double *__restrict__ G, int64_t ldg,
                                                      const double *__restrict__ Z, int64_t ldz,
@@ -71,7 +80,8 @@ __global__ __launch_bounds__(small::NT) void k_recon(int64_t h, int b,
                                                      double *__restrict__ T, int64_t ldt,
                                                      double *__restrict__ Minv,
                                                      double *__restrict__ stash,
-                                                     int *__restrict__ flag) {
+                                                     int *__restrict__ flag,
+                                                     const int *__restrict__ one_pass) {
     __shared__ small::Lds P[PB], Q[PB];
     __shared__ double sg[PB];
     __shared__ int fail;
@@ -89,85 +99,108 @@ __global__ __launch_bounds__(small::NT) void k_recon(int64_t h, int b,
     }
     const int tid = threadIdx.x, q = small::wave_id();
     const int l = small::lane_id(), li = l & 15, kq = l >> 4, arow = 16 * q + li;
-    if (tid == 0) fail = 0;
-    small::lds_load(P, G, ldg, b);
-    __syncthreads();
-    // R2 = chol(G2) -> P, R2^{-1} -> Q
-    sb::chol_inv(P, Q, b, [](int) { return 1e-2; }, &fail);
-    if (fail) {  // uniform: read after the barrier that ends chol_inv
-        if (tid == 0) *flag = 1;
-        __syncthreads();  // the LDS buffers are reused by the fallback
-        hh::panel_hh_body(h, b, A, lda, tau, T, ldt, fb);
-        return;
-    }
-    // tile row q of R2 as MFMA operands; R2^{-1}'s to the stash
-    double r2[4][4];
+    sb::v4d rr[4];  // tile row q of R2 R1
+    if (one_pass && *one_pass) {
+        // Q1 = Z_top -> P (padded with I), R2 R1 = R1
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+        for (int jb = 0; jb < 4; ++jb)
 #pragma unroll
-        for (int k4 = 0; k4 < 4; ++k4) {
-            const int c = 16 * k + 4 * k4 + kq;
-            r2[k][k4] = P[arow][c];
-            stash[arow + (int64_t)c * PB] = Q[arow][c];
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * q + sb::trow(r), c = 16 * jb + sb::tcol();
+                P[i][c] = (i < b && c < b) ? Z[i + (int64_t)c * ldz] : (i == c ? 1.0 : 0.0);
+                rr[jb][r] = R1[i + (int64_t)c * PB];
+            }
+        __syncthreads();
+        // LU of Q1 - S -> P; U^{-1} -> Q, which is Minv
+        sb::lu_sign_invu(P, Q, sg);
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * q + sb::trow(r), c = 16 * jb + sb::tcol();
+                Minv[i + (int64_t)c * PB] = i <= c ? Q[i][c] : 0.0;
+            }
+        __syncthreads();  // U^{-1} no longer read
+    } else {
+        if (tid == 0) fail = 0;
+        small::lds_load(P, G, ldg, b);
+        __syncthreads();
+        // R2 = chol(G2) -> P, R2^{-1} -> Q
+        sb::chol_inv(P, Q, b, [](int) { return 1e-2; }, &fail);
+        if (fail) {  // uniform: read after the barrier that ends chol_inv
+            if (tid == 0) *flag = 1;
+            __syncthreads();  // the LDS buffers are reused by the fallback
+            hh::panel_hh_body(h, b, A, lda, tau, T, ldt, fb);
+            return;
         }
-    // Q1 = Z_top R2^{-1} -> tile row q of P (Z_top padded with I like G2)
-    {
-        double z[4][4];
+        // tile row q of R2 as MFMA operands; R2^{-1}'s to the stash
+        double r2[4][4];
 #pragma unroll
         for (int k = 0; k < 4; ++k)
 #pragma unroll
             for (int k4 = 0; k4 < 4; ++k4) {
                 const int c = 16 * k + 4 * k4 + kq;
-                z[k][k4] = (arow < b && c < b) ? Z[arow + (int64_t)c * ldz] : (arow == c ? 1.0 : 0.0);
+                r2[k][k4] = P[arow][c];
+                stash[arow + (int64_t)c * PB] = Q[arow][c];
             }
-        sb::v4d acc[4];
-#pragma unroll
-        for (int jb = 0; jb < 4; ++jb) {
-            acc[jb] = sb::zero4();
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (k <= jb) acc[jb] = sb::tmma_reg(acc[jb], z[k], Q, k, jb);
-        }
-        // this wave's reads of its tile row of P are done before it writes it
-        __builtin_amdgcn_wave_barrier();
-#pragma unroll
-        for (int jb = 0; jb < 4; ++jb) sb::tstore(P, q, jb, acc[jb]);
-    }
-    __syncthreads();  // R2^{-1} no longer read
-    // RR = R2 R1, kept in registers (tile row q)
-    small::lds_load(Q, R1, PB, PB);
-    __syncthreads();
-    sb::v4d rr[4];
-#pragma unroll
-    for (int jb = 0; jb < 4; ++jb) {
-        rr[jb] = sb::zero4();
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (k >= q && k <= jb) rr[jb] = sb::tmma_reg(rr[jb], r2[k], Q, k, jb);
-    }
-    __syncthreads();  // R1 no longer read
-    // LU of Q1 - S -> P; U^{-1} -> Q
-    sb::lu_sign_invu(P, Q, sg);
-    // Minv = R2^{-1} U^{-1} straight to global, R2^{-1} back from the stash
-    {
-        double ri[4][4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-#pragma unroll
-            for (int k4 = 0; k4 < 4; ++k4)
-                ri[k][k4] = stash[arow + (int64_t)(16 * k + 4 * k4 + kq) * PB];
-#pragma unroll
-        for (int jb = 0; jb < 4; ++jb) {
-            sb::v4d acc = sb::zero4();
+        // Q1 = Z_top R2^{-1} -> tile row q of P (Z_top padded with I like G2)
+        {
+            double z[4][4];
 #pragma unroll
             for (int k = 0; k < 4; ++k)
-                if (k >= q && k <= jb) acc = sb::tmma_reg(acc, ri[k], Q, k, jb);
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-                Minv[16 * q + sb::trow(r) + (int64_t)(16 * jb + sb::tcol()) * PB] = acc[r];
+                for (int k4 = 0; k4 < 4; ++k4) {
+                    const int c = 16 * k + 4 * k4 + kq;
+                    z[k][k4] = (arow < b && c < b) ? Z[arow + (int64_t)c * ldz] : (arow == c ? 1.0 : 0.0);
+                }
+            sb::v4d acc[4];
+#pragma unroll
+            for (int jb = 0; jb < 4; ++jb) {
+                acc[jb] = sb::zero4();
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (k <= jb) acc[jb] = sb::tmma_reg(acc[jb], z[k], Q, k, jb);
+            }
+            // this wave's reads of its tile row of P are done before it writes it
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int jb = 0; jb < 4; ++jb) sb::tstore(P, q, jb, acc[jb]);
         }
+        __syncthreads();  // R2^{-1} no longer read
+        // RR = R2 R1, kept in registers (tile row q)
+        small::lds_load(Q, R1, PB, PB);
+        __syncthreads();
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb) {
+            rr[jb] = sb::zero4();
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (k >= q && k <= jb) rr[jb] = sb::tmma_reg(rr[jb], r2[k], Q, k, jb);
+        }
+        __syncthreads();  // R1 no longer read
+        // LU of Q1 - S -> P; U^{-1} -> Q
+        sb::lu_sign_invu(P, Q, sg);
+        // Minv = R2^{-1} U^{-1} straight to global, R2^{-1} back from the stash
+        {
+            double ri[4][4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int k4 = 0; k4 < 4; ++k4)
+                    ri[k][k4] = stash[arow + (int64_t)(16 * k + 4 * k4 + kq) * PB];
+#pragma unroll
+            for (int jb = 0; jb < 4; ++jb) {
+                sb::v4d acc = sb::zero4();
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (k >= q && k <= jb) acc = sb::tmma_reg(acc, ri[k], Q, k, jb);
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    Minv[16 * q + sb::trow(r) + (int64_t)(16 * jb + sb::tcol()) * PB] = acc[r];
+            }
+        }
+        __syncthreads();  // U^{-1} no longer read
     }
-    __syncthreads();  // U^{-1} no longer read
     // L^{-1} -> Q
     sb::inv_unit_lower(P, Q);
     // T = -(U S) L^{-T} (upper), R = S (R2 R1) with V1 strictly below
@@ -715,7 +748,9 @@ __global__ __launch_bounds__(small::NT) void k_apply_gram(int64_t h, int b,
 // C's top rows are read from Ctop (their copy, ld 64), since workgroup 0
 // overwrites them with Xt.
 // Writes: Xt to C's top rows (TC), X to the panel X (FP64, ldx; for an FP64
-// C, X = C + 64), R1, Z and the Gram partials of Z, the flag.  Breakdown:
+// C, X = C + 64), R1, Z and the Gram partials of Z, the flag, and flag[1]:
+// with one_pass_ok, 1 when Z needs no second pass (then no partials are
+// written; see the test after the Cholesky), else 0.  Breakdown:
 // all workgroups store X, meet at a grid barrier and take the grid
 // fallback of k_apply_gram; a cancelled barrier leaves the flag at 1 for
 // k_recon.
@@ -733,9 +768,10 @@ __global__ __launch_bounds__(small::NT) void k_upd_apply(int64_t h, const double
                                                          int *__restrict__ flag,
                                                          double *__restrict__ tau,
                                                          double *__restrict__ T, int64_t ldt,
-                                                         int *__restrict__ bar) {
+                                                         int *__restrict__ bar, int one_pass_ok) {
     __shared__ small::Lds As[RC], Bs[PB];
     __shared__ double thr[PB];
+    __shared__ double red[3][small::NT / 64];
     __shared__ int fail;
     __shared__ FbLds fbl;
     constexpr int b = PB;
@@ -805,6 +841,7 @@ __global__ __launch_bounds__(small::NT) void k_upd_apply(int64_t h, const double
                 if (row < h) X[row + (int64_t)(16 * jb + lcol(r)) * ldx] = xc[jb][r];
     }
     // G1 = Gold - Xt^T Xt -> As (upper tiles; the strictly lower ones zero)
+    double gtr = 0.0;  // this thread's share of trace(Gold)
     __syncthreads();
 #pragma unroll
     for (int jb = 0; jb < 4; ++jb) tstoreT(As, q, jb, xt[jb]);
@@ -823,6 +860,7 @@ __global__ __launch_bounds__(small::NT) void k_upd_apply(int64_t h, const double
                 go[u][r] = t < 10 ? Gold[16 * i + lrow() + (16 * j + lcol(r)) * PB] : 0.0;
         }
         const double gdiag = tid < PB ? Gold[tid * (PB + 1)] : 0.0;
+        gtr = gdiag;
         sb::v4d g[3] = {sb::zero4(), sb::zero4(), sb::zero4()};
         gram_acc(As, g);
         if (tid < PB) thr[tid] = 1e-10 * gdiag;
@@ -852,9 +890,35 @@ __global__ __launch_bounds__(small::NT) void k_upd_apply(int64_t h, const double
     __syncthreads();
     // R1 -> As, R1^{-1} -> Bs; ends with a barrier
     sb::chol_inv(As, Bs, b, [&](int i) { return thr[i]; }, &fail);
+    // One pass suffices when Z = X R1^{-1} is predicted orthonormal to 1e-9:
+    // G1's rounding, at most about u (tr Gold + tr G1) with the cancellation
+    // of Gold - Xt^T Xt, and CholeskyQR's own u kappa^2 term are both
+    // bounded by u |R1^{-1}|_F^2 (|R1|_F^2 + 2 tr Gold) (Frobenius norms
+    // over-estimate the 2-norms).  Every workgroup sums the same values in
+    // the same order, so all reach the same decision.
+    bool one = false;
+    if (one_pass_ok && !fail) {
+        double sr = 0.0, si = 0.0;
+        for (int e = tid; e < PB * PB; e += small::NT) {
+            const int i = e / PB, j = e % PB;
+            if (i <= j) {
+                sr = fma(As[i][j], As[i][j], sr);
+                si = fma(Bs[i][j], Bs[i][j], si);
+            }
+        }
+        sr = small::wave_sum(sr);
+        si = small::wave_sum(si);
+        const double gt = small::wave_sum(gtr);
+        if (l == 0) red[0][q] = sr, red[1][q] = si, red[2][q] = gt;
+        __syncthreads();
+        double a = 0.0, c = 0.0, t = 0.0;
+#pragma unroll
+        for (int w = 0; w < small::NT / 64; ++w) a += red[0][w], c += red[1][w], t += red[2][w];
+        one = 0x1p-53 * c * (a + 2.0 * t) <= 1e-9;  // false for NaN
+    }
     if (blockIdx.x == 0) {
         small::lds_store(As, R1, PB, PB, PB);
-        if (tid == 0) *flag = fail;
+        if (tid == 0) *flag = fail, flag[1] = one ? 1 : 0;
     }
     if (fail) {  // uniform: written before chol_inv's last barrier
         // every workgroup's rows of X are stored before the fallback reads them
@@ -882,6 +946,7 @@ __global__ __launch_bounds__(small::NT) void k_upd_apply(int64_t h, const double
 #pragma unroll
         for (int r = 0; r < 4; ++r)
             if (row < h) Z[row + (int64_t)(16 * jb + lcol(r)) * ldz] = acc[jb][r];
+    if (one) return;  // uniform: no second pass, so no Gram partial of Z
     __syncthreads();
 #pragma unroll
     for (int jb = 0; jb < 4; ++jb) tstoreT(As, q, jb, row < h ? acc[jb] : sb::zero4());
@@ -1148,7 +1213,7 @@ int cholqr_rest_a(hipStream_t s, int64_t h, int b, double *X, int64_t ld, double
         ProfScope ps(K_PANEL_SMALL, 8.0 * bb * bb * bb / 3.0, 8.0 * bb * bb * 6.0, s);
         hipLaunchKernelGGL(k_recon, dim3(1), dim3(small::NT), 0, s, h, b, (const double *)w.Gp,
                            (int64_t)PB, (const double *)w.Z, h, (const double *)w.R1, X, ld, tau,
-                           T, ldt, w.Mi, w.G1, w.flag);
+                           T, ldt, w.Mi, w.G1, w.flag, nullptr);
         MQR_LAUNCH_CHECK();
     }
     return MQR_SUCCESS;
@@ -1197,7 +1262,7 @@ int cholqr_finish(hipStream_t s, int64_t h, int b, double *X, int64_t ld, const 
 // follows as after cholqr_rest_a.  Needs h <= 256 x 64.
 int cholqr_fused_a(hipStream_t s, int64_t h, const double *Y, int64_t ldy, const double *WG,
                    const double *Ctop, void *C, int64_t ldc, bool c_f32, double *X, int64_t ldx,
-                   double *tau, double *T, int64_t ldt, void *ws) {
+                   double *tau, double *T, int64_t ldt, void *ws, bool one_pass) {
     const CqrWs w = carve(ws, h);
     const int parts = (int)((h + RC - 1) / RC);
     if (h <= 0 || parts > kMaxGramParts) return MQR_ERR_INTERNAL;
@@ -1208,19 +1273,21 @@ int cholqr_fused_a(hipStream_t s, int64_t h, const double *Y, int64_t ldy, const
         if (c_f32)
             hipLaunchKernelGGL(k_upd_apply<float>, dim3(parts), dim3(small::NT), 0, s, h, Y, ldy,
                                WG, Ctop, (float *)C, ldc, X, ldx, w.G1, w.R1, w.Z, h, w.Gp, w.flag, tau,
-                               T, ldt, w.bar);
+                               T, ldt, w.bar, (int)one_pass);
         else
             hipLaunchKernelGGL(k_upd_apply<double>, dim3(parts), dim3(small::NT), 0, s, h, Y, ldy,
                                WG, Ctop, (double *)C, ldc, X, ldx, w.G1, w.R1, w.Z, h, w.Gp, w.flag, tau,
-                               T, ldt, w.bar);
+                               T, ldt, w.bar, (int)one_pass);
         MQR_LAUNCH_CHECK();
     }
-    MQR_CHECK(reduce_partials((int64_t)PB * PB, w.Gp, parts, (int64_t)PB * PB, s));
+    // flag[1]: k_upd_apply's decision that the second pass is not needed
+    const int *skip = one_pass ? w.flag + 1 : nullptr;
+    MQR_CHECK(reduce_partials((int64_t)PB * PB, w.Gp, parts, (int64_t)PB * PB, s, nullptr, skip));
     {
         ProfScope ps(K_PANEL_SMALL, 8.0 * bb * bb * bb / 3.0, 8.0 * bb * bb * 6.0, s);
         hipLaunchKernelGGL(k_recon, dim3(1), dim3(small::NT), 0, s, h, (int)PB,
                            (const double *)w.Gp, (int64_t)PB, (const double *)w.Z, h,
-                           (const double *)w.R1, X, ldx, tau, T, ldt, w.Mi, w.G1, w.flag);
+                           (const double *)w.R1, X, ldx, tau, T, ldt, w.Mi, w.G1, w.flag, skip);
         MQR_LAUNCH_CHECK();
     }
     return MQR_SUCCESS;

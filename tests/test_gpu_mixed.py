@@ -137,6 +137,57 @@ def test_mixed_fp32_reaches_fp64_backward_error(h, orc, m, n, cond, resid):
     assert np.linalg.norm(x - xo) <= tol * np.linalg.norm(xo)
 
 
+def one_pass_estimates(A, nb=64):
+    """The estimate on which the mixed factorisation skips the second
+    CholeskyQR pass of a 64-column panel j >= 1 (one pass when <= 1e-9):
+    u ||R1^{-1}||_F^2 (||R1||_F^2 + 2 tr Gold), where R1 is the panel's R
+    factor after the earlier panels' reflectors (the diagonal block R_jj of
+    A's R) and Gold = C^T C of the panel's columns over the previous panel's
+    frame (the rows from the previous panel's top: R[j-1:j+1, j]^T R[j-1:j+1, j]
+    in 64-blocks).  Computed here from numpy's QR of A, up to rounding."""
+    R = np.linalg.qr(A, mode="r")
+    n = A.shape[1]
+    est = []
+    for j in range(nb, n - nb + 1, nb):
+        R1 = R[j:j + nb, j:j + nb]
+        gold = np.sum(R[j - nb:j + nb, j:j + nb] ** 2)
+        ri = np.linalg.inv(R1)
+        est.append(2.0 ** -53 * np.sum(ri ** 2) * (np.sum(R1 ** 2) + 2.0 * gold))
+    return np.array(est)
+
+
+@pytest.mark.parametrize("cond,panel_cond", [(1e3, 1.0), (10.0, 1e4)])
+def test_mixed_one_and_two_pass_panels(h, orc, cond, panel_cond):
+    """Well-conditioned panels take one CholeskyQR pass in the mixed
+    factorisation; panels whose columns are scaled over panel_cond = 1e4 take
+    the second pass.  The construction lands on the intended side of the
+    device's test (checked on the CPU), and both converge to the FP64
+    backward-error level without the fallback."""
+    m, n = 3000, 640
+    note(f"gels_mixed fp32 panel scaling {panel_cond:g}")
+    A, _ = ls_problem(m, n, cond, seed=77)
+    A = np.asfortranarray(A * np.tile(np.logspace(0, -np.log10(panel_cond), 64), n // 64))
+    rng = np.random.default_rng(78)
+    b0 = A @ rng.standard_normal(n)
+    e = rng.standard_normal(m)
+    b = b0 + 1e-10 * np.linalg.norm(b0) * e / np.linalg.norm(e)
+    est = one_pass_estimates(A)
+    if panel_cond > 1.0:
+        assert est.min() > 1e-7, est
+    else:
+        assert est.max() < 1e-10, est
+    x, info = run_mixed(h, A, b)
+    note(f"  iterations={info.iterations} converged={info.converged} "
+         f"fell_back={info.fell_back} omega={info.backward_error:.3e}")
+    assert info.converged == 1 and info.fell_back == 0
+    be = ls_backward_error(A, b, x)
+    assert be["kw"] <= 1e-14, be
+    xo = orc.gels(A, b)
+    cond = np.linalg.cond(A)
+    tol = 50 * np.sqrt(n) * cond * EPS * (1.0 + cond * 1e-10)
+    assert np.linalg.norm(x - xo) <= tol * np.linalg.norm(xo)
+
+
 def test_mixed_fp64_refinement(h, orc):
     A, b = ls_problem(1200, 400, 1e8, seed=3, resid=1e-8)
     x, info = run_mixed(h, A, b, prec="fp64")
