@@ -86,14 +86,29 @@ __global__ __launch_bounds__(256) void k_trsv_inv_block(int64_t i0, int bs,
                                                         double *__restrict__ xout) {
     __shared__ double cs[TRB], xs[TRB], part[4][TRB];
     const int tid = threadIdx.x, row = tid & (TRB - 1), qt = tid >> 6;
+    const int64_t r = (int64_t)blockIdx.x * TRB + row;
+    // every load first (c, the inverse's row piece and this thread's piece of
+    // R's rows above the block), so they share one memory round trip
+    double xv[16], rv[16];
+    {
+        const double *Xq = Xd + i0 * TRB;  // block q's inverse, ld TRB
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int k = qt * 16 + u;
+            xv[u] = (row < bs && k >= row && k < bs) ? Xq[row + (int64_t)k * TRB] : 0.0;
+            rv[u] = (r < i0 && k < bs) ? R[r + (i0 + k) * ldr] : 0.0;
+        }
+    }
     if (tid < TRB) cs[tid] = tid < bs ? c[i0 + tid] : 0.0;
     __syncthreads();
     {
-        const double *Xq = Xd + i0 * TRB;  // block q's inverse, ld TRB
         double acc = 0.0;
         if (row < bs)
-            for (int k = qt * 16; k < qt * 16 + 16; ++k)
-                if (k >= row && k < bs) acc = fma(Xq[row + (int64_t)k * TRB], cs[k], acc);
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int k = qt * 16 + u;
+                if (k >= row && k < bs) acc = fma(xv[u], cs[k], acc);
+            }
         part[qt][row] = acc;
     }
     __syncthreads();
@@ -105,11 +120,13 @@ __global__ __launch_bounds__(256) void k_trsv_inv_block(int64_t i0, int bs,
     __syncthreads();
     // rows [64 b, 64 b + 64) above the block: four threads per row, each a
     // quarter of the block's columns, partial sums added in a fixed order
-    const int64_t r = (int64_t)blockIdx.x * TRB + row;
     double acc = 0.0;
     if (r < i0)
-        for (int j = qt * 16; j < qt * 16 + 16; ++j)
-            if (j < bs) acc = fma(R[r + (i0 + j) * ldr], xs[j], acc);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int j = qt * 16 + u;
+            if (j < bs) acc = fma(rv[u], xs[j], acc);
+        }
     part[qt][row] = acc;
     __syncthreads();
     if (tid < TRB && r < i0) c[r] -= (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
