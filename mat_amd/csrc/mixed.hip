@@ -117,6 +117,51 @@ __global__ __launch_bounds__(256) void k_sumsq(int64_t m, int64_t n, const doubl
     if (threadIdx.x == 0) out[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
+// One pass over an m x n FP64 matrix: the sum of squares of rows
+// [1024 blockIdx.x, + 1024) of columns blockIdx.y, blockIdx.y + gridDim.y,
+// ... into out[blockIdx.y * gridDim.x + blockIdx.x], and, when B is given,
+// the FP32 copy B (ldb).  Four consecutive rows per thread, as one vector
+// access when lda (ldb) and the base allow it.
+__global__ __launch_bounds__(256) void k_cast_sumsq(int64_t m, int64_t n,
+                                                    const double *__restrict__ A, int64_t lda,
+                                                    float *__restrict__ B, int64_t ldb,
+                                                    bool vec, double *__restrict__ out) {
+    __shared__ double red[4];
+    const int64_t r = (int64_t)blockIdx.x * 1024 + 4 * threadIdx.x;
+    double s = 0.0;
+    for (int64_t c = blockIdx.y; c < n; c += gridDim.y) {
+        const double *a = A + r + c * lda;
+        double v[4];
+        if (vec && r + 3 < m) {
+            typedef double d4 __attribute__((ext_vector_type(4)));
+            const d4 x = __builtin_nontemporal_load(reinterpret_cast<const d4 *>(a));
+            v[0] = x[0], v[1] = x[1], v[2] = x[2], v[3] = x[3];
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = r + u < m ? a[u] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s = fma(v[u], v[u], s);
+        if (B) {
+            float *b = B + r + c * ldb;
+            if (vec && r + 3 < m) {
+                typedef float f4 __attribute__((ext_vector_type(4)));
+                const f4 y = {(float)v[0], (float)v[1], (float)v[2], (float)v[3]};
+                *reinterpret_cast<f4 *>(b) = y;
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (r + u < m) b[u] = (float)v[u];
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        out[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 // out[slot] = sum of in[0:cnt] (one workgroup)
 __global__ __launch_bounds__(256) void k_sum(int cnt, const double *__restrict__ in,
                                              double *__restrict__ out, int slot) {
@@ -476,7 +521,7 @@ int panel_fp32(mqr_handle_t hd, int64_t m, int64_t j, int b, MixedWs &w, double 
 // outer block's FP64 buffer (two buffers, by block parity), which the far
 // stream's merged T reads.
 int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t lda,
-                const double *b, MixedWs &w, int bf16_terms) {
+                const double *b, MixedWs &w, int bf16_terms, bool a_cast) {
     hipStream_t s0 = hd->stream, s1 = hd->side_stream, s2 = hd->far_stream;
     const int nb = std::min(hd->nb, kMaxNb);  // panel width; T internal with ld nb
     const int ob = w.ob;
@@ -485,8 +530,9 @@ int factor_fp32(mqr_handle_t hd, int64_t m, int64_t n, const double *A, int64_t 
     const int64_t pcap = (int64_t)kMaxSplits * nb * nc, pcap2 = (int64_t)kMaxSplits * nb * nb;
     {
         ProfScope ps(K_CAST, 0.0, 12.0 * m * nc, s0);
-        hipLaunchKernelGGL(k_cast_d2s, dim3(grid_for(m * n)), dim3(256), 0, s0, m, n, A, lda,
-                           w.Aw, m);
+        if (!a_cast)
+            hipLaunchKernelGGL(k_cast_d2s, dim3(grid_for(m * n)), dim3(256), 0, s0, m, n, A, lda,
+                               w.Aw, m);
         hipLaunchKernelGGL(k_cast_d2s, dim3(grid_for(m)), dim3(256), 0, s0, m, (int64_t)1, b, m,
                            w.Aw + n * m, m);
         MQR_LAUNCH_CHECK();
@@ -997,11 +1043,19 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
     MQR_CHECK(ws_reserve(hd, hd->ws_ir, w.bytes + 4096));
     w = carve((char *)hd->ws_ir.ptr, fp64, m, n, nb, nbt);
 
-    // ||A||_F^2 -> sc[0]
+    // ||A||_F^2 -> sc[0], in the same pass as the FP32 copy of A the
+    // low-precision factorisation starts from
     {
-        ProfScope ps(K_RESID, 2.0 * m * n, 8.0 * m * n, s);
-        hipLaunchKernelGGL(k_sumsq, dim3(1024), dim3(256), 0, s, m, n, A, lda, w.part);
-        hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, s, 1024, (const double *)w.part, w.sc, 0);
+        ProfScope ps(K_RESID, 2.0 * m * n, (fp64 ? 8.0 : 12.0) * m * n, s);
+        const unsigned gx = (unsigned)((m + 1023) / 1024);
+        const unsigned gy = (unsigned)std::min<int64_t>(n, std::max<int64_t>(1, 4096 / gx));
+        const bool vec = lda % 4 == 0 && m % 4 == 0 &&
+                         (reinterpret_cast<uintptr_t>(A) & 31) == 0 &&
+                         (reinterpret_cast<uintptr_t>(w.Aw) & 15) == 0;
+        hipLaunchKernelGGL(k_cast_sumsq, dim3(gx, gy), dim3(256), 0, s, m, n, A, lda,
+                           fp64 ? nullptr : w.Aw, m, vec, w.part);
+        hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, s, (int)(gx * gy),
+                           (const double *)w.part, w.sc, 0);
         MQR_LAUNCH_CHECK();
     }
     // factorisation, then X = R^{-1}
@@ -1018,7 +1072,8 @@ int gels_mixed(mqr_handle_t hd, mqr_precision_t prec, int64_t m, int64_t n, cons
             MQR_CHECK(geqrf_rhs_f64(hd, m, n, w.F, m, w.tau, w.T, nbt, w.r, m, 1));
         } else
             MQR_CHECK(factor_fp32(hd, m, n, A, lda, b, w,
-                                  prec == MQR_PREC_BF16X3 ? 3 : prec == MQR_PREC_BF16 ? 1 : 0));
+                                  prec == MQR_PREC_BF16X3 ? 3 : prec == MQR_PREC_BF16 ? 1 : 0,
+                                  true));
     }
     {
         Range rg("gels_mixed: R inverse");
