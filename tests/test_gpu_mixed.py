@@ -118,7 +118,7 @@ def run_mixed(h, A, b, prec="fp32", **kw):
 
 @pytest.mark.parametrize("m,n,cond,resid", [(2000, 500, 1e6, 1e-10), (1000, 333, 1e2, 1e-10),
                                             (1500, 300, 1e4, 1.0), (700, 700, 1e3, 0.0),
-                                            (4100, 130, 1e5, 1e-6)])
+                                            (4100, 130, 1e5, 1e-6), (1003, 257, 1e4, 1e-8)])
 def test_mixed_fp32_reaches_fp64_backward_error(h, orc, m, n, cond, resid):
     note(f"gels_mixed fp32 m={m} n={n} cond={cond:g} resid={resid:g}")
     A, b = ls_problem(m, n, cond, seed=m + n, resid=resid)
