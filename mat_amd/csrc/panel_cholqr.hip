@@ -83,7 +83,7 @@ __global__ __launch_bounds__(small::NT) void k_recon(int64_t h, int b,
                                                      int *__restrict__ flag,
                                                      const int *__restrict__ one_pass) {
     __shared__ small::Lds P[PB], Q[PB];
-    __shared__ double sg[PB];
+    __shared__ double sg[PB], red[small::NT / 64];
     __shared__ int fail;
     // the fallback's LDS: P, and its small buffers carved from Q
     double *const qa = &Q[0][0];
@@ -125,8 +125,63 @@ __global__ __launch_bounds__(small::NT) void k_recon(int64_t h, int b,
         if (tid == 0) fail = 0;
         small::lds_load(P, G, ldg, b);
         __syncthreads();
-        // R2 = chol(G2) -> P, R2^{-1} -> Q
-        sb::chol_inv(P, Q, b, [](int) { return 1e-2; }, &fail);
+        // e2 = ||G2 - I||_F^2, the same in every wave
+        double e2 = 0.0;
+        for (int e = tid; e < PB * PB; e += small::NT) {
+            const int i = e / PB, j = e % PB;
+            const double d = P[i][j] - (i == j ? 1.0 : 0.0);
+            e2 = fma(d, d, e2);
+        }
+        e2 = small::wave_sum(e2);
+        if (l == 0) red[q] = e2;
+        __syncthreads();
+        e2 = (red[0] + red[1]) + (red[2] + red[3]);
+        if (e2 <= 1e-12) {
+            // G2 = I + E with |E|_F <= 1e-6: R2 = I + F and R2^{-1} from
+            // series instead of the Cholesky's 64 dependent pivots.  With U(X)
+            // the strict upper triangle of X plus half its diagonal,
+            // F = U(E) - U(U(E)^T U(E)) satisfies F + F^T + F^T F = E up to
+            // O(|E|^3), and R2^{-1} = I - F + F^2 up to O(|E|^3) (<= 1e-18).
+            // U(E) -> Q
+            for (int e = tid; e < PB * PB; e += small::NT) {
+                const int i = e / PB, j = e % PB;
+                Q[i][j] = j > i ? P[i][j] : (j == i ? 0.5 * (P[i][i] - 1.0) : 0.0);
+            }
+            __syncthreads();
+            // F -> P, tile row q (U(E) is upper: k-tiles 0..q of U(E)^T U(E))
+#pragma unroll
+            for (int jb = 0; jb < 4; ++jb) {
+                sb::v4d sq = sb::zero4();
+                if (jb >= q)
+                    for (int k = 0; k <= q; ++k)
+                        sq = sb::tmma<true, false>(sq, Q, q, k, Q, k, jb);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = 16 * q + sb::trow(r), j = 16 * jb + sb::tcol();
+                    P[i][j] = j > i ? Q[i][j] - sq[r] : (j == i ? Q[i][i] - 0.5 * sq[r] : 0.0);
+                }
+            }
+            __syncthreads();
+            // R2^{-1} = I - F + F F -> Q, tile row q
+#pragma unroll
+            for (int jb = 0; jb < 4; ++jb) {
+                sb::v4d f2 = sb::zero4();
+                if (jb >= q)
+                    for (int k = q; k <= jb; ++k) f2 = sb::tmma<false, false>(f2, P, q, k, P, k, jb);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = 16 * q + sb::trow(r), j = 16 * jb + sb::tcol();
+                    Q[i][j] = j >= i ? (i == j ? 1.0 : 0.0) - P[i][j] + f2[r] : 0.0;
+                }
+            }
+            __syncthreads();
+            // R2 = I + F
+            for (int i = tid; i < PB; i += small::NT) P[i][i] += 1.0;
+            __syncthreads();
+        } else {
+            // R2 = chol(G2) -> P, R2^{-1} -> Q
+            sb::chol_inv(P, Q, b, [](int) { return 1e-2; }, &fail);
+        }
         if (fail) {  // uniform: read after the barrier that ends chol_inv
             if (tid == 0) *flag = 1;
             __syncthreads();  // the LDS buffers are reused by the fallback
