@@ -162,11 +162,11 @@ int form_y(hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv, cons
 // and a full Q).
 int block_w(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv,
             const double *C, int64_t ldc, int64_t nc, double *P, int64_t pcap, int kind_w,
-            bool fill = false) {
+            bool fill = false, int target = 0) {
     // W itself (splits = 1) must fit the partial buffer; callers chunk C
     if ((int64_t)b * nc > pcap) return MQR_ERR_INTERNAL;
     int splits = fill ? w_pick_splits(b, nc, h, hd->cu_count)
-                      : gemm_pick_splits(b, (int)nc, (int)h, target_blocks(hd));
+                      : gemm_pick_splits(b, (int)nc, (int)h, target > 0 ? target : target_blocks(hd));
     while (splits > 1 && (int64_t)splits * b * nc > pcap) --splits;
     GemmF64 g;
     g.M = b; g.N = (int)nc; g.K = (int)h;
@@ -180,7 +180,7 @@ int block_w(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double *Vp,
 
 int apply_block(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv,
                 const double *Y, double *C, int64_t ldc, int64_t nc, double *P, int64_t pcap,
-                int kind_w = K_VTA, int kind_u = K_UPDATE, bool fill = false) {
+                int kind_w = K_VTA, int kind_u = K_UPDATE, bool fill = false, int target = 0) {
     if (nc <= 0) return MQR_SUCCESS;
     // column chunks whose W (b x chunk) fits the partial buffer P, so that a
     // right-hand side wider than the matrix (least squares with many
@@ -189,7 +189,7 @@ int apply_block(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double 
     for (int64_t c0 = 0; c0 < nc; c0 += chunk) {
         const int64_t ncc = std::min<int64_t>(chunk, nc - c0);
         double *Cc = C + c0 * ldc;
-        MQR_CHECK(block_w(hd, st, h, b, Vp, ldv, Cc, ldc, ncc, P, pcap, kind_w, fill));
+        MQR_CHECK(block_w(hd, st, h, b, Vp, ldv, Cc, ldc, ncc, P, pcap, kind_w, fill, target));
         GemmF64 u;
         u.M = (int)h; u.N = (int)ncc; u.K = b;
         u.A = Y; u.lda = h; u.B = P; u.ldb = b;
@@ -204,7 +204,7 @@ int apply_block(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double 
 namespace {
 int merged_t(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double *V, int64_t ldv,
              const double *T, int64_t ldt, int64_t j0, double *G, double *Tb, double *tmp,
-             double *P, int64_t pcap);
+             double *P, int64_t pcap, int target = 0);
 }  // namespace
 
 int form_y_f64(hipStream_t st, int64_t h, int b, const double *Vp, int64_t ldv, const double *Tp,
@@ -362,6 +362,11 @@ int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda,
     // next-but-one block's columns [c2, c2 + ob) go first, and
     // ev.merged_a[J & 1] marks them: the deferred parts of block J + 1's
     // panels (side stream) wait for those columns only, not for the rest.
+    // the merged updates' split-K products aim at one workgroup per CU, not
+    // four: they run beside the panel chain and the side stream, and fewer,
+    // longer workgroups leave those more room (16384 x 4096 least squares
+    // 19.27 -> 19.08 ms)
+    const int far_target = hd->cu_count;
     auto merge_block = [&](int64_t J) -> int {
         const int64_t c0 = J * ob, bo = std::min<int64_t>(ob, k - c0);
         const int64_t h = m - c0;
@@ -376,7 +381,7 @@ int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda,
         int64_t ldtj = ldt;
         if (bo > ib) {
             MQR_CHECK(merged_t(hd, s2, h, (int)bo, V, lda, T, ldt, c0, w.G, w.Tb, w.tmp, w.P3,
-                               w.pcap3));
+                               w.pcap3, far_target));
             Tj = w.Tb;
             ldtj = bo;
         }
@@ -385,18 +390,19 @@ int geqrf_rhs_f64(mqr_handle_t hd, int64_t m, int64_t n, double *A, int64_t lda,
             if (far) {
                 const int64_t na = std::min<int64_t>(ob, n - c2);
                 MQR_CHECK(apply_block(hd, s2, h, (int)bo, V, lda, w.Yo, A + c0 + c2 * lda, lda,
-                                      na, w.P3, w.pcap3, K_LOOKAHEAD, K_LOOKAHEAD));
+                                      na, w.P3, w.pcap3, K_LOOKAHEAD, K_LOOKAHEAD, false,
+                                      far_target));
                 MQR_HIP(hipEventRecord(ev.merged_a[J & 1], s2));
                 if (n - c2 > na)
                     MQR_CHECK(apply_block(hd, s2, h, (int)bo, V, lda, w.Yo,
                                           A + c0 + (c2 + na) * lda, lda, n - c2 - na, w.P3,
-                                          w.pcap3));
+                                          w.pcap3, K_VTA, K_UPDATE, false, far_target));
             } else {
                 MQR_HIP(hipEventRecord(ev.merged_a[J & 1], s2));
             }
             if (B)
                 MQR_CHECK(apply_block(hd, s2, h, (int)bo, V, lda, w.Yo, B + c0, ldb, nrhs, w.P3,
-                                      w.pcap3, K_APPLYQ, K_APPLYQ));
+                                      w.pcap3, K_APPLYQ, K_APPLYQ, false, far_target));
         }
         if (!(far || B)) MQR_HIP(hipEventRecord(ev.merged_a[J & 1], s2));
         // nb > 64: T returns the merged nb x nb blocks
@@ -649,8 +655,8 @@ __global__ void k_merged_t_init(int nbig, const double *__restrict__ T, int64_t 
 // b/2 x b doubles; P (pcap doubles) holds split-K partials.
 int merged_t(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const double *V, int64_t ldv,
              const double *T, int64_t ldt, int64_t j0, double *G, double *Tb, double *tmp,
-             double *P, int64_t pcap) {
-    int splits = gemm_pick_splits(b, b, (int)h, target_blocks(hd));
+             double *P, int64_t pcap, int target) {
+    int splits = gemm_pick_splits(b, b, (int)h, target > 0 ? target : target_blocks(hd));
     while (splits > 1 && (int64_t)splits * b * b > pcap) --splits;
     GemmF64 g;
     g.M = b; g.N = b; g.K = (int)h;
