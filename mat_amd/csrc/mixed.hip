@@ -65,18 +65,33 @@ __global__ void k_cast_v(int64_t h, int b, const double *__restrict__ V, int64_t
 
 // P[0:count] = sum over s of P[s * stride + 0:count] (FP32 partials summed in
 // FP64)
-__global__ void k_reduce_f32(int64_t count, float *__restrict__ P, int splits, int64_t stride) {
-    const int64_t e = gidx();
-    if (e >= count) return;
-    // eight independent loads per round keep the partials' reads in flight
-    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    int s = 0;
-    for (; s + 7 < splits; s += 8) {
-#pragma unroll
-        for (int u = 0; u < 8; ++u) a[u] += (double)P[(int64_t)(s + u) * stride + e];
+// P[0:count] = sum of the splits partials P[s stride : s stride + count]
+// (FP64 sums).  G threads per entry (thread group g sums partials g, g + G,
+// ...; the G sums meet in LDS in a fixed order): few entries with many
+// partials (the next panel's 64 x 64 W from ~100 K chunks) still keep many
+// loads in flight.  Each workgroup reads and writes only its own entries.
+__global__ __launch_bounds__(256) void k_reduce_f32(int64_t count, float *__restrict__ P,
+                                                    int splits, int64_t stride, int G) {
+    __shared__ double part[256];
+    const int E = 256 / G;
+    const int t = threadIdx.x, g = t / E;
+    const int64_t e = (int64_t)blockIdx.x * E + (t % E);
+    double a0 = 0.0, a1 = 0.0;
+    if (e < count) {
+        int s = g;
+        for (; s + G < splits; s += 2 * G) {
+            a0 += (double)P[(int64_t)s * stride + e];
+            a1 += (double)P[(int64_t)(s + G) * stride + e];
+        }
+        if (s < splits) a0 += (double)P[(int64_t)s * stride + e];
     }
-    for (int u = 0; s < splits; ++s, ++u) a[u] += (double)P[(int64_t)s * stride + e];
-    P[e] = (float)(((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7])));
+    part[t] = a0 + a1;
+    __syncthreads();
+    if (g == 0 && e < count) {
+        double sum = part[t];
+        for (int k = 1; k < G; ++k) sum += part[t + k * E];
+        P[e] = (float)sum;
+    }
 }
 
 // R (ldr) entries of the block rows right of the diagonal blocks:
@@ -472,8 +487,12 @@ int apply_block_f32(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const flo
     MQR_CHECK(gemm_f32(f, st));
     if (sp > 1) {
         ProfScope ps(kind_w, 0.0, 4.0 * b * nc * (sp + 1), st);
-        hipLaunchKernelGGL(k_reduce_f32, dim3((unsigned)(((int64_t)b * nc + 255) / 256)),
-                           dim3(256), 0, st, (int64_t)b * nc, P, sp, (int64_t)b * nc);
+        const int64_t cnt = (int64_t)b * nc;
+        int G = 1;
+        while (G < 16 && 2 * G <= sp && cnt * G / 256 < 2048) G *= 2;
+        const int E = 256 / G;
+        hipLaunchKernelGGL(k_reduce_f32, dim3((unsigned)((cnt + E - 1) / E)), dim3(256), 0, st,
+                           cnt, P, sp, cnt, G);
         MQR_LAUNCH_CHECK();
     }
     GemmF32 u;
