@@ -306,11 +306,11 @@ int gemm_pick_splits(int M, int N, int K, int target_blocks) {
     return s < 1 ? 1 : s;
 }
 
-int w_pick_splits(int b, int64_t nc, int64_t h, int cus) {
+int w_pick_splits(int b, int64_t nc, int64_t h, int cus, int min_rows) {
     const int t = (b >= BIG && nc >= BIG) ? BIG : SMALL;
     const int64_t tiles = ((b + t - 1) / t) * ((nc + t - 1) / t);
     int64_t sp = (4 * (int64_t)cus + tiles - 1) / std::max<int64_t>(tiles, 1);
-    sp = std::min<int64_t>(sp, (h + 127) / 128);
+    sp = std::min<int64_t>(sp, (h + min_rows - 1) / min_rows);
     sp = std::min<int64_t>(sp, 256);
     return (int)std::max<int64_t>(sp, 1);
 }

@@ -228,9 +228,10 @@ int scale_f64(int64_t m, int64_t n, double beta, double *C, int64_t ldc, hipStre
 // Number of K splits giving roughly `target_blocks` workgroups.
 int gemm_pick_splits(int M, int N, int K, int target_blocks);
 // K splits of W = V^T C (b x nc, K = h rows): enough workgroups for four per
-// CU, K chunks of at least 128 rows, at most 256 splits.  A narrow C (the next panel's 64 columns) is one tile column,
-// which gemm_pick_splits' 256-row chunks would leave on a quarter of the CUs.
-int w_pick_splits(int b, int64_t nc, int64_t h, int cus);
+// CU (of `cus`), K chunks of at least min_rows rows, at most 256 splits.  A
+// narrow C (the next panel's 64 columns) is one tile column, which
+// gemm_pick_splits' 256-row chunks would leave on a quarter of the CUs.
+int w_pick_splits(int b, int64_t nc, int64_t h, int cus, int min_rows = 128);
 
 // ---- panel / T / block reflector ------------------------------------------
 // Every panel routine factors the h x b panel P (ld) in place (R on and above
