@@ -236,7 +236,66 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_f32(GemmF32 gin, int kchunk) {
             for (int j = 0; j < BW; ++j) acc[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
     }
 
-    if (kb < ke) {
+    // one slice of the FP32 mode: four K = 4 steps of BW x BW MFMAs
+    auto mma_f32 = [&](int bf) {
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            float a[BW], b[BW];
+#pragma unroll
+            for (int i = 0; i < BW; ++i) a[i] = As[bf][kk + (lane >> 4)][arow + i];
+#pragma unroll
+            for (int j = 0; j < BW; ++j)
+                b[j] = Bs[bf][kk + (lane >> 4)][wn * HALF + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < BW; ++i)
+#pragma unroll
+                for (int j = 0; j < BW; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(b[j], a[i], acc[i][j], 0, 0, 0);
+        }
+    };
+
+    if constexpr (TILE == 64 && MODE == 0) {
+        // The narrow FP32 tile spends 16 MFMAs per wave on a slice, less
+        // than the latency of the loads that bring the next one: two slices
+        // are kept in flight, in two register sets that trade roles every
+        // slice.  At the top of the loop slice k0 is in As/Bs[buf], slice
+        // k0 + BK (when there is one) in wa/wb, and va/vb are free.
+        if (kb < ke) {
+            float va[NV], vb[NV], wa[NV], wb[NV];
+            fetch<TILE, TA, BKM>(g.A, g.lda, m0, kb, g.M, ke, tid, va);
+            fetch<TILE, !TB, BKM>(g.B, g.ldb, n0, kb, g.N, ke, tid, vb);
+            if (kb + BK < ke) {
+                fetch<TILE, TA, BKM>(g.A, g.lda, m0, kb + BK, g.M, ke, tid, wa);
+                fetch<TILE, !TB, BKM>(g.B, g.ldb, n0, kb + BK, g.N, ke, tid, wb);
+            }
+            stash<TILE, TA, BKM>(As[0], tid, va, m0, kb, g.M, ke, g.alpha);
+            stash<TILE, !TB, BKM>(Bs[0], tid, vb, n0, kb, g.N, ke, 1.f);
+            __syncthreads();
+            int buf = 0;
+            for (int k0 = kb; k0 < ke; k0 += 2 * BK) {
+                if (k0 + 2 * BK < ke) {
+                    fetch<TILE, TA, BKM>(g.A, g.lda, m0, k0 + 2 * BK, g.M, ke, tid, va);
+                    fetch<TILE, !TB, BKM>(g.B, g.ldb, n0, k0 + 2 * BK, g.N, ke, tid, vb);
+                }
+                mma_f32(buf);
+                if (k0 + BK >= ke) break;
+                stash<TILE, TA, BKM>(As[buf ^ 1], tid, wa, m0, k0 + BK, g.M, ke, g.alpha);
+                stash<TILE, !TB, BKM>(Bs[buf ^ 1], tid, wb, n0, k0 + BK, g.N, ke, 1.f);
+                __syncthreads();
+                buf ^= 1;
+                if (k0 + 3 * BK < ke) {
+                    fetch<TILE, TA, BKM>(g.A, g.lda, m0, k0 + 3 * BK, g.M, ke, tid, wa);
+                    fetch<TILE, !TB, BKM>(g.B, g.ldb, n0, k0 + 3 * BK, g.N, ke, tid, wb);
+                }
+                mma_f32(buf);
+                if (k0 + 2 * BK >= ke) break;
+                stash<TILE, TA, BKM>(As[buf ^ 1], tid, va, m0, k0 + 2 * BK, g.M, ke, g.alpha);
+                stash<TILE, !TB, BKM>(Bs[buf ^ 1], tid, vb, n0, k0 + 2 * BK, g.N, ke, 1.f);
+                __syncthreads();
+                buf ^= 1;
+            }
+        }
+    } else if (kb < ke) {
         float va[NV], vb[NV];
         fetch<TILE, TA, BKM>(g.A, g.lda, m0, kb, g.M, ke, tid, va);
         fetch<TILE, !TB, BKM>(g.B, g.ldb, n0, kb, g.N, ke, tid, vb);
@@ -251,21 +310,7 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_f32(GemmF32 gin, int kchunk) {
                 fetch<TILE, !TB, BKM>(g.B, g.ldb, n0, k0 + BKM, g.N, ke, tid, vb);
             }
             if constexpr (MODE == 0) {
-#pragma unroll
-                for (int kk = 0; kk < BK; kk += 4) {
-                    float a[BW], b[BW];
-#pragma unroll
-                    for (int i = 0; i < BW; ++i) a[i] = As[buf][kk + (lane >> 4)][arow + i];
-#pragma unroll
-                    for (int j = 0; j < BW; ++j)
-                        b[j] = Bs[buf][kk + (lane >> 4)][wn * HALF + j * 16 + (lane & 15)];
-#pragma unroll
-                    for (int i = 0; i < BW; ++i)
-#pragma unroll
-                        for (int j = 0; j < BW; ++j)
-                            acc[i][j] =
-                                __builtin_amdgcn_mfma_f32_16x16x4f32(b[j], a[i], acc[i][j], 0, 0, 0);
-                }
+                mma_f32(buf);
             } else {
                 // one 32-deep MFMA per tile and term: lane l supplies
                 // k = 8 (l >> 4) + q, q = 0..7, of row / column l & 15 (A and
