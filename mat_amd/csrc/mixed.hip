@@ -474,13 +474,16 @@ int apply_block_f32(mqr_handle_t hd, hipStream_t st, int64_t h, int b, const flo
                     const float *Y32, float *C, int64_t ldc, int64_t nc, float *P, int64_t pcap,
                     int kind_w, int kind_u, int bf16_terms) {
     if (nc <= 0) return MQR_SUCCESS;
-    // the bulk of a panel's update splits K for eight workgroups per CU
-    // (w_pick_splits aims at four): 16384 x 4096 mixed LS 15.80 -> 15.65 ms,
-    // against 16.9 and 19.7 ms at two and one per CU and 16.0 at sixteen.
+    // the bulk of a panel's update splits K for six workgroups per CU
+    // (w_pick_splits aims at four).  The 64 x 64 FP32 tile holds 70 VGPRs,
+    // seven workgroups per CU: at six the W launch is one round beside the
+    // panel kernels, at seven and eight a tail round follows (16384 x 4096
+    // mixed LS 14.67-14.77 ms at five and six per CU, 15.12-15.28 at seven
+    // and eight; at four, 15.27-15.46 with a fifth refinement step).
     // K chunks down to 64 rows (128 in the FP64 path): the next panel's
     // 64 x 64 W is one tile, and shorter chunks give it more workgroups
     // (15.45 -> 15.35 ms)
-    int sp = w_pick_splits(b, nc, h, nc > 64 ? 2 * hd->cu_count : hd->cu_count, 64);
+    int sp = w_pick_splits(b, nc, h, nc > 64 ? 3 * hd->cu_count / 2 : hd->cu_count, 64);
     while (sp > 1 && (int64_t)sp * b * nc > pcap) --sp;
     GemmF32 f;
     f.M = b; f.N = (int)nc; f.K = (int)h; f.ta = true;
